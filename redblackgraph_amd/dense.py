@@ -1,0 +1,135 @@
+"""Device-level dense operations: Floyd-Warshall closure, AVOS product,
+relational composition.  Inputs and outputs are torch tensors resident on the
+GPU, in the widened device domain (see _dtypes).  The reference-shaped API in
+core.py wraps these.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+from . import _lib
+from ._dtypes import ElemType, msb
+
+FW_TILE = 128
+_ws_cache: dict = {}
+
+
+def _workspace(npad: int, elem: int):
+    t = _lib.torch()
+    dev = _lib.device()
+    key = (dev.index, npad, elem)
+    ws = _ws_cache.get(key)
+    if ws is None:
+        nbytes = int(_lib.load().rbg_fw_workspace_bytes(npad, elem))
+        ws = t.empty(nbytes, dtype=t.uint8, device=dev)
+        _ws_cache.clear()  # keep one workspace per device
+        _ws_cache[key] = ws
+    return ws
+
+
+def padded_order(n: int) -> int:
+    return max(FW_TILE, ((n + FW_TILE - 1) // FW_TILE) * FW_TILE)
+
+
+@dataclass
+class FwOutcome:
+    status: _lib.RbgStatus
+    npad: int
+    mode: str  # "fast" | "general"
+
+    @property
+    def diameter(self) -> int:
+        return msb(self.status.diameter_max)
+
+    def key_kij(self, key: int):
+        n = self.npad
+        return key // (n * n), (key // n) % n, key % n
+
+
+def _run(work, et: ElemType, mode: int, st: _lib.Status, stream):
+    t = _lib.torch()
+    npad = work.shape[0]
+    elem = work.element_size()
+    ws = _workspace(npad, elem)
+    fn = "rbg_fw_dense_i32" if work.dtype == t.int32 else "rbg_fw_dense_i64"
+    _lib.call(fn, _lib.ptr(work), npad, npad, et.width, int(et.unsigned), mode,
+              _lib.ptr(ws), ws.numel(), st.p, _lib.stream_handle(stream))
+
+
+def fw_closure(W, et: ElemType, stream=None, force_general: bool = False):
+    """Close an n x n device matrix.  Returns (closed n x n tensor, FwOutcome).
+
+    The input tensor is not modified.  Runs the fast kernels when the input is
+    inside their domain (rbg_fw_precheck) and falls back to the general kernels
+    otherwise -- both on the GPU.
+    """
+    t = _lib.torch()
+    n = W.shape[0]
+    npad = padded_order(n)
+    if npad == n:
+        work = W.clone()
+    else:
+        work = t.zeros((npad, npad), dtype=W.dtype, device=W.device)
+        work[:n, :n] = W
+    st = _lib.Status()
+    st.reset(stream)
+    fast = et.native and not force_general
+    if fast:
+        fn = "rbg_fw_precheck_i32" if work.dtype == t.int32 else "rbg_fw_precheck_i64"
+        _lib.call(fn, _lib.ptr(work), npad, npad, st.p, _lib.stream_handle(stream))
+        s = st.read(stream)
+        if s.flags & _lib.F_PRECOND_FAIL:
+            fast = False
+        st.reset(stream)
+    src = work.clone() if fast else None
+    _run(work, et, 0 if fast else 1, st, stream)
+    s = st.read(stream)
+    mode = "fast" if fast else "general"
+    if fast and (s.flags & _lib.F_NEED_GENERAL):
+        work = src
+        st.reset(stream)
+        _run(work, et, 1, st, stream)
+        s = st.read(stream)
+        mode = "general"
+    return work[:n, :n], FwOutcome(s, npad, mode)
+
+
+def matmul(A, B, et: ElemType, stream=None):
+    """C = A (x) B for device matrices (M x K) and (K x N)."""
+    t = _lib.torch()
+    M, K = A.shape
+    K2, N = B.shape
+    assert K == K2
+    C = t.empty((M, N), dtype=A.dtype, device=A.device)
+    st = _lib.Status()
+    st.reset(stream)
+    fn = "rbg_matmul_i32" if A.dtype == t.int32 else "rbg_matmul_i64"
+    A = A.contiguous()
+    B = B.contiguous()
+    _lib.call(fn, _lib.ptr(A), K, _lib.ptr(B), N, _lib.ptr(C), N, M, K, N, et.width, int(et.unsigned),
+              st.p, _lib.stream_handle(stream))
+    return C, st.read(stream)
+
+
+def edge_compose(R, alpha: int, beta: int, rel: int, et: ElemType, stream=None):
+    t = _lib.torch()
+    n = R.shape[0]
+    out = t.empty_like(R)
+    st = _lib.Status()
+    st.reset(stream)
+    fn = "rbg_edge_compose_i32" if R.dtype == t.int32 else "rbg_edge_compose_i64"
+    _lib.call(fn, _lib.ptr(R.contiguous()), n, alpha, beta, int(rel), _lib.ptr(out), et.width,
+              int(et.unsigned), st.p, _lib.stream_handle(stream))
+    return out, st.read(stream)
+
+
+def vertex_compose(u, R, v, color: int, et: ElemType, stream=None):
+    t = _lib.torch()
+    n = R.shape[0]
+    out = t.zeros((n + 1, n + 1), dtype=R.dtype, device=R.device)
+    st = _lib.Status()
+    st.reset(stream)
+    fn = "rbg_vertex_compose_i32" if R.dtype == t.int32 else "rbg_vertex_compose_i64"
+    _lib.call(fn, _lib.ptr(u.contiguous()), _lib.ptr(R.contiguous()), _lib.ptr(v.contiguous()), int(color), n,
+              _lib.ptr(out), et.width, int(et.unsigned), st.p, _lib.stream_handle(stream))
+    return out, st.read(stream)

@@ -1,0 +1,274 @@
+"""GPU parity of the dense path (element-wise AVOS, Floyd-Warshall closure,
+AVOS matrix product, relational composition) against the CPU oracle, the
+reference's known answers and size-independent properties.  Every call goes
+through librbg_hip.so's C ABI."""
+import itertools
+
+import numpy as np
+import pytest
+
+import redblackgraph_amd as rb
+from redblackgraph_amd import dense, synth, _lib
+from redblackgraph_amd._dtypes import elem_type
+
+pytestmark = pytest.mark.gpu
+
+DTYPES = [np.int8, np.uint8, np.int16, np.uint16, np.int32, np.uint32, np.int64, np.uint64]
+
+
+def _as(dt, v):
+    return np.array(v).astype(dt) if np.dtype(dt).kind == "u" else np.array(v, dtype=dt)
+
+
+def _fw(A, force_general=False):
+    et = elem_type(A.dtype)
+    from redblackgraph_amd._dtypes import widen, narrow
+    W, out = dense.fw_closure(_lib.to_device(widen(A, et)), et, force_general=force_general)
+    return narrow(W.cpu().numpy(), et), out
+
+
+# ---------------------------------------------------------------------------
+# element-wise kernels
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("dt", [np.int32, np.int64])
+def test_elementwise_vs_oracle(orc, dt):
+    from redblackgraph_amd.gpu import avos_sum_gpu, avos_product_gpu
+    vals = list(range(-1, 70))
+    x = np.array([a for a, b in itertools.product(vals, vals)], dtype=dt)
+    y = np.array([b for a, b in itertools.product(vals, vals)], dtype=dt)
+    zs = avos_sum_gpu(x, y)
+    zp = avos_product_gpu(x, y)
+    for i in range(0, len(x), 7):
+        assert zs[i] == orc.avos_sum(int(x[i]), int(y[i]), dt)
+        assert zp[i] == orc.avos_product(int(x[i]), int(y[i]), dt)[0]
+
+
+def test_elementwise_table(golden):
+    from redblackgraph_amd.gpu import avos_product_gpu, avos_sum_gpu
+    g = golden["product_table"]
+    r = list(range(g["range"][0], g["range"][1] + 1))
+    x = np.array([a for a, b in itertools.product(r, r)], dtype=np.int32)
+    y = np.array([b for a, b in itertools.product(r, r)], dtype=np.int32)
+    np.testing.assert_array_equal(avos_product_gpu(x, y), g["values"])
+    g = golden["sum_table"]
+    r = list(range(g["range"][0], g["range"][1] + 1))
+    x = np.array([a for a, b in itertools.product(r, r)], dtype=np.int32)
+    y = np.array([b for a, b in itertools.product(r, r)], dtype=np.int32)
+    np.testing.assert_array_equal(avos_sum_gpu(x, y), g["values"])
+
+
+# ---------------------------------------------------------------------------
+# Floyd-Warshall
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("dt", DTYPES)
+def test_fw_apsp5(golden, dt):
+    g = golden["apsp_5"]
+    A = rb.array(g["A"], dtype=dt)
+    res = A.transitive_closure()
+    np.testing.assert_array_equal(res.W, _as(dt, g["W"]))
+    assert res.diameter == g["diameter"]
+    assert isinstance(res.W, rb.array)
+
+
+def test_fw_apsp5_module_function(golden):
+    res = rb.transitive_closure(rb.array(golden["apsp_5"]["A"]))
+    np.testing.assert_array_equal(res.W, golden["apsp_5"]["W"])
+    assert res.diameter == 3
+
+
+def test_fw_apsp7(golden):
+    import scipy.sparse as sp
+    g = golden["apsp_7"]
+    for dt in (np.int32, np.uint32):
+        A = sp.coo_matrix((_as(dt, g["data"]), (g["rows"], g["cols"])), shape=(7, 7)).toarray()
+        res = rb.array(A).transitive_closure()
+        np.testing.assert_array_equal(res.W.astype(np.int64), _as(dt, g["W"]).astype(np.int64))
+        assert res.diameter == g["diameter"]
+
+
+def test_fw_cycle_message(orc, golden):
+    A = np.array(golden["cycle_5"]["A"], dtype=np.int32)
+    ref = orc.warshall(A).report
+    with pytest.raises(ValueError) as e:
+        rb.array(A).transitive_closure()
+    msg = str(e.value)
+    assert golden["cycle_5"]["substring"] in msg
+    k, i = ref["last_k"], ref["last_i"]
+    exp = (f"Error: cycle detected! Vertex {i} has a path to itself. "
+           f"A({i},{k})={ref['last_a'] & 0xFFFFFFFF}, A({k},{i})={ref['last_b'] & 0xFFFFFFFF}")
+    assert msg == exp
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_fw_dense512_config(orc, seed):
+    """The N=512 int32 pedigree workload: W and diameter bit-exact vs the oracle."""
+    A = synth.dense_pedigree(synth.DENSE_512, seed)
+    ref = orc.warshall(A)
+    W, out = _fw(A)
+    assert out.mode == "fast"
+    np.testing.assert_array_equal(W, ref.W)
+    assert out.diameter == ref.diameter
+    assert out.status.cycle_count == 0 and out.status.events == 0
+
+
+@pytest.mark.parametrize("seed", [3, 4])
+def test_fw_general_equals_fast(orc, seed):
+    A = synth.dense_pedigree(synth.PedigreeSpec(384, 12, 0.8, 4), seed)
+    Wf, of = _fw(A)
+    Wg, og = _fw(A, force_general=True)
+    assert of.mode == "fast" and og.mode == "general"
+    np.testing.assert_array_equal(Wf, Wg)
+    assert of.diameter == og.diameter
+    ref = orc.warshall(A)
+    np.testing.assert_array_equal(Wf, ref.W)
+    assert of.diameter == ref.diameter
+
+
+@pytest.mark.parametrize("seed,n", [(42, 20), (123, 15), (456, 15), (7, 200)])
+def test_fw_reference_random_dags(orc, seed, n):
+    """The reference's parity-inconsistent random DAGs (off-diagonal 1s): general mode."""
+    A = synth.random_upper_dag(n, seed)
+    ref = orc.warshall(A)
+    W, out = _fw(A)
+    assert out.mode == "general"
+    np.testing.assert_array_equal(W, ref.W)
+    assert out.diameter == ref.diameter
+
+
+def test_fw_int64(orc):
+    A = synth.dense_pedigree(synth.PedigreeSpec(300, 14, 0.75, 4), 9, dtype=np.int64)
+    ref = orc.warshall(A)
+    W, out = _fw(A)
+    assert out.mode == "fast"
+    np.testing.assert_array_equal(W, ref.W)
+    assert out.diameter == ref.diameter
+
+
+def test_fw_cyclic_pedigree_errors(orc):
+    """A pedigree with back edges: last (core) and first (csgraph) detections
+    and their operands must equal the naive loop's."""
+    A = synth.dense_pedigree(synth.PedigreeSpec(260, 10, 0.8, 3), 5)
+    rng = np.random.default_rng(5)
+    for _ in range(6):
+        i, j = sorted(rng.integers(0, 260, 2))
+        if i != j:
+            A[i, j] = 2 if A[j, j] == -1 else 3
+    ref = orc.warshall(A).report
+    assert ref["n_cycle"] > 0
+    W, out = _fw(A)
+    s = out.status
+    assert s.cycle_count > 0
+    assert out.key_kij(s.cycle_first_key)[:2] == (ref["first_k"], ref["first_i"])
+    assert out.key_kij(s.cycle_last_key)[:2] == (ref["last_k"], ref["last_i"])
+    assert (s.cycle_first_a, s.cycle_first_b) == (ref["first_a"], ref["first_b"])
+    assert (s.cycle_last_a, s.cycle_last_b) == (ref["last_a"], ref["last_b"])
+    from redblackgraph_amd.sparse.csgraph import CycleError
+    with pytest.raises(ValueError):
+        rb.array(A).transitive_closure()
+    with pytest.raises(CycleError) as e:
+        rb.sparse.transitive_closure_floyd_warshall(rb.array(A))
+    assert e.value.vertex == ref["first_i"]
+
+
+def test_fw_1024_vs_oracle(orc):
+    A = synth.dense_pedigree(synth.PedigreeSpec(1024, 16, 0.75, 4), 11)
+    ref = orc.warshall(A)
+    W, out = _fw(A)
+    np.testing.assert_array_equal(W, ref.W)
+    assert out.diameter == ref.diameter
+
+
+def test_fw_8192_properties():
+    """Full-size workload: fast vs general arithmetic agree, the closure is a
+    fixed point (closing it again changes nothing), lower-triangular input
+    stays lower triangular."""
+    t = _lib.torch()
+    A = synth.dense_pedigree(synth.DENSE_8192, 0)
+    et = elem_type(A.dtype)
+    Ad = _lib.to_device(A)
+    W1, o1 = dense.fw_closure(Ad, et)
+    assert o1.mode == "fast"
+    W2, o2 = dense.fw_closure(W1, et)
+    assert bool(t.equal(W1, W2))
+    assert bool((t.triu(W1, diagonal=1) == 0).all())
+    diag = t.diagonal(W1)
+    assert bool(t.equal(diag, t.diagonal(Ad)))
+    # spot-check rows against the Ahnentafel rule: W[i][p] for each direct parent p
+    Wc = W1.cpu().numpy()
+    rows, cols = np.nonzero(np.tril(A, -1))
+    np.testing.assert_array_equal(Wc[rows, cols], A[rows, cols])
+    assert o1.diameter >= 1
+
+
+# ---------------------------------------------------------------------------
+# AVOS product and composition
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("dt", DTYPES)
+def test_matmul_known(golden, dt):
+    g = golden["matmul_5"]
+    A = rb.array(g["A"], dtype=dt)
+    np.testing.assert_array_equal(A @ A, _as(dt, g["AA"]))
+    np.testing.assert_array_equal(A @ A @ A, _as(dt, g["A_star"]))
+    np.testing.assert_array_equal((A @ A) @ A, _as(dt, g["A_star"]))
+    u = rb.array(g["u"], dtype=dt).reshape((1, 5))
+    v = rb.array(g["v"], dtype=dt).reshape((5, 1))
+    S = rb.array(g["A_star"], dtype=dt)
+    np.testing.assert_array_equal(u @ S, np.array(g["u_lambda"]).reshape(1, 5))
+    np.testing.assert_array_equal(S @ v, np.array(g["v_lambda"]).reshape(5, 1))
+
+
+def test_vector_products(golden):
+    g = golden["vector_product"]
+    assert rb.array(g["u"]) @ rb.array(g["v"]) == g["result"]
+    g = golden["vecmat"]
+    np.testing.assert_array_equal(rb.array(g["u"]) @ rb.array(g["A"]), g["uA"])
+    np.testing.assert_array_equal(np.array(g["u"]) @ rb.array(g["A"]), g["uA"])
+    np.testing.assert_array_equal(rb.array(g["A"]) @ rb.array(g["v"]), g["Av"])
+
+
+def test_matmul_random_vs_oracle(orc):
+    rng = np.random.default_rng(2)
+    A = synth.dense_pedigree(synth.PedigreeSpec(200, 8, 0.8, 3), 1)
+    W = orc.warshall(A).W
+    B = np.where(rng.random((200, 70)) < 0.1, rng.integers(2, 30, (200, 70)), 0).astype(np.int32)
+    ref, ev = orc.matmul(W, B)
+    np.testing.assert_array_equal(rb.array(W) @ rb.array(B), ref)
+
+
+def test_compositions(golden):
+    for key in ("vertex_compose_1", "vertex_compose_2", "vertex_compose_14"):
+        g = golden[key]
+        A = rb.array(g["A"], dtype=np.int32)
+        out = rb.vertex_relational_composition(np.array(g["u"]).reshape(1, -1), A, np.array(g["v"]).reshape(-1, 1),
+                                               g["color"])
+        np.testing.assert_array_equal(out, g["out"])
+    g = golden["vertex_cycle"]
+    A_star = rb.array(g["A"]).transitive_closure().W
+    with pytest.raises(ValueError) as e:
+        A_star.vertex_relational_composition(rb.array(g["u"]), rb.array(g["v"]), g["color"])
+    assert str(e.value) == g["message"]
+    for key in ("edge_compose_simple", "edge_compose_15"):
+        g = golden[key]
+        out = rb.edge_relational_composition(rb.array(g["R"]), g["alpha"], g["beta"], g["rel"])
+        np.testing.assert_array_equal(out, g["out"])
+
+
+def test_edge_composition_equals_reclosure(orc):
+    """Incremental edge insertion == closing the graph with the edge added."""
+    A = synth.dense_pedigree(synth.PedigreeSpec(300, 10, 0.7, 3), 21)
+    R = orc.warshall(A).W
+    rng = np.random.default_rng(3)
+    done = 0
+    for _ in range(200):
+        c, p = int(rng.integers(50, 300)), int(rng.integers(0, 50))
+        if A[c, p] == 0 and R[p, c] == 0 and not A[c].tolist().count(2 if A[p, p] == -1 else 3):
+            rel = 2 if A[p, p] == -1 else 3
+            A2 = A.copy()
+            A2[c, p] = rel
+            exp = orc.warshall(A2).W
+            got = rb.edge_relational_composition(rb.array(R), c, p, rel)
+            np.testing.assert_array_equal(got, exp)
+            done += 1
+            if done == 3:
+                break
+    assert done > 0
