@@ -1,0 +1,178 @@
+#!/usr/bin/env python3
+"""AVOS closure throughput on one MI355X.
+
+Default workload: dense Floyd-Warshall closure of the seeded N=8192 int32
+pedigree (24 layers, parents known with p=0.75 from the previous 1-4 layers).
+A step is one full closure of that matrix, input already resident in HBM,
+through the same entry point users call (redblackgraph_amd.dense.fw_closure:
+copy, fast-path precheck, closure, status read-back).
+
+Prints one JSON line (the bench contract) with:
+  value          closure entries/s = N^2 / seconds per closure
+  roofline       the dominant kernel (k_fw_tile_fast<int>, phase 3): integer
+                 VALU ops it must execute (2.5 per relaxation) over its average
+                 launch duration, measured with hipEvents on its launch stream
+  cpu_baseline   the oracle's single-core naive loop (a C restatement of the
+                 reference's warshall.c.in) on a bounded sample, scaled to the
+                 workload by the N^3 law
+Other workloads (--workload): dense4096_i64, sparse1m, batched.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "AVOS closure entries/sec, dense N=8192 & sparse 1M-vertex; batched vs plain"
+PEAK_VALU_OPS = 256 * 64 * 2.4e9  # int32 VALU lane-ops/s (256 CU x 64 lanes x 2.4 GHz)
+PEAK_HBM = 8.0e12
+OPS_PER_RELAX = {"int32": 2.5, "int64": 9.0}
+FW_BLK = 32
+
+
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def _read_traffic(name: str):
+    """HBM bytes per launch from a committed rocprofv3 PMC summary, or None."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    with open(path) as f:
+        for line in f:
+            if ":" in line and not line.startswith("#"):
+                k, v = line.split(":", 1)
+                try:
+                    vals[k.strip()] = float(v.split()[0])
+                except (ValueError, IndexError):
+                    pass
+    if "traffic_bytes_per_launch" in vals:
+        return vals["traffic_bytes_per_launch"]
+    return None
+
+
+def cpu_baseline_dense(n_sample: int, n_target: int, dtype):
+    """Single-core naive FW (oracle restatement of warshall.c.in) on an
+    n_sample pedigree; relaxations/s scaled to entries/s at n_target."""
+    import numpy as np
+    from oracle import oracle
+    from redblackgraph_amd import synth
+    A = synth.dense_pedigree(synth.PedigreeSpec(n_sample, max(4, round(24 * n_sample / 8192)), 0.75, 4), 1,
+                             dtype=dtype)
+    oracle.lib()
+    t0 = time.perf_counter()
+    oracle.warshall(A)
+    dt = time.perf_counter() - t0
+    rate = n_sample ** 3 / dt  # relaxations per second
+    return {
+        "value": rate / n_target,  # entries/s = N^2 / (N^3 / rate)
+        "unit": "entries/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"oracle naive k-i-j FW (C restatement of warshall.c.in, gcc -O2, 1 thread) on a "
+                   f"seeded N={n_sample} {np.dtype(dtype).name} pedigree: {dt:.2f} s = {rate:.3e} relaxations/s; "
+                   f"value = that rate / {n_target} (N^3 scaling to N={n_target}); host CPU: {_cpu_model()}"),
+    }
+
+
+def bench_dense(args, n: int, dtype_name: str):
+    import numpy as np
+    from redblackgraph_amd import _lib, dense, synth
+    from redblackgraph_amd._dtypes import elem_type
+    t = _lib.torch()
+    dtype = np.int32 if dtype_name == "int32" else np.int64
+    spec = synth.DENSE_8192 if n == 8192 else synth.PedigreeSpec(n, 24, 0.75, 4)
+    A = synth.dense_pedigree(spec, args.seed, dtype=dtype)
+    et = elem_type(dtype)
+    Ad = _lib.to_device(A)
+    L = _lib.load()
+    for _ in range(args.warmup):
+        W, out = dense.fw_closure(Ad, et)
+    t.cuda.synchronize()
+    L.rbg_fw_prof_enable(1)
+    t.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        W, out = dense.fw_closure(Ad, et)
+    t.cuda.synchronize()
+    t1 = time.perf_counter()
+    import ctypes
+    pm, pl, tm, tl = ctypes.c_double(), ctypes.c_int64(), ctypes.c_double(), ctypes.c_int64()
+    _lib.call("rbg_fw_prof_read", ctypes.byref(pm), ctypes.byref(pl), ctypes.byref(tm), ctypes.byref(tl))
+    L.rbg_fw_prof_enable(0)
+    if out.mode != "fast" or out.status.cycle_count or out.status.events:
+        raise RuntimeError(f"unexpected closure outcome: mode={out.mode} status={out.status.flags}")
+    ms = (t1 - t0) * 1e3 / args.steps
+    npad = out.npad
+    relax_per_launch = (npad - FW_BLK) ** 2 * FW_BLK
+    tile_avg_s = tm.value / max(tl.value, 1) / 1e3
+    achieved = relax_per_launch * OPS_PER_RELAX[dtype_name] / tile_avg_s
+    kname = f"k_fw_tile_fast<{'int' if dtype_name == 'int32' else 'long'}>"
+    traffic = _read_traffic(f"fw_tile_fast_dense{n}_{dtype_name}_pmc.txt")
+    roof = {
+        "bound": "valu", "kernel": kname,
+        "achieved": achieved / 1e12, "peak": PEAK_VALU_OPS / 1e12, "unit": "TOP/s",
+        "frac": achieved / PEAK_VALU_OPS,
+        "traffic": traffic,
+        "ops_per_relaxation": OPS_PER_RELAX[dtype_name],
+        "relaxations_per_launch": relax_per_launch,
+        "avg_launch_ms": tile_avg_s * 1e3, "launches": tl.value,
+        "panel_avg_ms": pm.value / max(pl.value, 1),
+        "kernel_share_of_step": (tm.value + pm.value) / (ms * args.steps),
+    }
+    res = {
+        "metric": METRIC, "value": n * n / (ms / 1e3), "unit": "entries/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": dtype_name, "data": "synthetic",
+        "config": {"workload": f"dense Floyd-Warshall closure, N={n} {dtype_name}, seeded layered pedigree "
+                               f"({spec.layers} layers, p_parent={spec.p_parent}, lookback {spec.lookback}), seed {args.seed}",
+                   "n": n, "diameter": out.diameter, "relaxations": npad ** 3},
+        "roofline": roof,
+    }
+    if not args.no_cpu_baseline:
+        res["cpu_baseline"] = cpu_baseline_dense(args.cpu_sample, n, dtype)
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--workload", default="dense8192",
+                    choices=["dense8192", "dense4096_i64", "dense512"])
+    ap.add_argument("--cpu-sample", type=int, default=1024)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (each GPU box has one GPU); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    if args.workload == "dense8192":
+        res = bench_dense(args, 8192, "int32")
+    elif args.workload == "dense4096_i64":
+        res = bench_dense(args, 4096, "int64")
+    else:
+        res = bench_dense(args, 512, "int32")
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -97,6 +97,12 @@ uint64_t rbg_host_avos_product_u64(int64_t a, int64_t b, uint32_t *events);
  * and st->flags & RBG_F_NEED_GENERAL asks for a general-mode rerun on the
  * original input. */
 int64_t rbg_fw_workspace_bytes(int64_t n, int32_t elem_bytes);
+/* Profiling (no reference counterpart; used by bench.py): while enabled, every
+ * panel and tile launch of rbg_fw_dense_* is bracketed by hipEvents on its
+ * stream.  rbg_fw_prof_read synchronises those events and returns the summed
+ * durations (ms) and launch counts since the last read, then clears them. */
+int rbg_fw_prof_enable(int32_t on);
+int rbg_fw_prof_read(double *panel_ms, int64_t *panel_launches, double *tile_ms, int64_t *tile_launches);
 int rbg_fw_precheck_i32(const int32_t *W, int64_t n, int64_t ld, rbg_status *st, rbg_stream_t stream);
 int rbg_fw_precheck_i64(const int64_t *W, int64_t n, int64_t ld, rbg_status *st, rbg_stream_t stream);
 int rbg_fw_dense_i32(int32_t *W, int64_t n, int64_t ld, int32_t width, int32_t is_unsigned, int32_t mode,

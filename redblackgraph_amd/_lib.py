@@ -51,6 +51,9 @@ SIGNATURES = {
     "rbg_host_avos_sum_i64": (I64, [I64, I64]),
     "rbg_host_avos_product_u64": (U64, [I64, I64, ctypes.POINTER(ctypes.c_uint32)]),
     "rbg_fw_workspace_bytes": (I64, [I64, I32]),
+    "rbg_fw_prof_enable": (I32, [I32]),
+    "rbg_fw_prof_read": (I32, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64),
+                               ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]),
     "rbg_fw_precheck_i32": (I32, [P, I64, I64, P, P]),
     "rbg_fw_precheck_i64": (I32, [P, I64, I64, P, P]),
     "rbg_fw_dense_i32": (I32, [P, I64, I64, I32, I32, I32, P, I64, P, P]),

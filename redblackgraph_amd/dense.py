@@ -5,6 +5,7 @@ core.py wraps these.
 """
 from __future__ import annotations
 
+import threading
 from dataclasses import dataclass
 
 from . import _lib
@@ -56,6 +57,18 @@ def _run(work, et: ElemType, mode: int, st: _lib.Status, stream):
               _lib.ptr(ws), ws.numel(), st.p, _lib.stream_handle(stream))
 
 
+_tls = threading.local()
+
+
+def status() -> _lib.Status:
+    """A per-thread reusable status block (device + pinned host mirror)."""
+    st = getattr(_tls, "status", None)
+    if st is None:
+        st = _lib.Status()
+        _tls.status = st
+    return st
+
+
 def fw_closure(W, et: ElemType, stream=None, force_general: bool = False):
     """Close an n x n device matrix.  Returns (closed n x n tensor, FwOutcome).
 
@@ -66,12 +79,16 @@ def fw_closure(W, et: ElemType, stream=None, force_general: bool = False):
     t = _lib.torch()
     n = W.shape[0]
     npad = padded_order(n)
-    if npad == n:
-        work = W.clone()
-    else:
-        work = t.zeros((npad, npad), dtype=W.dtype, device=W.device)
-        work[:n, :n] = W
-    st = _lib.Status()
+
+    def fresh():
+        if npad == n:
+            return W.clone()
+        w = t.zeros((npad, npad), dtype=W.dtype, device=W.device)
+        w[:n, :n] = W
+        return w
+
+    work = fresh()
+    st = status()
     st.reset(stream)
     fast = et.native and not force_general
     if fast:
@@ -81,12 +98,11 @@ def fw_closure(W, et: ElemType, stream=None, force_general: bool = False):
         if s.flags & _lib.F_PRECOND_FAIL:
             fast = False
         st.reset(stream)
-    src = work.clone() if fast else None
     _run(work, et, 0 if fast else 1, st, stream)
     s = st.read(stream)
     mode = "fast" if fast else "general"
     if fast and (s.flags & _lib.F_NEED_GENERAL):
-        work = src
+        work = fresh()
         st.reset(stream)
         _run(work, et, 1, st, stream)
         s = st.read(stream)
@@ -101,7 +117,7 @@ def matmul(A, B, et: ElemType, stream=None):
     K2, N = B.shape
     assert K == K2
     C = t.empty((M, N), dtype=A.dtype, device=A.device)
-    st = _lib.Status()
+    st = status()
     st.reset(stream)
     fn = "rbg_matmul_i32" if A.dtype == t.int32 else "rbg_matmul_i64"
     A = A.contiguous()
@@ -115,7 +131,7 @@ def edge_compose(R, alpha: int, beta: int, rel: int, et: ElemType, stream=None):
     t = _lib.torch()
     n = R.shape[0]
     out = t.empty_like(R)
-    st = _lib.Status()
+    st = status()
     st.reset(stream)
     fn = "rbg_edge_compose_i32" if R.dtype == t.int32 else "rbg_edge_compose_i64"
     _lib.call(fn, _lib.ptr(R.contiguous()), n, alpha, beta, int(rel), _lib.ptr(out), et.width,
@@ -127,7 +143,7 @@ def vertex_compose(u, R, v, color: int, et: ElemType, stream=None):
     t = _lib.torch()
     n = R.shape[0]
     out = t.zeros((n + 1, n + 1), dtype=R.dtype, device=R.device)
-    st = _lib.Status()
+    st = status()
     st.reset(stream)
     fn = "rbg_vertex_compose_i32" if R.dtype == t.int32 else "rbg_vertex_compose_i64"
     _lib.call(fn, _lib.ptr(u.contiguous()), _lib.ptr(R.contiguous()), _lib.ptr(v.contiguous()), int(color), n,
