@@ -318,6 +318,184 @@ __global__ __launch_bounds__(NT) void k_fw_panel(T* W, int64_t n, int64_t ld, in
 }
 
 // ---------------------------------------------------------------------------
+// Panel kernel, fast mode: same schedule as k_fw_panel, E-domain arithmetic.
+// At step k the pivot row and column do not change (they are skipped), so the
+// step reads them while it updates every other cell and needs one barrier.
+// Operands off the pivot row/column are never +-1 (precheck), so the product
+// is (x << MSB(y)) | low(y) there.  A per-product MSB check raises
+// RBG_F_NEED_GENERAL before any value could overflow.
+// A product with an absent operand is some value with the top bit set; the
+// min can store it into an absent cell.  Such a value must not be shifted
+// again (the shift could clear its top bit), so every operand read here is
+// first canonicalised to all ones, and so is every snapshot phase 3 reads.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(NT) void k_fw_panel_fast(T* W, int64_t n, int64_t ld, int64_t K, FwWork ws,
+                                                      int width, rbg_status* st) {
+  using U = typename unsigned_of<T>::type;
+  constexpr U TOP = (U)1 << (sizeof(U) * 8 - 1);
+  __shared__ U Dg[BLK][BLK + 1];
+  __shared__ U Pn[BLK][BLK + 1];
+  // snapshots of step kk, staged in LDS and written out after the loop so the
+  // per-step barrier never waits on global stores
+  __shared__ U sDc[BLK][BLK + 1];  // [kk][i] = diag W[i][kk]
+  __shared__ U sDr[BLK][BLK + 1];  // [kk][j] = diag W[kk][j]
+  __shared__ U sP[BLK][BLK + 1];   // row panel: [kk][j] = W[kk][j]; column panel: [kk][i] = W[i][kk]
+  __shared__ int sbad;
+  const int tid = threadIdx.x;
+  const int nb = (int)(n / BLK);
+  const int64_t k0 = K * BLK;
+  const int par = (int)(K & 1);
+  T* colT = ws.col<T>(par);
+  T* rowS = ws.row<T>(par);
+  int* guard = ws.guard(par);
+
+  int role = 0, J = 0;
+  if (nb > 1) {
+    int t = blockIdx.x;
+    role = t < nb - 1 ? 1 : 2;
+    int idx = role == 1 ? t : t - (nb - 1);
+    J = idx < K ? idx : idx + 1;
+  }
+  const bool owner = blockIdx.x == 0;
+  if (owner && tid == 0) {
+    if (K > 0) capture_block<T, true>(st, ws.col<T>(par ^ 1), ws.row<T>(par ^ 1), k0 - BLK, n);
+    int* gnext = ws.guard(par ^ 1);
+    for (int q = 0; q < 2 * BLK; q++) gnext[q] = 0;
+  }
+  if (tid == 0) sbad = 0;
+
+  const int64_t pr0 = role == 2 ? (int64_t)J * BLK : k0;
+  const int64_t pc0 = role == 1 ? (int64_t)J * BLK : k0;
+  auto canon = [](U x) -> U { return (T)x < 0 ? ~(U)0 : x; };
+  for (int e = tid; e < BLK * BLK; e += NT) {
+    int r = e / BLK, c = e % BLK;
+    Dg[r][c] = canon((U)W[(k0 + r) * ld + k0 + c]);
+    if (role) Pn[r][c] = canon((U)W[(pr0 + r) * ld + pc0 + c]);
+  }
+  __syncthreads();
+
+  const int i = tid >> 3;
+  const int j0 = (tid & 7) * 4;
+  const int lim = width - 2;
+  U dv[4], pv[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    dv[q] = Dg[i][j0 + q];
+    pv[q] = role ? Pn[i][j0 + q] : (U)0;
+  }
+  T dmax = 0;
+  bool bad = false;
+  for (int kk = 0; kk < BLK; kk++) {
+    const int64_t k = k0 + kk;
+    const U aD = canon(Dg[i][kk]);
+    U bD[4], bP[4], aP = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) bD[q] = canon(Dg[kk][j0 + q]);
+    if (role == 1) {
+      aP = aD;
+#pragma unroll
+      for (int q = 0; q < 4; q++) bP[q] = canon(Pn[kk][j0 + q]);
+    } else if (role == 2) {
+      aP = canon(Pn[i][kk]);
+#pragma unroll
+      for (int q = 0; q < 4; q++) bP[q] = bD[q];
+    }
+    if ((tid & 7) == 0) {
+      sDc[kk][i] = aD;
+      if (role == 2) sP[kk][i] = aP;
+    }
+    if (i == 0) {
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        sDr[kk][j0 + q] = bD[q];
+        if (role == 1) sP[kk][j0 + q] = bP[q];
+      }
+    }
+    // diagonal tile: row kk and column kk are invariant at step kk
+    const int maD = (T)aD > 1 ? msb_u(aD) : 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int j = j0 + q;
+      if (i != kk && j != kk) {
+        U m, y;
+        if ((T)bD[q] < 0) {
+          m = 0;
+          y = TOP;
+        } else {
+          m = (U)msb_u(bD[q]);
+          y = bD[q] & (((U)1 << m) - 1);
+        }
+        if ((T)bD[q] > 1 && maD + (int)m > lim) bad = true;
+        U p = (aD << m) | y;
+        if (owner && i == j && (T)p >= 0) record_cycle(st, naive_key(k, k0 + i, k0 + i, n));
+        dv[q] = dv[q] < p ? dv[q] : p;
+        Dg[i][j] = dv[q];
+      }
+      if (owner && (T)dv[q] > dmax) dmax = (T)dv[q];
+    }
+    if (role) {
+      const int maP = (T)aP > 1 ? msb_u(aP) : 0;
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int j = j0 + q;
+        const bool pivot = role == 1 ? (i == kk) : (j == kk);
+        if (!pivot) {
+          U m, y;
+          if ((T)bP[q] < 0) {
+            m = 0;
+            y = TOP;
+          } else {
+            m = (U)msb_u(bP[q]);
+            y = bP[q] & (((U)1 << m) - 1);
+          }
+          if ((T)bP[q] > 1 && maP + (int)m > lim) bad = true;
+          U p = (aP << m) | y;
+          pv[q] = pv[q] < p ? pv[q] : p;
+          Pn[i][j] = pv[q];
+        }
+        if ((T)pv[q] > dmax) dmax = (T)pv[q];
+      }
+    }
+    __syncthreads();
+  }
+  if (bad) sbad = 1;
+  __syncthreads();
+  if (sbad) {
+    if (tid == 0) atomicOr(&st->flags, (uint32_t)RBG_F_NEED_GENERAL);
+    return;
+  }
+  // tiles and snapshots out, coalesced
+  for (int e = tid; e < BLK * BLK; e += NT) {
+    int r = e / BLK, c = e % BLK;
+    if (owner) {
+      W[(k0 + r) * ld + k0 + c] = (T)Dg[r][c];
+      colT[r * n + k0 + c] = (T)sDc[r][c];
+      rowS[r * n + k0 + c] = (T)sDr[r][c];
+    }
+    if (role == 1) {
+      W[(pr0 + r) * ld + pc0 + c] = (T)Pn[r][c];
+      rowS[r * n + pc0 + c] = (T)sP[r][c];
+    } else if (role == 2) {
+      W[(pr0 + r) * ld + pc0 + c] = (T)Pn[r][c];
+      colT[r * n + pr0 + c] = (T)sP[r][c];
+    }
+  }
+  // per-k MSB guard for phase 3 from this panel's snapshots
+  if (role && tid < BLK) {
+    int mm = 0;
+    for (int c = 0; c < BLK; c++) {
+      U v = sP[tid][c];
+      int m = (T)v > 1 ? msb_u(v) : 0;
+      mm = m > mm ? m : mm;
+    }
+    if (mm) atomicMax(&guard[(role == 1 ? BLK : 0) + tid], mm);
+  }
+  int64_t dm = wave_max_i64((int64_t)dmax);
+  if ((tid & 63) == 0) record_diameter(st, dm);
+}
+
+// ---------------------------------------------------------------------------
 // Phase 3, fast mode (E domain).  One 128x128 tile per workgroup, 8x8 cells
 // per thread held in VGPRs for the whole pivot block; per k a thread reads 8
 // left operands and 8 (shift, low-bits) pairs from LDS.
@@ -354,10 +532,11 @@ __global__ __launch_bounds__(NT) void k_fw_tile_fast(T* W, int64_t n, int64_t ld
       for (int c = 0; c < CPT; c++) w[r][c] = (U)W[(r0 + r) * ld + c0 + c];
   }
 
-  if (tid == 0) sbad = 0;
+  // a workgroup-uniform exit when an earlier kernel already asked for general mode
+  if (tid == 0) sbad = (st->flags & RBG_F_NEED_GENERAL) ? 2 : 0;
   __syncthreads();
   if (tid < BLK) {
-    if (guard[tid] + guard[BLK + tid] > width - 2) sbad = 1;
+    if (guard[tid] + guard[BLK + tid] > width - 2) atomicOr(&sbad, 1);
   }
   for (int e = tid; e < BLK * TILE; e += NT) {
     int kk = e / TILE, x = e % TILE;
@@ -376,7 +555,7 @@ __global__ __launch_bounds__(NT) void k_fw_tile_fast(T* W, int64_t n, int64_t ld
   }
   __syncthreads();
   if (sbad) {
-    if (tid == 0) atomicOr(&st->flags, (uint32_t)RBG_F_NEED_GENERAL);
+    if (tid == 0 && (sbad & 1)) atomicOr(&st->flags, (uint32_t)RBG_F_NEED_GENERAL);
     return;
   }
   if (skip) return;
@@ -515,8 +694,8 @@ static int fw_dense(T* W, int64_t n, int64_t ld, int width, int is_uns, int mode
   for (int K = 0; K < nb; K++) {
     int p0 = prof_start(s);
     if (fast) {
-      hipLaunchKernelGGL((k_fw_panel<T, true>), dim3(panel_grid), dim3(NT), 0, s, W, n, ld, (int64_t)K, ws,
-                         width, is_uns, st);
+      hipLaunchKernelGGL((k_fw_panel_fast<T>), dim3(panel_grid), dim3(NT), 0, s, W, n, ld, (int64_t)K, ws, width,
+                         st);
     } else {
       hipLaunchKernelGGL((k_fw_panel<T, false>), dim3(panel_grid), dim3(NT), 0, s, W, n, ld, (int64_t)K, ws,
                          width, is_uns, st);

@@ -272,3 +272,28 @@ def test_edge_composition_equals_reclosure(orc):
             if done == 3:
                 break
     assert done > 0
+
+
+def _overflow_chain(n, dt):
+    """A paternal chain of n vertices whose middle vertex has the largest
+    index: the naive loop meets MSB(x) + MSB(y) = width only at the last step,
+    with both operands still small, so the reference raises OverflowError
+    (instead of first storing a wrapped negative it would later loop on)."""
+    mid = (n - 1) // 2
+    idx = lambda t: t if t < mid else (n - 1 if t == mid else t - 1)
+    A = np.zeros((n, n), dtype=dt)
+    np.fill_diagonal(A, -1)
+    for t in range(n - 1):
+        A[idx(t + 1), idx(t)] = 2
+    return A
+
+
+@pytest.mark.parametrize("dt,n", [(np.int32, 33), (np.int8, 9), (np.int64, 65)])
+def test_fw_overflow_chain(orc, dt, n):
+    A = _overflow_chain(n, dt)
+    rep = orc.warshall(A).report
+    assert rep["n_overflow"] > 0 and not rep["hang"] and rep["last_is_overflow"]
+    with pytest.raises(OverflowError):
+        rb.array(A).transitive_closure()
+    W, out = _fw(A)
+    assert out.mode == "general" and out.status.events != 0
