@@ -150,6 +150,81 @@ def bench_dense(args, n: int, dtype_name: str):
     return res
 
 
+def cpu_baseline_sparse(A, nnz_c: int):
+    """Single-core oracle DAG closure (C restatement of _transitive_closure_dag.pyx)
+    on the full workload."""
+    from oracle import oracle
+    oracle.lib()
+    t0 = time.perf_counter()
+    r = oracle.dag_closure(A, 64)
+    dt = time.perf_counter() - t0
+    assert r.indptr[-1] == nnz_c
+    return {
+        "value": nnz_c / dt, "unit": "entries/s", "cores": 1, "kind": "port",
+        "sample": (f"oracle sparse DAG closure (C restatement of _transitive_closure_dag.pyx:168-351, gcc -O2, "
+                   f"1 thread) on the full 1M-vertex workload: {dt:.2f} s for {nnz_c} closure entries; "
+                   f"host CPU: {_cpu_model()}"),
+    }
+
+
+def bench_sparse(args):
+    import ctypes
+    import numpy as np
+    from redblackgraph_amd import _lib, synth
+    from redblackgraph_amd import sparse_dev as sd
+    t = _lib.torch()
+    A = synth.csr_pedigree(synth.SPARSE_1M, args.seed)
+    Ad = sd.from_scipy(A, np.int64)
+    plan = None
+    L = _lib.load()
+    for _ in range(max(1, args.warmup)):
+        C, info = sd.dag_closure(sd.canonicalize(Ad), plan=plan)
+    plan = sd.DagPlan(A.shape[0], t.int64, int(info.nnz * 1.25) + 4096, int(info.nnz) // 2 + 4096)
+    C, info = sd.dag_closure(sd.canonicalize(Ad), plan=plan)
+    t.cuda.synchronize()
+    L.rbg_prof_enable(1)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        C, info = sd.dag_closure(sd.canonicalize(Ad), plan=plan)
+    t.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_, nl = ctypes.c_double(), ctypes.c_int64()
+    _lib.call("rbg_prof_read", 2, ctypes.byref(ms_), ctypes.byref(nl))
+    L.rbg_prof_enable(0)
+    ms = (t1 - t0) * 1e3 / args.steps
+    n = A.shape[0]
+    clen = np.diff(C.indptr.cpu().numpy())
+    rows = np.repeat(np.arange(n), np.diff(A.indptr))
+    off = A.indices != rows
+    reads = int(clen[A.indices[off]].sum())
+    bytes_alg = 12 * A.nnz + 12 * reads + 12 * info.nnz + 8 * (n + 1)
+    kernel_s = ms_.value / 1e3 / args.steps
+    achieved = bytes_alg / kernel_s
+    roof = {
+        "bound": "hbm", "kernel": "k_dag_level<long>",
+        "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s", "frac": achieved / PEAK_HBM,
+        "traffic": _read_traffic("dag_level_sparse1m_pmc.txt"),
+        "algorithmic_bytes_per_closure": bytes_alg,
+        "launches_per_closure": nl.value / args.steps, "avg_launch_ms": ms_.value / max(nl.value, 1),
+        "kernel_ms_per_closure": kernel_s * 1e3,
+        "kernel_share_of_step": kernel_s * 1e3 / ms,
+    }
+    res = {
+        "metric": METRIC, "value": info.nnz / (ms / 1e3), "unit": "entries/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int64", "data": "synthetic",
+        "config": {"workload": f"sparse DAG closure, 1M-vertex pedigree (15 layers of ~66,667, p_parent=0.7, "
+                               f"parents from the previous layer), int64 values / int32 indices, seed {args.seed}",
+                   "n": n, "nnz_in": int(A.nnz), "nnz_closure": info.nnz, "levels": info.nlevels,
+                   "diameter": info.diameter_candidates},
+        "roofline": roof,
+    }
+    if not args.no_cpu_baseline:
+        res["cpu_baseline"] = cpu_baseline_sparse(A, info.nnz)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -157,7 +232,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--workload", default="dense8192",
-                    choices=["dense8192", "dense4096_i64", "dense512"])
+                    choices=["dense8192", "dense4096_i64", "dense512", "sparse1m"])
     ap.add_argument("--cpu-sample", type=int, default=1024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -167,6 +242,8 @@ def main():
         sys.exit(2)
     if args.workload == "dense8192":
         res = bench_dense(args, 8192, "int32")
+    elif args.workload == "sparse1m":
+        res = bench_sparse(args)
     elif args.workload == "dense4096_i64":
         res = bench_dense(args, 4096, "int64")
     else:

@@ -102,6 +102,11 @@ int64_t rbg_fw_workspace_bytes(int64_t n, int32_t elem_bytes);
  * stream.  rbg_fw_prof_read synchronises those events and returns the summed
  * durations (ms) and launch counts since the last read, then clears them. */
 int rbg_fw_prof_enable(int32_t on);
+/* Same facility for every timed launch kind: 0 FW panel, 1 FW tile, 2 DAG
+ * level, 3 SpGEMM rows, 4 batched FW.  rbg_prof_read returns (and clears) the
+ * summed duration and count of one kind. */
+int rbg_prof_enable(int32_t on);
+int rbg_prof_read(int32_t kind, double *total_ms, int64_t *launches);
 int rbg_fw_prof_read(double *panel_ms, int64_t *panel_launches, double *tile_ms, int64_t *tile_launches);
 int rbg_fw_precheck_i32(const int32_t *W, int64_t n, int64_t ld, rbg_status *st, rbg_stream_t stream);
 int rbg_fw_precheck_i64(const int64_t *W, int64_t n, int64_t ld, rbg_status *st, rbg_stream_t stream);
@@ -136,6 +141,86 @@ int rbg_vertex_compose_i32(const int32_t *u, const int32_t *R, const int32_t *v,
 int rbg_vertex_compose_i64(const int64_t *u, const int64_t *R, const int64_t *v, int64_t color,
                            int64_t n, int64_t *out, int32_t width, int32_t is_unsigned,
                            rbg_status *st, rbg_stream_t stream);
+
+/* ---- sparse CSR helpers ---------------------------------------------------- */
+/* Device CSR on this ABI: int64 row pointers, int32 column indices, int32/int64 values. */
+int64_t rbg_csr_tmp_bytes(int32_t n);
+/* Sort rows, combine duplicate columns with the AVOS sum, drop explicit zeros
+ * (rows are sorted in place in idx/val); writes the canonical CSR.  With
+ * out_idx == NULL only out_indptr is produced (to size the outputs).
+ * Replaces the canonicalisation rb_matrix does before its SpGEMM (sparse/rbm.py:49-56). */
+int rbg_csr_canonicalize_i32(const int64_t *indptr, int32_t *idx, int32_t *val, int32_t n, void *tmp,
+                             int64_t *out_indptr, int32_t *out_idx, int32_t *out_val, rbg_stream_t stream);
+int rbg_csr_canonicalize_i64(const int64_t *indptr, int32_t *idx, int64_t *val, int32_t n, void *tmp,
+                             int64_t *out_indptr, int32_t *out_idx, int64_t *out_val, rbg_stream_t stream);
+/* CSRMatrixGPU._validate_csr (gpu/csr_gpu.py:93-147): monotone indptr, columns in
+ * range; a violation sets RBG_F_CAPACITY in st->flags. */
+int rbg_csr_validate(const int64_t *indptr, const int32_t *idx, int32_t n, int32_t ncols, rbg_status *st,
+                     rbg_stream_t stream);
+
+/* ---- sparse DAG closure (_transitive_closure_dag.pyx:168-351,
+ *      gpu/transitive_closure.py:474-652) --------------------------------------- */
+int64_t rbg_dag_workspace_bytes(int32_t n);
+/* Bytes of the per-level candidate scratch for `cap` candidate entries. */
+int64_t rbg_dag_cand_bytes(int64_t cap, int32_t elem_bytes);
+/* Closure of a DAG (canonical CSR: int64 row pointers, sorted rows, no explicit
+ * zeros).  Rounds r = 0, 1, ...: the vertices whose non-self successors all sit
+ * in levels < r form level r, and their closure rows are built by an
+ * element-parallel merge of their direct entries with A[v][w] (x) C[w] for each
+ * successor w.  All sizes live on the device; the host synchronises once every
+ * `batch` rounds to test completion.  Results: closure rows in the arena (see
+ * rbg_dag_gather_*); st->aux[2] max |candidate| (Cython diameter rule), [3] max
+ * |final value| > 1 (GPU diameter rule), [4] arena entries used, [6] levels.
+ * A cycle sets RBG_F_CYCLE with st->aux[0] = a vertex on the cycle; a full
+ * arena or candidate scratch sets RBG_F_CAPACITY (grow it and call again). */
+int rbg_dag_closure_i32(const int64_t *indptr, const int32_t *indices, const int32_t *data, int32_t n,
+                        void *workspace, int64_t ws_bytes, int32_t *arena_cols, int32_t *arena_vals,
+                        int64_t arena_cap, void *cand, int64_t cand_cap, int32_t batch, int32_t width,
+                        rbg_status *st, rbg_stream_t stream);
+int rbg_dag_closure_i64(const int64_t *indptr, const int32_t *indices, const int64_t *data, int32_t n,
+                        void *workspace, int64_t ws_bytes, int32_t *arena_cols, int64_t *arena_vals,
+                        int64_t arena_cap, void *cand, int64_t cand_cap, int32_t batch, int32_t width,
+                        rbg_status *st, rbg_stream_t stream);
+/* Gather arena rows into CSR: out_indptr (n+1, int64) always; indices/data when
+ * non-null.  st->aux: [1] nnz, [2] max |candidate|, [3] max |final value| > 1,
+ * [4] arena entries used, [5] temp entries used. */
+int rbg_dag_gather_i32(void *workspace, int32_t n, const int32_t *arena_cols, const int32_t *arena_vals,
+                       int64_t *out_indptr, int32_t *out_indices, int32_t *out_data, rbg_status *st,
+                       rbg_stream_t stream);
+int rbg_dag_gather_i64(void *workspace, int32_t n, const int32_t *arena_cols, const int64_t *arena_vals,
+                       int64_t *out_indptr, int32_t *out_indices, int64_t *out_data, rbg_status *st,
+                       rbg_stream_t stream);
+
+/* ---- AVOS SpGEMM (sparse/sparsetools/rbm_math.h:116-222, gpu/spgemm*.py) ---------- */
+/* Rows of C = A (x) B into the row arena (rbg_dag_workspace_bytes(m) workspace,
+ * sized for m = rows of A); then rbg_dag_gather_* writes the CSR.  A and B use
+ * int64 row pointers, rows sorted.  mask_upper keeps columns >= row only.
+ * Zero sums are dropped (the C++ pass 2 rule). */
+int rbg_spgemm_rows_i32(const int64_t *ap, const int32_t *aj, const int32_t *ax, int32_t m, const int64_t *bp,
+                        const int32_t *bj, const int32_t *bx, int32_t mask_upper, void *workspace,
+                        int32_t *arena_cols, int32_t *arena_vals, int64_t arena_cap, int32_t *tmp_cols,
+                        int32_t *tmp_vals, int64_t tmp_cap, int32_t width, int32_t is_unsigned, rbg_status *st,
+                        rbg_stream_t stream);
+int rbg_spgemm_rows_i64(const int64_t *ap, const int32_t *aj, const int64_t *ax, int32_t m, const int64_t *bp,
+                        const int32_t *bj, const int64_t *bx, int32_t mask_upper, void *workspace,
+                        int32_t *arena_cols, int64_t *arena_vals, int64_t arena_cap, int32_t *tmp_cols,
+                        int64_t *tmp_vals, int64_t tmp_cap, int32_t width, int32_t is_unsigned, rbg_status *st,
+                        rbg_stream_t stream);
+
+/* Element-wise sum of two canonical CSR matrices (sparse_avos_sum_gpu,
+ * gpu/transitive_closure.py:77-156; _sparse_avos_sum, sparse/csgraph/
+ * transitive_closure.py:272-357).  combine: 0 = AVOS sum, 1 = smallest |value|
+ * (the squaring merge of the reference).  cat_ptr = a_ptr + b_ptr; cat_idx /
+ * cat_val hold nnz(A)+nnz(B) entries of scratch.  Two calls: first with
+ * out_idx == NULL to size the output (out_indptr[n] = nnz), then with outputs. */
+int rbg_csr_add_i32(const int64_t *ap, const int32_t *aj, const int32_t *ax, const int64_t *bp, const int32_t *bj,
+                    const int32_t *bx, int32_t n, int32_t combine, const int64_t *cat_ptr, int32_t *cat_idx,
+                    int32_t *cat_val, void *tmp, int64_t *out_indptr, int32_t *out_idx, int32_t *out_val,
+                    rbg_stream_t stream);
+int rbg_csr_add_i64(const int64_t *ap, const int32_t *aj, const int64_t *ax, const int64_t *bp, const int32_t *bj,
+                    const int64_t *bx, int32_t n, int32_t combine, const int64_t *cat_ptr, int32_t *cat_idx,
+                    int64_t *cat_val, void *tmp, int64_t *out_indptr, int32_t *out_idx, int64_t *out_val,
+                    rbg_stream_t stream);
 
 #ifdef __cplusplus
 }

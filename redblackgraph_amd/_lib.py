@@ -52,6 +52,8 @@ SIGNATURES = {
     "rbg_host_avos_product_u64": (U64, [I64, I64, ctypes.POINTER(ctypes.c_uint32)]),
     "rbg_fw_workspace_bytes": (I64, [I64, I32]),
     "rbg_fw_prof_enable": (I32, [I32]),
+    "rbg_prof_enable": (I32, [I32]),
+    "rbg_prof_read": (I32, [I32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]),
     "rbg_fw_prof_read": (I32, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64),
                                ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]),
     "rbg_fw_precheck_i32": (I32, [P, I64, I64, P, P]),
@@ -64,6 +66,20 @@ SIGNATURES = {
     "rbg_edge_compose_i64": (I32, [P, I64, I64, I64, I64, P, I32, I32, P, P]),
     "rbg_vertex_compose_i32": (I32, [P, P, P, I32, I64, P, I32, I32, P, P]),
     "rbg_vertex_compose_i64": (I32, [P, P, P, I64, I64, P, I32, I32, P, P]),
+    "rbg_csr_tmp_bytes": (I64, [I32]),
+    "rbg_csr_canonicalize_i32": (I32, [P, P, P, I32, P, P, P, P, P]),
+    "rbg_csr_canonicalize_i64": (I32, [P, P, P, I32, P, P, P, P, P]),
+    "rbg_csr_validate": (I32, [P, P, I32, I32, P, P]),
+    "rbg_dag_workspace_bytes": (I64, [I32]),
+    "rbg_dag_cand_bytes": (I64, [I64, I32]),
+    "rbg_dag_closure_i32": (I32, [P, P, P, I32, P, I64, P, P, I64, P, I64, I32, I32, P, P]),
+    "rbg_dag_closure_i64": (I32, [P, P, P, I32, P, I64, P, P, I64, P, I64, I32, I32, P, P]),
+    "rbg_dag_gather_i32": (I32, [P, I32, P, P, P, P, P, P, P]),
+    "rbg_dag_gather_i64": (I32, [P, I32, P, P, P, P, P, P, P]),
+    "rbg_spgemm_rows_i32": (I32, [P, P, P, I32, P, P, P, I32, P, P, P, I64, P, P, I64, I32, I32, P, P]),
+    "rbg_csr_add_i32": (I32, [P, P, P, P, P, P, I32, I32, P, P, P, P, P, P, P, P]),
+    "rbg_csr_add_i64": (I32, [P, P, P, P, P, P, I32, I32, P, P, P, P, P, P, P, P]),
+    "rbg_spgemm_rows_i64": (I32, [P, P, P, I32, P, P, P, I32, P, P, P, I64, P, P, I64, I32, I32, P, P]),
 }
 
 

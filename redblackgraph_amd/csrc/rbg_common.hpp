@@ -4,6 +4,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+#include <utility>
+#include <vector>
+
 #include "../../include/rbg_hip.h"
 #include "avos.hpp"
 
@@ -14,6 +18,47 @@
   } while (0)
 
 namespace rbg {
+
+// ---- optional per-launch timing ---------------------------------------------
+// While enabled, selected launches are bracketed by hipEvents on their stream
+// (bench and profiling only).  Kinds: 0 FW panel, 1 FW tile, 2 DAG level,
+// 3 SpGEMM rows, 4 batched FW.
+constexpr int PROF_KINDS = 8;
+struct Prof {
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  std::vector<std::pair<int, int>> pairs[PROF_KINDS];
+  int used = 0;
+  std::mutex mu;
+  hipEvent_t take() {
+    if (used == (int)pool.size()) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return nullptr;
+      pool.push_back(e);
+    }
+    return pool[used++];
+  }
+};
+inline Prof g_prof;
+
+inline int prof_start(hipStream_t s) {
+  if (!g_prof.on) return -1;
+  std::lock_guard<std::mutex> g(g_prof.mu);
+  int idx = g_prof.used;
+  hipEvent_t e = g_prof.take();
+  if (!e) return -1;
+  (void)hipEventRecord(e, s);
+  return idx;
+}
+inline void prof_stop(int kind, int start_idx, hipStream_t s) {
+  if (start_idx < 0) return;
+  std::lock_guard<std::mutex> g(g_prof.mu);
+  int idx = g_prof.used;
+  hipEvent_t e = g_prof.take();
+  if (!e) return;
+  (void)hipEventRecord(e, s);
+  g_prof.pairs[kind].push_back({start_idx, idx});
+}
 
 // ---- event recording -------------------------------------------------------
 __device__ inline void record_cycle(rbg_status* st, uint64_t key) {

@@ -26,10 +26,6 @@
 //            otherwise the kernels raise RBG_F_NEED_GENERAL.
 #include "rbg_common.hpp"
 
-#include <mutex>
-#include <utility>
-#include <vector>
-
 namespace rbg {
 
 constexpr int BLK = 32;    // pivot block (k values per step)
@@ -57,42 +53,6 @@ struct FwWork {
   }
   static int64_t bytes(int64_t n, int elem) { return (int64_t)4 * BLK * n * elem + 4 * BLK * 4 + 256; }
 };
-
-// Optional per-launch timing of the panel and tile kernels with hipEvents on
-// the launch stream (bench and profiling only; off by default).
-struct FwProf {
-  bool on = false;
-  std::vector<hipEvent_t> pool;
-  std::vector<std::pair<int, int>> pairs[2];  // [0] panel, [1] tile: (start, stop) pool indices
-  int used = 0;
-  hipEvent_t take() {
-    if (used == (int)pool.size()) {
-      hipEvent_t e;
-      if (hipEventCreate(&e) != hipSuccess) return nullptr;
-      pool.push_back(e);
-    }
-    return pool[used++];
-  }
-};
-static FwProf g_prof;
-static std::mutex g_prof_mu;
-
-static inline int prof_start(hipStream_t s) {
-  if (!g_prof.on) return -1;
-  int idx = g_prof.used;
-  hipEvent_t e = g_prof.take();
-  if (!e) return -1;
-  (void)hipEventRecord(e, s);
-  return idx;
-}
-static inline void prof_stop(int kind, int start_idx, hipStream_t s) {
-  if (start_idx < 0) return;
-  int idx = g_prof.used;
-  hipEvent_t e = g_prof.take();
-  if (!e) return;
-  (void)hipEventRecord(e, s);
-  g_prof.pairs[kind].push_back({start_idx, idx});
-}
 
 __device__ inline uint64_t naive_key(int64_t k, int64_t i, int64_t j, int64_t n) {
   return ((uint64_t)k * (uint64_t)n + (uint64_t)i) * (uint64_t)n + (uint64_t)j;
@@ -749,36 +709,41 @@ int rbg_status_init(rbg_status* st, rbg_stream_t stream) {
 
 int64_t rbg_fw_workspace_bytes(int64_t n, int32_t elem_bytes) { return FwWork::bytes(n, elem_bytes); }
 
-int rbg_fw_prof_enable(int32_t on) {
-  std::lock_guard<std::mutex> g(g_prof_mu);
+int rbg_prof_enable(int32_t on) {
+  std::lock_guard<std::mutex> g(g_prof.mu);
   g_prof.on = on != 0;
   g_prof.used = 0;
-  g_prof.pairs[0].clear();
-  g_prof.pairs[1].clear();
+  for (int k = 0; k < PROF_KINDS; k++) g_prof.pairs[k].clear();
   return 0;
 }
 
-int rbg_fw_prof_read(double* panel_ms, int64_t* panel_launches, double* tile_ms, int64_t* tile_launches) {
-  std::lock_guard<std::mutex> g(g_prof_mu);
-  double tot[2] = {0.0, 0.0};
-  for (int kind = 0; kind < 2; kind++) {
-    for (auto& pr : g_prof.pairs[kind]) {
-      hipError_t e = hipEventSynchronize(g_prof.pool[pr.second]);
-      if (e != hipSuccess) return (int)e;
-      float ms = 0.f;
-      e = hipEventElapsedTime(&ms, g_prof.pool[pr.first], g_prof.pool[pr.second]);
-      if (e != hipSuccess) return (int)e;
-      tot[kind] += ms;
-    }
+int rbg_prof_read(int32_t kind, double* total_ms, int64_t* launches) {
+  if (kind < 0 || kind >= PROF_KINDS) return RBG_E_ARG;
+  std::lock_guard<std::mutex> g(g_prof.mu);
+  double tot = 0.0;
+  for (auto& pr : g_prof.pairs[kind]) {
+    hipError_t e = hipEventSynchronize(g_prof.pool[pr.second]);
+    if (e != hipSuccess) return (int)e;
+    float ms = 0.f;
+    e = hipEventElapsedTime(&ms, g_prof.pool[pr.first], g_prof.pool[pr.second]);
+    if (e != hipSuccess) return (int)e;
+    tot += ms;
   }
-  if (panel_ms) *panel_ms = tot[0];
-  if (tile_ms) *tile_ms = tot[1];
-  if (panel_launches) *panel_launches = (int64_t)g_prof.pairs[0].size();
-  if (tile_launches) *tile_launches = (int64_t)g_prof.pairs[1].size();
-  g_prof.used = 0;
-  g_prof.pairs[0].clear();
-  g_prof.pairs[1].clear();
+  if (total_ms) *total_ms = tot;
+  if (launches) *launches = (int64_t)g_prof.pairs[kind].size();
+  g_prof.pairs[kind].clear();
+  bool empty = true;
+  for (int k = 0; k < PROF_KINDS; k++) empty = empty && g_prof.pairs[k].empty();
+  if (empty) g_prof.used = 0;
   return 0;
+}
+
+int rbg_fw_prof_enable(int32_t on) { return rbg_prof_enable(on); }
+
+int rbg_fw_prof_read(double* panel_ms, int64_t* panel_launches, double* tile_ms, int64_t* tile_launches) {
+  int rc = rbg_prof_read(0, panel_ms, panel_launches);
+  if (rc) return rc;
+  return rbg_prof_read(1, tile_ms, tile_launches);
 }
 
 int rbg_fw_precheck_i32(const int32_t* W, int64_t n, int64_t ld, rbg_status* st, rbg_stream_t stream) {

@@ -1,0 +1,1169 @@
+// Sparse AVOS closure of a DAG, level-synchronous on the GPU.
+//
+// Reference: redblackgraph/sparse/csgraph/_transitive_closure_dag.pyx:168-351
+// (closure row of v = direct edges of v (+) A[v][w] (x) C[w] over successors
+// w != v, rows built sinks first) and redblackgraph/gpu/transitive_closure.py:
+// 474-652 (level-by-level expansion).  Here:
+//   * levels: a vertex joins level r once every non-self successor sits in a
+//     level < r ("pull" rounds over the shrinking set of unplaced vertices);
+//     a round that places nothing while vertices remain means a cycle;
+//   * one wavefront closes one vertex: the direct edges and each successor's
+//     (already sorted) closure row, mapped through the AVOS product, are
+//     folded together with a merge-path merge in LDS, duplicates combined
+//     with the AVOS sum and zero products dropped; rows land sorted;
+//   * row storage comes from a bump-allocated arena (atomicAdd), so a level
+//     needs no host round trip; the final CSR is gathered at the end.
+#include "rbg_common.hpp"
+
+namespace rbg {
+
+constexpr int DAG_WAVES = 4;          // waves per workgroup in the merge kernel
+constexpr int DAG_CAP = 512;          // LDS entries per buffer per wave
+constexpr int SCAN_NT = 256;
+
+// ---------------------------------------------------------------------------
+// exclusive scan of int64 (3 kernels: block sums, scan of sums, add)
+// ---------------------------------------------------------------------------
+__global__ void k_scan_blocks(const int64_t* in, int64_t* out, int64_t n, int64_t* block_sums, int per_thread) {
+  __shared__ int64_t s[SCAN_NT];
+  const int tid = threadIdx.x;
+  const int64_t base = ((int64_t)blockIdx.x * SCAN_NT + tid) * per_thread;
+  int64_t local = 0;
+  for (int q = 0; q < per_thread; q++) {
+    int64_t e = base + q;
+    if (e < n) local += in[e];
+  }
+  s[tid] = local;
+  __syncthreads();
+  for (int off = 1; off < SCAN_NT; off <<= 1) {
+    int64_t v = tid >= off ? s[tid - off] : 0;
+    __syncthreads();
+    s[tid] += v;
+    __syncthreads();
+  }
+  int64_t run = s[tid] - local;  // exclusive prefix of this thread within the block
+  for (int q = 0; q < per_thread; q++) {
+    int64_t e = base + q;
+    if (e < n) {
+      int64_t v = in[e];
+      out[e] = run;
+      run += v;
+    }
+  }
+  if (tid == SCAN_NT - 1) block_sums[blockIdx.x] = s[tid];
+}
+
+__global__ void k_scan_sums(int64_t* block_sums, int nblocks, int64_t* total) {
+  // single workgroup, sequential chunks
+  __shared__ int64_t s[SCAN_NT];
+  const int tid = threadIdx.x;
+  int64_t carry = 0;
+  for (int c0 = 0; c0 < nblocks; c0 += SCAN_NT) {
+    int idx = c0 + tid;
+    int64_t v = idx < nblocks ? block_sums[idx] : 0;
+    s[tid] = v;
+    __syncthreads();
+    for (int off = 1; off < SCAN_NT; off <<= 1) {
+      int64_t t = tid >= off ? s[tid - off] : 0;
+      __syncthreads();
+      s[tid] += t;
+      __syncthreads();
+    }
+    if (idx < nblocks) block_sums[idx] = carry + s[tid] - v;
+    carry += s[SCAN_NT - 1];
+    __syncthreads();
+  }
+  if (tid == 0 && total) *total = carry;
+}
+
+__global__ void k_scan_add(int64_t* out, int64_t n, const int64_t* block_sums, int per_thread) {
+  const int64_t base = ((int64_t)blockIdx.x * SCAN_NT + threadIdx.x) * per_thread;
+  const int64_t add = block_sums[blockIdx.x];
+  for (int q = 0; q < per_thread; q++) {
+    int64_t e = base + q;
+    if (e < n) out[e] += add;
+  }
+}
+
+static int scan_blocks_for(int64_t n, int per_thread) {
+  return (int)((n + (int64_t)SCAN_NT * per_thread - 1) / ((int64_t)SCAN_NT * per_thread));
+}
+
+// out[0..n) = exclusive scan of in[0..n); *total = sum.  tmp holds >= nblocks int64.
+static int scan_exclusive(const int64_t* in, int64_t* out, int64_t n, int64_t* tmp, int64_t* total, hipStream_t s) {
+  const int per = 8;
+  int nb = scan_blocks_for(n, per);
+  if (nb < 1) nb = 1;
+  hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(SCAN_NT), 0, s, in, out, n, tmp, per);
+  RBG_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_NT), 0, s, tmp, nb, total);
+  RBG_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(SCAN_NT), 0, s, out, n, (const int64_t*)tmp, per);
+  RBG_CHECK_LAUNCH();
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// exclusive scan with a device-side element count: block-local scans plus a
+// scan of the block sums; a consumer reads element i as
+// local[i] + sums[i / SCAN_CHUNK].
+// ---------------------------------------------------------------------------
+constexpr int SCAN_PER = 8;
+constexpr int64_t SCAN_CHUNK = (int64_t)SCAN_NT * SCAN_PER;
+
+__global__ void k_scan_local(const int64_t* __restrict__ in, int64_t* __restrict__ local, const int64_t* n_ptr,
+                             int64_t extra, int64_t max_n, int64_t* __restrict__ sums) {
+  __shared__ int64_t sh[SCAN_NT];
+  int64_t n = *n_ptr + extra;
+  n = n < max_n ? n : max_n;
+  const int tid = threadIdx.x;
+  for (int64_t blk = blockIdx.x; blk * SCAN_CHUNK < n; blk += gridDim.x) {
+    const int64_t base = blk * SCAN_CHUNK + (int64_t)tid * SCAN_PER;
+    int64_t v[SCAN_PER];
+    int64_t acc = 0;
+#pragma unroll
+    for (int q = 0; q < SCAN_PER; q++) {
+      v[q] = (base + q < n) ? in[base + q] : 0;
+      acc += v[q];
+    }
+    sh[tid] = acc;
+    __syncthreads();
+    for (int off = 1; off < SCAN_NT; off <<= 1) {
+      int64_t t = tid >= off ? sh[tid - off] : 0;
+      __syncthreads();
+      sh[tid] += t;
+      __syncthreads();
+    }
+    int64_t run = sh[tid] - acc;
+#pragma unroll
+    for (int q = 0; q < SCAN_PER; q++) {
+      if (base + q < n) local[base + q] = run;
+      run += v[q];
+    }
+    if (tid == SCAN_NT - 1) sums[blk] = sh[tid];
+    __syncthreads();
+  }
+}
+
+// scan of the block sums in place (single workgroup); *total = sum of all
+__global__ void k_scan_blocksums(int64_t* sums, const int64_t* n_ptr, int64_t extra, int64_t max_n, int64_t* total) {
+  __shared__ int64_t sh[SCAN_NT];
+  const int tid = threadIdx.x;
+  int64_t n = *n_ptr + extra;
+  n = n < max_n ? n : max_n;
+  const int64_t nb = (n + SCAN_CHUNK - 1) / SCAN_CHUNK;
+  int64_t carry = 0;
+  for (int64_t c0 = 0; c0 < nb; c0 += SCAN_NT) {
+    int64_t idx = c0 + tid;
+    int64_t v = idx < nb ? sums[idx] : 0;
+    sh[tid] = v;
+    __syncthreads();
+    for (int off = 1; off < SCAN_NT; off <<= 1) {
+      int64_t t = tid >= off ? sh[tid - off] : 0;
+      __syncthreads();
+      sh[tid] += t;
+      __syncthreads();
+    }
+    if (idx < nb) sums[idx] = carry + sh[tid] - v;
+    carry += sh[SCAN_NT - 1];
+    __syncthreads();
+  }
+  if (tid == 0 && total) *total = carry;
+}
+
+__device__ inline int64_t scan_at(const int64_t* local, const int64_t* sums, int64_t i) {
+  return local[i] + sums[i / SCAN_CHUNK];
+}
+
+static int scan_dev(const int64_t* in, int64_t* local, int64_t* sums, const int64_t* n_ptr, int64_t extra,
+                    int64_t max_n, int64_t* total, hipStream_t s) {
+  int64_t nb = (max_n + extra + SCAN_CHUNK - 1) / SCAN_CHUNK;
+  int grid = (int)(nb < 1 ? 1 : (nb > 4096 ? 4096 : nb));
+  hipLaunchKernelGGL(k_scan_local, dim3(grid), dim3(SCAN_NT), 0, s, in, local, n_ptr, extra, max_n + extra, sums);
+  RBG_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_scan_blocksums, dim3(1), dim3(SCAN_NT), 0, s, sums, n_ptr, extra, max_n + extra, total);
+  RBG_CHECK_LAUNCH();
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// workspace layout
+// ---------------------------------------------------------------------------
+enum {
+  C_PLACED = 0,   // vertices placed so far
+  C_M0 = 1,       // unplaced count, even rounds read C_M0 and write C_M1
+  C_M1 = 2,
+  C_LSTART = 3,   // first index in `order` of the current level
+  C_LSIZE = 4,    // vertices in the current level
+  C_M = 5,        // candidate entries of the current level
+  C_KEPT = 6,     // closure entries of the current level
+  C_ARENA = 7,    // arena entries used
+  C_CMAX = 8,     // max |candidate|
+  C_FMAX = 9,     // max |final value| > 1
+  C_STUCK = 10,   // a round placed nothing while vertices remain (cycle)
+  C_CAP = 11,     // a capacity was exceeded
+  C_NLEV = 12,    // levels so far
+  C_NNZ = 13,     // total closure entries (gather)
+  C_DONE = 14,    // every vertex placed
+  C_ROUND = 15,   // round counter
+  C_CYCV = 16,    // a vertex on a cycle
+  C_TEMP = 17,    // temp arena entries used (SpGEMM long rows)
+  C_NCTR = 32,
+};
+
+struct DagWork {
+  int32_t* level;      // [n] level of each vertex (INT32_MAX = unplaced)
+  int32_t* order;      // [n] vertices grouped by level, sinks first
+  int32_t* rem[2];     // [n] unplaced vertex lists (ping-pong)
+  int64_t* row_off;    // [n] arena offset of each closure row
+  int64_t* row_len;    // [n+1] closure row lengths
+  int64_t* cnt;        // [n+1] candidate counts of the current level's vertices
+  int64_t* cand_loc;   // [n+1] block-local scan of cnt
+  int64_t* sums_a;     // scan block sums (vertex scans)
+  int64_t* counters;   // [C_NCTR]
+  static int64_t bytes(int64_t n) {
+    int64_t b = 4 * n * 4 + 8 * n + 8 * (n + 1) * 3 + 8 * (n / SCAN_CHUNK + 8) + 8 * C_NCTR;
+    return b + 16 * 256;
+  }
+  static DagWork carve(void* base, int64_t n) {
+    DagWork w;
+    char* p = (char*)(((uintptr_t)base + 255) & ~(uintptr_t)255);
+    auto take = [&](int64_t bytes) {
+      char* r = p;
+      p += (bytes + 255) & ~(int64_t)255;
+      return r;
+    };
+    w.level = (int32_t*)take(4 * n);
+    w.order = (int32_t*)take(4 * n);
+    w.rem[0] = (int32_t*)take(4 * n);
+    w.rem[1] = (int32_t*)take(4 * n);
+    w.row_off = (int64_t*)take(8 * n);
+    w.row_len = (int64_t*)take(8 * (n + 1));
+    w.cnt = (int64_t*)take(8 * (n + 1));
+    w.cand_loc = (int64_t*)take(8 * (n + 1));
+    w.sums_a = (int64_t*)take(8 * (n / SCAN_CHUNK + 8));
+    w.counters = (int64_t*)take(8 * C_NCTR);
+    return w;
+  }
+};
+
+// per-level candidate scratch (capacity cap entries)
+template <typename T>
+struct DagCand {
+  int32_t* col;    // [cap] merged columns (duplicates adjacent)
+  T* val;          // [cap] combined values at run heads
+  int64_t* keep;   // [cap+1] 1 = closure entry
+  int64_t* kloc;   // [cap+1] block-local scan of keep
+  int64_t* sums;   // scan block sums
+  int64_t cap;
+  static int64_t bytes(int64_t cap) { return cap * (4 + (int64_t)sizeof(T) + 16) + 16 + 8 * (cap / SCAN_CHUNK + 8) + 8 * 256; }
+  static DagCand carve(void* base, int64_t cap) {
+    DagCand c;
+    char* p = (char*)(((uintptr_t)base + 255) & ~(uintptr_t)255);
+    auto take = [&](int64_t bytes) {
+      char* r = p;
+      p += (bytes + 255) & ~(int64_t)255;
+      return r;
+    };
+    c.col = (int32_t*)take(4 * cap);
+    c.val = (T*)take((int64_t)sizeof(T) * cap);
+    c.keep = (int64_t*)take(8 * (cap + 1));
+    c.kloc = (int64_t*)take(8 * (cap + 1));
+    c.sums = (int64_t*)take(8 * (cap / SCAN_CHUNK + 8));
+    c.cap = cap;
+    return c;
+  }
+};
+
+__global__ void k_dag_init(int32_t* level, int32_t* rem, int64_t* row_len, int64_t* counters, int32_t n) {
+  int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (v < n) {
+    level[v] = INT32_MAX;
+    rem[v] = (int32_t)v;
+    row_len[v] = 0;
+  }
+  if (v == 0) {
+    for (int q = 0; q < C_NCTR; q++) counters[q] = 0;
+    counters[C_M0] = n;
+    counters[C_CYCV] = -1;
+  }
+}
+
+__device__ inline bool dag_halted(const int64_t* c) { return c[C_STUCK] | c[C_CAP] | c[C_DONE]; }
+
+// Round r, step 1: every unplaced vertex whose non-self successors all sit in
+// levels < r joins level r.  Placement offsets come from one atomic per
+// workgroup (a workgroup-local count first), not one per vertex.
+__global__ __launch_bounds__(256) void k_round_place(const int64_t* __restrict__ indptr,
+                                                     const int32_t* __restrict__ indices, DagWork ws, int32_t r) {
+  __shared__ int s_ready, s_wait;
+  __shared__ int64_t s_base_ready, s_base_wait;
+  int64_t* ctr = ws.counters;
+  if (dag_halted(ctr)) return;
+  const int64_t m = ctr[(r & 1) ? C_M1 : C_M0];
+  const int32_t* rem_in = ws.rem[r & 1];
+  int32_t* rem_out = ws.rem[(r & 1) ^ 1];
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < m; base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t t = base + threadIdx.x;
+    if (threadIdx.x == 0) s_ready = s_wait = 0;
+    __syncthreads();
+    int32_t v = -1;
+    bool ready = false;
+    int slot = 0;
+    if (t < m) {
+      v = rem_in[t];
+      ready = true;
+      for (int64_t e = indptr[v]; e < indptr[v + 1]; e++) {
+        int32_t w = indices[e];
+        if (w != v && ws.level[w] >= r) {
+          ready = false;
+          break;
+        }
+      }
+      slot = ready ? atomicAdd(&s_ready, 1) : atomicAdd(&s_wait, 1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      s_base_ready = s_ready ? (int64_t)atomicAdd((unsigned long long*)&ctr[C_PLACED], (unsigned long long)s_ready) : 0;
+      s_base_wait = s_wait ? (int64_t)atomicAdd((unsigned long long*)&ctr[(r & 1) ? C_M0 : C_M1],
+                                                (unsigned long long)s_wait) : 0;
+    }
+    __syncthreads();
+    if (t < m) {
+      if (ready) {
+        ws.level[v] = r;
+        ws.order[s_base_ready + slot] = v;
+      } else {
+        rem_out[s_base_wait + slot] = v;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Round r, step 2: candidate count of every vertex of the level: its direct
+// entries plus the closure rows of its non-self successors.
+__global__ void k_lvl_count(const int64_t* __restrict__ indptr, const int32_t* __restrict__ indices, DagWork ws) {
+  int64_t* ctr = ws.counters;
+  if (dag_halted(ctr)) return;
+  const int64_t L0 = ctr[C_LSTART], Ls = ctr[C_LSIZE];
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t <= Ls; t += (int64_t)gridDim.x * blockDim.x) {
+    if (t == Ls) {
+      ws.cnt[t] = 0;
+      continue;
+    }
+    const int32_t v = ws.order[L0 + t];
+    int64_t c = 0;
+    for (int64_t e = indptr[v]; e < indptr[v + 1]; e++) {
+      int32_t w = indices[e];
+      c += (w == v) ? 1 : 1 + ws.row_len[w];
+    }
+    ws.cnt[t] = c;
+  }
+}
+
+__global__ void k_lvl_size(DagWork ws) {
+  int64_t* ctr = ws.counters;
+  if (dag_halted(ctr)) return;
+  ctr[C_LSIZE] = ctr[C_PLACED] - ctr[C_LSTART];
+}
+
+__global__ void k_lvl_fix_count(DagWork ws, int64_t cand_cap) {
+  // after the count scan (whose total landed in C_M): level size and capacity
+  int64_t* ctr = ws.counters;
+  if (dag_halted(ctr)) return;
+  if (ctr[C_M] > cand_cap) ctr[C_CAP] = 1;
+}
+
+// upper bound (first index with value > x) / lower bound over a sorted column list
+__device__ inline int64_t ub_col(const int32_t* c, int64_t n, int32_t x) {
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    int64_t mid = (lo + hi) >> 1;
+    if (c[mid] <= x) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+__device__ inline int64_t lb_col(const int32_t* c, int64_t n, int32_t x) {
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    int64_t mid = (lo + hi) >> 1;
+    if (c[mid] < x) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+template <typename T>
+struct DagArena {
+  int32_t* cols;
+  T* vals;
+  int64_t cap;
+};
+
+// Round r, step 3: one lane per candidate entry.  The candidate lists of a
+// vertex v are list 0 = its direct entries and list i = A[v][w_i] (x) C[w_i]
+// (each sorted by column, no repeats).  An entry's place in the merged order
+// is its index in its own list plus, for every other list, the number of
+// entries that come before it (ties go to the lower list); an entry whose
+// column also appears in a lower list is a follower and is dropped; a run
+// head takes the AVOS sum of every list's value for its column.
+template <typename T>
+__global__ __launch_bounds__(256) void k_lvl_place(const int64_t* __restrict__ indptr,
+                                                   const int32_t* __restrict__ indices, const T* __restrict__ data,
+                                                   DagWork ws, DagArena<T> ar, DagCand<T> cd, int width,
+                                                   rbg_status* st) {
+  __shared__ int64_t s_cmax;
+  int64_t* ctr = ws.counters;
+  if (dag_halted(ctr)) return;
+  const int64_t L0 = ctr[C_LSTART], Ls = ctr[C_LSIZE], M = ctr[C_M];
+  if (threadIdx.x == 0) s_cmax = 0;
+  __syncthreads();
+  T cmax = 0;
+  uint32_t evs = 0;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < M; idx += (int64_t)gridDim.x * blockDim.x) {
+    // vertex of this candidate: last t with cand_off[t] <= idx
+    int64_t lo = 0, hi = Ls;
+    while (hi - lo > 1) {
+      int64_t mid = (lo + hi) >> 1;
+      if (scan_at(ws.cand_loc, ws.sums_a, mid) <= idx) lo = mid;
+      else hi = mid;
+    }
+    const int64_t t = lo;
+    const int64_t seg = scan_at(ws.cand_loc, ws.sums_a, t);
+    const int32_t v = ws.order[L0 + t];
+    const int64_t e0 = indptr[v], e1 = indptr[v + 1];
+    int64_t local = idx - seg;
+    // locate own list: list index li (0 = direct), position p, column c, value x
+    int li = 0;
+    int64_t p = 0;
+    int32_t c = 0;
+    T x = 0;
+    const int64_t nd = e1 - e0;
+    if (local < nd) {
+      li = 0;
+      p = local;
+      c = indices[e0 + p];
+      x = data[e0 + p];
+    } else {
+      local -= nd;
+      int k = 0;
+      for (int64_t e = e0; e < e1; e++) {
+        const int32_t w = indices[e];
+        if (w == v) continue;
+        k++;
+        const int64_t lw = ws.row_len[w];
+        if (local < lw) {
+          li = k;
+          p = local;
+          const int64_t o = ws.row_off[w] + p;
+          c = ar.cols[o];
+          uint32_t ev = 0;
+          x = avos_product<T>(data[e], ar.vals[o], width, false, ev);
+          evs |= ev;
+          break;
+        }
+        local -= lw;
+      }
+    }
+    {
+      T a = x < 0 ? (T)-x : x;
+      cmax = a > cmax ? a : cmax;
+    }
+    // rank against every other list
+    int64_t pos = p;
+    bool head = true;
+    T comb = x;
+    {
+      // list 0 (direct entries)
+      if (li != 0) {
+        const int32_t* dc = indices + e0;
+        int64_t r0 = ub_col(dc, nd, c);  // list 0 is lower: ties precede
+        pos += r0;
+        if (r0 > 0 && dc[r0 - 1] == c) head = false;
+      }
+      int k = 0;
+      for (int64_t e = e0; e < e1; e++) {
+        const int32_t w = indices[e];
+        if (w == v) continue;
+        k++;
+        if (k == li) continue;
+        const int64_t lw = ws.row_len[w];
+        const int32_t* wc = ar.cols + ws.row_off[w];
+        if (k < li) {
+          int64_t rk = ub_col(wc, lw, c);
+          pos += rk;
+          if (rk > 0 && wc[rk - 1] == c) head = false;
+        } else {
+          int64_t rk = lb_col(wc, lw, c);
+          pos += rk;
+          if (head && rk < lw && wc[rk] == c) {
+            uint32_t ev = 0;
+            T y = avos_product<T>(data[e], ar.vals[ws.row_off[w] + rk], width, false, ev);
+            evs |= ev;
+            comb = avos_sum<T>(comb, y);
+          }
+        }
+      }
+      if (head && li == 0) {
+        // direct entries take later lists' values for the same column
+      }
+    }
+    const int64_t mp = seg + pos;
+    cd.col[mp] = c;
+    cd.val[mp] = comb;
+    cd.keep[mp] = (head && comb != 0) ? 1 : 0;
+  }
+  if (cmax) atomicMax((long long*)&s_cmax, (long long)cmax);
+  __syncthreads();
+  if (threadIdx.x == 0 && s_cmax) atomicMax((long long*)&ctr[C_CMAX], (long long)s_cmax);
+  if (evs) atomicOr(&st->events, evs);
+  if (blockIdx.x == 0 && threadIdx.x == 0) cd.keep[M] = 0;
+}
+
+__global__ void k_lvl_check_arena(DagWork ws, int64_t arena_cap) {
+  int64_t* ctr = ws.counters;
+  if (dag_halted(ctr)) return;
+  if (ctr[C_ARENA] + ctr[C_KEPT] > arena_cap) ctr[C_CAP] = 1;
+}
+
+// Round r, step 4: kept entries to the arena; row offsets and lengths.
+template <typename T>
+__global__ __launch_bounds__(256) void k_lvl_scatter(DagWork ws, DagArena<T> ar, DagCand<T> cd) {
+  __shared__ int64_t s_fmax;
+  int64_t* ctr = ws.counters;
+  if (dag_halted(ctr)) return;
+  const int64_t L0 = ctr[C_LSTART], Ls = ctr[C_LSIZE], M = ctr[C_M], base = ctr[C_ARENA];
+  if (threadIdx.x == 0) s_fmax = 0;
+  __syncthreads();
+  T fmax = 0;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < M; idx += (int64_t)gridDim.x * blockDim.x) {
+    if (cd.keep[idx]) {
+      const int64_t o = base + scan_at(cd.kloc, cd.sums, idx);
+      T v = cd.val[idx];
+      ar.cols[o] = cd.col[idx];
+      ar.vals[o] = v;
+      T a = v < 0 ? (T)-v : v;
+      if (a > 1 && a > fmax) fmax = a;
+    }
+  }
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < Ls; t += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t v = ws.order[L0 + t];
+    const int64_t s0 = scan_at(ws.cand_loc, ws.sums_a, t), s1 = scan_at(ws.cand_loc, ws.sums_a, t + 1);
+    const int64_t k0 = scan_at(cd.kloc, cd.sums, s0), k1 = scan_at(cd.kloc, cd.sums, s1);
+    ws.row_off[v] = base + k0;
+    ws.row_len[v] = k1 - k0;
+  }
+  if (fmax) atomicMax((long long*)&s_fmax, (long long)fmax);
+  __syncthreads();
+  if (threadIdx.x == 0 && s_fmax) atomicMax((long long*)&ctr[C_FMAX], (long long)s_fmax);
+}
+
+// Round r, step 5 (single lane): advance the arena, count the level, detect a
+// cycle (nothing placed while vertices remain) and completion, and set up
+// round r + 1.
+__global__ void k_round_end(DagWork ws, int32_t r) {
+  int64_t* ctr = ws.counters;
+  if (ctr[C_STUCK] | ctr[C_CAP] | ctr[C_DONE]) return;
+  const int64_t placed_now = ctr[C_PLACED] - ctr[C_LSTART];
+  const int64_t m_next = ctr[(r & 1) ? C_M0 : C_M1];
+  ctr[C_ARENA] += ctr[C_KEPT];
+  if (placed_now > 0) ctr[C_NLEV] = r + 1;
+  if (m_next == 0) {
+    ctr[C_DONE] = 1;
+  } else if (placed_now == 0) {
+    ctr[C_STUCK] = 1;
+  }
+  ctr[C_LSTART] = ctr[C_PLACED];
+  ctr[(r & 1) ? C_M1 : C_M0] = 0;  // becomes the output count of round r + 1
+  ctr[C_ROUND] = r + 1;
+}
+
+// A vertex on a cycle among the unplaced ones: start at the smallest unplaced
+// vertex and keep following its first unplaced successor; the first vertex
+// met twice lies on a cycle.  Single lane; error path only.
+__global__ void k_dag_cycle_vertex(const int64_t* indptr, const int32_t* indices, DagWork ws, int32_t n,
+                                   int32_t* stamp, rbg_status* st) {
+  int32_t v = -1;
+  for (int32_t u = 0; u < n; u++)
+    if (ws.level[u] == INT32_MAX) {
+      v = u;
+      break;
+    }
+  if (v < 0) return;
+  for (int32_t u = 0; u < n; u++) stamp[u] = 0;
+  for (int64_t step = 0; step <= n; step++) {
+    if (stamp[v]) break;
+    stamp[v] = 1;
+    int32_t nxt = -1;
+    for (int64_t e = indptr[v]; e < indptr[v + 1]; e++) {
+      int32_t w = indices[e];
+      if (w != v && ws.level[w] == INT32_MAX) {
+        nxt = w;
+        break;
+      }
+    }
+    if (nxt < 0) break;
+    v = nxt;
+  }
+  st->aux[0] = v;
+  atomicOr(&st->flags, (uint32_t)RBG_F_CYCLE);
+}
+
+__global__ void k_dag_report(DagWork ws, rbg_status* st) {
+  const int64_t* c = ws.counters;
+  if (c[C_CAP]) atomicOr(&st->flags, (uint32_t)RBG_F_CAPACITY);
+  st->aux[2] = c[C_CMAX];
+  st->aux[3] = c[C_FMAX];
+  st->aux[4] = c[C_ARENA];
+  st->aux[6] = c[C_NLEV];
+  st->aux[7] = c[C_DONE];
+}
+
+// ---------------------------------------------------------------------------
+// wave-level helpers
+// ---------------------------------------------------------------------------
+__device__ inline int wave_lane() { return threadIdx.x & 63; }
+
+// Make this wave's LDS / global writes visible to its other lanes before they
+// read them (the merge buffers may live in either address space).
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+}
+
+template <typename V>
+__device__ inline V wave_bcast(V v, int src) {
+  return __shfl(v, src, 64);
+}
+
+__device__ inline int64_t wave_sum64(int64_t v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Stable merge of two column-sorted lists (ties: list A first) into out.
+template <typename T>
+__device__ void wave_merge(const int32_t* ac, const T* av, int na, const int32_t* bc, const T* bv, int nb,
+                           int32_t* oc, T* ov) {
+  const int lane = wave_lane();
+  const int total = na + nb;
+  const int per = (total + 63) >> 6;
+  int d0 = lane * per;
+  if (d0 > total) d0 = total;
+  int d1 = d0 + per;
+  if (d1 > total) d1 = total;
+  // merge-path split for diagonal d0: smallest i with i + j = d0 and A[i] > B[j-1]
+  int lo = d0 > nb ? d0 - nb : 0, hi = d0 < na ? d0 : na;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    // take A[mid] before B[d0-1-mid]?  A wins ties.
+    if (ac[mid] <= bc[d0 - 1 - mid]) lo = mid + 1;
+    else hi = mid;
+  }
+  int i = lo, j = d0 - lo;
+  for (int d = d0; d < d1; d++) {
+    bool takeA = (j >= nb) || (i < na && ac[i] <= bc[j]);
+    if (takeA) {
+      oc[d] = ac[i];
+      ov[d] = av[i];
+      i++;
+    } else {
+      oc[d] = bc[j];
+      ov[d] = bv[j];
+      j++;
+    }
+  }
+}
+
+// Combine adjacent equal columns with the AVOS sum, drop zero values, compact
+// into dst.  Returns the new length (wave-uniform).
+template <typename T>
+__device__ int wave_dedup(const int32_t* sc, const T* sv, int n, int32_t* dc, T* dv) {
+  const int lane = wave_lane();
+  int out = 0;
+  for (int base = 0; base < n; base += 64) {
+    int e = base + lane;
+    bool keep = false;
+    int32_t c = 0;
+    T v = 0;
+    if (e < n) {
+      c = sc[e];
+      v = sv[e];
+      bool first = (e == 0) || (sc[e - 1] != c);
+      if (first) {
+        int f = e + 1;
+        while (f < n && sc[f] == c) {
+          v = avos_sum<T>(v, sv[f]);
+          f++;
+        }
+        keep = v != 0;
+      }
+    }
+    uint64_t mask = __ballot(keep);
+    int before = __popcll(mask & ((1ull << lane) - 1ull));
+    if (keep) {
+      dc[out + before] = c;
+      dv[out + before] = v;
+    }
+    out += __popcll(mask);
+  }
+  return out;
+}
+
+template <typename T>
+struct RowArena {
+  int32_t* cols;
+  T* vals;
+  int64_t cap;   // entries in the row arena
+  int32_t* tcols;
+  T* tvals;
+  int64_t tcap;  // entries in the temp arena (rows longer than DAG_CAP)
+};
+
+// ---------------------------------------------------------------------------
+// AVOS SpGEMM C = A (x) B, one wave per row of A: the row is the AVOS sum of
+// A[i][k] (x) B[k][:] over the entries of A's row, folded one B row at a time
+// with the same merge-path merge.  With mask_upper, columns < i are dropped
+// (the j >= i mask of spgemm_upper_triangular, gpu/spgemm.py:122-149).
+// Reference: sparse/sparsetools/rbm_math.h:164-222 (values),
+//            gpu/spgemm_numeric.py (GPU driver).  Zero sums are dropped.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(64 * DAG_WAVES) void k_spgemm_rows(
+    const int64_t* __restrict__ ap, const int32_t* __restrict__ aj, const T* __restrict__ ax, int32_t m,
+    const int64_t* __restrict__ bp, const int32_t* __restrict__ bj, const T* __restrict__ bx, int mask_upper,
+    DagWork ws, RowArena<T> ar, int width, int is_uns, rbg_status* st) {
+  __shared__ int32_t lc[DAG_WAVES][3][DAG_CAP];
+  __shared__ T lv[DAG_WAVES][3][DAG_CAP];
+  const int wv = threadIdx.x >> 6;
+  const int lane = wave_lane();
+  const int64_t i = (int64_t)blockIdx.x * DAG_WAVES + wv;
+  if (i >= m) return;
+  const int64_t e0 = ap[i], e1 = ap[i + 1];
+  int64_t ub = 0;
+  for (int64_t e = e0 + lane; e < e1; e += 64) {
+    int32_t k = aj[e];
+    ub += bp[k + 1] - bp[k];
+  }
+  ub = wave_sum64(ub);
+  int64_t off = 0;
+  if (lane == 0) off = (int64_t)atomicAdd((unsigned long long*)&ws.counters[C_ARENA], (unsigned long long)ub);
+  off = wave_bcast(off, 0);
+  if (off + ub > ar.cap) {
+    if (lane == 0) {
+      atomicOr(&st->flags, (uint32_t)RBG_F_CAPACITY);
+      ws.row_len[i] = 0;
+      ws.row_off[i] = 0;
+    }
+    return;
+  }
+  const bool big = ub > DAG_CAP;
+  int32_t *c0p, *c1p, *c2p;
+  T *v0p, *v1p, *v2p;
+  if (!big) {
+    c0p = lc[wv][0]; c1p = lc[wv][1]; c2p = lc[wv][2];
+    v0p = lv[wv][0]; v1p = lv[wv][1]; v2p = lv[wv][2];
+  } else {
+    int64_t toff = 0;
+    if (lane == 0) toff = (int64_t)atomicAdd((unsigned long long*)&ws.counters[C_TEMP], (unsigned long long)(3 * ub));
+    toff = wave_bcast(toff, 0);
+    if (toff + 3 * ub > ar.tcap) {
+      if (lane == 0) {
+        atomicOr(&st->flags, (uint32_t)RBG_F_CAPACITY);
+        ws.row_len[i] = 0;
+        ws.row_off[i] = 0;
+      }
+      return;
+    }
+    c0p = ar.tcols + toff; c1p = c0p + ub; c2p = c1p + ub;
+    v0p = ar.tvals + toff; v1p = v0p + ub; v2p = v1p + ub;
+  }
+  int na = 0;
+  uint32_t evs = 0;
+  for (int64_t e = e0; e < e1; e++) {
+    const int32_t k = aj[e];
+    const T a = ax[e];
+    const int64_t bo = bp[k];
+    const int nb = (int)(bp[k + 1] - bo);
+    int nw = 0;
+    // stage a (x) B[k] (masked) into buffer 1
+    for (int base = 0; base < nb; base += 64) {
+      int q = base + lane;
+      bool keep = false;
+      int32_t c = 0;
+      T p = 0;
+      if (q < nb) {
+        c = bj[bo + q];
+        uint32_t ev = 0;
+        p = avos_product<T>(a, bx[bo + q], width, is_uns != 0, ev);
+        evs |= ev;
+        keep = !(mask_upper && c < i);
+      }
+      uint64_t msk = __ballot(keep);
+      int before = __popcll(msk & ((1ull << lane) - 1ull));
+      if (keep) {
+        c1p[nw + before] = c;
+        v1p[nw + before] = p;
+      }
+      nw += __popcll(msk);
+    }
+    wave_sync();
+    wave_merge<T>(c0p, v0p, na, c1p, v1p, nw, c2p, v2p);
+    wave_sync();
+    na = wave_dedup<T>(c2p, v2p, na + nw, c0p, v0p);
+    wave_sync();
+  }
+  for (int q = lane; q < na; q += 64) {
+    ar.cols[off + q] = c0p[q];
+    ar.vals[off + q] = v0p[q];
+  }
+  if (lane == 0) {
+    ws.row_off[i] = off;
+    ws.row_len[i] = na;
+  }
+  uint32_t evall = evs;
+  for (int o = 32; o > 0; o >>= 1) evall |= (uint32_t)__shfl_xor((int)evall, o, 64);
+  if (lane == 0 && evall) atomicOr(&st->events, evall);
+}
+
+__global__ void k_rows_init(int64_t* row_len, int64_t* counters, int32_t n) {
+  int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (v < n) row_len[v] = 0;
+  if (v == 0)
+    for (int q = 0; q < C_NCTR; q++) counters[q] = 0;
+}
+
+template <typename T>
+__global__ void k_dag_gather(const DagWork ws, const DagArena<T> ar, const int64_t* __restrict__ out_ptr, int32_t n,
+                             int32_t* __restrict__ out_idx, T* __restrict__ out_val) {
+  const int lane = wave_lane();
+  const int64_t v = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (v >= n) return;
+  const int64_t src = ws.row_off[v], len = ws.row_len[v], dst = out_ptr[v];
+  for (int64_t q = lane; q < len; q += 64) {
+    out_idx[dst + q] = ar.cols[src + q];
+    out_val[dst + q] = ar.vals[src + q];
+  }
+}
+
+__global__ void k_dag_finish_status(const DagWork ws, rbg_status* st) {
+  st->aux[1] = ws.counters[C_NNZ];    // nnz of the gathered CSR
+  st->aux[2] = ws.counters[C_CMAX];   // max |candidate|
+  st->aux[3] = ws.counters[C_FMAX];   // max |final value| > 1
+  st->aux[4] = ws.counters[C_ARENA];  // arena entries used
+  st->aux[5] = ws.counters[C_TEMP];   // temp arena entries used
+}
+
+__global__ void k_copy_i64(const int64_t* src, int64_t* dst, int64_t n) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+__global__ void k_set_last(int64_t* p, int64_t idx, const int64_t* val) { p[idx] = *val; }
+
+// ---------------------------------------------------------------------------
+// CSR canonicalisation: sort each row by column, combine duplicate columns
+// with the AVOS sum and drop explicit zeros (one lane per row; insertion sort
+// in place).  Row lengths go to len[], then a scan + gather compacts.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_csr_sort_rows(const int64_t* __restrict__ indptr, int32_t* __restrict__ idx, T* __restrict__ val,
+                                int32_t n, int64_t* __restrict__ len) {
+  int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int64_t a = indptr[r], b = indptr[r + 1];
+  for (int64_t x = a + 1; x < b; x++) {
+    int32_t c = idx[x];
+    T v = val[x];
+    int64_t y = x - 1;
+    while (y >= a && idx[y] > c) {
+      idx[y + 1] = idx[y];
+      val[y + 1] = val[y];
+      y--;
+    }
+    idx[y + 1] = c;
+    val[y + 1] = v;
+  }
+  int64_t out = a;
+  for (int64_t x = a; x < b;) {
+    int32_t c = idx[x];
+    T v = val[x];
+    int64_t y = x + 1;
+    while (y < b && idx[y] == c) {
+      v = avos_sum<T>(v, val[y]);
+      y++;
+    }
+    if (v != 0) {
+      idx[out] = c;
+      val[out] = v;
+      out++;
+    }
+    x = y;
+  }
+  len[r] = out - a;
+}
+
+template <typename T>
+__global__ void k_csr_compact(const int64_t* __restrict__ indptr, const int32_t* __restrict__ idx,
+                              const T* __restrict__ val, const int64_t* __restrict__ len, const int64_t* __restrict__ optr,
+                              int32_t n, int32_t* __restrict__ oidx, T* __restrict__ oval) {
+  const int lane = threadIdx.x & 63;
+  const int64_t r = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (r >= n) return;
+  const int64_t a = indptr[r], l = len[r], o = optr[r];
+  for (int64_t q = lane; q < l; q += 64) {
+    oidx[o + q] = idx[a + q];
+    oval[o + q] = val[a + q];
+  }
+}
+
+__global__ void k_csr_check(const int64_t* __restrict__ indptr, const int32_t* __restrict__ idx, int32_t n,
+                            int32_t ncols, rbg_status* st) {
+  int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int64_t a = indptr[r], b = indptr[r + 1];
+  bool bad = b < a;
+  for (int64_t x = a; x < b && !bad; x++)
+    if (idx[x] < 0 || idx[x] >= ncols) bad = true;
+  if (bad) atomicOr(&st->flags, (uint32_t)RBG_F_CAPACITY);
+}
+
+// Element-wise sum of two canonical CSR matrices, one lane per row: a
+// sequential merge of the two sorted rows into the concatenated slot
+// (cat_ptr = a_ptr + b_ptr), equal columns combined by `combine`
+// (0: AVOS sum, 1: smallest |value|, first operand on ties), zeros dropped.
+template <typename T>
+__global__ void k_csr_add_rows(const int64_t* __restrict__ ap, const int32_t* __restrict__ aj, const T* __restrict__ ax,
+                               const int64_t* __restrict__ bp, const int32_t* __restrict__ bj, const T* __restrict__ bx,
+                               int32_t n, int combine, const int64_t* __restrict__ cp, int32_t* __restrict__ cj,
+                               T* __restrict__ cx, int64_t* __restrict__ len) {
+  int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  int64_t i = ap[r], ie = ap[r + 1], j = bp[r], je = bp[r + 1], o = cp[r], o0 = o;
+  while (i < ie || j < je) {
+    int32_t c;
+    T v;
+    if (j >= je || (i < ie && aj[i] < bj[j])) {
+      c = aj[i];
+      v = ax[i++];
+    } else if (i >= ie || bj[j] < aj[i]) {
+      c = bj[j];
+      v = bx[j++];
+    } else {
+      c = aj[i];
+      T x = ax[i++], y = bx[j++];
+      if (combine == 0) {
+        v = avos_sum<T>(x, y);
+      } else {
+        T ax_ = x < 0 ? -x : x, ay_ = y < 0 ? -y : y;
+        v = ay_ < ax_ ? y : x;
+      }
+    }
+    if (v != 0) {
+      cj[o] = c;
+      cx[o] = v;
+      o++;
+    }
+  }
+  len[r] = o - o0;
+}
+
+}  // namespace rbg
+
+using namespace rbg;
+
+extern "C" {
+
+int64_t rbg_dag_workspace_bytes(int32_t n) { return DagWork::bytes(n > 0 ? n : 1); }
+int64_t rbg_dag_cand_bytes(int64_t cap, int32_t elem_bytes) {
+  return elem_bytes == 4 ? DagCand<int32_t>::bytes(cap) : DagCand<int64_t>::bytes(cap);
+}
+
+#define RBG_DAG_ENTRY(SUF, T)                                                                                   \
+  int rbg_dag_closure_##SUF(const int64_t* indptr, const int32_t* indices, const T* data, int32_t n,            \
+                            void* workspace, int64_t ws_bytes, int32_t* arena_cols, T* arena_vals,              \
+                            int64_t arena_cap, void* cand, int64_t cand_cap, int32_t batch, int32_t width,      \
+                            rbg_status* st, rbg_stream_t stream) {                                              \
+    hipStream_t s = (hipStream_t)stream;                                                                        \
+    if (n <= 0 || !workspace || !arena_cols || !arena_vals || !cand || cand_cap < 1) return RBG_E_ARG;          \
+    if (ws_bytes < DagWork::bytes(n)) return RBG_E_WORKSPACE;                                                   \
+    if (batch < 1) batch = 8;                                                                                   \
+    DagWork ws = DagWork::carve(workspace, n);                                                                  \
+    DagCand<T> cd = DagCand<T>::carve(cand, cand_cap);                                                          \
+    DagArena<T> ar{arena_cols, arena_vals, arena_cap};                                                          \
+    hipLaunchKernelGGL(k_dag_init, dim3((n + 255) / 256), dim3(256), 0, s, ws.level, ws.rem[0], ws.row_len,     \
+                       ws.counters, n);                                                                         \
+    RBG_CHECK_LAUNCH();                                                                                         \
+    const int gv = (int)((n + 255) / 256 < 2048 ? (n + 255) / 256 : 2048);                                      \
+    const int gc = 4096;                                                                                        \
+    int64_t host[C_NCTR];                                                                                       \
+    for (int32_t r = 0; r <= n; r++) {                                                                          \
+      hipLaunchKernelGGL(k_round_place, dim3(gv), dim3(256), 0, s, indptr, indices, ws, r);                     \
+      RBG_CHECK_LAUNCH();                                                                                       \
+      hipLaunchKernelGGL(k_lvl_size, dim3(1), dim3(1), 0, s, ws);                                               \
+      RBG_CHECK_LAUNCH();                                                                                       \
+      hipLaunchKernelGGL(k_lvl_count, dim3(gv), dim3(256), 0, s, indptr, indices, ws);                          \
+      RBG_CHECK_LAUNCH();                                                                                       \
+      int rc = scan_dev(ws.cnt, ws.cand_loc, ws.sums_a, ws.counters + C_LSIZE, 1, n, ws.counters + C_M, s);    \
+      if (rc) return rc;                                                                                        \
+      hipLaunchKernelGGL(k_lvl_fix_count, dim3(1), dim3(1), 0, s, ws, cand_cap);                                \
+      RBG_CHECK_LAUNCH();                                                                                       \
+      int pi = prof_start(s);                                                                                   \
+      hipLaunchKernelGGL((k_lvl_place<T>), dim3(gc), dim3(256), 0, s, indptr, indices, data, ws, ar, cd, width, \
+                         st);                                                                                   \
+      RBG_CHECK_LAUNCH();                                                                                       \
+      rc = scan_dev(cd.keep, cd.kloc, cd.sums, ws.counters + C_M, 1, cand_cap, ws.counters + C_KEPT, s);        \
+      if (rc) return rc;                                                                                        \
+      hipLaunchKernelGGL(k_lvl_check_arena, dim3(1), dim3(1), 0, s, ws, arena_cap);                             \
+      RBG_CHECK_LAUNCH();                                                                                       \
+      hipLaunchKernelGGL((k_lvl_scatter<T>), dim3(gc), dim3(256), 0, s, ws, ar, cd);                            \
+      RBG_CHECK_LAUNCH();                                                                                       \
+      prof_stop(2, pi, s);                                                                                      \
+      hipLaunchKernelGGL(k_round_end, dim3(1), dim3(1), 0, s, ws, r);                                           \
+      RBG_CHECK_LAUNCH();                                                                                       \
+      if ((r + 1) % batch == 0) {                                                                               \
+        hipError_t e = hipMemcpyAsync(host, ws.counters, sizeof(host), hipMemcpyDeviceToHost, s);              \
+        if (e != hipSuccess) return (int)e;                                                                     \
+        e = hipStreamSynchronize(s);                                                                            \
+        if (e != hipSuccess) return (int)e;                                                                     \
+        if (host[C_DONE] || host[C_STUCK] || host[C_CAP]) break;                                                \
+      }                                                                                                         \
+    }                                                                                                           \
+    hipLaunchKernelGGL(k_dag_report, dim3(1), dim3(1), 0, s, ws, st);                                           \
+    RBG_CHECK_LAUNCH();                                                                                         \
+    hipError_t e = hipMemcpyAsync(host, ws.counters, sizeof(host), hipMemcpyDeviceToHost, s);                  \
+    if (e != hipSuccess) return (int)e;                                                                         \
+    e = hipStreamSynchronize(s);                                                                                \
+    if (e != hipSuccess) return (int)e;                                                                         \
+    if (host[C_STUCK]) {                                                                                        \
+      hipLaunchKernelGGL(k_dag_cycle_vertex, dim3(1), dim3(1), 0, s, indptr, indices, ws, n, ws.rem[0], st);    \
+      RBG_CHECK_LAUNCH();                                                                                       \
+    }                                                                                                           \
+    return 0;                                                                                                   \
+  }                                                                                                             \
+  int rbg_dag_gather_##SUF(void* workspace, int32_t n, const int32_t* arena_cols, const T* arena_vals,          \
+                           int64_t* out_indptr, int32_t* out_indices, T* out_data, rbg_status* st,              \
+                           rbg_stream_t stream) {                                                               \
+    hipStream_t s = (hipStream_t)stream;                                                                        \
+    if (n <= 0 || !workspace || !out_indptr) return RBG_E_ARG;                                                  \
+    DagWork ws = DagWork::carve(workspace, n);                                                                  \
+    DagArena<T> ar{(int32_t*)arena_cols, (T*)arena_vals, 0};                                                    \
+    int rc = scan_exclusive(ws.row_len, out_indptr, n, ws.sums_a, ws.counters + C_NNZ, s);                      \
+    if (rc) return rc;                                                                                          \
+    hipLaunchKernelGGL(k_set_last, dim3(1), dim3(1), 0, s, out_indptr, (int64_t)n,                              \
+                       (const int64_t*)(ws.counters + C_NNZ));                                                  \
+    RBG_CHECK_LAUNCH();                                                                                         \
+    if (out_indices && out_data) {                                                                              \
+      hipLaunchKernelGGL((k_dag_gather<T>), dim3((n + 3) / 4), dim3(256), 0, s, ws, ar,                         \
+                         (const int64_t*)out_indptr, n, out_indices, out_data);                                 \
+      RBG_CHECK_LAUNCH();                                                                                       \
+    }                                                                                                           \
+    hipLaunchKernelGGL(k_dag_finish_status, dim3(1), dim3(1), 0, s, ws, st);                                     \
+    RBG_CHECK_LAUNCH();                                                                                         \
+    return 0;                                                                                                   \
+  }
+
+RBG_DAG_ENTRY(i32, int32_t)
+RBG_DAG_ENTRY(i64, int64_t)
+
+}  // extern "C"
+
+extern "C" {
+// Sort rows, combine duplicates with the AVOS sum, drop explicit zeros, in
+// place on (idx, val); the canonical matrix is written to (out_indptr,
+// out_idx, out_val).  tmp needs rbg_csr_tmp_bytes(n).
+int64_t rbg_csr_tmp_bytes(int32_t n) { return 8 * ((int64_t)n + 1) + 8 * ((int64_t)n / 256 + 8) + 8 * 4 + 1024; }
+
+#define RBG_CSR_ENTRY(SUF, T)                                                                                   \
+  int rbg_csr_canonicalize_##SUF(const int64_t* indptr, int32_t* idx, T* val, int32_t n, void* tmp,            \
+                                 int64_t* out_indptr, int32_t* out_idx, T* out_val, rbg_stream_t stream) {     \
+    hipStream_t s = (hipStream_t)stream;                                                                        \
+    if (n < 0 || !indptr || !tmp || !out_indptr) return RBG_E_ARG;                                              \
+    if (n == 0) return 0;                                                                                       \
+    int64_t* len = (int64_t*)(((uintptr_t)tmp + 255) & ~(uintptr_t)255);                                        \
+    int64_t* stmp = len + ((n + 1 + 31) & ~31);                                                                 \
+    int64_t* total = stmp + (n / 256 + 8);                                                                      \
+    hipLaunchKernelGGL((k_csr_sort_rows<T>), dim3((n + 255) / 256), dim3(256), 0, s, indptr, idx, val, n, len); \
+    RBG_CHECK_LAUNCH();                                                                                         \
+    int rc = scan_exclusive(len, out_indptr, n, stmp, total, s);                                                \
+    if (rc) return rc;                                                                                          \
+    hipLaunchKernelGGL(k_set_last, dim3(1), dim3(1), 0, s, out_indptr, (int64_t)n, (const int64_t*)total);      \
+    RBG_CHECK_LAUNCH();                                                                                         \
+    if (out_idx && out_val) {                                                                                   \
+      hipLaunchKernelGGL((k_csr_compact<T>), dim3((n + 3) / 4), dim3(256), 0, s, indptr, (const int32_t*)idx,     \
+                         (const T*)val, (const int64_t*)len, (const int64_t*)out_indptr, n, out_idx, out_val);   \
+      RBG_CHECK_LAUNCH();                                                                                       \
+    }                                                                                                           \
+    return 0;                                                                                                   \
+  }
+RBG_CSR_ENTRY(i32, int32_t)
+RBG_CSR_ENTRY(i64, int64_t)
+
+// Validate a CSR structure: monotone indptr, column indices in [0, ncols).
+// Sets RBG_F_CAPACITY in st->flags on a violation.
+int rbg_csr_validate(const int64_t* indptr, const int32_t* idx, int32_t n, int32_t ncols, rbg_status* st,
+                     rbg_stream_t stream) {
+  if (n < 0 || !indptr || !st) return RBG_E_ARG;
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_csr_check, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, indptr, idx, n, ncols,
+                     st);
+  RBG_CHECK_LAUNCH();
+  return 0;
+}
+}  // extern "C"
+
+extern "C" {
+#define RBG_SPGEMM_ENTRY(SUF, T)                                                                                \
+  int rbg_spgemm_rows_##SUF(const int64_t* ap, const int32_t* aj, const T* ax, int32_t m, const int64_t* bp,     \
+                            const int32_t* bj, const T* bx, int32_t mask_upper, void* workspace,                 \
+                            int32_t* arena_cols, T* arena_vals, int64_t arena_cap, int32_t* tmp_cols,            \
+                            T* tmp_vals, int64_t tmp_cap, int32_t width, int32_t is_unsigned, rbg_status* st,    \
+                            rbg_stream_t stream) {                                                              \
+    hipStream_t s = (hipStream_t)stream;                                                                        \
+    if (m <= 0 || !workspace || !arena_cols || !arena_vals) return RBG_E_ARG;                                   \
+    DagWork ws = DagWork::carve(workspace, m);                                                                  \
+    RowArena<T> ar{arena_cols, arena_vals, arena_cap, tmp_cols, tmp_vals, tmp_cap};                             \
+    hipLaunchKernelGGL(k_rows_init, dim3((m + 255) / 256), dim3(256), 0, s, ws.row_len, ws.counters, m);        \
+    RBG_CHECK_LAUNCH();                                                                                         \
+    int pi = prof_start(s);                                                                                     \
+    hipLaunchKernelGGL((k_spgemm_rows<T>), dim3((m + DAG_WAVES - 1) / DAG_WAVES), dim3(64 * DAG_WAVES), 0, s,   \
+                       ap, aj, ax, m, bp, bj, bx, mask_upper, ws, ar, width, is_unsigned, st);                  \
+    RBG_CHECK_LAUNCH();                                                                                         \
+    prof_stop(3, pi, s);                                                                                        \
+    return 0;                                                                                                   \
+  }
+RBG_SPGEMM_ENTRY(i32, int32_t)
+RBG_SPGEMM_ENTRY(i64, int64_t)
+}  // extern "C"
+
+extern "C" {
+#define RBG_CSR_ADD_ENTRY(SUF, T)                                                                               \
+  int rbg_csr_add_##SUF(const int64_t* ap, const int32_t* aj, const T* ax, const int64_t* bp, const int32_t* bj, \
+                        const T* bx, int32_t n, int32_t combine, const int64_t* cat_ptr, int32_t* cat_idx,      \
+                        T* cat_val, void* tmp, int64_t* out_indptr, int32_t* out_idx, T* out_val,               \
+                        rbg_stream_t stream) {                                                                  \
+    hipStream_t s = (hipStream_t)stream;                                                                        \
+    if (n < 0 || !tmp || !out_indptr) return RBG_E_ARG;                                                         \
+    if (n == 0) return 0;                                                                                       \
+    int64_t* len = (int64_t*)(((uintptr_t)tmp + 255) & ~(uintptr_t)255);                                        \
+    int64_t* stmp = len + ((n + 1 + 31) & ~31);                                                                 \
+    int64_t* total = stmp + (n / 256 + 8);                                                                      \
+    hipLaunchKernelGGL((k_csr_add_rows<T>), dim3((n + 255) / 256), dim3(256), 0, s, ap, aj, ax, bp, bj, bx, n,   \
+                       combine, cat_ptr, cat_idx, cat_val, len);                                                \
+    RBG_CHECK_LAUNCH();                                                                                         \
+    int rc = scan_exclusive(len, out_indptr, n, stmp, total, s);                                                \
+    if (rc) return rc;                                                                                          \
+    hipLaunchKernelGGL(k_set_last, dim3(1), dim3(1), 0, s, out_indptr, (int64_t)n, (const int64_t*)total);      \
+    RBG_CHECK_LAUNCH();                                                                                         \
+    if (out_idx && out_val) {                                                                                   \
+      hipLaunchKernelGGL((k_csr_compact<T>), dim3((n + 3) / 4), dim3(256), 0, s, cat_ptr, (const int32_t*)cat_idx, \
+                         (const T*)cat_val, (const int64_t*)len, (const int64_t*)out_indptr, n, out_idx, out_val); \
+      RBG_CHECK_LAUNCH();                                                                                       \
+    }                                                                                                           \
+    return 0;                                                                                                   \
+  }
+RBG_CSR_ADD_ENTRY(i32, int32_t)
+RBG_CSR_ADD_ENTRY(i64, int64_t)
+}  // extern "C"

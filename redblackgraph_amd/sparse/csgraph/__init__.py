@@ -1,5 +1,15 @@
 """csgraph closure family (sparse/csgraph/__init__.py of the reference)."""
 from .cycleerror import CycleError
-from .transitive_closure import transitive_closure, transitive_closure_floyd_warshall
+from .transitive_closure import (
+    transitive_closure,
+    transitive_closure_floyd_warshall,
+    transitive_closure_dijkstra,
+    transitive_closure_squaring,
+    transitive_closure_adaptive,
+    transitive_closure_dag_sparse,
+    DensificationError,
+)
 
-__all__ = ["CycleError", "transitive_closure", "transitive_closure_floyd_warshall"]
+__all__ = ["CycleError", "DensificationError", "transitive_closure", "transitive_closure_floyd_warshall",
+           "transitive_closure_dijkstra", "transitive_closure_squaring", "transitive_closure_adaptive",
+           "transitive_closure_dag_sparse"]

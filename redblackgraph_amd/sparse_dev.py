@@ -1,0 +1,260 @@
+"""Device-level sparse operations on CSR matrices held in torch tensors.
+
+Device CSR convention: int64 row pointers, int32 column indices, int32 or
+int64 values.  The reference-shaped classes (gpu.CSRMatrixGPU,
+sparse.rb_matrix) wrap these.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+
+
+@dataclass
+class DevCSR:
+    indptr: object   # torch int64 [n_rows + 1]
+    indices: object  # torch int32 [nnz]
+    data: object     # torch int32/int64 [nnz]
+    shape: tuple
+
+    @property
+    def nnz(self) -> int:
+        return int(self.indices.numel())
+
+    @property
+    def n_rows(self) -> int:
+        return int(self.shape[0])
+
+
+def from_scipy(m, dtype=None) -> DevCSR:
+    import scipy.sparse as sp
+    m = sp.csr_matrix(m)
+    t = _lib.torch()
+    dev = _lib.device()
+    dt = np.dtype(dtype) if dtype is not None else (np.dtype(np.int64) if m.dtype.itemsize > 4 else np.dtype(np.int32))
+    return DevCSR(
+        t.from_numpy(np.ascontiguousarray(m.indptr, dtype=np.int64)).to(dev),
+        t.from_numpy(np.ascontiguousarray(m.indices, dtype=np.int32)).to(dev),
+        t.from_numpy(np.ascontiguousarray(m.data.astype(dt))).to(dev),
+        tuple(m.shape))
+
+
+def to_scipy(c: DevCSR):
+    import scipy.sparse as sp
+    indptr = c.indptr.cpu().numpy()
+    idx_dt = np.int32 if (indptr.size == 0 or indptr[-1] < 2 ** 31) else np.int64
+    return sp.csr_matrix((c.data.cpu().numpy(), c.indices.cpu().numpy(), indptr.astype(idx_dt)), shape=c.shape)
+
+
+def canonicalize(c: DevCSR, stream=None) -> DevCSR:
+    """Sorted rows, duplicates combined with the AVOS sum, explicit zeros dropped."""
+    t = _lib.torch()
+    n = c.n_rows
+    if n == 0:
+        return c
+    idx = c.indices.clone()
+    val = c.data.clone()
+    tmp = t.empty(int(_lib.load().rbg_csr_tmp_bytes(n)), dtype=t.uint8, device=idx.device)
+    optr = t.empty(n + 1, dtype=t.int64, device=idx.device)
+    fn = "rbg_csr_canonicalize_i32" if val.dtype == t.int32 else "rbg_csr_canonicalize_i64"
+    sh = _lib.stream_handle(stream)
+    _lib.call(fn, _lib.ptr(c.indptr), _lib.ptr(idx), _lib.ptr(val), n, _lib.ptr(tmp), _lib.ptr(optr), None, None, sh)
+    nnz = int(optr[-1].item())
+    oi = t.empty(nnz, dtype=t.int32, device=idx.device)
+    ov = t.empty(nnz, dtype=val.dtype, device=idx.device)
+    _lib.call(fn, _lib.ptr(c.indptr), _lib.ptr(idx), _lib.ptr(val), n, _lib.ptr(tmp), _lib.ptr(optr), _lib.ptr(oi),
+              _lib.ptr(ov), sh)
+    return DevCSR(optr, oi, ov, c.shape)
+
+
+def validate(c: DevCSR, stream=None) -> bool:
+    from .dense import status
+    st = status()
+    st.reset(stream)
+    if c.n_rows:
+        _lib.call("rbg_csr_validate", _lib.ptr(c.indptr), _lib.ptr(c.indices), c.n_rows, int(c.shape[1]), st.p,
+                  _lib.stream_handle(stream))
+    s = st.read(stream)
+    return not (s.flags & _lib.F_CAPACITY)
+
+
+@dataclass
+class DagInfo:
+    nnz: int
+    cand_max: int
+    final_max: int
+    cycle_vertex: int
+    nlevels: int
+    events: int
+
+    @property
+    def diameter_candidates(self) -> int:
+        """_transitive_closure_dag.pyx:347-349: MSB(max |candidate|) when > 1."""
+        return int(self.cand_max).bit_length() - 1 if self.cand_max > 1 else 0
+
+    @property
+    def diameter_final(self) -> int:
+        """gpu/transitive_closure.py:640-646: floor(log2(max |final| > 1))."""
+        return int(self.final_max).bit_length() - 1 if self.final_max > 1 else 0
+
+
+class DagPlan:
+    """Reusable device buffers for closing graphs of one size: workspace, row
+    arena (closure entries) and per-level candidate scratch."""
+
+    def __init__(self, n: int, value_dtype, arena_entries: int, cand_entries: int):
+        t = _lib.torch()
+        dev = _lib.device()
+        L = _lib.load()
+        self.n = n
+        self.vdt = value_dtype
+        self.ws = t.empty(int(L.rbg_dag_workspace_bytes(max(n, 1))), dtype=t.uint8, device=dev)
+        self.arena_cols = t.empty(arena_entries, dtype=t.int32, device=dev)
+        self.arena_vals = t.empty(arena_entries, dtype=value_dtype, device=dev)
+        elem = 4 if value_dtype == t.int32 else 8
+        self.cand_cap = int(cand_entries)
+        self.cand = t.empty(int(L.rbg_dag_cand_bytes(self.cand_cap, elem)), dtype=t.uint8, device=dev)
+
+
+def dag_closure(A: DevCSR, width: int | None = None, stream=None, plan: DagPlan | None = None,
+                arena_factor: float = 48.0, batch: int = 8):
+    """Closure of a DAG held as a canonical device CSR.
+
+    Returns (C: DevCSR, DagInfo).  On a cycle returns (None, info) with
+    info.cycle_vertex >= 0.  When a capacity flag comes back the buffers grow
+    and the closure runs again.
+    """
+    from .dense import status
+    t = _lib.torch()
+    n = A.n_rows
+    vdt = A.data.dtype
+    if width is None:
+        width = 32 if vdt == t.int32 else 64
+    if n == 0:
+        return A, DagInfo(0, 0, 0, -1, 0, 0)
+    st = status()
+    sh = _lib.stream_handle(stream)
+    if plan is None or plan.n != n or plan.vdt != vdt:
+        arena = int(max(4096, arena_factor * (A.nnz + n)))
+        plan = DagPlan(n, vdt, arena, arena)
+    sfx = "i32" if vdt == t.int32 else "i64"
+    for _attempt in range(8):
+        st.reset(stream)
+        _lib.call(f"rbg_dag_closure_{sfx}", _lib.ptr(A.indptr), _lib.ptr(A.indices), _lib.ptr(A.data), n,
+                  _lib.ptr(plan.ws), plan.ws.numel(), _lib.ptr(plan.arena_cols), _lib.ptr(plan.arena_vals),
+                  plan.arena_cols.numel(), _lib.ptr(plan.cand), plan.cand_cap, batch, width, st.p, sh)
+        s = st.read(stream)
+        if s.flags & _lib.F_CYCLE:
+            return None, DagInfo(0, 0, 0, int(s.aux[0]), int(s.aux[6]), 0)
+        if s.flags & _lib.F_CAPACITY:
+            plan = DagPlan(n, vdt, 2 * plan.arena_cols.numel(), 2 * plan.cand_cap)
+            continue
+        info = DagInfo(0, int(s.aux[2]), int(s.aux[3]), -1, int(s.aux[6]), int(s.events))
+        optr = t.empty(n + 1, dtype=t.int64, device=A.indices.device)
+        _lib.call(f"rbg_dag_gather_{sfx}", _lib.ptr(plan.ws), n, _lib.ptr(plan.arena_cols),
+                  _lib.ptr(plan.arena_vals), _lib.ptr(optr), None, None, st.p, sh)
+        nnz = int(optr[-1].item())
+        oi = t.empty(nnz, dtype=t.int32, device=A.indices.device)
+        ov = t.empty(nnz, dtype=vdt, device=A.indices.device)
+        _lib.call(f"rbg_dag_gather_{sfx}", _lib.ptr(plan.ws), n, _lib.ptr(plan.arena_cols),
+                  _lib.ptr(plan.arena_vals), _lib.ptr(optr), _lib.ptr(oi), _lib.ptr(ov), st.p, sh)
+        info.nnz = nnz
+        return DevCSR(optr, oi, ov, A.shape), info
+    raise MemoryError("DAG closure buffers could not be sized")
+
+
+def spgemm(A: DevCSR, B: DevCSR, mask_upper: bool = False, width: int | None = None, is_unsigned: bool = False,
+           stream=None, arena_factor: float = 4.0):
+    """C = A (x) B (AVOS semiring) for canonical device CSR operands.  Returns (C, events)."""
+    from .dense import status
+    t = _lib.torch()
+    m = A.n_rows
+    if A.shape[1] != B.shape[0]:
+        raise ValueError(f"Incompatible shapes: A is {A.shape}, B is {B.shape}. A.shape[1] must equal B.shape[0].")
+    vdt = A.data.dtype
+    if B.data.dtype != vdt:
+        B = DevCSR(B.indptr, B.indices, B.data.to(vdt), B.shape)
+    if width is None:
+        width = 32 if vdt == t.int32 else 64
+    dev = A.indices.device
+    if m == 0 or A.nnz == 0 or B.nnz == 0:
+        return DevCSR(t.zeros(m + 1, dtype=t.int64, device=dev), t.empty(0, dtype=t.int32, device=dev),
+                      t.empty(0, dtype=vdt, device=dev), (m, B.shape[1])), 0
+    # the row bounds sum to the expansion size: sum over A's entries of |B row|
+    blen = (B.indptr[1:] - B.indptr[:-1])
+    expand = int(blen[A.indices.long()].sum().item())
+    st = status()
+    sh = _lib.stream_handle(stream)
+    sfx = "i32" if vdt == t.int32 else "i64"
+    ws = t.empty(int(_lib.load().rbg_dag_workspace_bytes(m)), dtype=t.uint8, device=dev)
+    cap = max(1024, expand)
+    tcap = max(1024, 3 * expand)
+    ac = t.empty(cap, dtype=t.int32, device=dev)
+    av = t.empty(cap, dtype=vdt, device=dev)
+    tc = t.empty(tcap, dtype=t.int32, device=dev)
+    tv = t.empty(tcap, dtype=vdt, device=dev)
+    st.reset(stream)
+    _lib.call(f"rbg_spgemm_rows_{sfx}", _lib.ptr(A.indptr), _lib.ptr(A.indices), _lib.ptr(A.data), m,
+              _lib.ptr(B.indptr), _lib.ptr(B.indices), _lib.ptr(B.data), int(mask_upper), _lib.ptr(ws),
+              _lib.ptr(ac), _lib.ptr(av), cap, _lib.ptr(tc), _lib.ptr(tv), tcap, width, int(is_unsigned), st.p, sh)
+    optr = t.empty(m + 1, dtype=t.int64, device=dev)
+    _lib.call(f"rbg_dag_gather_{sfx}", _lib.ptr(ws), m, _lib.ptr(ac), _lib.ptr(av), _lib.ptr(optr), None, None, st.p, sh)
+    s = st.read(stream)
+    if s.flags & _lib.F_CAPACITY:
+        raise MemoryError("SpGEMM arena too small")  # bounds are exact; not expected
+    nnz = int(s.aux[1])
+    oi = t.empty(nnz, dtype=t.int32, device=dev)
+    ov = t.empty(nnz, dtype=vdt, device=dev)
+    _lib.call(f"rbg_dag_gather_{sfx}", _lib.ptr(ws), m, _lib.ptr(ac), _lib.ptr(av), _lib.ptr(optr), _lib.ptr(oi),
+              _lib.ptr(ov), st.p, sh)
+    s = st.read(stream)
+    return DevCSR(optr, oi, ov, (m, B.shape[1])), int(s.events)
+
+
+def avos_sum_csr(A: DevCSR, B: DevCSR, min_abs: bool = False, stream=None) -> DevCSR:
+    """Element-wise sum of two canonical device CSR matrices of one shape.
+
+    Default: the AVOS sum (nonzero min, RED_ONE absorbing).  min_abs: keep the
+    entry of smallest |value| (first operand on ties), the merge rule of the
+    reference's squaring closures (gpu/transitive_closure.py:225-257,
+    sparse/csgraph/transitive_closure.py:331-344).  Zero results are dropped."""
+    t = _lib.torch()
+    if A.shape != B.shape:
+        raise ValueError(f"Shape mismatch: {A.shape} vs {B.shape}")
+    n = A.n_rows
+    dev = A.indices.device
+    vdt = A.data.dtype
+    if n == 0:
+        return A
+    bdata = B.data.to(vdt)
+    tmp = t.empty(int(_lib.load().rbg_csr_tmp_bytes(n)), dtype=t.uint8, device=dev)
+    cat_ptr = A.indptr + B.indptr
+    cat_idx = t.empty(A.nnz + B.nnz, dtype=t.int32, device=dev)
+    cat_val = t.empty(A.nnz + B.nnz, dtype=vdt, device=dev)
+    optr = t.empty(n + 1, dtype=t.int64, device=dev)
+    fn = "rbg_csr_add_i32" if vdt == t.int32 else "rbg_csr_add_i64"
+    sh = _lib.stream_handle(stream)
+    args = [_lib.ptr(A.indptr), _lib.ptr(A.indices), _lib.ptr(A.data), _lib.ptr(B.indptr), _lib.ptr(B.indices),
+            _lib.ptr(bdata), n, int(min_abs), _lib.ptr(cat_ptr), _lib.ptr(cat_idx), _lib.ptr(cat_val), _lib.ptr(tmp),
+            _lib.ptr(optr)]
+    _lib.call(fn, *args, None, None, sh)
+    nnz = int(optr[-1].item())
+    oi = t.empty(nnz, dtype=t.int32, device=dev)
+    ov = t.empty(nnz, dtype=vdt, device=dev)
+    _lib.call(fn, *args, _lib.ptr(oi), _lib.ptr(ov), sh)
+    return DevCSR(optr, oi, ov, A.shape)
+
+
+def csr_equal(A: DevCSR, B: DevCSR) -> bool:
+    """sparse_equal_gpu (gpu/transitive_closure.py:159-184)."""
+    t = _lib.torch()
+    if A.shape != B.shape or A.nnz != B.nnz:
+        return False
+    if A.nnz == 0:
+        return True
+    return bool(t.equal(A.indptr, B.indptr)) and bool(t.equal(A.indices, B.indices)) and \
+        bool(t.equal(A.data, B.data.to(A.data.dtype)))
