@@ -179,7 +179,7 @@ def bench_sparse(args):
     L = _lib.load()
     for _ in range(max(1, args.warmup)):
         C, info = sd.dag_closure(sd.canonicalize(Ad), plan=plan)
-    plan = sd.DagPlan(A.shape[0], t.int64, int(info.nnz * 1.25) + 4096, int(info.nnz) // 2 + 4096)
+    plan = sd.DagPlan(A.shape[0], t.int64, info.arena + 4096)
     C, info = sd.dag_closure(sd.canonicalize(Ad), plan=plan)
     t.cuda.synchronize()
     L.rbg_prof_enable(1)
@@ -201,7 +201,7 @@ def bench_sparse(args):
     kernel_s = ms_.value / 1e3 / args.steps
     achieved = bytes_alg / kernel_s
     roof = {
-        "bound": "hbm", "kernel": "k_dag_level<long>",
+        "bound": "hbm", "kernel": "k_lvl_merge<long>",
         "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s", "frac": achieved / PEAK_HBM,
         "traffic": _read_traffic("dag_level_sparse1m_pmc.txt"),
         "algorithmic_bytes_per_closure": bytes_alg,

@@ -161,26 +161,25 @@ int rbg_csr_validate(const int64_t *indptr, const int32_t *idx, int32_t n, int32
 /* ---- sparse DAG closure (_transitive_closure_dag.pyx:168-351,
  *      gpu/transitive_closure.py:474-652) --------------------------------------- */
 int64_t rbg_dag_workspace_bytes(int32_t n);
-/* Bytes of the per-level candidate scratch for `cap` candidate entries. */
-int64_t rbg_dag_cand_bytes(int64_t cap, int32_t elem_bytes);
 /* Closure of a DAG (canonical CSR: int64 row pointers, sorted rows, no explicit
  * zeros).  Rounds r = 0, 1, ...: the vertices whose non-self successors all sit
- * in levels < r form level r, and their closure rows are built by an
- * element-parallel merge of their direct entries with A[v][w] (x) C[w] for each
- * successor w.  All sizes live on the device; the host synchronises once every
- * `batch` rounds to test completion.  Results: closure rows in the arena (see
- * rbg_dag_gather_*); st->aux[2] max |candidate| (Cython diameter rule), [3] max
- * |final value| > 1 (GPU diameter rule), [4] arena entries used, [6] levels.
- * A cycle sets RBG_F_CYCLE with st->aux[0] = a vertex on the cycle; a full
- * arena or candidate scratch sets RBG_F_CAPACITY (grow it and call again). */
+ * in levels < r form level r, and each of their closure rows is merged by one
+ * wavefront from its direct entries and A[v][w] (x) C[w] for each successor w.
+ * Every level reserves its candidate count in the arena (rows are compacted
+ * into the front of their slot), so the arena needs the total candidate count
+ * sum_v (nnz(A[v]) + sum_w nnz(C[w])), not nnz(C).  All sizes live on the
+ * device; the host synchronises once every `batch` rounds to test completion.
+ * Results: closure rows in the arena (see rbg_dag_gather_*); st->aux[2] max
+ * |candidate| (Cython diameter rule), [3] max |final value| > 1 (GPU diameter
+ * rule), [4] arena entries used, [6] levels.  A cycle sets RBG_F_CYCLE with
+ * st->aux[0] = a vertex on the cycle; a full arena sets RBG_F_CAPACITY (grow
+ * it and call again). */
 int rbg_dag_closure_i32(const int64_t *indptr, const int32_t *indices, const int32_t *data, int32_t n,
                         void *workspace, int64_t ws_bytes, int32_t *arena_cols, int32_t *arena_vals,
-                        int64_t arena_cap, void *cand, int64_t cand_cap, int32_t batch, int32_t width,
-                        rbg_status *st, rbg_stream_t stream);
+                        int64_t arena_cap, int32_t batch, int32_t width, rbg_status *st, rbg_stream_t stream);
 int rbg_dag_closure_i64(const int64_t *indptr, const int32_t *indices, const int64_t *data, int32_t n,
                         void *workspace, int64_t ws_bytes, int32_t *arena_cols, int64_t *arena_vals,
-                        int64_t arena_cap, void *cand, int64_t cand_cap, int32_t batch, int32_t width,
-                        rbg_status *st, rbg_stream_t stream);
+                        int64_t arena_cap, int32_t batch, int32_t width, rbg_status *st, rbg_stream_t stream);
 /* Gather arena rows into CSR: out_indptr (n+1, int64) always; indices/data when
  * non-null.  st->aux: [1] nnz, [2] max |candidate|, [3] max |final value| > 1,
  * [4] arena entries used, [5] temp entries used. */

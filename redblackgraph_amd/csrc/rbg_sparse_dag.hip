@@ -196,7 +196,7 @@ enum {
   C_LSTART = 3,   // first index in `order` of the current level
   C_LSIZE = 4,    // vertices in the current level
   C_M = 5,        // candidate entries of the current level
-  C_KEPT = 6,     // closure entries of the current level
+  C_UNUSED6 = 6,
   C_ARENA = 7,    // arena entries used
   C_CMAX = 8,     // max |candidate|
   C_FMAX = 9,     // max |final value| > 1
@@ -244,34 +244,6 @@ struct DagWork {
     w.sums_a = (int64_t*)take(8 * (n / SCAN_CHUNK + 8));
     w.counters = (int64_t*)take(8 * C_NCTR);
     return w;
-  }
-};
-
-// per-level candidate scratch (capacity cap entries)
-template <typename T>
-struct DagCand {
-  int32_t* col;    // [cap] merged columns (duplicates adjacent)
-  T* val;          // [cap] combined values at run heads
-  int64_t* keep;   // [cap+1] 1 = closure entry
-  int64_t* kloc;   // [cap+1] block-local scan of keep
-  int64_t* sums;   // scan block sums
-  int64_t cap;
-  static int64_t bytes(int64_t cap) { return cap * (4 + (int64_t)sizeof(T) + 16) + 16 + 8 * (cap / SCAN_CHUNK + 8) + 8 * 256; }
-  static DagCand carve(void* base, int64_t cap) {
-    DagCand c;
-    char* p = (char*)(((uintptr_t)base + 255) & ~(uintptr_t)255);
-    auto take = [&](int64_t bytes) {
-      char* r = p;
-      p += (bytes + 255) & ~(int64_t)255;
-      return r;
-    };
-    c.col = (int32_t*)take(4 * cap);
-    c.val = (T*)take((int64_t)sizeof(T) * cap);
-    c.keep = (int64_t*)take(8 * (cap + 1));
-    c.kloc = (int64_t*)take(8 * (cap + 1));
-    c.sums = (int64_t*)take(8 * (cap / SCAN_CHUNK + 8));
-    c.cap = cap;
-    return c;
   }
 };
 
@@ -341,12 +313,13 @@ __global__ __launch_bounds__(256) void k_round_place(const int64_t* __restrict__
   }
 }
 
-// Round r, step 2: candidate count of every vertex of the level: its direct
-// entries plus the closure rows of its non-self successors.
+// Round r, step 2: level size, and the candidate count of every vertex of the
+// level: its direct entries plus the closure rows of its non-self successors.
 __global__ void k_lvl_count(const int64_t* __restrict__ indptr, const int32_t* __restrict__ indices, DagWork ws) {
   int64_t* ctr = ws.counters;
   if (dag_halted(ctr)) return;
-  const int64_t L0 = ctr[C_LSTART], Ls = ctr[C_LSIZE];
+  const int64_t L0 = ctr[C_LSTART], Ls = ctr[C_PLACED] - L0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctr[C_LSIZE] = Ls;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t <= Ls; t += (int64_t)gridDim.x * blockDim.x) {
     if (t == Ls) {
       ws.cnt[t] = 0;
@@ -360,19 +333,6 @@ __global__ void k_lvl_count(const int64_t* __restrict__ indptr, const int32_t* _
     }
     ws.cnt[t] = c;
   }
-}
-
-__global__ void k_lvl_size(DagWork ws) {
-  int64_t* ctr = ws.counters;
-  if (dag_halted(ctr)) return;
-  ctr[C_LSIZE] = ctr[C_PLACED] - ctr[C_LSTART];
-}
-
-__global__ void k_lvl_fix_count(DagWork ws, int64_t cand_cap) {
-  // after the count scan (whose total landed in C_M): level size and capacity
-  int64_t* ctr = ws.counters;
-  if (dag_halted(ctr)) return;
-  if (ctr[C_M] > cand_cap) ctr[C_CAP] = 1;
 }
 
 // upper bound (first index with value > x) / lower bound over a sorted column list
@@ -402,162 +362,262 @@ struct DagArena {
   int64_t cap;
 };
 
-// Round r, step 3: one lane per candidate entry.  The candidate lists of a
-// vertex v are list 0 = its direct entries and list i = A[v][w_i] (x) C[w_i]
-// (each sorted by column, no repeats).  An entry's place in the merged order
-// is its index in its own list plus, for every other list, the number of
-// entries that come before it (ties go to the lower list); an entry whose
-// column also appears in a lower list is a follower and is dropped; a run
-// head takes the AVOS sum of every list's value for its column.
+constexpr int LV_WAVES = 4;                    // waves per workgroup in the level kernel
+constexpr int LV_NT = 64 * LV_WAVES;
+constexpr int LV_SMALL = 512;                  // candidates a wave merges in its LDS share
+constexpr int LV_BIG = LV_WAVES * LV_SMALL;    // candidates the workgroup merges in LDS
+
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// One closure row.  Candidate lists of v: list 0 = its direct entries, list k
+// = A[v][w_k] (x) C[w_k] for its k-th non-self successor w_k, each sorted by
+// column without repeats.  `lcol`, when non-null, holds the lists' columns
+// concatenated (staged in LDS); otherwise they are read from the arena.
 template <typename T>
-__global__ __launch_bounds__(256) void k_lvl_place(const int64_t* __restrict__ indptr,
-                                                   const int32_t* __restrict__ indices, const T* __restrict__ data,
-                                                   DagWork ws, DagArena<T> ar, DagCand<T> cd, int width,
-                                                   rbg_status* st) {
-  __shared__ int64_t s_cmax;
-  int64_t* ctr = ws.counters;
-  if (dag_halted(ctr)) return;
-  const int64_t L0 = ctr[C_LSTART], Ls = ctr[C_LSIZE], M = ctr[C_M];
-  if (threadIdx.x == 0) s_cmax = 0;
-  __syncthreads();
-  T cmax = 0;
-  uint32_t evs = 0;
-  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < M; idx += (int64_t)gridDim.x * blockDim.x) {
-    // vertex of this candidate: last t with cand_off[t] <= idx
-    int64_t lo = 0, hi = Ls;
-    while (hi - lo > 1) {
-      int64_t mid = (lo + hi) >> 1;
-      if (scan_at(ws.cand_loc, ws.sums_a, mid) <= idx) lo = mid;
-      else hi = mid;
-    }
-    const int64_t t = lo;
-    const int64_t seg = scan_at(ws.cand_loc, ws.sums_a, t);
-    const int32_t v = ws.order[L0 + t];
-    const int64_t e0 = indptr[v], e1 = indptr[v + 1];
-    int64_t local = idx - seg;
-    // locate own list: list index li (0 = direct), position p, column c, value x
+struct RowJob {
+  int32_t v;
+  int64_t e0, e1, nd, cnt, dst;
+};
+
+template <typename T>
+__device__ inline void stage_lists(const RowJob<T>& J, const int32_t* __restrict__ indices, const DagWork& ws,
+                                   const DagArena<T>& ar, int32_t* lcol, int lane, int nl) {
+  for (int64_t q = lane; q < J.nd; q += nl) lcol[q] = indices[J.e0 + q];
+  int64_t off = J.nd;
+  for (int64_t e = J.e0; e < J.e1; e++) {
+    const int32_t w = indices[e];
+    if (w == J.v) continue;
+    const int64_t lw = ws.row_len[w], ro = ws.row_off[w];
+    for (int64_t q = lane; q < lw; q += nl) lcol[off + q] = ar.cols[ro + q];
+    off += lw;
+  }
+}
+
+// Place every candidate: its position in the merged order is its index in its
+// own list plus, for every other list, the number of entries that come before
+// it (ties go to the lower list).  A candidate whose column appears in a lower
+// list is a follower (written as column -1); a run head takes the AVOS sum of
+// the values every list holds for its column, and a zero sum is dropped too.
+template <typename T>
+__device__ inline void merge_lists(const RowJob<T>& J, const int32_t* __restrict__ indices,
+                                   const T* __restrict__ data, const DagWork& ws, const DagArena<T>& ar,
+                                   const int32_t* lcol, int32_t* mcol, T* mval, int lane, int nl, int width,
+                                   T& cmax, uint32_t& evs) {
+  const int32_t* l0 = lcol ? lcol : indices + J.e0;
+  for (int64_t idx = lane; idx < J.cnt; idx += nl) {
     int li = 0;
-    int64_t p = 0;
+    int64_t p = idx;
     int32_t c = 0;
     T x = 0;
-    const int64_t nd = e1 - e0;
-    if (local < nd) {
-      li = 0;
-      p = local;
-      c = indices[e0 + p];
-      x = data[e0 + p];
+    if (idx < J.nd) {
+      c = l0[idx];
+      x = data[J.e0 + idx];
     } else {
-      local -= nd;
+      int64_t local = idx - J.nd, off = J.nd;
       int k = 0;
-      for (int64_t e = e0; e < e1; e++) {
+      for (int64_t e = J.e0; e < J.e1; e++) {
         const int32_t w = indices[e];
-        if (w == v) continue;
+        if (w == J.v) continue;
         k++;
         const int64_t lw = ws.row_len[w];
         if (local < lw) {
+          const int64_t o = ws.row_off[w] + local;
           li = k;
           p = local;
-          const int64_t o = ws.row_off[w] + p;
-          c = ar.cols[o];
+          c = lcol ? lcol[off + local] : ar.cols[o];
           uint32_t ev = 0;
           x = avos_product<T>(data[e], ar.vals[o], width, false, ev);
           evs |= ev;
           break;
         }
         local -= lw;
+        off += lw;
       }
     }
     {
-      T a = x < 0 ? (T)-x : x;
+      const T a = x < 0 ? (T)-x : x;
       cmax = a > cmax ? a : cmax;
     }
-    // rank against every other list
     int64_t pos = p;
     bool head = true;
     T comb = x;
-    {
-      // list 0 (direct entries)
-      if (li != 0) {
-        const int32_t* dc = indices + e0;
-        int64_t r0 = ub_col(dc, nd, c);  // list 0 is lower: ties precede
-        pos += r0;
-        if (r0 > 0 && dc[r0 - 1] == c) head = false;
-      }
-      int k = 0;
-      for (int64_t e = e0; e < e1; e++) {
-        const int32_t w = indices[e];
-        if (w == v) continue;
-        k++;
-        if (k == li) continue;
-        const int64_t lw = ws.row_len[w];
-        const int32_t* wc = ar.cols + ws.row_off[w];
+    if (li != 0) {
+      const int64_t r0 = ub_col(l0, J.nd, c);
+      pos += r0;
+      if (r0 > 0 && l0[r0 - 1] == c) head = false;
+    }
+    int64_t off = J.nd;
+    int k = 0;
+    for (int64_t e = J.e0; e < J.e1; e++) {
+      const int32_t w = indices[e];
+      if (w == J.v) continue;
+      k++;
+      const int64_t lw = ws.row_len[w];
+      if (k != li) {
+        const int32_t* wc = lcol ? lcol + off : ar.cols + ws.row_off[w];
         if (k < li) {
-          int64_t rk = ub_col(wc, lw, c);
+          const int64_t rk = ub_col(wc, lw, c);
           pos += rk;
           if (rk > 0 && wc[rk - 1] == c) head = false;
         } else {
-          int64_t rk = lb_col(wc, lw, c);
+          const int64_t rk = lb_col(wc, lw, c);
           pos += rk;
           if (head && rk < lw && wc[rk] == c) {
             uint32_t ev = 0;
-            T y = avos_product<T>(data[e], ar.vals[ws.row_off[w] + rk], width, false, ev);
+            const T y = avos_product<T>(data[e], ar.vals[ws.row_off[w] + rk], width, false, ev);
             evs |= ev;
             comb = avos_sum<T>(comb, y);
           }
         }
       }
-      if (head && li == 0) {
-        // direct entries take later lists' values for the same column
+      off += lw;
+    }
+    mcol[pos] = (head && comb != 0) ? c : -1;
+    mval[pos] = comb;
+  }
+}
+
+// Round r, step 3: closure rows of the level.  Each workgroup takes the level's
+// vertices four at a time: a vertex with at most LV_SMALL candidates is merged
+// by one wave in that wave's LDS share; a longer one by the whole workgroup,
+// in LDS up to LV_BIG candidates, else in its arena slot.  The compaction
+// writes the row to the front of the slot [abase + seg, abase + seg + cnt)
+// that the count scan reserved for the vertex; when the row was merged in the
+// slot itself every store of a pass lands at or below the entries the pass
+// has already loaded.
+template <typename T>
+__global__ __launch_bounds__(LV_NT) void k_lvl_merge(const int64_t* __restrict__ indptr,
+                                                     const int32_t* __restrict__ indices,
+                                                     const T* __restrict__ data, DagWork ws, DagArena<T> ar,
+                                                     int width, rbg_status* st) {
+  __shared__ int32_t s_lcol[LV_BIG];
+  __shared__ int32_t s_mcol[LV_BIG];
+  __shared__ T s_mval[LV_BIG];
+  __shared__ int64_t s_seg[LV_WAVES + 1];
+  __shared__ int s_wsum[LV_WAVES];
+  __shared__ int64_t s_cmax, s_fmax;
+  int64_t* ctr = ws.counters;
+  if (dag_halted(ctr)) return;
+  const int64_t L0 = ctr[C_LSTART], Ls = ctr[C_LSIZE], M = ctr[C_M], abase = ctr[C_ARENA];
+  if (abase + M > ar.cap) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctr[C_CAP] = 1;
+    return;
+  }
+  const int wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  if (threadIdx.x == 0) s_cmax = s_fmax = 0;
+  T cmax = 0, fmax = 0;
+  uint32_t evs = 0;
+  for (int64_t t0 = (int64_t)blockIdx.x * LV_WAVES; t0 < Ls; t0 += (int64_t)gridDim.x * LV_WAVES) {
+    if (threadIdx.x <= LV_WAVES) {
+      const int64_t t = t0 + threadIdx.x < Ls ? t0 + threadIdx.x : Ls;
+      s_seg[threadIdx.x] = scan_at(ws.cand_loc, ws.sums_a, t);
+    }
+    __syncthreads();
+    // vertices merged by one wave
+    {
+      const int64_t t = t0 + wv;
+      const int64_t cnt = s_seg[wv + 1] - s_seg[wv];
+      if (t < Ls && cnt <= LV_SMALL) {
+        RowJob<T> J;
+        J.v = ws.order[L0 + t];
+        J.e0 = indptr[J.v];
+        J.e1 = indptr[J.v + 1];
+        J.nd = J.e1 - J.e0;
+        J.cnt = cnt;
+        J.dst = abase + s_seg[wv];
+        int32_t* lcol = s_lcol + wv * LV_SMALL;
+        int32_t* mcol = s_mcol + wv * LV_SMALL;
+        T* mval = s_mval + wv * LV_SMALL;
+        stage_lists<T>(J, indices, ws, ar, lcol, lane, 64);
+        wave_sync();
+        merge_lists<T>(J, indices, data, ws, ar, lcol, mcol, mval, lane, 64, width, cmax, evs);
+        wave_sync();
+        int64_t out = 0;
+        for (int64_t b0 = 0; b0 < cnt; b0 += 64) {
+          const int64_t i = b0 + lane;
+          const int32_t c = i < cnt ? mcol[i] : -1;
+          const T val = i < cnt ? mval[i] : (T)0;
+          const uint64_t msk = __ballot(c >= 0);
+          if (c >= 0) {
+            const int64_t o = J.dst + out + __popcll(msk & ((1ull << lane) - 1ull));
+            ar.cols[o] = c;
+            ar.vals[o] = val;
+            const T a = val < 0 ? (T)-val : val;
+            if (a > 1 && a > fmax) fmax = a;
+          }
+          out += __popcll(msk);
+        }
+        if (lane == 0) {
+          ws.row_off[J.v] = J.dst;
+          ws.row_len[J.v] = out;
+        }
       }
     }
-    const int64_t mp = seg + pos;
-    cd.col[mp] = c;
-    cd.val[mp] = comb;
-    cd.keep[mp] = (head && comb != 0) ? 1 : 0;
+    __syncthreads();
+    // vertices merged by the workgroup
+    for (int j = 0; j < LV_WAVES; j++) {
+      const int64_t t = t0 + j;
+      const int64_t cnt = s_seg[j + 1] - s_seg[j];
+      if (t >= Ls || cnt <= LV_SMALL) continue;
+      RowJob<T> J;
+      J.v = ws.order[L0 + t];
+      J.e0 = indptr[J.v];
+      J.e1 = indptr[J.v + 1];
+      J.nd = J.e1 - J.e0;
+      J.cnt = cnt;
+      J.dst = abase + s_seg[j];
+      const bool in_lds = cnt <= LV_BIG;
+      int32_t* mcol = in_lds ? s_mcol : ar.cols + J.dst;
+      T* mval = in_lds ? s_mval : ar.vals + J.dst;
+      if (in_lds) {
+        stage_lists<T>(J, indices, ws, ar, s_lcol, threadIdx.x, LV_NT);
+        __syncthreads();
+      }
+      merge_lists<T>(J, indices, data, ws, ar, in_lds ? s_lcol : nullptr, mcol, mval, threadIdx.x, LV_NT, width,
+                     cmax, evs);
+      __syncthreads();
+      int64_t out = 0;
+      for (int64_t b0 = 0; b0 < cnt; b0 += LV_NT) {
+        const int64_t i = b0 + threadIdx.x;
+        const int32_t c = i < cnt ? mcol[i] : -1;
+        const T val = i < cnt ? mval[i] : (T)0;
+        const uint64_t msk = __ballot(c >= 0);
+        if (lane == 0) s_wsum[wv] = __popcll(msk);
+        __syncthreads();
+        int64_t before = 0, total = 0;
+        for (int q = 0; q < LV_WAVES; q++) {
+          before += q < wv ? s_wsum[q] : 0;
+          total += s_wsum[q];
+        }
+        if (c >= 0) {
+          const int64_t o = J.dst + out + before + __popcll(msk & ((1ull << lane) - 1ull));
+          ar.cols[o] = c;
+          ar.vals[o] = val;
+          const T a = val < 0 ? (T)-val : val;
+          if (a > 1 && a > fmax) fmax = a;
+        }
+        out += total;
+        __syncthreads();
+      }
+      if (threadIdx.x == 0) {
+        ws.row_off[J.v] = J.dst;
+        ws.row_len[J.v] = out;
+      }
+    }
+    __syncthreads();  // s_seg is rewritten by the next chunk
   }
   if (cmax) atomicMax((long long*)&s_cmax, (long long)cmax);
-  __syncthreads();
-  if (threadIdx.x == 0 && s_cmax) atomicMax((long long*)&ctr[C_CMAX], (long long)s_cmax);
-  if (evs) atomicOr(&st->events, evs);
-  if (blockIdx.x == 0 && threadIdx.x == 0) cd.keep[M] = 0;
-}
-
-__global__ void k_lvl_check_arena(DagWork ws, int64_t arena_cap) {
-  int64_t* ctr = ws.counters;
-  if (dag_halted(ctr)) return;
-  if (ctr[C_ARENA] + ctr[C_KEPT] > arena_cap) ctr[C_CAP] = 1;
-}
-
-// Round r, step 4: kept entries to the arena; row offsets and lengths.
-template <typename T>
-__global__ __launch_bounds__(256) void k_lvl_scatter(DagWork ws, DagArena<T> ar, DagCand<T> cd) {
-  __shared__ int64_t s_fmax;
-  int64_t* ctr = ws.counters;
-  if (dag_halted(ctr)) return;
-  const int64_t L0 = ctr[C_LSTART], Ls = ctr[C_LSIZE], M = ctr[C_M], base = ctr[C_ARENA];
-  if (threadIdx.x == 0) s_fmax = 0;
-  __syncthreads();
-  T fmax = 0;
-  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < M; idx += (int64_t)gridDim.x * blockDim.x) {
-    if (cd.keep[idx]) {
-      const int64_t o = base + scan_at(cd.kloc, cd.sums, idx);
-      T v = cd.val[idx];
-      ar.cols[o] = cd.col[idx];
-      ar.vals[o] = v;
-      T a = v < 0 ? (T)-v : v;
-      if (a > 1 && a > fmax) fmax = a;
-    }
-  }
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < Ls; t += (int64_t)gridDim.x * blockDim.x) {
-    const int32_t v = ws.order[L0 + t];
-    const int64_t s0 = scan_at(ws.cand_loc, ws.sums_a, t), s1 = scan_at(ws.cand_loc, ws.sums_a, t + 1);
-    const int64_t k0 = scan_at(cd.kloc, cd.sums, s0), k1 = scan_at(cd.kloc, cd.sums, s1);
-    ws.row_off[v] = base + k0;
-    ws.row_len[v] = k1 - k0;
-  }
   if (fmax) atomicMax((long long*)&s_fmax, (long long)fmax);
   __syncthreads();
-  if (threadIdx.x == 0 && s_fmax) atomicMax((long long*)&ctr[C_FMAX], (long long)s_fmax);
+  if (threadIdx.x == 0) {
+    if (s_cmax) atomicMax((long long*)&ctr[C_CMAX], (long long)s_cmax);
+    if (s_fmax) atomicMax((long long*)&ctr[C_FMAX], (long long)s_fmax);
+  }
+  if (evs) atomicOr(&st->events, evs);
 }
 
 // Round r, step 5 (single lane): advance the arena, count the level, detect a
@@ -568,7 +628,7 @@ __global__ void k_round_end(DagWork ws, int32_t r) {
   if (ctr[C_STUCK] | ctr[C_CAP] | ctr[C_DONE]) return;
   const int64_t placed_now = ctr[C_PLACED] - ctr[C_LSTART];
   const int64_t m_next = ctr[(r & 1) ? C_M0 : C_M1];
-  ctr[C_ARENA] += ctr[C_KEPT];
+  ctr[C_ARENA] += ctr[C_M];  // the level's reserved slots
   if (placed_now > 0) ctr[C_NLEV] = r + 1;
   if (m_next == 0) {
     ctr[C_DONE] = 1;
@@ -628,10 +688,6 @@ __device__ inline int wave_lane() { return threadIdx.x & 63; }
 
 // Make this wave's LDS / global writes visible to its other lanes before they
 // read them (the merge buffers may live in either address space).
-__device__ inline void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
-  __builtin_amdgcn_wave_barrier();
-}
 
 template <typename V>
 __device__ inline V wave_bcast(V v, int src) {
@@ -977,48 +1033,34 @@ using namespace rbg;
 extern "C" {
 
 int64_t rbg_dag_workspace_bytes(int32_t n) { return DagWork::bytes(n > 0 ? n : 1); }
-int64_t rbg_dag_cand_bytes(int64_t cap, int32_t elem_bytes) {
-  return elem_bytes == 4 ? DagCand<int32_t>::bytes(cap) : DagCand<int64_t>::bytes(cap);
-}
 
 #define RBG_DAG_ENTRY(SUF, T)                                                                                   \
   int rbg_dag_closure_##SUF(const int64_t* indptr, const int32_t* indices, const T* data, int32_t n,            \
                             void* workspace, int64_t ws_bytes, int32_t* arena_cols, T* arena_vals,              \
-                            int64_t arena_cap, void* cand, int64_t cand_cap, int32_t batch, int32_t width,      \
-                            rbg_status* st, rbg_stream_t stream) {                                              \
+                            int64_t arena_cap, int32_t batch, int32_t width, rbg_status* st,                    \
+                            rbg_stream_t stream) {                                                              \
     hipStream_t s = (hipStream_t)stream;                                                                        \
-    if (n <= 0 || !workspace || !arena_cols || !arena_vals || !cand || cand_cap < 1) return RBG_E_ARG;          \
+    if (n <= 0 || !workspace || !arena_cols || !arena_vals || arena_cap < 0) return RBG_E_ARG;                  \
     if (ws_bytes < DagWork::bytes(n)) return RBG_E_WORKSPACE;                                                   \
     if (batch < 1) batch = 8;                                                                                   \
     DagWork ws = DagWork::carve(workspace, n);                                                                  \
-    DagCand<T> cd = DagCand<T>::carve(cand, cand_cap);                                                          \
     DagArena<T> ar{arena_cols, arena_vals, arena_cap};                                                          \
     hipLaunchKernelGGL(k_dag_init, dim3((n + 255) / 256), dim3(256), 0, s, ws.level, ws.rem[0], ws.row_len,     \
                        ws.counters, n);                                                                         \
     RBG_CHECK_LAUNCH();                                                                                         \
     const int gv = (int)((n + 255) / 256 < 2048 ? (n + 255) / 256 : 2048);                                      \
-    const int gc = 4096;                                                                                        \
+    const int gw = (int)((n + LV_WAVES - 1) / LV_WAVES < 4096 ? (n + LV_WAVES - 1) / LV_WAVES : 4096);          \
     int64_t host[C_NCTR];                                                                                       \
     for (int32_t r = 0; r <= n; r++) {                                                                          \
       hipLaunchKernelGGL(k_round_place, dim3(gv), dim3(256), 0, s, indptr, indices, ws, r);                     \
-      RBG_CHECK_LAUNCH();                                                                                       \
-      hipLaunchKernelGGL(k_lvl_size, dim3(1), dim3(1), 0, s, ws);                                               \
       RBG_CHECK_LAUNCH();                                                                                       \
       hipLaunchKernelGGL(k_lvl_count, dim3(gv), dim3(256), 0, s, indptr, indices, ws);                          \
       RBG_CHECK_LAUNCH();                                                                                       \
       int rc = scan_dev(ws.cnt, ws.cand_loc, ws.sums_a, ws.counters + C_LSIZE, 1, n, ws.counters + C_M, s);    \
       if (rc) return rc;                                                                                        \
-      hipLaunchKernelGGL(k_lvl_fix_count, dim3(1), dim3(1), 0, s, ws, cand_cap);                                \
-      RBG_CHECK_LAUNCH();                                                                                       \
       int pi = prof_start(s);                                                                                   \
-      hipLaunchKernelGGL((k_lvl_place<T>), dim3(gc), dim3(256), 0, s, indptr, indices, data, ws, ar, cd, width, \
-                         st);                                                                                   \
-      RBG_CHECK_LAUNCH();                                                                                       \
-      rc = scan_dev(cd.keep, cd.kloc, cd.sums, ws.counters + C_M, 1, cand_cap, ws.counters + C_KEPT, s);        \
-      if (rc) return rc;                                                                                        \
-      hipLaunchKernelGGL(k_lvl_check_arena, dim3(1), dim3(1), 0, s, ws, arena_cap);                             \
-      RBG_CHECK_LAUNCH();                                                                                       \
-      hipLaunchKernelGGL((k_lvl_scatter<T>), dim3(gc), dim3(256), 0, s, ws, ar, cd);                            \
+      hipLaunchKernelGGL((k_lvl_merge<T>), dim3(gw), dim3(LV_NT), 0, s, indptr, indices, data, ws, ar,          \
+                         width, st);                                                                            \
       RBG_CHECK_LAUNCH();                                                                                       \
       prof_stop(2, pi, s);                                                                                      \
       hipLaunchKernelGGL(k_round_end, dim3(1), dim3(1), 0, s, ws, r);                                           \

@@ -90,6 +90,7 @@ class DagInfo:
     cycle_vertex: int
     nlevels: int
     events: int
+    arena: int = 0      # arena entries the run reserved (sum of candidate counts)
 
     @property
     def diameter_candidates(self) -> int:
@@ -103,10 +104,12 @@ class DagInfo:
 
 
 class DagPlan:
-    """Reusable device buffers for closing graphs of one size: workspace, row
-    arena (closure entries) and per-level candidate scratch."""
+    """Reusable device buffers for closing graphs of one size: the workspace
+    and the row arena.  Every level reserves its candidate count in the arena,
+    so it needs sum_v (nnz(A[v]) + sum_w nnz(C[w])) entries (DagInfo.arena
+    after a run), somewhat more than nnz(C)."""
 
-    def __init__(self, n: int, value_dtype, arena_entries: int, cand_entries: int):
+    def __init__(self, n: int, value_dtype, arena_entries: int):
         t = _lib.torch()
         dev = _lib.device()
         L = _lib.load()
@@ -115,9 +118,6 @@ class DagPlan:
         self.ws = t.empty(int(L.rbg_dag_workspace_bytes(max(n, 1))), dtype=t.uint8, device=dev)
         self.arena_cols = t.empty(arena_entries, dtype=t.int32, device=dev)
         self.arena_vals = t.empty(arena_entries, dtype=value_dtype, device=dev)
-        elem = 4 if value_dtype == t.int32 else 8
-        self.cand_cap = int(cand_entries)
-        self.cand = t.empty(int(L.rbg_dag_cand_bytes(self.cand_cap, elem)), dtype=t.uint8, device=dev)
 
 
 def dag_closure(A: DevCSR, width: int | None = None, stream=None, plan: DagPlan | None = None,
@@ -139,21 +139,21 @@ def dag_closure(A: DevCSR, width: int | None = None, stream=None, plan: DagPlan 
     st = status()
     sh = _lib.stream_handle(stream)
     if plan is None or plan.n != n or plan.vdt != vdt:
-        arena = int(max(4096, arena_factor * (A.nnz + n)))
-        plan = DagPlan(n, vdt, arena, arena)
+        plan = DagPlan(n, vdt, int(max(4096, arena_factor * (A.nnz + n))))
     sfx = "i32" if vdt == t.int32 else "i64"
     for _attempt in range(8):
         st.reset(stream)
         _lib.call(f"rbg_dag_closure_{sfx}", _lib.ptr(A.indptr), _lib.ptr(A.indices), _lib.ptr(A.data), n,
                   _lib.ptr(plan.ws), plan.ws.numel(), _lib.ptr(plan.arena_cols), _lib.ptr(plan.arena_vals),
-                  plan.arena_cols.numel(), _lib.ptr(plan.cand), plan.cand_cap, batch, width, st.p, sh)
+                  plan.arena_cols.numel(), batch, width, st.p, sh)
         s = st.read(stream)
         if s.flags & _lib.F_CYCLE:
             return None, DagInfo(0, 0, 0, int(s.aux[0]), int(s.aux[6]), 0)
         if s.flags & _lib.F_CAPACITY:
-            plan = DagPlan(n, vdt, 2 * plan.arena_cols.numel(), 2 * plan.cand_cap)
+            plan = DagPlan(n, vdt, 2 * plan.arena_cols.numel())
             continue
         info = DagInfo(0, int(s.aux[2]), int(s.aux[3]), -1, int(s.aux[6]), int(s.events))
+        info.arena = int(s.aux[4])
         optr = t.empty(n + 1, dtype=t.int64, device=A.indices.device)
         _lib.call(f"rbg_dag_gather_{sfx}", _lib.ptr(plan.ws), n, _lib.ptr(plan.arena_cols),
                   _lib.ptr(plan.arena_vals), _lib.ptr(optr), None, None, st.p, sh)

@@ -113,21 +113,28 @@ def test_dag_1m_workload_vs_oracle(orc):
     assert np.all(np.abs(d) >= 1)
 
 
-def test_dag_long_rows_use_global_buffers(orc):
-    """Rows longer than the per-wave LDS capacity (512 entries) take the
-    global-memory merge buffers: vertex 0 points at 799 vertices, vertex 1 at
-    400 of them."""
-    n = 800
+@pytest.mark.parametrize("vdt,width", [(np.int64, 64), (np.int32, 32)])
+def test_dag_long_rows_all_merge_paths(orc, vdt, width):
+    """Rows of every merge path: vertex 0 points at 2999 vertices (more than
+    the workgroup's 2048-entry LDS buffers: merged in its arena slot), vertex 1
+    at 1200 (workgroup in LDS), vertex 2 at 300 (one wave in LDS), and vertex 3
+    reaches vertices 0..2's rows through successors 1 and 2."""
+    n = 3000
     rng = np.random.default_rng(4)
     A = np.zeros((n, n), dtype=np.int64)
     np.fill_diagonal(A, rng.choice([-1, 1], n))
     A[0, 1:] = rng.integers(2, 4, n - 1)
-    A[1, 2:402] = rng.integers(2, 4, 400)
+    A[1, 4:1204] = rng.integers(2, 4, 1200)
+    A[2, 1000:1300] = rng.integers(2, 4, 300)
+    A[3, [1, 2]] = [2, 3]
+    A[3, 2500:2600] = 2
     Acsr = sp.csr_matrix(A)
-    C, info = sd.dag_closure(sd.canonicalize(sd.from_scipy(Acsr, np.int64)))
-    ref = orc.dag_closure(Acsr, 64)
+    C, info = sd.dag_closure(sd.canonicalize(sd.from_scipy(Acsr, vdt)))
+    ref = orc.dag_closure(Acsr, width)
+    assert ref.events == 0
     np.testing.assert_array_equal(sd.to_scipy(C).toarray(), ref.toarray(n))
     assert info.diameter_candidates == ref.diameter_cython
+    assert (info.cand_max, info.final_max) == (ref.cand_max, ref.final_max)
 
 
 def test_dag_wide_fanout(orc):
