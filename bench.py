@@ -225,6 +225,132 @@ def bench_sparse(args):
     return res
 
 
+def cpu_baseline_batched(sizes, diag, edges, budget_s: float = 15.0):
+    """Single-core naive FW (oracle) on the batch's components in order of
+    size, smallest first, until about budget_s of CPU time; the measured
+    relaxations/s give the whole batch's CPU time by the n^3 law."""
+    import numpy as np
+    from oracle import oracle
+    oracle.lib()
+    order = np.argsort(sizes, kind="stable")
+    done_relax, spent, used = 0, 0.0, 0
+    for i in order[:: max(1, len(order) // 256)]:
+        n = int(sizes[i])
+        A = np.zeros((n, n), dtype=np.int32)
+        r, c, v = edges[i]
+        A[r, c] = v
+        np.fill_diagonal(A, diag[i])
+        t0 = time.perf_counter()
+        oracle.warshall(A)
+        spent += time.perf_counter() - t0
+        done_relax += n ** 3
+        used += 1
+        if spent > budget_s:
+            break
+    rate = done_relax / spent
+    entries = float(np.sum(np.asarray(sizes, dtype=np.float64) ** 2))
+    relax = float(np.sum(np.asarray(sizes, dtype=np.float64) ** 3))
+    return {
+        "value": entries / (relax / rate), "unit": "entries/s", "cores": 1, "kind": "port",
+        "sample": (f"oracle naive FW (C restatement of warshall.c.in, gcc -O2, 1 thread) on {used} of the batch's "
+                   f"components (every {max(1, len(order) // 256)}th by size, smallest first): {spent:.2f} s, "
+                   f"{rate:.3e} relaxations/s; value = sum n^2 / (sum n^3 / rate); host CPU: {_cpu_model()}"),
+    }
+
+
+def bench_batched(args):
+    import ctypes
+    import numpy as np
+    from redblackgraph_amd import _lib, dense, synth
+    from redblackgraph_amd._dtypes import elem_type
+    from redblackgraph_amd.batched import BatchedClosure
+    t = _lib.torch()
+    sizes, diag, edges = synth.batched_workload(count=args.batch_count, seed=args.seed)
+    t_tune0 = time.perf_counter()
+    b = BatchedClosure(sizes)
+    tune_s = time.perf_counter() - t_tune0
+    b.load_edges(diag, edges)
+    pristine = b.pool.clone()
+    t.cuda.synchronize()
+    info = b.info()
+    L = _lib.load()
+    # eager run with per-step tile timing (prof events do not go into graphs)
+    L.rbg_prof_enable(1)
+    b.run(graph=False)
+    b.stream.synchronize()
+    ms_, nl = ctypes.c_double(), ctypes.c_int64()
+    _lib.call("rbg_prof_read", 4, ctypes.byref(ms_), ctypes.byref(nl))
+    L.rbg_prof_enable(0)
+    outs = b.outcomes()
+    if any(o.mode != "fast" or o.status.cycle_count or o.status.events for o in outs):
+        raise RuntimeError("unexpected component outcome in the batched workload")
+    eager_result = b.pool.clone()
+    t.cuda.synchronize()  # the clone (current stream) must finish before the batch stream rewrites the pool
+    with t.cuda.stream(b.stream):
+        for _ in range(args.warmup):
+            b.pool.copy_(pristine)
+            b.run(graph=True)
+    b.stream.synchronize()
+    t.cuda.synchronize()
+    t0 = time.perf_counter()
+    with t.cuda.stream(b.stream):
+        for _ in range(args.steps):
+            b.pool.copy_(pristine)
+            b.run(graph=True)
+    b.stream.synchronize()
+    t.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms = (t1 - t0) * 1e3 / args.steps
+    identical_graph = bool(t.equal(b.pool, eager_result))
+    # plain path: one closure per component through the single-matrix entry point
+    et = elem_type(np.int32)
+    views = []
+    for i, n in enumerate(sizes):
+        p, o = int(b.layout.npad[i]), int(b.layout.offset[i])
+        views.append(pristine[o:o + p * p].view(p, p)[:int(n), :int(n)])
+    t.cuda.synchronize()
+    tp0 = time.perf_counter()
+    plain = [dense.fw_closure(V, et)[0] for V in views]
+    t.cuda.synchronize()
+    plain_ms = (time.perf_counter() - tp0) * 1e3
+    identical_plain = all(bool(t.equal(plain[i], b.result(i))) for i in range(len(sizes)))
+    del plain
+    P = b.layout.npad.astype(np.float64)
+    relax_exec = float(np.sum((P - FW_BLK) ** 2 * P))
+    tile_s = ms_.value / 1e3
+    achieved = relax_exec * OPS_PER_RELAX["int32"] / tile_s
+    entries = float(np.sum(sizes.astype(np.float64) ** 2))
+    roof = {
+        "bound": "valu", "kernel": "k_bfw_tile<8>/<4> (batched phase 3)",
+        "achieved": achieved / 1e12, "peak": PEAK_VALU_OPS / 1e12, "unit": "TOP/s",
+        "frac": achieved / PEAK_VALU_OPS, "traffic": _read_traffic("bfw_tile_batched4096_pmc.txt"),
+        "ops_per_relaxation": OPS_PER_RELAX["int32"], "relaxations_executed": relax_exec,
+        "tile_ms_per_batch": ms_.value, "tile_launches": nl.value,
+        "kernel_share_of_step": ms_.value / ms,
+    }
+    res = {
+        "metric": METRIC, "value": entries / (ms / 1e3), "unit": "entries/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int32", "data": "synthetic",
+        "config": {"workload": (f"batched dense closure of {len(sizes)} pedigree components, sizes log-uniform "
+                                f"in [16, 4096], layers = ceil(log2 n), p_parent=0.8, seed {args.seed}; one "
+                                f"hipGraph replay per step (input reloaded by one device copy inside the step)"),
+                   "components": len(sizes), "sum_n": int(sizes.sum()), "sum_n2": entries,
+                   "closures_per_s": len(sizes) / (ms / 1e3),
+                   "plain_ms": plain_ms, "plain_closures_per_s": len(sizes) / (plain_ms / 1e3),
+                   "batched_vs_plain": plain_ms / ms,
+                   "identical_to_plain": identical_plain, "graph_replay_identical_to_eager": identical_graph,
+                   "launches_per_batch": info["launches"], "pivot_steps": info["steps"],
+                   "tiles": {str(k): v for k, v in sorted(b.tiles.items())}, "autotune_s": tune_s},
+        "roofline": roof,
+    }
+    if not args.no_cpu_baseline:
+        res["cpu_baseline"] = cpu_baseline_batched(sizes, diag, edges)
+    b.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -232,7 +358,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--workload", default="dense8192",
-                    choices=["dense8192", "dense4096_i64", "dense512", "sparse1m"])
+                    choices=["dense8192", "dense4096_i64", "dense512", "sparse1m", "batched"])
+    ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--cpu-sample", type=int, default=1024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -244,6 +371,8 @@ def main():
         res = bench_dense(args, 8192, "int32")
     elif args.workload == "sparse1m":
         res = bench_sparse(args)
+    elif args.workload == "batched":
+        res = bench_batched(args)
     elif args.workload == "dense4096_i64":
         res = bench_dense(args, 4096, "int64")
     else:

@@ -221,6 +221,39 @@ int rbg_csr_add_i64(const int64_t *ap, const int32_t *aj, const int64_t *ax, con
                     int64_t *cat_val, void *tmp, int64_t *out_indptr, int32_t *out_idx, int64_t *out_val,
                     rbg_stream_t stream);
 
+/* ---- batched dense closure -------------------------------------------------
+ * Replaces closing components one at a time (sparse/csgraph/
+ * transitive_closure.py:85-195 component_wise_closure, each through
+ * warshall.c.in:50-107).  Every component is an int32 matrix of padded order
+ * npad (multiple of 128, rows contiguous, zero padding), closed in place with
+ * the fast kernels' arithmetic; `snap` is its snapshot workspace
+ * (rbg_batch_snap_bytes(npad)), `st` its status, `tile` the phase-3 tile edge
+ * (128 or 64) for its size bucket.  After a run each status holds what the
+ * single-matrix fast path would report; RBG_F_PRECOND_FAIL, RBG_F_NEED_GENERAL
+ * or a nonzero cycle_count mean the component must be closed again from its
+ * input by rbg_fw_dense_i32 in general mode (its matrix is then unspecified).
+ * rbg_batch_create copies the descriptors and builds the per-step work lists
+ * (device memory owned by the handle); run launches the whole schedule on
+ * `stream`; capture records it into a hipGraph owned by the handle (stream
+ * must not be the legacy null stream) and replay launches that graph.
+ * info: [0] count [1] pivot-block steps [2] launches [3] panel items
+ *       [4] 128-tiles [5] 64-tiles [6] 128x128 chunks [7] graph captured. */
+typedef struct rbg_batch rbg_batch;
+typedef struct {
+  int32_t *W;
+  void *snap;
+  rbg_status *st;
+  int32_t npad;
+  int32_t tile;
+} rbg_batch_comp;
+int64_t rbg_batch_snap_bytes(int32_t npad);
+int rbg_batch_create(const rbg_batch_comp *comps, int32_t count, rbg_batch **out);
+int rbg_batch_info(const rbg_batch *b, int64_t *info);
+int rbg_batch_run(rbg_batch *b, rbg_stream_t stream);
+int rbg_batch_capture(rbg_batch *b, rbg_stream_t stream);
+int rbg_batch_replay(rbg_batch *b, rbg_stream_t stream);
+int rbg_batch_destroy(rbg_batch *b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -79,7 +79,20 @@ SIGNATURES = {
     "rbg_csr_add_i32": (I32, [P, P, P, P, P, P, I32, I32, P, P, P, P, P, P, P, P]),
     "rbg_csr_add_i64": (I32, [P, P, P, P, P, P, I32, I32, P, P, P, P, P, P, P, P]),
     "rbg_spgemm_rows_i64": (I32, [P, P, P, I32, P, P, P, I32, P, P, P, I64, P, P, I64, I32, I32, P, P]),
+    "rbg_batch_snap_bytes": (I64, [I32]),
+    "rbg_batch_create": (I32, [P, I32, P]),
+    "rbg_batch_info": (I32, [P, P]),
+    "rbg_batch_run": (I32, [P, P]),
+    "rbg_batch_capture": (I32, [P, P]),
+    "rbg_batch_replay": (I32, [P, P]),
+    "rbg_batch_destroy": (I32, [P]),
 }
+
+
+class RbgBatchComp(ctypes.Structure):
+    """rbg_batch_comp: one component of a batched dense closure."""
+    _fields_ = [("W", ctypes.c_void_p), ("snap", ctypes.c_void_p), ("st", ctypes.c_void_p),
+                ("npad", ctypes.c_int32), ("tile", ctypes.c_int32)]
 
 
 class LibraryMissing(ImportError):

@@ -1,0 +1,659 @@
+#pragma once
+// Dense AVOS transitive closure: snapshot-blocked Floyd-Warshall kernels for
+// gfx950, shared by the single-matrix and the batched launchers.
+//
+// Reference: redblackgraph/core/src/redblackgraph/warshall.c.in:50-107 (naive
+// in-place k-i-j loop, diameter = MSB of the largest value ever written, cycle
+// detection on i == j).  The blocked schedule reproduces that loop's
+// trajectory cell by cell:
+//
+//   * At step k the naive loop reads W[i][k] and W[k][j]; for inputs >= -1 and
+//     no overflow neither row k nor column k changes during step k (x (x) y >= y
+//     for x >= 2, the identities are neutral).  So step k is a pure function of
+//     the step k-1 values of row k and column k.
+//   * The panel kernel walks the pivot block's k sequentially and records, for
+//     every k, column k and row k as they are at step k ("snapshots").  Phase 3
+//     applies W[i][j] = W[i][j] (+) col_k[i] (x) row_k[j] for k ascending, which
+//     is exactly the naive sequence of values of (i, j).
+//   * Hence W, the diameter (max over every written value) and the (k, i) order
+//     of cycle detections equal the naive loop's.
+//
+// Two arithmetic modes:
+//   GENERAL  natural values, full AVOS product with the C core's overflow rules,
+//            every event recorded with its naive-order key.
+//   FAST     matrix kept in the "E domain" (rbg_common.hpp): a relaxation is
+//            v_lshl_or_b32 + v_min_u32 and the trajectory max one v_max3 per
+//            two steps.  Valid when no off-diagonal entry is +-1 or < -1 (checked
+//            by rbg_fw_precheck) and the per-block MSB guard rules out overflow;
+//            otherwise the kernels raise RBG_F_NEED_GENERAL.
+#include "rbg_common.hpp"
+
+namespace rbg {
+
+constexpr int BLK = 32;    // pivot block (k values per step)
+constexpr int TILE = 128;  // phase-3 tile edge
+constexpr int NT = 256;    // threads per workgroup
+constexpr int RPT = 8;     // phase-3 rows per thread
+constexpr int CPT = 8;     // phase-3 cols per thread
+
+struct FwWork {
+  // two snapshot buffers (block parity), each BLK x n for columns and rows,
+  // then 2 x 2 x BLK guard words
+  void* base;
+  int64_t n;
+  int elem;
+  template <typename T>
+  __host__ __device__ T* col(int parity) const {
+    return (T*)base + (size_t)parity * 2 * BLK * n;
+  }
+  template <typename T>
+  __host__ __device__ T* row(int parity) const {
+    return (T*)base + (size_t)parity * 2 * BLK * n + (size_t)BLK * n;
+  }
+  __host__ __device__ int* guard(int parity) const {  // [0..BLK) col msb, [BLK..2BLK) row msb
+    return (int*)((char*)base + (size_t)4 * BLK * n * elem) + parity * 2 * BLK;
+  }
+  static int64_t bytes(int64_t n, int elem) { return (int64_t)4 * BLK * n * elem + 4 * BLK * 4 + 256; }
+};
+
+__device__ inline uint64_t naive_key(int64_t k, int64_t i, int64_t j, int64_t n) {
+  return ((uint64_t)k * (uint64_t)n + (uint64_t)i) * (uint64_t)n + (uint64_t)j;
+}
+
+template <typename T, bool FAST>
+__device__ inline T snap_enc(T v) {
+  return FAST ? e_enc<T>(v) : v;
+}
+template <typename T, bool FAST>
+__device__ inline T snap_dec(T v) {
+  return FAST ? e_dec<T>(v) : v;
+}
+
+// Copy the operands of detections that happened in block [k0, k0+BLK) out of
+// that block's snapshots (single thread).
+template <typename T, bool FAST>
+__device__ void capture_block(rbg_status* st, const T* colT, const T* rowS, int64_t k0, int64_t n) {
+  const uint64_t nn = (uint64_t)n * (uint64_t)n;
+  if (st->cycle_count) {
+    uint64_t fk = st->cycle_first_key;
+    int64_t k = (int64_t)(fk / nn), i = (int64_t)((fk / n) % n);
+    if (k >= k0 && k < k0 + BLK) {
+      st->cycle_first_a = (int64_t)snap_dec<T, FAST>(colT[(k - k0) * n + i]);
+      st->cycle_first_b = (int64_t)snap_dec<T, FAST>(rowS[(k - k0) * n + i]);
+    }
+    uint64_t lk = st->cycle_last_key;
+    k = (int64_t)(lk / nn);
+    i = (int64_t)((lk / n) % n);
+    if (k >= k0 && k < k0 + BLK) {
+      st->cycle_last_a = (int64_t)snap_dec<T, FAST>(colT[(k - k0) * n + i]);
+      st->cycle_last_b = (int64_t)snap_dec<T, FAST>(rowS[(k - k0) * n + i]);
+    }
+  }
+  if (st->ov_first_key != ~0ull) {
+    uint64_t lk = st->ov_last_key;
+    int64_t k = (int64_t)(lk / nn), i = (int64_t)((lk / n) % n), j = (int64_t)(lk % n);
+    if (k >= k0 && k < k0 + BLK) {
+      st->ov_last_a = (int64_t)snap_dec<T, FAST>(colT[(k - k0) * n + i]);
+      st->ov_last_b = (int64_t)snap_dec<T, FAST>(rowS[(k - k0) * n + j]);
+    }
+  }
+}
+
+static __global__ void k_status_init(rbg_status* st) {
+  st->events = 0;
+  st->flags = 0;
+  st->cycle_count = 0;
+  st->cycle_first_key = ~0ull;
+  st->cycle_last_key = 0;
+  st->ov_first_key = ~0ull;
+  st->ov_last_key = 0;
+  st->cycle_first_a = st->cycle_first_b = st->cycle_last_a = st->cycle_last_b = 0;
+  st->ov_last_a = st->ov_last_b = 0;
+  st->diameter_max = 0;
+  for (int i = 0; i < 8; i++) st->aux[i] = 0;
+}
+
+static __global__ void k_guard_init(int* g) {
+  if (threadIdx.x < 4 * BLK) g[threadIdx.x] = 0;
+}
+
+// Precondition of the fast kernels: every value >= -1, no off-diagonal +-1,
+// every diagonal value in {-1, 0, 1}.  With such diagonals the products that
+// involve a pivot cell (k, k) can neither overflow nor reveal a cycle, which
+// is what lets the fast kernels skip the pivot row and column.
+template <typename T>
+__global__ void k_fw_precheck(const T* W, int64_t n, int64_t ld, rbg_status* st) {
+  int64_t total = n * n;
+  bool bad = false;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    int64_t i = e / n, j = e % n;
+    T v = W[i * ld + j];
+    if (v < (T)-1) bad = true;
+    if (i != j && (v == (T)1 || v == (T)-1)) bad = true;
+    if (i == j && v > (T)1) bad = true;
+  }
+  if (bad) atomicOr(&st->flags, (uint32_t)RBG_F_PRECOND_FAIL);
+}
+
+template <typename T, bool ENC>
+__global__ void k_e_convert(T* W, int64_t n, int64_t ld) {
+  int64_t total = n * n;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    int64_t i = e / n, j = e % n;
+    T v = W[i * ld + j];
+    W[i * ld + j] = ENC ? e_enc<T>(v) : e_dec<T>(v);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Panel kernel: phases 1 and 2 of pivot block K fused.  Every workgroup walks
+// the diagonal tile (redundantly; workgroup 0 owns its results) together with
+// one row- or column-panel tile, one k at a time, in the natural domain with
+// the full arithmetic.  It writes the snapshots phase 3 consumes.
+// ---------------------------------------------------------------------------
+template <typename T, bool FAST>
+__global__ __launch_bounds__(NT) void k_fw_panel(T* W, int64_t n, int64_t ld, int64_t K, FwWork ws,
+                                                 int width, int is_uns, rbg_status* st) {
+  __shared__ T Dg[BLK][BLK + 1];
+  __shared__ T Pn[BLK][BLK + 1];
+  __shared__ int sguard[BLK];
+  const int tid = threadIdx.x;
+  const int nb = (int)(n / BLK);
+  const int64_t k0 = K * BLK;
+  const int par = (int)(K & 1);
+  T* colT = ws.col<T>(par);
+  T* rowS = ws.row<T>(par);
+  int* guard = ws.guard(par);
+
+  // panel role: 0 = diagonal only, 1 = row panel (K, J), 2 = column panel (J, K)
+  int role = 0, J = 0;
+  if (nb > 1) {
+    int t = blockIdx.x;
+    role = t < nb - 1 ? 1 : 2;
+    int idx = role == 1 ? t : t - (nb - 1);
+    J = idx < K ? idx : idx + 1;
+  }
+  const bool owner = blockIdx.x == 0;
+
+  if (owner && tid == 0) {
+    if (K > 0) capture_block<T, FAST>(st, ws.col<T>(par ^ 1), ws.row<T>(par ^ 1), k0 - BLK, n);
+    int* gnext = ws.guard(par ^ 1);
+    for (int q = 0; q < 2 * BLK; q++) gnext[q] = 0;
+  }
+  if (tid < BLK) sguard[tid] = 0;
+
+  // load tiles (natural domain)
+  const int64_t pr0 = role == 2 ? (int64_t)J * BLK : k0;  // panel row origin
+  const int64_t pc0 = role == 1 ? (int64_t)J * BLK : k0;  // panel col origin
+  for (int e = tid; e < BLK * BLK; e += NT) {
+    int r = e / BLK, c = e % BLK;
+    Dg[r][c] = snap_dec<T, FAST>(W[(k0 + r) * ld + k0 + c]);
+    if (role) Pn[r][c] = snap_dec<T, FAST>(W[(pr0 + r) * ld + pc0 + c]);
+  }
+  __syncthreads();
+
+  const int i = tid >> 3;         // 0..31
+  const int j0 = (tid & 7) * 4;   // 0..28
+  int64_t dmax = 0;
+  uint32_t evacc = 0;
+
+  for (int kk = 0; kk < BLK; kk++) {
+    const int64_t k = k0 + kk;
+    T aD = Dg[i][kk];
+    T bD[4], aP = 0, bP[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) bD[q] = Dg[kk][j0 + q];
+    if (role == 1) {
+      aP = aD;
+#pragma unroll
+      for (int q = 0; q < 4; q++) bP[q] = Pn[kk][j0 + q];
+    } else if (role == 2) {
+      aP = Pn[i][kk];
+#pragma unroll
+      for (int q = 0; q < 4; q++) bP[q] = bD[q];
+    }
+    // snapshots of step k
+    if (owner) {
+      if ((tid & 7) == 0) colT[kk * n + k0 + i] = snap_enc<T, FAST>(aD);
+      if (i == 0) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) rowS[kk * n + k0 + j0 + q] = snap_enc<T, FAST>(bD[q]);
+      }
+    }
+    if (role == 1 && i == 0) {
+      int mm = 0;
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        rowS[kk * n + pc0 + j0 + q] = snap_enc<T, FAST>(bP[q]);
+        int m = bP[q] > 1 ? msb<T>(bP[q]) : 0;
+        mm = m > mm ? m : mm;
+      }
+      atomicMax(&sguard[kk], mm);
+    }
+    if (role == 2 && (tid & 7) == 0) {
+      colT[kk * n + pr0 + i] = snap_enc<T, FAST>(aP);
+      atomicMax(&sguard[kk], aP > 1 ? msb<T>(aP) : 0);
+    }
+    __syncthreads();
+    // diagonal tile
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int j = j0 + q;
+      uint32_t ev = 0;
+      T p = avos_product<T>(aD, bD[q], width, is_uns != 0, ev);
+      if (owner) {
+        if (ev) {
+          record_overflow(st, naive_key(k, k0 + i, k0 + j, n), ev);
+          evacc |= ev;
+        }
+        if (i == j && !(p == (T)-1 || p == 0 || p == 1)) record_cycle(st, naive_key(k, k0 + i, k0 + i, n));
+      }
+      T v = avos_sum<T>(Dg[i][j], p);
+      Dg[i][j] = v;
+      if (owner && (int64_t)v > dmax) dmax = (int64_t)v;
+    }
+    if (role) {
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int j = j0 + q;
+        uint32_t ev = 0;
+        T p = avos_product<T>(aP, bP[q], width, is_uns != 0, ev);
+        if (ev) record_overflow(st, naive_key(k, pr0 + i, pc0 + j, n), ev);
+        T v = avos_sum<T>(Pn[i][j], p);
+        Pn[i][j] = v;
+        if ((int64_t)v > dmax) dmax = (int64_t)v;
+      }
+    }
+    __syncthreads();
+  }
+  (void)evacc;
+  // write back
+  for (int e = tid; e < BLK * BLK; e += NT) {
+    int r = e / BLK, c = e % BLK;
+    if (owner) W[(k0 + r) * ld + k0 + c] = snap_enc<T, FAST>(Dg[r][c]);
+    if (role) W[(pr0 + r) * ld + pc0 + c] = snap_enc<T, FAST>(Pn[r][c]);
+  }
+  if (tid < BLK) {
+    int g = sguard[tid];
+    if (g) atomicMax(&guard[(role == 1 ? BLK : 0) + tid], g);
+  }
+  dmax = wave_max_i64(dmax);
+  if ((tid & 63) == 0) record_diameter(st, dmax);
+}
+
+// ---------------------------------------------------------------------------
+// Panel kernel, fast mode: same schedule as k_fw_panel, E-domain arithmetic.
+// At step k the pivot row and column do not change (they are skipped), so the
+// step reads them while it updates every other cell and needs one barrier.
+// Operands off the pivot row/column are never +-1 (precheck), so the product
+// is (x << MSB(y)) | low(y) there.  A per-product MSB check raises
+// RBG_F_NEED_GENERAL before any value could overflow.
+// A product with an absent operand is some value with the top bit set; the
+// min can store it into an absent cell.  Such a value must not be shifted
+// again (the shift could clear its top bit), so every operand read here is
+// first canonicalised to all ones, and so is every snapshot phase 3 reads.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ inline void fw_panel_fast_body(T* W, int64_t n, int64_t ld, int64_t K, FwWork ws, int width,
+                                          rbg_status* st, int bid) {
+  using U = typename unsigned_of<T>::type;
+  constexpr U TOP = (U)1 << (sizeof(U) * 8 - 1);
+  __shared__ U Dg[BLK][BLK + 1];
+  __shared__ U Pn[BLK][BLK + 1];
+  // snapshots of step kk, staged in LDS and written out after the loop so the
+  // per-step barrier never waits on global stores
+  __shared__ U sDc[BLK][BLK + 1];  // [kk][i] = diag W[i][kk]
+  __shared__ U sDr[BLK][BLK + 1];  // [kk][j] = diag W[kk][j]
+  __shared__ U sP[BLK][BLK + 1];   // row panel: [kk][j] = W[kk][j]; column panel: [kk][i] = W[i][kk]
+  __shared__ int sbad;
+  const int tid = threadIdx.x;
+  const int nb = (int)(n / BLK);
+  const int64_t k0 = K * BLK;
+  const int par = (int)(K & 1);
+  T* colT = ws.col<T>(par);
+  T* rowS = ws.row<T>(par);
+  int* guard = ws.guard(par);
+
+  int role = 0, J = 0;
+  if (nb > 1) {
+    int t = bid;
+    role = t < nb - 1 ? 1 : 2;
+    int idx = role == 1 ? t : t - (nb - 1);
+    J = idx < K ? idx : idx + 1;
+  }
+  const bool owner = bid == 0;
+  // a workgroup-uniform exit once the matrix is known to need general mode
+  if (tid == 0) sbad = (st->flags & (RBG_F_NEED_GENERAL | RBG_F_PRECOND_FAIL)) ? 2 : 0;
+  __syncthreads();
+  if (sbad) return;
+  if (owner && tid == 0) {
+    if (K > 0) capture_block<T, true>(st, ws.col<T>(par ^ 1), ws.row<T>(par ^ 1), k0 - BLK, n);
+    int* gnext = ws.guard(par ^ 1);
+    for (int q = 0; q < 2 * BLK; q++) gnext[q] = 0;
+  }
+
+  const int64_t pr0 = role == 2 ? (int64_t)J * BLK : k0;
+  const int64_t pc0 = role == 1 ? (int64_t)J * BLK : k0;
+  auto canon = [](U x) -> U { return (T)x < 0 ? ~(U)0 : x; };
+  for (int e = tid; e < BLK * BLK; e += NT) {
+    int r = e / BLK, c = e % BLK;
+    Dg[r][c] = canon((U)W[(k0 + r) * ld + k0 + c]);
+    if (role) Pn[r][c] = canon((U)W[(pr0 + r) * ld + pc0 + c]);
+  }
+  __syncthreads();
+
+  const int i = tid >> 3;
+  const int j0 = (tid & 7) * 4;
+  const int lim = width - 2;
+  U dv[4], pv[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    dv[q] = Dg[i][j0 + q];
+    pv[q] = role ? Pn[i][j0 + q] : (U)0;
+  }
+  T dmax = 0;
+  bool bad = false;
+  for (int kk = 0; kk < BLK; kk++) {
+    const int64_t k = k0 + kk;
+    const U aD = canon(Dg[i][kk]);
+    U bD[4], bP[4], aP = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) bD[q] = canon(Dg[kk][j0 + q]);
+    if (role == 1) {
+      aP = aD;
+#pragma unroll
+      for (int q = 0; q < 4; q++) bP[q] = canon(Pn[kk][j0 + q]);
+    } else if (role == 2) {
+      aP = canon(Pn[i][kk]);
+#pragma unroll
+      for (int q = 0; q < 4; q++) bP[q] = bD[q];
+    }
+    if ((tid & 7) == 0) {
+      sDc[kk][i] = aD;
+      if (role == 2) sP[kk][i] = aP;
+    }
+    if (i == 0) {
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        sDr[kk][j0 + q] = bD[q];
+        if (role == 1) sP[kk][j0 + q] = bP[q];
+      }
+    }
+    // diagonal tile: row kk and column kk are invariant at step kk
+    const int maD = (T)aD > 1 ? msb_u(aD) : 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int j = j0 + q;
+      if (i != kk && j != kk) {
+        U m, y;
+        if ((T)bD[q] < 0) {
+          m = 0;
+          y = TOP;
+        } else {
+          m = (U)msb_u(bD[q]);
+          y = bD[q] & (((U)1 << m) - 1);
+        }
+        if ((T)bD[q] > 1 && maD + (int)m > lim) bad = true;
+        U p = (aD << m) | y;
+        if (owner && i == j && (T)p >= 0) record_cycle(st, naive_key(k, k0 + i, k0 + i, n));
+        dv[q] = dv[q] < p ? dv[q] : p;
+        Dg[i][j] = dv[q];
+      }
+      if (owner && (T)dv[q] > dmax) dmax = (T)dv[q];
+    }
+    if (role) {
+      const int maP = (T)aP > 1 ? msb_u(aP) : 0;
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int j = j0 + q;
+        const bool pivot = role == 1 ? (i == kk) : (j == kk);
+        if (!pivot) {
+          U m, y;
+          if ((T)bP[q] < 0) {
+            m = 0;
+            y = TOP;
+          } else {
+            m = (U)msb_u(bP[q]);
+            y = bP[q] & (((U)1 << m) - 1);
+          }
+          if ((T)bP[q] > 1 && maP + (int)m > lim) bad = true;
+          U p = (aP << m) | y;
+          pv[q] = pv[q] < p ? pv[q] : p;
+          Pn[i][j] = pv[q];
+        }
+        if ((T)pv[q] > dmax) dmax = (T)pv[q];
+      }
+    }
+    __syncthreads();
+  }
+  if (bad) sbad = 1;
+  __syncthreads();
+  if (sbad) {
+    if (tid == 0) atomicOr(&st->flags, (uint32_t)RBG_F_NEED_GENERAL);
+    return;
+  }
+  // tiles and snapshots out, coalesced
+  for (int e = tid; e < BLK * BLK; e += NT) {
+    int r = e / BLK, c = e % BLK;
+    if (owner) {
+      W[(k0 + r) * ld + k0 + c] = (T)Dg[r][c];
+      colT[r * n + k0 + c] = (T)sDc[r][c];
+      rowS[r * n + k0 + c] = (T)sDr[r][c];
+    }
+    if (role == 1) {
+      W[(pr0 + r) * ld + pc0 + c] = (T)Pn[r][c];
+      rowS[r * n + pc0 + c] = (T)sP[r][c];
+    } else if (role == 2) {
+      W[(pr0 + r) * ld + pc0 + c] = (T)Pn[r][c];
+      colT[r * n + pr0 + c] = (T)sP[r][c];
+    }
+  }
+  // per-k MSB guard for phase 3 from this panel's snapshots
+  if (role && tid < BLK) {
+    int mm = 0;
+    for (int c = 0; c < BLK; c++) {
+      U v = sP[tid][c];
+      int m = (T)v > 1 ? msb_u(v) : 0;
+      mm = m > mm ? m : mm;
+    }
+    if (mm) atomicMax(&guard[(role == 1 ? BLK : 0) + tid], mm);
+  }
+  int64_t dm = wave_max_i64((int64_t)dmax);
+  if ((tid & 63) == 0) record_diameter(st, dm);
+}
+
+// ---------------------------------------------------------------------------
+// Phase 3, fast mode (E domain).  One 128x128 tile per workgroup, 8x8 cells
+// per thread held in VGPRs for the whole pivot block; per k a thread reads 8
+// left operands and 8 (shift, low-bits) pairs from LDS.
+// ---------------------------------------------------------------------------
+template <typename T, int RPT, int CPT>
+__device__ inline void fw_tile_fast_body(T* W, int64_t n, int64_t ld, int64_t K, FwWork ws, int width,
+                                         rbg_status* st, int bid) {
+  using U = typename unsigned_of<T>::type;
+  constexpr U TOP = (U)1 << (sizeof(U) * 8 - 1);
+  constexpr int TL = 16 * CPT;  // tile edge: 16 x 16 threads
+  static_assert(RPT == CPT, "square thread blocks keep the diagonal cells on one thread");
+  __shared__ U As[BLK][TL];
+  __shared__ U Ms[BLK][TL];
+  __shared__ U Ys[BLK][TL];
+  __shared__ int sbad;
+  const int tid = threadIdx.x;
+  const int nt = (int)(n / TL);
+  const int I = bid / nt, J = bid % nt;
+  const int64_t k0 = K * BLK;
+  const int par = (int)(K & 1);
+  const T* colT = ws.col<T>(par);
+  const T* rowS = ws.row<T>(par);
+  const int* guard = ws.guard(par);
+
+  const int tr = tid >> 4, tc = tid & 15;
+  const int64_t r0 = (int64_t)I * TL + tr * RPT;
+  const int64_t c0 = (int64_t)J * TL + tc * CPT;
+  const bool skip = (r0 >= k0 && r0 < k0 + BLK) || (c0 >= k0 && c0 < k0 + BLK);
+
+  // issue the tile loads first
+  U w[RPT][CPT];
+  if (!skip) {
+#pragma unroll
+    for (int r = 0; r < RPT; r++)
+#pragma unroll
+      for (int c = 0; c < CPT; c++) w[r][c] = (U)W[(r0 + r) * ld + c0 + c];
+  }
+
+  // a workgroup-uniform exit when an earlier kernel already asked for general mode
+  if (tid == 0) sbad = (st->flags & (RBG_F_NEED_GENERAL | RBG_F_PRECOND_FAIL)) ? 2 : 0;
+  __syncthreads();
+  if (tid < BLK) {
+    if (guard[tid] + guard[BLK + tid] > width - 2) atomicOr(&sbad, 1);
+  }
+  for (int e = tid; e < BLK * TL; e += NT) {
+    int kk = e / TL, x = e % TL;
+    As[kk][x] = (U)colT[kk * n + (int64_t)I * TL + x];
+    T b = rowS[kk * n + (int64_t)J * TL + x];
+    U m, y;
+    if (b < 0) {  // absent
+      m = 0;
+      y = TOP;
+    } else {
+      m = (U)msb_u((U)b);
+      y = (U)b & (((U)1 << m) - 1);
+    }
+    Ms[kk][x] = m;
+    Ys[kk][x] = y;
+  }
+  __syncthreads();
+  if (sbad) {
+    if (tid == 0 && (sbad & 1)) atomicOr(&st->flags, (uint32_t)RBG_F_NEED_GENERAL);
+    return;
+  }
+  if (skip) return;
+
+  const bool diag_thread = (I == J) && (tr * RPT == tc * CPT);
+  T d0 = 0, d1 = 0;
+  for (int kk = 0; kk < BLK; kk += 2) {
+    U a0[RPT], a1[RPT], m0[CPT], m1[CPT], y0[CPT], y1[CPT];
+#pragma unroll
+    for (int r = 0; r < RPT; r++) {
+      a0[r] = As[kk][tr * RPT + r];
+      a1[r] = As[kk + 1][tr * RPT + r];
+    }
+#pragma unroll
+    for (int c = 0; c < CPT; c++) {
+      m0[c] = Ms[kk][tc * CPT + c];
+      y0[c] = Ys[kk][tc * CPT + c];
+      m1[c] = Ms[kk + 1][tc * CPT + c];
+      y1[c] = Ys[kk + 1][tc * CPT + c];
+    }
+    if (diag_thread) {
+#pragma unroll
+      for (int r = 0; r < RPT; r++) {
+        U p0 = (a0[r] << m0[r]) | y0[r];
+        U p1 = (a1[r] << m1[r]) | y1[r];
+        if ((T)p0 >= 0) record_cycle(st, naive_key(k0 + kk, r0 + r, r0 + r, n));
+        if ((T)p1 >= 0) record_cycle(st, naive_key(k0 + kk + 1, r0 + r, r0 + r, n));
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < RPT; r++) {
+#pragma unroll
+      for (int c = 0; c < CPT; c++) {
+        U p0 = (a0[r] << m0[c]) | y0[c];
+        U p1 = (a1[r] << m1[c]) | y1[c];
+        U v1 = w[r][c] < p0 ? w[r][c] : p0;
+        U v2 = v1 < p1 ? v1 : p1;
+        w[r][c] = v2;
+        T t1 = (T)v1, t2 = (T)v2;
+        T tm = t1 > t2 ? t1 : t2;
+        if (c & 1) d1 = tm > d1 ? tm : d1;
+        else d0 = tm > d0 ? tm : d0;
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < RPT; r++)
+#pragma unroll
+    for (int c = 0; c < CPT; c++) W[(r0 + r) * ld + c0 + c] = (T)w[r][c];
+  int64_t dm = (int64_t)(d0 > d1 ? d0 : d1);
+  dm = wave_max_i64(dm);
+  if ((tid & 63) == 0) record_diameter(st, dm);
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void k_fw_panel_fast(T* W, int64_t n, int64_t ld, int64_t K, FwWork ws,
+                                                      int width, rbg_status* st) {
+  fw_panel_fast_body<T>(W, n, ld, K, ws, width, st, blockIdx.x);
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void k_fw_tile_fast(T* W, int64_t n, int64_t ld, int64_t K, FwWork ws,
+                                                     int width, rbg_status* st) {
+  fw_tile_fast_body<T, RPT, CPT>(W, n, ld, K, ws, width, st, blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// Phase 3, general mode (natural domain, full arithmetic and event keys).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(NT) void k_fw_tile_general(T* W, int64_t n, int64_t ld, int64_t K, FwWork ws,
+                                                        int width, int is_uns, rbg_status* st) {
+  __shared__ T As[BLK][TILE];
+  __shared__ T Bs[BLK][TILE];
+  const int tid = threadIdx.x;
+  const int nt = (int)(n / TILE);
+  const int I = blockIdx.x / nt, J = blockIdx.x % nt;
+  const int64_t k0 = K * BLK;
+  const int par = (int)(K & 1);
+  const T* colT = ws.col<T>(par);
+  const T* rowS = ws.row<T>(par);
+  for (int e = tid; e < BLK * TILE; e += NT) {
+    int kk = e / TILE, x = e % TILE;
+    As[kk][x] = colT[kk * n + (int64_t)I * TILE + x];
+    Bs[kk][x] = rowS[kk * n + (int64_t)J * TILE + x];
+  }
+  __syncthreads();
+  const int tr = tid >> 4, tc = tid & 15;
+  const int64_t r0 = (int64_t)I * TILE + tr * RPT;
+  const int64_t c0 = (int64_t)J * TILE + tc * CPT;
+  if ((r0 >= k0 && r0 < k0 + BLK) || (c0 >= k0 && c0 < k0 + BLK)) return;
+  T w[RPT][CPT];
+#pragma unroll
+  for (int r = 0; r < RPT; r++)
+#pragma unroll
+    for (int c = 0; c < CPT; c++) w[r][c] = W[(r0 + r) * ld + c0 + c];
+  int64_t dmax = 0;
+  for (int kk = 0; kk < BLK; kk++) {
+    const int64_t k = k0 + kk;
+#pragma unroll
+    for (int r = 0; r < RPT; r++) {
+      T a = As[kk][tr * RPT + r];
+#pragma unroll
+      for (int c = 0; c < CPT; c++) {
+        uint32_t ev = 0;
+        T p = avos_product<T>(a, Bs[kk][tc * CPT + c], width, is_uns != 0, ev);
+        if (ev) record_overflow(st, naive_key(k, r0 + r, c0 + c, n), ev);
+        if (r0 + r == c0 + c && !(p == (T)-1 || p == 0 || p == 1))
+          record_cycle(st, naive_key(k, r0 + r, r0 + r, n));
+        T v = avos_sum<T>(w[r][c], p);
+        w[r][c] = v;
+        if ((int64_t)v > dmax) dmax = (int64_t)v;
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < RPT; r++)
+#pragma unroll
+    for (int c = 0; c < CPT; c++) W[(r0 + r) * ld + c0 + c] = w[r][c];
+  dmax = wave_max_i64(dmax);
+  if ((tid & 63) == 0) record_diameter(st, dmax);
+}
+
+template <typename T, bool FAST>
+__global__ void k_fw_capture_last(rbg_status* st, FwWork ws, int64_t K, int64_t n) {
+  const int par = (int)(K & 1);
+  capture_block<T, FAST>(st, ws.col<T>(par), ws.row<T>(par), K * BLK, n);
+}
+
+}  // namespace rbg

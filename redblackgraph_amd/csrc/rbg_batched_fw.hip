@@ -1,0 +1,291 @@
+// Batched dense closure: many independent matrices ("components") closed by one
+// schedule of launches, recordable as a single hipGraph.
+//
+// Reference: the per-graph closure is warshall.c.in:50-107 (through
+// core/redblack.py:49-50); the reference closes components one at a time
+// (sparse/csgraph/transitive_closure.py:85-195 component_wise_closure).  Here
+// every component is padded to a multiple of 128 and stored contiguously in a
+// pool; components are ordered by padded order, largest first, so that at
+// pivot-block step K the components still running (npad / 32 > K) form a
+// prefix of every work list.  Step K is one panel launch and one tile launch
+// per tile shape, each over the work items of all running components, so a
+// launch carries the panels (or tiles) of thousands of components at once.
+// Arithmetic, snapshots, event keys and the MSB guard are the single-matrix
+// fast kernels' (fw_kernels.hpp), per component.  A component whose input is
+// outside the fast domain, or that needs general arithmetic or saw a cycle,
+// is flagged in its status; the caller reruns it through the single-matrix
+// path.
+#include <algorithm>
+#include <numeric>
+
+#include "fw_kernels.hpp"
+
+namespace rbg {
+
+struct BComp {
+  int32_t* W;
+  void* snap;
+  rbg_status* st;
+  int32_t npad;
+  int32_t tile;
+};
+
+struct Item {
+  int32_t comp;
+  int32_t bid;
+};
+
+constexpr int CHUNK = 128;  // precheck / domain conversion granule
+
+static __global__ void k_bfw_init(const BComp* comps, int32_t count) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= count) return;
+  rbg_status* st = comps[c].st;
+  st->events = 0;
+  st->flags = 0;
+  st->cycle_count = 0;
+  st->cycle_first_key = ~0ull;
+  st->cycle_last_key = 0;
+  st->ov_first_key = ~0ull;
+  st->ov_last_key = 0;
+  st->cycle_first_a = st->cycle_first_b = st->cycle_last_a = st->cycle_last_b = 0;
+  st->ov_last_a = st->ov_last_b = 0;
+  st->diameter_max = 0;
+  for (int i = 0; i < 8; i++) st->aux[i] = 0;
+  FwWork ws{comps[c].snap, comps[c].npad, 4};
+  int* g = ws.guard(0);
+  for (int q = 0; q < 4 * BLK; q++) g[q] = 0;
+}
+
+// Fast-domain precheck of one 128 x 128 chunk (rules of k_fw_precheck).
+static __global__ __launch_bounds__(NT) void k_bfw_precheck(const BComp* comps, const Item* items) {
+  __shared__ int sbad;
+  const Item it = items[blockIdx.x];
+  const BComp c = comps[it.comp];
+  const int nc = c.npad / CHUNK;
+  const int64_t i0 = (int64_t)(it.bid / nc) * CHUNK, j0 = (int64_t)(it.bid % nc) * CHUNK;
+  if (threadIdx.x == 0) sbad = 0;
+  __syncthreads();
+  bool bad = false;
+  for (int e = threadIdx.x; e < CHUNK * CHUNK; e += NT) {
+    const int64_t i = i0 + e / CHUNK, j = j0 + e % CHUNK;
+    const int32_t v = c.W[i * c.npad + j];
+    if (v < -1) bad = true;
+    if (i != j && (v == 1 || v == -1)) bad = true;
+    if (i == j && v > 1) bad = true;
+  }
+  if (bad) sbad = 1;
+  __syncthreads();
+  if (threadIdx.x == 0 && sbad) atomicOr(&c.st->flags, (uint32_t)RBG_F_PRECOND_FAIL);
+}
+
+template <bool ENC>
+static __global__ __launch_bounds__(NT) void k_bfw_convert(const BComp* comps, const Item* items) {
+  const Item it = items[blockIdx.x];
+  const BComp c = comps[it.comp];
+  const int nc = c.npad / CHUNK;
+  const int64_t i0 = (int64_t)(it.bid / nc) * CHUNK, j0 = (int64_t)(it.bid % nc) * CHUNK;
+  for (int e = threadIdx.x; e < CHUNK * CHUNK; e += NT) {
+    int32_t* p = c.W + (i0 + e / CHUNK) * c.npad + j0 + e % CHUNK;
+    *p = ENC ? e_enc<int32_t>(*p) : e_dec<int32_t>(*p);
+  }
+}
+
+static __global__ __launch_bounds__(NT) void k_bfw_panel(const BComp* comps, const Item* items, int64_t K) {
+  const Item it = items[blockIdx.x];
+  const BComp c = comps[it.comp];
+  FwWork ws{c.snap, c.npad, 4};
+  fw_panel_fast_body<int32_t>(c.W, c.npad, c.npad, K, ws, 32, c.st, it.bid);
+}
+
+template <int RC>
+static __global__ __launch_bounds__(NT) void k_bfw_tile(const BComp* comps, const Item* items, int64_t K) {
+  const Item it = items[blockIdx.x];
+  const BComp c = comps[it.comp];
+  FwWork ws{c.snap, c.npad, 4};
+  fw_tile_fast_body<int32_t, RC, RC>(c.W, c.npad, c.npad, K, ws, 32, c.st, it.bid);
+}
+
+// Operands of the last block's detections (the panel of the next block would
+// have copied them; there is none).
+static __global__ void k_bfw_capture(const BComp* comps, int32_t count) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= count) return;
+  const BComp b = comps[c];
+  const int64_t nb = b.npad / BLK;
+  const int par = (int)((nb - 1) & 1);
+  FwWork ws{b.snap, b.npad, 4};
+  capture_block<int32_t, true>(b.st, ws.col<int32_t>(par), ws.row<int32_t>(par), (nb - 1) * BLK, b.npad);
+}
+
+}  // namespace rbg
+
+using namespace rbg;
+
+struct rbg_batch {
+  int32_t count = 0;
+  int32_t maxnb = 0;
+  BComp* d_comps = nullptr;
+  Item* d_items = nullptr;  // chunks | panels | tiles 128 | tiles 64
+  int64_t n_chunk = 0, n_panel = 0, n_t128 = 0, n_t64 = 0;
+  std::vector<int64_t> panel_cnt, t128_cnt, t64_cnt;  // per step: items of running components
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+static int batch_launch(rbg_batch* b, hipStream_t s) {
+  const int cgrid = (b->count + 255) / 256;
+  hipLaunchKernelGGL(k_bfw_init, dim3(cgrid), dim3(256), 0, s, b->d_comps, b->count);
+  RBG_CHECK_LAUNCH();
+  const Item* chunks = b->d_items;
+  const Item* panels = chunks + b->n_chunk;
+  const Item* t128 = panels + b->n_panel;
+  const Item* t64 = t128 + b->n_t128;
+  hipLaunchKernelGGL(k_bfw_precheck, dim3(b->n_chunk), dim3(NT), 0, s, b->d_comps, chunks);
+  RBG_CHECK_LAUNCH();
+  hipLaunchKernelGGL((k_bfw_convert<true>), dim3(b->n_chunk), dim3(NT), 0, s, b->d_comps, chunks);
+  RBG_CHECK_LAUNCH();
+  for (int K = 0; K < b->maxnb; K++) {
+    hipLaunchKernelGGL(k_bfw_panel, dim3(b->panel_cnt[K]), dim3(NT), 0, s, b->d_comps, panels, (int64_t)K);
+    RBG_CHECK_LAUNCH();
+    int pi = prof_start(s);
+    if (b->t128_cnt[K]) {
+      hipLaunchKernelGGL((k_bfw_tile<8>), dim3(b->t128_cnt[K]), dim3(NT), 0, s, b->d_comps, t128, (int64_t)K);
+      RBG_CHECK_LAUNCH();
+    }
+    if (b->t64_cnt[K]) {
+      hipLaunchKernelGGL((k_bfw_tile<4>), dim3(b->t64_cnt[K]), dim3(NT), 0, s, b->d_comps, t64, (int64_t)K);
+      RBG_CHECK_LAUNCH();
+    }
+    prof_stop(4, pi, s);
+  }
+  hipLaunchKernelGGL(k_bfw_capture, dim3(cgrid), dim3(256), 0, s, b->d_comps, b->count);
+  RBG_CHECK_LAUNCH();
+  hipLaunchKernelGGL((k_bfw_convert<false>), dim3(b->n_chunk), dim3(NT), 0, s, b->d_comps, chunks);
+  RBG_CHECK_LAUNCH();
+  return 0;
+}
+
+extern "C" {
+
+int64_t rbg_batch_snap_bytes(int32_t npad) { return FwWork::bytes(npad, 4); }
+
+int rbg_batch_create(const rbg_batch_comp* comps, int32_t count, rbg_batch** out) {
+  if (!comps || count <= 0 || !out) return RBG_E_ARG;
+  *out = nullptr;
+  std::vector<BComp> c(count);
+  for (int i = 0; i < count; i++) {
+    const rbg_batch_comp& q = comps[i];
+    if (!q.W || !q.snap || !q.st || q.npad < CHUNK || q.npad % CHUNK) return RBG_E_ARG;
+    if (q.tile != 128 && q.tile != 64) return RBG_E_ARG;
+    c[i] = BComp{q.W, q.snap, q.st, q.npad, q.tile};
+  }
+  // largest first: the components running at step K are a prefix
+  std::stable_sort(c.begin(), c.end(), [](const BComp& a, const BComp& b) { return a.npad > b.npad; });
+  auto* b = new rbg_batch();
+  b->count = count;
+  b->maxnb = c[0].npad / BLK;
+  std::vector<Item> chunks, panels, t128, t64;
+  for (int i = 0; i < count; i++) {
+    const int nc = c[i].npad / CHUNK, nb = c[i].npad / BLK, nt = c[i].npad / c[i].tile;
+    for (int q = 0; q < nc * nc; q++) chunks.push_back({i, q});
+    for (int q = 0; q < 2 * (nb - 1); q++) panels.push_back({i, q});
+    auto& tl = c[i].tile == 128 ? t128 : t64;
+    for (int q = 0; q < nt * nt; q++) tl.push_back({i, q});
+  }
+  b->n_chunk = (int64_t)chunks.size();
+  b->n_panel = (int64_t)panels.size();
+  b->n_t128 = (int64_t)t128.size();
+  b->n_t64 = (int64_t)t64.size();
+  b->panel_cnt.assign(b->maxnb, 0);
+  b->t128_cnt.assign(b->maxnb, 0);
+  b->t64_cnt.assign(b->maxnb, 0);
+  for (int i = 0; i < count; i++) {
+    const int nb = c[i].npad / BLK, nt = c[i].npad / c[i].tile;
+    for (int K = 0; K < nb; K++) {
+      b->panel_cnt[K] += 2 * (nb - 1);
+      (c[i].tile == 128 ? b->t128_cnt : b->t64_cnt)[K] += (int64_t)nt * nt;
+    }
+  }
+  std::vector<Item> all;
+  all.reserve(chunks.size() + panels.size() + t128.size() + t64.size());
+  for (auto* v : {&chunks, &panels, &t128, &t64}) all.insert(all.end(), v->begin(), v->end());
+  hipError_t e = hipMalloc((void**)&b->d_comps, sizeof(BComp) * count);
+  if (e == hipSuccess) e = hipMalloc((void**)&b->d_items, sizeof(Item) * all.size());
+  if (e == hipSuccess) e = hipMemcpy(b->d_comps, c.data(), sizeof(BComp) * count, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(b->d_items, all.data(), sizeof(Item) * all.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(b->d_comps);
+    (void)hipFree(b->d_items);
+    delete b;
+    return (int)e;
+  }
+  *out = b;
+  return 0;
+}
+
+int rbg_batch_info(const rbg_batch* b, int64_t* info) {
+  if (!b || !info) return RBG_E_ARG;
+  int64_t launches = 5;
+  for (int K = 0; K < b->maxnb; K++) launches += 1 + (b->t128_cnt[K] > 0) + (b->t64_cnt[K] > 0);
+  info[0] = b->count;
+  info[1] = b->maxnb;
+  info[2] = launches;
+  info[3] = b->n_panel;
+  info[4] = b->n_t128;
+  info[5] = b->n_t64;
+  info[6] = b->n_chunk;
+  info[7] = b->exec != nullptr;
+  return 0;
+}
+
+int rbg_batch_run(rbg_batch* b, rbg_stream_t stream) {
+  if (!b) return RBG_E_ARG;
+  return batch_launch(b, (hipStream_t)stream);
+}
+
+int rbg_batch_capture(rbg_batch* b, rbg_stream_t stream) {
+  if (!b || !stream) return RBG_E_ARG;  // the legacy null stream cannot be captured
+  hipStream_t s = (hipStream_t)stream;
+  if (b->exec) {
+    (void)hipGraphExecDestroy(b->exec);
+    (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+  }
+  hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return (int)e;
+  int rc = batch_launch(b, s);
+  hipGraph_t g = nullptr;
+  e = hipStreamEndCapture(s, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return (int)e;
+  e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    return (int)e;
+  }
+  b->graph = g;
+  return 0;
+}
+
+int rbg_batch_replay(rbg_batch* b, rbg_stream_t stream) {
+  if (!b || !b->exec) return RBG_E_ARG;
+  hipError_t e = hipGraphLaunch(b->exec, (hipStream_t)stream);
+  return e == hipSuccess ? 0 : (int)e;
+}
+
+int rbg_batch_destroy(rbg_batch* b) {
+  if (!b) return 0;
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  (void)hipFree(b->d_comps);
+  (void)hipFree(b->d_items);
+  delete b;
+  return 0;
+}
+
+}  // extern "C"

@@ -101,7 +101,10 @@ def fw_closure(W, et: ElemType, stream=None, force_general: bool = False):
     _run(work, et, 0 if fast else 1, st, stream)
     s = st.read(stream)
     mode = "fast" if fast else "general"
-    if fast and (s.flags & _lib.F_NEED_GENERAL):
+    # A cycle makes a diagonal value >= 2, after which products through the
+    # pivot cell (skipped by the fast kernels) can raise events of their own:
+    # the error path reruns in general mode, which records every event.
+    if fast and (s.flags & _lib.F_NEED_GENERAL or s.cycle_count):
         work = fresh()
         st.reset(stream)
         _run(work, et, 1, st, stream)

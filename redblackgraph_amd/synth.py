@@ -109,10 +109,34 @@ def component_spec(size: int, p_parent: float = 0.8, lookback: int = 3) -> Pedig
     return PedigreeSpec(int(size), max(1, math.ceil(math.log2(max(2, size)))), p_parent, lookback)
 
 
+def component_seed(seed: int, i: int) -> int:
+    return seed * 1_000_003 + i
+
+
 def batched_components(count: int = 4096, lo: int = 16, hi: int = 4096, seed: int = 0, dtype=np.int32):
     """List of dense component matrices for the batched workload."""
     sizes = batched_sizes(count, lo, hi, seed)
-    return [dense_pedigree(component_spec(s), seed * 1_000_003 + i, dtype) for i, s in enumerate(sizes)]
+    return [dense_pedigree(component_spec(s), component_seed(seed, i), dtype) for i, s in enumerate(sizes)]
+
+
+def component_edges(spec: PedigreeSpec, seed: int):
+    """(diagonal[n] of -1/1, (child, parent, value)) of one pedigree: the same
+    graph dense_pedigree(spec, seed) builds, without the n x n matrix."""
+    is_red, child, parent, value = pedigree_edges(spec, seed)
+    return np.where(is_red, -1, 1).astype(np.int32), (child, parent, value)
+
+
+def batched_workload(count: int = 4096, lo: int = 16, hi: int = 4096, seed: int = 0):
+    """Sizes, diagonals and edge lists of the batched workload (4096
+    components, sizes log-uniform in [16, 4096], layers = ceil(log2(size)),
+    each parent known with p=0.8 from the previous 1-3 layers)."""
+    sizes = batched_sizes(count, lo, hi, seed)
+    diag, edges = [], []
+    for i, s in enumerate(sizes):
+        d, e = component_edges(component_spec(int(s)), component_seed(seed, i))
+        diag.append(d)
+        edges.append(e)
+    return sizes, diag, edges
 
 
 def random_upper_dag(n: int, seed: int, hi: int = 4) -> np.ndarray:

@@ -99,3 +99,17 @@ def dense_triple_loop(A, B, prod, add):
                 acc = add(acc, prod(int(A[i, j]), int(B[j, k])))
             C[i, k] = acc
     return C
+
+
+def overflow_chain(n, dt):
+    """A paternal chain of n vertices whose middle vertex has the largest
+    index: the naive loop meets MSB(x) + MSB(y) = width only at the last step,
+    with both operands still small, so the reference raises OverflowError
+    (instead of first storing a wrapped negative it would later loop on)."""
+    mid = (n - 1) // 2
+    idx = lambda t: t if t < mid else (n - 1 if t == mid else t - 1)
+    A = np.zeros((n, n), dtype=dt)
+    np.fill_diagonal(A, -1)
+    for t in range(n - 1):
+        A[idx(t + 1), idx(t)] = 2
+    return A

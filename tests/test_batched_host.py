@@ -1,0 +1,34 @@
+"""Host-side logic of the batched closure: pool layout and the autotuner's
+cache (no GPU calls)."""
+import json
+
+import numpy as np
+
+from redblackgraph_amd import autotune
+from redblackgraph_amd.batched import BatchLayout
+
+
+def test_layout_offsets_and_padding():
+    sizes = [16, 129, 4096, 300, 128]
+    L = BatchLayout.build(sizes)
+    assert list(L.npad) == [128, 256, 4096, 384, 128]
+    ends = L.offset + L.npad ** 2
+    assert L.offset[0] == 0 and np.all(L.offset[1:] == ends[:-1]) and L.pool_elems == ends[-1]
+    assert np.all(L.offset % (128 * 128) == 0)
+    assert np.all(L.snap_offset % 256 == 0)
+    assert np.all(np.diff(L.snap_offset) >= 4 * 32 * L.npad[:-1] * 4)
+
+
+def test_tuning_batch_sizes():
+    assert autotune.tuning_components(128) == 256
+    assert autotune.tuning_components(4096) == 1
+    assert autotune.tuning_components(1024) == 16
+
+
+def test_tile_plan_reads_cache(tmp_path, monkeypatch):
+    cache = tmp_path / "tune.json"
+    dev = "TestGPU|gfx950|256cu"
+    cache.write_text(json.dumps({f"{dev}|batched_fw_tile|256": {"tile": 64, "ms": {}}}))
+    monkeypatch.setenv("RBG_AUTOTUNE_CACHE", str(cache))
+    plan = autotune.tile_plan([256, 512, 256], measure=False, dev=dev)
+    assert plan == {256: 64, 512: 128}

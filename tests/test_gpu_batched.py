@@ -1,0 +1,115 @@
+"""GPU parity of the batched dense closure: every component equals the CPU
+oracle's naive loop (W and diameter) and the single-matrix GPU path, for both
+tile shapes, eagerly and through hipGraph replays; components outside the fast
+domain fall back to the general kernels with the single-matrix errors."""
+import numpy as np
+import pytest
+
+from redblackgraph_amd import _lib, dense, synth
+from redblackgraph_amd._dtypes import elem_type
+from redblackgraph_amd.batched import BatchedClosure, transitive_closure_batch
+from helpers import overflow_chain
+
+pytestmark = pytest.mark.gpu
+
+
+def _components(count, lo, hi, seed):
+    sizes = synth.batched_sizes(count, lo, hi, seed)
+    return [synth.dense_pedigree(synth.component_spec(int(s)), synth.component_seed(seed, i))
+            for i, s in enumerate(sizes)]
+
+
+@pytest.mark.parametrize("tile", [128, 64])
+def test_batched_vs_oracle(orc, tile):
+    mats = _components(40, 16, 400, 1)
+    b = BatchedClosure([A.shape[0] for A in mats], tiles={p: tile for p in range(128, 4097, 128)})
+    try:
+        b.load_dense(mats)
+        b.run(graph=False)
+        outs = b.outcomes()
+        for i, A in enumerate(mats):
+            ref = orc.warshall(A)
+            assert outs[i].mode == "fast"
+            np.testing.assert_array_equal(b.result(i).cpu().numpy(), ref.W, err_msg=f"component {i}")
+            assert outs[i].diameter == ref.diameter, i
+            assert outs[i].status.cycle_count == 0
+    finally:
+        b.close()
+
+
+def test_batched_equals_single_matrix_path_with_graph_replays():
+    """A few hundred components up to 2048: the batched result equals
+    dense.fw_closure per component; graph replays, with eager work and
+    synchronisations between them, reproduce the eager run every time."""
+    t = _lib.torch()
+    sizes, diag, edges = synth.batched_workload(count=160, lo=16, hi=2048, seed=3)
+    b = BatchedClosure(sizes)
+    try:
+        b.load_edges(diag, edges)
+        b.run(graph=False)
+        b.stream.synchronize()  # run() is asynchronous on the batch's stream
+        eager = b.pool.clone()
+        st0 = [(s.diameter_max, s.cycle_count, s.flags) for s in b.statuses()]
+        et = elem_type(np.int32)
+        for i in range(0, len(sizes), 7):
+            n = int(sizes[i])
+            A = t.zeros((n, n), dtype=t.int32, device=b.pool.device)
+            r, c, v = edges[i]
+            A[t.from_numpy(r).to(A.device), t.from_numpy(c).to(A.device)] = t.from_numpy(v.astype(np.int32)).to(A.device)
+            A.diagonal().copy_(t.from_numpy(diag[i]).to(A.device))
+            W, out = dense.fw_closure(A, et)
+            assert bool(t.equal(W, b.result(i))), i
+            assert out.status.diameter_max == st0[i][0]
+        for rep in range(3):
+            b.load_edges(diag, edges)
+            b.run(graph=True)
+            b.stream.synchronize()
+            assert bool(t.equal(b.pool, eager)), f"replay {rep}"
+            assert [(s.diameter_max, s.cycle_count, s.flags) for s in b.statuses()] == st0
+            _ = (t.arange(1000, device=b.pool.device) * 3).sum().item()  # unrelated eager work
+    finally:
+        b.close()
+
+
+def test_batched_fallbacks_and_errors(orc):
+    """Components the fast kernels cannot close (an off-diagonal 1, a cycle, a
+    value chain that overflows int32) come back through the general kernels
+    with the single-matrix path's results and errors."""
+    good = _components(6, 16, 300, 5)
+    off1 = good[1].copy()
+    off1[5, 2] = 1
+    cyc = good[2].copy()
+    i, j = np.nonzero(np.tril(cyc, -1))
+    cyc[j[0], i[0]] = 2
+    chain = overflow_chain(33, np.int32)
+    mats = [good[0], off1, cyc, chain, good[4], good[5]]
+    b = BatchedClosure([A.shape[0] for A in mats])
+    try:
+        b.load_dense(mats)
+        b.run(graph=False)
+        sts = b.statuses()
+        assert sts[1].flags & _lib.F_PRECOND_FAIL
+        assert sts[2].cycle_count > 0 or sts[2].flags & _lib.F_NEED_GENERAL
+        assert sts[3].flags & _lib.F_NEED_GENERAL
+        outs = b.outcomes(lambda q: _lib.to_device(mats[q]))
+        et = elem_type(np.int32)
+        for q, A in enumerate(mats):
+            W, single = dense.fw_closure(_lib.to_device(A), et)
+            np.testing.assert_array_equal(b.result(q).cpu().numpy(), W.cpu().numpy(), err_msg=f"component {q}")
+            assert outs[q].mode == single.mode
+            assert (outs[q].status.cycle_count, outs[q].status.cycle_last_key, outs[q].status.events) == \
+                (single.status.cycle_count, single.status.cycle_last_key, single.status.events)
+            if q not in (2, 3):
+                ref = orc.warshall(A)
+                np.testing.assert_array_equal(W.cpu().numpy(), ref.W)
+                assert outs[q].diameter == ref.diameter
+        assert [o.mode for o in outs] == ["fast", "general", "general", "general", "fast", "fast"]
+    finally:
+        b.close()
+    with pytest.raises(ValueError, match="Error: cycle detected!"):
+        transitive_closure_batch(mats)
+    with pytest.raises(OverflowError):
+        transitive_closure_batch([good[0], chain])
+    res = transitive_closure_batch([good[0], off1])
+    np.testing.assert_array_equal(res[1].W, orc.warshall(off1).W)
+    assert res[0].diameter == orc.warshall(good[0]).diameter

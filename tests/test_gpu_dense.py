@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 import redblackgraph_amd as rb
+from helpers import overflow_chain
 from redblackgraph_amd import dense, synth, _lib
 from redblackgraph_amd._dtypes import elem_type
 
@@ -170,6 +171,40 @@ def test_fw_cyclic_pedigree_errors(orc):
     assert e.value.vertex == ref["first_i"]
 
 
+def _pivot_cases():
+    # vertex 1 has an absent diagonal and a 2-cycle with vertex 0: the naive
+    # loop's last detection is the pivot cell (k=1, i=1), W[1][1] (x) W[1][1]
+    two = np.zeros((6, 6), dtype=np.int32)
+    np.fill_diagonal(two, [-1, 0, 1, -1, 1, -1])
+    two[0, 1], two[1, 0], two[2, 3] = 2, 3, 2
+    # a diagonal value >= 2 in the input is a cycle at the very first step
+    diag = synth.dense_pedigree(synth.PedigreeSpec(150, 6, 0.8, 2), 3)
+    diag[7, 7] = 2
+    return {"two_cycle_absent_diag": two, "diag_two": diag}
+
+
+@pytest.mark.parametrize("case", ["two_cycle_absent_diag", "diag_two"])
+def test_fw_pivot_cell_detections(orc, case):
+    """Detections made by products through the pivot cell (k, k): the fast
+    kernels skip that row and column, so these inputs must come out of the
+    general kernels with every event of the naive loop."""
+    A = _pivot_cases()[case]
+    ref = orc.warshall(A)
+    rep = ref.report
+    assert rep["n_cycle"] > 0
+    W, out = _fw(A)
+    s = out.status
+    assert out.mode == "general"
+    np.testing.assert_array_equal(W, ref.W)
+    assert s.cycle_count == rep["n_cycle"]
+    assert out.key_kij(s.cycle_first_key)[:2] == (rep["first_k"], rep["first_i"])
+    assert out.key_kij(s.cycle_last_key)[:2] == (rep["last_k"], rep["last_i"])
+    assert (s.cycle_first_a, s.cycle_first_b) == (rep["first_a"], rep["first_b"])
+    assert (s.cycle_last_a, s.cycle_last_b) == (rep["last_a"], rep["last_b"])
+    if case == "two_cycle_absent_diag":
+        assert (rep["last_k"], rep["last_i"]) == (1, 1)
+
+
 def test_fw_1024_vs_oracle(orc):
     A = synth.dense_pedigree(synth.PedigreeSpec(1024, 16, 0.75, 4), 11)
     ref = orc.warshall(A)
@@ -274,23 +309,9 @@ def test_edge_composition_equals_reclosure(orc):
     assert done > 0
 
 
-def _overflow_chain(n, dt):
-    """A paternal chain of n vertices whose middle vertex has the largest
-    index: the naive loop meets MSB(x) + MSB(y) = width only at the last step,
-    with both operands still small, so the reference raises OverflowError
-    (instead of first storing a wrapped negative it would later loop on)."""
-    mid = (n - 1) // 2
-    idx = lambda t: t if t < mid else (n - 1 if t == mid else t - 1)
-    A = np.zeros((n, n), dtype=dt)
-    np.fill_diagonal(A, -1)
-    for t in range(n - 1):
-        A[idx(t + 1), idx(t)] = 2
-    return A
-
-
 @pytest.mark.parametrize("dt,n", [(np.int32, 33), (np.int8, 9), (np.int64, 65)])
 def test_fw_overflow_chain(orc, dt, n):
-    A = _overflow_chain(n, dt)
+    A = overflow_chain(n, dt)
     rep = orc.warshall(A).report
     assert rep["n_overflow"] > 0 and not rep["hang"] and rep["last_is_overflow"]
     with pytest.raises(OverflowError):
