@@ -351,6 +351,80 @@ def bench_batched(args):
     return res
 
 
+def bench_serve(args):
+    """Served closure stream: Poisson arrivals, dense int32 below 4096
+    vertices and CSR int64 above, 1-64 edge insertions into resident graphs;
+    value = closure entries produced per second of makespan."""
+    import numpy as np
+    from redblackgraph_amd import _lib, dense, synth
+    from redblackgraph_amd._dtypes import elem_type
+    from redblackgraph_amd.serving import ClosureServer, summarize, synthetic_trace
+    t = _lib.torch()
+    trace = synthetic_trace(count=args.serve_count, rate=args.serve_rate, seed=args.seed)
+    warm = synthetic_trace(count=12, rate=1000.0, seed=args.seed + 1000)
+    srv = ClosureServer(hbm_budget=args.serve_budget_gib << 30, slots=args.serve_slots)
+    try:
+        srv.serve(warm)
+        srv.residents.clear()
+        srv.used = 0
+        t.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = srv.serve(trace)
+        t.cuda.synchronize()
+        span = time.perf_counter() - t0
+    finally:
+        srv.close()
+    summ = summarize(res, span)
+    # edge insertion roofline: one 64-edge insertion into a resident N=4096 closure
+    n = 4096
+    A = synth.dense_pedigree(synth.component_spec(n), 77)
+    et = elem_type(np.int32)
+    W, _ = dense.fw_closure(_lib.to_device(A), et)
+    W = W.contiguous()
+    rng = np.random.default_rng(1)
+    diag = np.diagonal(A)
+    edges = []
+    for _ in range(64):
+        a = int(rng.integers(n // 2, n))
+        rel = int(rng.choice([2, 3]))
+        cand = np.nonzero(diag[:a] == (-1 if rel == 2 else 1))[0]
+        edges.append((a, int(rng.choice(cand)), rel))
+    ev0, ev1 = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
+    V = W
+    for a, b, rel in edges[:4]:
+        V, _ = dense.edge_compose(V, a, b, rel, et)
+    t.cuda.synchronize()
+    V = W
+    ev0.record()
+    for a, b, rel in edges:
+        V, _ = dense.edge_compose(V, a, b, rel, et)
+    ev1.record()
+    ev1.synchronize()
+    per_edge_s = ev0.elapsed_time(ev1) / 1e3 / len(edges)
+    b_edge = 8 * n * n
+    roof = {"bound": "hbm", "kernel": "edge insertion (k_edge_row + k_edge_update), N=4096 int32",
+            "achieved": b_edge / per_edge_s / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+            "frac": b_edge / per_edge_s / PEAK_HBM, "traffic": None,
+            "algorithmic_bytes_per_edge": b_edge, "ms_per_edge": per_edge_s * 1e3}
+    closes = [r for r in res if r.kind == "close" and r.error is None]
+    return {
+        "metric": METRIC, "value": summ["entries_per_s"], "unit": "entries/s",
+        "n_gpus": 1, "steps": len(trace), "warmup": len(warm), "ms_per_step": span * 1e3 / max(1, len(trace)),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32/int64",
+        "data": "synthetic",
+        "config": {"workload": (f"served closure stream: {len(trace)} requests, Poisson arrivals at "
+                                f"{args.serve_rate}/s, new pedigrees (size log-uniform in [32, 16384]; dense int32 "
+                                f"below 4096, CSR int64 above) or 1-64 parent-edge insertions into a resident "
+                                f"closure (p=0.35); {args.serve_slots} slots, HBM budget {args.serve_budget_gib} GiB, "
+                                f"seed {args.seed}"),
+                   **{k: v for k, v in summ.items()},
+                   "dense_closes": sum(1 for r in closes if r.n < 4096),
+                   "sparse_closes": sum(1 for r in closes if r.n >= 4096),
+                   "peak_reserved_gib": srv.peak / 2 ** 30, "evictions": srv.evictions},
+        "roofline": roof,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -358,7 +432,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--workload", default="dense8192",
-                    choices=["dense8192", "dense4096_i64", "dense512", "sparse1m", "batched"])
+                    choices=["dense8192", "dense4096_i64", "dense512", "sparse1m", "batched", "serve"])
+    ap.add_argument("--serve-count", type=int, default=300)
+    ap.add_argument("--serve-rate", type=float, default=150.0)
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-budget-gib", type=int, default=64)
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--cpu-sample", type=int, default=1024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -371,6 +449,8 @@ def main():
         res = bench_dense(args, 8192, "int32")
     elif args.workload == "sparse1m":
         res = bench_sparse(args)
+    elif args.workload == "serve":
+        res = bench_serve(args)
     elif args.workload == "batched":
         res = bench_batched(args)
     elif args.workload == "dense4096_i64":

@@ -146,28 +146,115 @@ __global__ void k_matvec(const T* __restrict__ R, int64_t n, const T* __restrict
   if (lane == 0) out[a * out_stride] = r;
 }
 
-// Edge composition, step 2 (relational_composition.c.in:110-137):
-// out[m][p] = m == alpha ? u[p] : (u[p] ? R[m][p] (+) R[m][alpha] (x) u[p] : R[m][p])
+// Keys for combining AVOS sums with an unsigned atomicMin: RED_ONE (-1) maps
+// to 0 (absorbs every sum), "absent" (0) to all ones (loses every sum), any
+// other value to itself read as unsigned (the sum of two such values is their
+// unsigned minimum).
 template <typename T>
-__global__ void k_edge_update(const T* __restrict__ R, int64_t n, int64_t alpha, const T* __restrict__ u,
-                              T* __restrict__ out, int width, int is_uns, rbg_status* st) {
-  int64_t total = n * n;
-  uint32_t evs = 0;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-    int64_t m = e / n, p = e % n;
-    T up = u[p];
-    T rv = R[e];
-    T o;
-    if (m == alpha) {
-      o = up;
-    } else if (up != 0) {
-      uint32_t ev = 0;
-      o = avos_sum<T>(rv, avos_product<T>(R[m * n + alpha], up, width, is_uns != 0, ev));
-      evs |= ev;
-    } else {
-      o = rv;
+__device__ inline typename unsigned_of<T>::type sum_key(T v) {
+  using U = typename unsigned_of<T>::type;
+  return v == (T)-1 ? (U)0 : (v == 0 ? ~(U)0 : (U)v);
+}
+template <typename T>
+__device__ inline T sum_unkey(typename unsigned_of<T>::type k) {
+  using U = typename unsigned_of<T>::type;
+  return k == 0 ? (T)-1 : (k == ~(U)0 ? (T)0 : (T)k);
+}
+
+// Edge composition, step 1 (relational_composition.c.in:52-69, 92-101): the
+// composed row u = lhs (x) R, lhs = row alpha of R with entry beta replaced by
+// the relationship.  u is accumulated as sum keys in row alpha of the output.
+// k_edge_begin also flags a cycle when R[beta][alpha] != 0 (the loop check of
+// core/redblack.py:93-95).
+template <typename T>
+__global__ void k_edge_begin(const T* __restrict__ R, int64_t n, int64_t alpha, int64_t beta,
+                             typename unsigned_of<T>::type* __restrict__ ukey, rbg_status* st) {
+  using U = typename unsigned_of<T>::type;
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x)
+    ukey[p] = ~(U)0;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && R[beta * n + alpha] != 0) atomicOr(&st->flags, (uint32_t)RBG_F_CYCLE);
+}
+
+constexpr int EC_ROWS = 256;  // rows of R per workgroup in the composed-row product
+
+// Workgroup (x, y): columns [256x, 256x + 256), rows [256y, 256y + 256) of R.
+// The nonzero left operands of the row chunk are gathered in LDS (pedigree
+// rows are sparse: few ancestors), then every lane folds its column.
+template <typename T>
+__global__ __launch_bounds__(256) void k_edge_row(const T* __restrict__ R, int64_t n, int64_t alpha, int64_t beta,
+                                                  T np, typename unsigned_of<T>::type* __restrict__ ukey, int width,
+                                                  int is_uns, rbg_status* st) {
+  __shared__ int32_t s_idx[EC_ROWS];
+  __shared__ T s_lhs[EC_ROWS];
+  __shared__ int s_cnt;
+  const int tid = threadIdx.x;
+  if (tid == 0) s_cnt = 0;
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.y * EC_ROWS + tid;
+  if (i < n) {
+    T lhs = R[alpha * n + i];
+    if (np != 0 && i == beta) lhs = np;
+    if (lhs != 0) {
+      int q = atomicAdd(&s_cnt, 1);
+      s_idx[q] = (int32_t)i;
+      s_lhs[q] = lhs;
     }
-    out[e] = o;
+  }
+  __syncthreads();
+  const int cnt = s_cnt;
+  if (cnt == 0) return;
+  const int64_t p = (int64_t)blockIdx.x * 256 + tid;
+  if (p >= n) return;
+  T acc = 0;
+  uint32_t evs = 0;
+  for (int k = 0; k < cnt; k++) {
+    uint32_t ev = 0;
+    acc = avos_sum<T>(acc, avos_product<T>(s_lhs[k], R[(int64_t)s_idx[k] * n + p], width, is_uns != 0, ev));
+    evs |= ev;
+  }
+  if (acc != 0) atomicMin(&ukey[p], sum_key<T>(acc));
+  if (evs) atomicOr(&st->events, evs);
+}
+
+template <typename T>
+__global__ void k_edge_row_end(typename unsigned_of<T>::type* __restrict__ ukey, int64_t n) {
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x)
+    ukey[p] = (typename unsigned_of<T>::type)sum_unkey<T>(ukey[p]);
+}
+
+// Edge composition, step 2 (relational_composition.c.in:110-137):
+// out[m][p] = u[p] ? R[m][p] (+) R[m][alpha] (x) u[p] : R[m][p] for m != alpha
+// (row alpha already holds u).  One workgroup row per matrix row, four
+// consecutive columns per lane.
+template <typename T>
+__global__ __launch_bounds__(256) void k_edge_update(const T* __restrict__ R, int64_t n, int64_t alpha,
+                                                     T* __restrict__ out, int width, int is_uns, rbg_status* st) {
+  const T* u = out + alpha * n;
+  uint32_t evs = 0;
+  for (int64_t m = blockIdx.y; m < n; m += gridDim.y) {
+    if (m == alpha) continue;
+    const T rma = R[m * n + alpha];
+    const T* Rm = R + m * n;
+    T* Om = out + m * n;
+    for (int64_t p0 = ((int64_t)blockIdx.x * 256 + threadIdx.x) * 4; p0 < n; p0 += (int64_t)gridDim.x * 256 * 4) {
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int64_t p = p0 + q;
+        if (p < n) {
+          const T rv = Rm[p];
+          T o = rv;
+          if (rma != 0) {
+            const T up = u[p];
+            if (up != 0) {
+              uint32_t ev = 0;
+              o = avos_sum<T>(rv, avos_product<T>(rma, up, width, is_uns != 0, ev));
+              evs |= ev;
+            }
+          }
+          Om[p] = o;
+        }
+      }
+    }
   }
   if (evs) atomicOr(&st->events, evs);
 }
@@ -228,13 +315,20 @@ static int matmul(const T* A, int64_t lda, const T* B, int64_t ldb, T* C, int64_
 template <typename T>
 static int edge_compose(const T* R, int64_t n, int64_t alpha, int64_t beta, T rel, T* out, int width, int is_uns,
                         rbg_status* st, hipStream_t s) {
-  if (!R || !out || !st || n <= 0 || alpha < 0 || alpha >= n || beta < 0 || beta >= n) return RBG_E_ARG;
-  // row alpha of the output is the composed row; computed first into out[alpha]
-  hipLaunchKernelGGL((k_vecmat<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, R, n, alpha, beta, rel,
-                     (const T*)nullptr, out + alpha * n, width, is_uns, st);
+  if (!R || !out || !st || n <= 0 || alpha < 0 || alpha >= n || beta < 0 || beta >= n || R == out) return RBG_E_ARG;
+  using U = typename unsigned_of<T>::type;
+  U* ukey = (U*)(out + alpha * n);  // row alpha of the output holds u
+  const unsigned cols = (unsigned)((n + 255) / 256);
+  hipLaunchKernelGGL((k_edge_begin<T>), dim3(cols), dim3(256), 0, s, R, n, alpha, beta, ukey, st);
   RBG_CHECK_LAUNCH();
-  hipLaunchKernelGGL((k_edge_update<T>), dim3((unsigned)ew_grid(n * n)), dim3(EW_NT), 0, s, R, n, alpha,
-                     (const T*)(out + alpha * n), out, width, is_uns, st);
+  hipLaunchKernelGGL((k_edge_row<T>), dim3(cols, (unsigned)((n + EC_ROWS - 1) / EC_ROWS)), dim3(256), 0, s, R, n,
+                     alpha, beta, rel, ukey, width, is_uns, st);
+  RBG_CHECK_LAUNCH();
+  hipLaunchKernelGGL((k_edge_row_end<T>), dim3(cols), dim3(256), 0, s, ukey, n);
+  RBG_CHECK_LAUNCH();
+  const unsigned gx = (unsigned)((n + 1023) / 1024);
+  const unsigned gy = (unsigned)(n < 65535 ? n : 65535);
+  hipLaunchKernelGGL((k_edge_update<T>), dim3(gx, gy), dim3(256), 0, s, R, n, alpha, out, width, is_uns, st);
   RBG_CHECK_LAUNCH();
   return 0;
 }

@@ -12,19 +12,24 @@ from . import _lib
 from ._dtypes import ElemType, msb
 
 FW_TILE = 128
-_ws_cache: dict = {}
+_tls = threading.local()
 
 
 def _workspace(npad: int, elem: int):
+    """The calling thread's snapshot workspace (one per thread and device:
+    threads run closures on their own streams and must not share it)."""
     t = _lib.torch()
     dev = _lib.device()
+    cache = getattr(_tls, "ws_cache", None)
+    if cache is None:
+        cache = _tls.ws_cache = {}
     key = (dev.index, npad, elem)
-    ws = _ws_cache.get(key)
+    ws = cache.get(key)
     if ws is None:
         nbytes = int(_lib.load().rbg_fw_workspace_bytes(npad, elem))
         ws = t.empty(nbytes, dtype=t.uint8, device=dev)
-        _ws_cache.clear()  # keep one workspace per device
-        _ws_cache[key] = ws
+        cache.clear()
+        cache[key] = ws
     return ws
 
 
@@ -55,9 +60,6 @@ def _run(work, et: ElemType, mode: int, st: _lib.Status, stream):
     fn = "rbg_fw_dense_i32" if work.dtype == t.int32 else "rbg_fw_dense_i64"
     _lib.call(fn, _lib.ptr(work), npad, npad, et.width, int(et.unsigned), mode,
               _lib.ptr(ws), ws.numel(), st.p, _lib.stream_handle(stream))
-
-
-_tls = threading.local()
 
 
 def status() -> _lib.Status:
@@ -140,6 +142,28 @@ def edge_compose(R, alpha: int, beta: int, rel: int, et: ElemType, stream=None):
     _lib.call(fn, _lib.ptr(R.contiguous()), n, alpha, beta, int(rel), _lib.ptr(out), et.width,
               int(et.unsigned), st.p, _lib.stream_handle(stream))
     return out, st.read(stream)
+
+
+def edge_compose_many(R, edges, et: ElemType, stream=None):
+    """Apply edge compositions one after another without host round trips:
+    each edge reads the previous result (two device buffers alternate; R is
+    not modified).  Returns (closure, status); status.flags has RBG_F_CYCLE
+    when some edge would have closed a cycle, and the result is then
+    meaningless."""
+    t = _lib.torch()
+    n = R.shape[0]
+    st = status()
+    st.reset(stream)
+    fn = "rbg_edge_compose_i32" if R.dtype == t.int32 else "rbg_edge_compose_i64"
+    sh = _lib.stream_handle(stream)
+    cur = R.contiguous()
+    bufs = [t.empty_like(cur), t.empty_like(cur)]
+    for q, (a, b, rel) in enumerate(edges):
+        out = bufs[q & 1]
+        _lib.call(fn, _lib.ptr(cur), n, int(a), int(b), int(rel), _lib.ptr(out), et.width, int(et.unsigned),
+                  st.p, sh)
+        cur = out
+    return cur, st.read(stream)
 
 
 def vertex_compose(u, R, v, color: int, et: ElemType, stream=None):

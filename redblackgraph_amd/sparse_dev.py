@@ -258,3 +258,47 @@ def csr_equal(A: DevCSR, B: DevCSR) -> bool:
         return True
     return bool(t.equal(A.indptr, B.indptr)) and bool(t.equal(A.indices, B.indices)) and \
         bool(t.equal(A.data, B.data.to(A.data.dtype)))
+
+
+def edge_insert(C: DevCSR, alpha: int, beta: int, relationship: int, stream=None) -> DevCSR:
+    """Closure of C with the edge alpha -> beta (value ``relationship``) added,
+    for a closed canonical device CSR C: the sparse form of the edge
+    relational composition (core/redblack.py:74-97,
+    relational_composition.c.in:71-137).
+
+    u = (row alpha of C with entry beta set to the relationship) (x) C is the
+    new row alpha; every row m then becomes C[m] (+) C[m][alpha] (x) u (the
+    row alpha itself included: its diagonal +-1 leaves u unchanged).  Both
+    steps run on the GPU as AVOS SpGEMMs and one CSR AVOS sum.  Raises
+    ValueError when C[beta][alpha] != 0 (the edge would close a cycle).
+    """
+    t = _lib.torch()
+    n = C.n_rows
+    if not (0 <= alpha < n and 0 <= beta < n):
+        raise ValueError("vertex index out of range")
+    dev = C.indices.device
+    vdt = C.data.dtype
+    b0, b1 = int(C.indptr[beta].item()), int(C.indptr[beta + 1].item())
+    if bool((C.indices[b0:b1] == alpha).any()):
+        raise ValueError("Relational composition would result in a cycle.")
+    a0, a1 = int(C.indptr[alpha].item()), int(C.indptr[alpha + 1].item())
+    cols, vals = C.indices[a0:a1], C.data[a0:a1].clone()
+    hit = cols == beta
+    if bool(hit.any()):
+        vals[hit] = relationship
+    else:
+        cols = t.cat([cols, t.tensor([beta], dtype=t.int32, device=dev)])
+        vals = t.cat([vals, t.tensor([relationship], dtype=vdt, device=dev)])
+        order = t.argsort(cols)
+        cols, vals = cols[order], vals[order]
+    row = DevCSR(t.tensor([0, cols.numel()], dtype=t.int64, device=dev), cols.contiguous(), vals.contiguous(), (1, n))
+    u, _ = spgemm(row, C, stream=stream)
+    # column alpha of C as an n x 1 matrix
+    pos = t.nonzero(C.indices == alpha).flatten()
+    rows = t.searchsorted(C.indptr, pos, right=True) - 1
+    xptr = t.zeros(n + 1, dtype=t.int64, device=dev)
+    xptr[rows + 1] = 1
+    xptr = t.cumsum(xptr, 0)
+    X = DevCSR(xptr, t.zeros(pos.numel(), dtype=t.int32, device=dev), C.data[pos].contiguous(), (n, 1))
+    R1, _ = spgemm(X, u, stream=stream)
+    return avos_sum_csr(C, R1, stream=stream)

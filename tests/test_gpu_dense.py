@@ -309,6 +309,39 @@ def test_edge_composition_equals_reclosure(orc):
     assert done > 0
 
 
+@pytest.mark.parametrize("dt,n", [(np.int32, 1100), (np.int64, 700)])
+def test_edge_compose_many_equals_reclosure(orc, dt, n):
+    """A run of edge insertions on the device (no host round trips) equals
+    closing the graph with every edge added; an edge that would close a cycle
+    flags the run."""
+    A = synth.dense_pedigree(synth.component_spec(n), 4, dtype=dt)
+    et = elem_type(A.dtype)
+    W, _ = dense.fw_closure(_lib.to_device(A), et)
+    rng = np.random.default_rng(8)
+    diag = np.diagonal(A)
+    edges = []
+    A2 = A.copy()
+    for _ in range(16):
+        a = int(rng.integers(n // 2, n))
+        rel = int(rng.choice([2, 3]))
+        cand = np.nonzero(diag[:a] == (-1 if rel == 2 else 1))[0]
+        b = int(rng.choice(cand))
+        edges.append((a, b, rel))
+        A2[a, b] = rel
+    got, st = dense.edge_compose_many(W.contiguous(), edges, et)
+    assert not (st.flags & _lib.F_CYCLE) and st.events == 0
+    if n <= 700:
+        exp = orc.warshall(A2).W
+    else:
+        exp = dense.fw_closure(_lib.to_device(A2), et)[0].cpu().numpy()
+    np.testing.assert_array_equal(got.cpu().numpy(), exp)
+    # descendant -> ancestor is fine; ancestor -> descendant closes a cycle
+    Wc = got.cpu().numpy()
+    i, j = [int(x) for x in np.argwhere(np.tril(Wc, -1) > 0)[0]]
+    _, st = dense.edge_compose_many(got, [edges[0], (j, i, 2)], et)
+    assert st.flags & _lib.F_CYCLE
+
+
 @pytest.mark.parametrize("dt,n", [(np.int32, 33), (np.int8, 9), (np.int64, 65)])
 def test_fw_overflow_chain(orc, dt, n):
     A = overflow_chain(n, dt)

@@ -318,3 +318,31 @@ def test_csr_gpu_plumbing():
     prod = (g @ g).to_cpu().toarray()
     from oracle.oracle import py_mat_avos
     np.testing.assert_array_equal(prod, np.triu(py_mat_avos(A.toarray(), A.toarray())))
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_sparse_edge_insert_equals_reclosure(orc, seed):
+    """The sparse edge relational composition on a resident closure equals
+    closing the edited graph from scratch; an edge closing a cycle raises."""
+    spec = synth.PedigreeSpec(3000, 12, 0.8, 3)
+    A = synth.csr_pedigree(spec, seed)
+    C, _ = sd.dag_closure(sd.canonicalize(sd.from_scipy(A, np.int64)))
+    rng = np.random.default_rng(seed)
+    diag = A.diagonal()
+    B = A.tolil()
+    for _ in range(5):
+        a = int(rng.integers(1500, 3000))
+        rel = int(rng.choice([2, 3]))
+        cand = np.nonzero(diag[:a] == (-1 if rel == 2 else 1))[0]
+        b = int(rng.choice(cand))
+        C = sd.edge_insert(C, a, b, rel)
+        B[a, b] = rel
+    ref = orc.dag_closure(B.tocsr(), 64)
+    np.testing.assert_array_equal(C.indptr.cpu().numpy(), ref.indptr)
+    np.testing.assert_array_equal(C.indices.cpu().numpy(), ref.indices)
+    np.testing.assert_array_equal(C.data.cpu().numpy(), ref.data)
+    # an ancestor pointing back at its descendant
+    rows = np.repeat(np.arange(3000), np.diff(ref.indptr))
+    k = int(np.nonzero(ref.indices < rows)[0][0])
+    with pytest.raises(ValueError, match="cycle"):
+        sd.edge_insert(C, int(ref.indices[k]), int(rows[k]), 2)
