@@ -203,7 +203,7 @@ def bench_sparse(args):
     roof = {
         "bound": "hbm", "kernel": "k_lvl_merge<long>",
         "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s", "frac": achieved / PEAK_HBM,
-        "traffic": _read_traffic("dag_level_sparse1m_pmc.txt"),
+        "traffic": _read_traffic("lvl_merge_sparse1m_pmc.txt"),
         "algorithmic_bytes_per_closure": bytes_alg,
         "launches_per_closure": nl.value / args.steps, "avg_launch_ms": ms_.value / max(nl.value, 1),
         "kernel_ms_per_closure": kernel_s * 1e3,
@@ -425,6 +425,29 @@ def bench_serve(args):
     }
 
 
+def bench_hbmcopy(args):
+    """A plain device-to-device copy of 1 GiB (reads 1 GiB, writes 1 GiB):
+    the calibration run for the HBM byte counters (FETCH_SIZE, WRITE_SIZE)."""
+    from redblackgraph_amd import _lib
+    t = _lib.torch()
+    nbytes = 1 << 30
+    src = t.empty(nbytes // 4, dtype=t.int32, device=_lib.device()).fill_(7)
+    dst = t.empty_like(src)
+    for _ in range(args.warmup):
+        dst.copy_(src)
+    t.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dst.copy_(src)
+    t.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    return {"metric": "device copy bandwidth (calibration)", "value": 2 * nbytes / (ms / 1e3) / 1e9,
+            "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+            "data": "synthetic", "config": {"workload": "1 GiB int32 device-to-device copy", "bytes_read": nbytes,
+                                            "bytes_written": nbytes}}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -432,7 +455,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--workload", default="dense8192",
-                    choices=["dense8192", "dense4096_i64", "dense512", "sparse1m", "batched", "serve"])
+                    choices=["dense8192", "dense4096_i64", "dense512", "sparse1m", "batched", "serve", "hbmcopy"])
     ap.add_argument("--serve-count", type=int, default=300)
     ap.add_argument("--serve-rate", type=float, default=150.0)
     ap.add_argument("--serve-slots", type=int, default=4)
@@ -449,6 +472,8 @@ def main():
         res = bench_dense(args, 8192, "int32")
     elif args.workload == "sparse1m":
         res = bench_sparse(args)
+    elif args.workload == "hbmcopy":
+        res = bench_hbmcopy(args)
     elif args.workload == "serve":
         res = bench_serve(args)
     elif args.workload == "batched":

@@ -254,6 +254,16 @@ int rbg_batch_capture(rbg_batch *b, rbg_stream_t stream);
 int rbg_batch_replay(rbg_batch *b, rbg_stream_t stream);
 int rbg_batch_destroy(rbg_batch *b);
 
+/* ---- connected components ----------------------------------------------------
+ * Replaces sparse/csgraph/_components.pyx:86-177 (find_components_sparse) and
+ * :15-81 (find_components): components of the graph with edges taken in
+ * both directions.  labels[v] = the smallest vertex of v's component (the
+ * reference numbers components by that vertex, in increasing order).
+ * flags: two int32 of device scratch.  The host waits once per hooking round
+ * (*rounds of them). */
+int rbg_components(const int64_t *indptr, const int32_t *indices, int32_t n, int32_t *labels, int32_t *flags,
+                   int32_t *rounds, rbg_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -86,6 +86,7 @@ SIGNATURES = {
     "rbg_batch_capture": (I32, [P, P]),
     "rbg_batch_replay": (I32, [P, P]),
     "rbg_batch_destroy": (I32, [P]),
+    "rbg_components": (I32, [P, P, I32, P, P, ctypes.POINTER(ctypes.c_int32), P]),
 }
 
 

@@ -9,7 +9,16 @@ from .transitive_closure import (
     transitive_closure_dag_sparse,
     DensificationError,
 )
+from ._components import (
+    component_wise_closure,
+    find_components,
+    find_components_sparse,
+    extract_submatrix,
+    merge_component_matrices,
+    get_component_vertices,
+)
 
 __all__ = ["CycleError", "DensificationError", "transitive_closure", "transitive_closure_floyd_warshall",
            "transitive_closure_dijkstra", "transitive_closure_squaring", "transitive_closure_adaptive",
-           "transitive_closure_dag_sparse"]
+           "transitive_closure_dag_sparse", "component_wise_closure", "find_components", "find_components_sparse",
+           "extract_submatrix", "merge_component_matrices", "get_component_vertices"]

@@ -148,13 +148,19 @@ def transitive_closure_adaptive(A, method: str = "auto", density_threshold: floa
         return transitive_closure(A, method="FW")
     if method == "D":
         return transitive_closure(A, method="D")
+    from ._components import component_wise_closure, find_components_sparse
     if method == "component":
-        from .components import component_wise_closure
         return component_wise_closure(A)
     if method == "squaring":
         return transitive_closure_squaring(A)
     if method == "dag_sparse":
         return transitive_closure_dag_sparse(A)
+    q = {}
+    find_components_sparse(A, q)
+    if len(q) > component_threshold:
+        return component_wise_closure(A)
+    if A.nnz / (n * n) < density_threshold and n > size_threshold:
+        return transitive_closure(A, method="D")
     if _is_upper(A) or _is_lower(A):
         return transitive_closure_dag_sparse(A)
     return transitive_closure(A, method="FW")
