@@ -118,6 +118,12 @@ def bench_dense(args, n: int, dtype_name: str):
     if out.mode != "fast" or out.status.cycle_count or out.status.events:
         raise RuntimeError(f"unexpected closure outcome: mode={out.mode} status={out.status.flags}")
     ms = (t1 - t0) * 1e3 / args.steps
+    # host-boundary rate: numpy matrix in, numpy closure out (PCIe both ways)
+    t.cuda.synchronize()
+    h0 = time.perf_counter()
+    Wh = dense.fw_closure(_lib.to_device(A), et)[0].cpu().numpy()
+    pcie_ms = (time.perf_counter() - h0) * 1e3
+    del Wh
     npad = out.npad
     relax_per_launch = (npad - FW_BLK) ** 2 * FW_BLK
     tile_avg_s = tm.value / max(tl.value, 1) / 1e3
@@ -142,7 +148,8 @@ def bench_dense(args, n: int, dtype_name: str):
         "dtype": dtype_name, "data": "synthetic",
         "config": {"workload": f"dense Floyd-Warshall closure, N={n} {dtype_name}, seeded layered pedigree "
                                f"({spec.layers} layers, p_parent={spec.p_parent}, lookback {spec.lookback}), seed {args.seed}",
-                   "n": n, "diameter": out.diameter, "relaxations": npad ** 3},
+                   "n": n, "diameter": out.diameter, "relaxations": npad ** 3,
+                   "host_boundary_ms": pcie_ms, "host_boundary_entries_per_s": n * n / (pcie_ms / 1e3)},
         "roofline": roof,
     }
     if not args.no_cpu_baseline:
@@ -192,6 +199,13 @@ def bench_sparse(args):
     _lib.call("rbg_prof_read", 2, ctypes.byref(ms_), ctypes.byref(nl))
     L.rbg_prof_enable(0)
     ms = (t1 - t0) * 1e3 / args.steps
+    # host-boundary rate: scipy CSR in, scipy CSR out (PCIe both ways)
+    t.cuda.synchronize()
+    h0 = time.perf_counter()
+    Ch, _ = sd.dag_closure(sd.canonicalize(sd.from_scipy(A, np.int64)), plan=plan)
+    Ch = sd.to_scipy(Ch)
+    pcie_ms = (time.perf_counter() - h0) * 1e3
+    del Ch
     n = A.shape[0]
     clen = np.diff(C.indptr.cpu().numpy())
     rows = np.repeat(np.arange(n), np.diff(A.indptr))
@@ -217,7 +231,8 @@ def bench_sparse(args):
         "config": {"workload": f"sparse DAG closure, 1M-vertex pedigree (15 layers of ~66,667, p_parent=0.7, "
                                f"parents from the previous layer), int64 values / int32 indices, seed {args.seed}",
                    "n": n, "nnz_in": int(A.nnz), "nnz_closure": info.nnz, "levels": info.nlevels,
-                   "diameter": info.diameter_candidates},
+                   "diameter": info.diameter_candidates, "host_boundary_ms": pcie_ms,
+                   "host_boundary_entries_per_s": info.nnz / (pcie_ms / 1e3)},
         "roofline": roof,
     }
     if not args.no_cpu_baseline:

@@ -6,7 +6,9 @@
  *     comment says "host".  No function allocates device memory unless it says
  *     so; workspace sizes come from the *_workspace_bytes queries.
  *   - Every call is asynchronous on `stream` and performs no host
- *     synchronisation, so it may be captured into a hipGraph.
+ *     synchronisation, so it may be captured into a hipGraph -- except the
+ *     ones whose comment says they wait (the DAG closure's completion test,
+ *     the components' round test, the batched plan's descriptor upload).
  *   - Return value: 0 on success, an RBG_E_* code for argument errors, or a
  *     hipError_t (>0) from a failed launch.
  *   - Arithmetic errors (cycles, overflow) are reported through the device

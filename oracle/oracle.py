@@ -8,7 +8,7 @@ What each function restates (reference file:line) and what pins it
 independently of itself:
 
 =====================  ====================================================  =========================================
-function               restates                                              pinned by (tests/test_oracle_*.py)
+function               restates                                              pinned by (tests/test_oracle.py)
 =====================  ====================================================  =========================================
 avos_sum / product     rbg_math.c.in:29-121 (C core), rbm_math.h:13-110      17x17 / 5x5 tables of tests/avos/test_math.py:23-68;
                                                                              64-bit edge + overflow cases (:87-111);
