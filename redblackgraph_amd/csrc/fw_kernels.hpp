@@ -464,18 +464,45 @@ __device__ inline void fw_panel_fast_body(T* W, int64_t n, int64_t ld, int64_t K
   if ((tid & 63) == 0) record_diameter(st, dm);
 }
 
+// 16-byte vector moves of four consecutive elements (addresses are multiples
+// of four elements: padded orders are multiples of 128).
+__device__ inline void ld4(const uint32_t* p, uint32_t (&o)[4]) {
+  const uint4 v = *reinterpret_cast<const uint4*>(p);
+  o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+}
+__device__ inline void ld4(const uint64_t* p, uint64_t (&o)[4]) {
+  const ulonglong2 a = *reinterpret_cast<const ulonglong2*>(p);
+  const ulonglong2 b = *reinterpret_cast<const ulonglong2*>(p + 2);
+  o[0] = a.x; o[1] = a.y; o[2] = b.x; o[3] = b.y;
+}
+__device__ inline void st4(uint32_t* p, const uint32_t (&v)[4]) {
+  *reinterpret_cast<uint4*>(p) = make_uint4(v[0], v[1], v[2], v[3]);
+}
+__device__ inline void st4(uint64_t* p, const uint64_t (&v)[4]) {
+  *reinterpret_cast<ulonglong2*>(p) = make_ulonglong2(v[0], v[1]);
+  *reinterpret_cast<ulonglong2*>(p + 2) = make_ulonglong2(v[2], v[3]);
+}
+
 // ---------------------------------------------------------------------------
-// Phase 3, fast mode (E domain).  One 128x128 tile per workgroup, 8x8 cells
-// per thread held in VGPRs for the whole pivot block; per k a thread reads 8
-// left operands and 8 (shift, low-bits) pairs from LDS.
+// Phase 3, fast mode (E domain).  One (16 RPT) x (16 CPT) tile per workgroup
+// of 16 x 16 threads; a thread holds RPT rows x CPT columns in VGPRs for the
+// whole pivot block.  Its columns are CPT/4 chunks of four, 64 columns apart,
+// so that the eight lanes an LDS read serves together cover all 32 banks.
+// Per k a thread reads RPT left operands and CPT (shift, low-bits) pairs from
+// LDS; a relaxation is one shift-or and one unsigned min, the trajectory
+// maximum one signed max3 per two steps.  Threads whose rows lie in the pivot
+// block skip (the panel kernel owns those rows); cells of the pivot columns
+// are relaxed again with the same snapshot products, which leaves them and
+// the trajectory maximum unchanged.
 // ---------------------------------------------------------------------------
 template <typename T, int RPT, int CPT>
 __device__ inline void fw_tile_fast_body(T* W, int64_t n, int64_t ld, int64_t K, FwWork ws, int width,
                                          rbg_status* st, int bid) {
   using U = typename unsigned_of<T>::type;
   constexpr U TOP = (U)1 << (sizeof(U) * 8 - 1);
-  constexpr int TL = 16 * CPT;  // tile edge: 16 x 16 threads
-  static_assert(RPT == CPT, "square thread blocks keep the diagonal cells on one thread");
+  constexpr int TL = 16 * CPT;  // tile edge
+  constexpr int NCH = CPT / 4;  // column chunks per thread
+  static_assert(RPT == CPT && CPT % 4 == 0, "square thread blocks of whole chunks");
   __shared__ U As[BLK][TL];
   __shared__ U Ms[BLK][TL];
   __shared__ U Ys[BLK][TL];
@@ -485,22 +512,28 @@ __device__ inline void fw_tile_fast_body(T* W, int64_t n, int64_t ld, int64_t K,
   const int I = bid / nt, J = bid % nt;
   const int64_t k0 = K * BLK;
   const int par = (int)(K & 1);
-  const T* colT = ws.col<T>(par);
-  const T* rowS = ws.row<T>(par);
+  const U* colT = (const U*)ws.col<T>(par);
+  const U* rowS = (const U*)ws.row<T>(par);
   const int* guard = ws.guard(par);
 
   const int tr = tid >> 4, tc = tid & 15;
   const int64_t r0 = (int64_t)I * TL + tr * RPT;
-  const int64_t c0 = (int64_t)J * TL + tc * CPT;
-  const bool skip = (r0 >= k0 && r0 < k0 + BLK) || (c0 >= k0 && c0 < k0 + BLK);
+  const int64_t cb = (int64_t)J * TL + tc * 4;  // column of chunk 0; chunk h adds 64 h
+  const bool skip = r0 >= k0 && r0 < k0 + BLK;
 
   // issue the tile loads first
   U w[RPT][CPT];
+  U* Wu = (U*)W;
   if (!skip) {
 #pragma unroll
     for (int r = 0; r < RPT; r++)
 #pragma unroll
-      for (int c = 0; c < CPT; c++) w[r][c] = (U)W[(r0 + r) * ld + c0 + c];
+      for (int h = 0; h < NCH; h++) {
+        U q[4];
+        ld4(Wu + (r0 + r) * ld + cb + 64 * h, q);
+#pragma unroll
+        for (int x = 0; x < 4; x++) w[r][4 * h + x] = q[x];
+      }
   }
 
   // a workgroup-uniform exit when an earlier kernel already asked for general mode
@@ -509,20 +542,24 @@ __device__ inline void fw_tile_fast_body(T* W, int64_t n, int64_t ld, int64_t K,
   if (tid < BLK) {
     if (guard[tid] + guard[BLK + tid] > width - 2) atomicOr(&sbad, 1);
   }
-  for (int e = tid; e < BLK * TL; e += NT) {
-    int kk = e / TL, x = e % TL;
-    As[kk][x] = (U)colT[kk * n + (int64_t)I * TL + x];
-    T b = rowS[kk * n + (int64_t)J * TL + x];
-    U m, y;
-    if (b < 0) {  // absent
-      m = 0;
-      y = TOP;
-    } else {
-      m = (U)msb_u((U)b);
-      y = (U)b & (((U)1 << m) - 1);
+  for (int e = tid; e < BLK * TL / 4; e += NT) {
+    const int kk = e / (TL / 4), x = (e % (TL / 4)) * 4;
+    U a[4], b[4], m[4], y[4];
+    ld4(colT + kk * n + (int64_t)I * TL + x, a);
+    ld4(rowS + kk * n + (int64_t)J * TL + x, b);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      if ((T)b[q] < 0) {  // absent
+        m[q] = 0;
+        y[q] = TOP;
+      } else {
+        m[q] = (U)msb_u(b[q]);
+        y[q] = b[q] & (((U)1 << m[q]) - 1);
+      }
     }
-    Ms[kk][x] = m;
-    Ys[kk][x] = y;
+    st4(&As[kk][x], a);
+    st4(&Ms[kk][x], m);
+    st4(&Ys[kk][x], y);
   }
   __syncthreads();
   if (sbad) {
@@ -531,7 +568,17 @@ __device__ inline void fw_tile_fast_body(T* W, int64_t n, int64_t ld, int64_t K,
   }
   if (skip) return;
 
-  const bool diag_thread = (I == J) && (tr * RPT == tc * CPT);
+  // diagonal cells of this thread (tiles with I == J): four at most, rows
+  // dr..dr+3 against chunk dh's columns; mode = 2 dh + dr / 4, or -1
+  int dmode = -1;
+  if (I == J) {
+    const int d = tc * 4 - tr * RPT;
+#pragma unroll
+    for (int h = 0; h < NCH; h++) {
+      const int dr = d + 64 * h;
+      if (dr == 0 || (RPT == 8 && dr == 4)) dmode = 2 * h + dr / 4;
+    }
+  }
   T d0 = 0, d1 = 0;
   for (int kk = 0; kk < BLK; kk += 2) {
     U a0[RPT], a1[RPT], m0[CPT], m1[CPT], y0[CPT], y1[CPT];
@@ -542,18 +589,27 @@ __device__ inline void fw_tile_fast_body(T* W, int64_t n, int64_t ld, int64_t K,
     }
 #pragma unroll
     for (int c = 0; c < CPT; c++) {
-      m0[c] = Ms[kk][tc * CPT + c];
-      y0[c] = Ys[kk][tc * CPT + c];
-      m1[c] = Ms[kk + 1][tc * CPT + c];
-      y1[c] = Ys[kk + 1][tc * CPT + c];
+      const int x = 64 * (c >> 2) + tc * 4 + (c & 3);
+      m0[c] = Ms[kk][x];
+      y0[c] = Ys[kk][x];
+      m1[c] = Ms[kk + 1][x];
+      y1[c] = Ys[kk + 1][x];
     }
-    if (diag_thread) {
+    if (dmode >= 0) {
 #pragma unroll
-      for (int r = 0; r < RPT; r++) {
-        U p0 = (a0[r] << m0[r]) | y0[r];
-        U p1 = (a1[r] << m1[r]) | y1[r];
-        if ((T)p0 >= 0) record_cycle(st, naive_key(k0 + kk, r0 + r, r0 + r, n));
-        if ((T)p1 >= 0) record_cycle(st, naive_key(k0 + kk + 1, r0 + r, r0 + r, n));
+      for (int mode = 0; mode < 2 * NCH; mode++) {
+        if (mode == dmode) {
+          const int dr = (mode & 1) * 4, dc = (mode >> 1) * 4;
+#pragma unroll
+          for (int q = 0; q < 4; q++) {
+            if (dr + q < RPT) {
+              const U p0 = (a0[dr + q] << m0[dc + q]) | y0[dc + q];
+              const U p1 = (a1[dr + q] << m1[dc + q]) | y1[dc + q];
+              if ((T)p0 >= 0) record_cycle(st, naive_key(k0 + kk, r0 + dr + q, r0 + dr + q, n));
+              if ((T)p1 >= 0) record_cycle(st, naive_key(k0 + kk + 1, r0 + dr + q, r0 + dr + q, n));
+            }
+          }
+        }
       }
     }
 #pragma unroll
@@ -575,7 +631,12 @@ __device__ inline void fw_tile_fast_body(T* W, int64_t n, int64_t ld, int64_t K,
 #pragma unroll
   for (int r = 0; r < RPT; r++)
 #pragma unroll
-    for (int c = 0; c < CPT; c++) W[(r0 + r) * ld + c0 + c] = (T)w[r][c];
+    for (int h = 0; h < NCH; h++) {
+      U q[4];
+#pragma unroll
+      for (int x = 0; x < 4; x++) q[x] = w[r][4 * h + x];
+      st4(Wu + (r0 + r) * ld + cb + 64 * h, q);
+    }
   int64_t dm = (int64_t)(d0 > d1 ? d0 : d1);
   dm = wave_max_i64(dm);
   if ((tid & 63) == 0) record_diameter(st, dm);
