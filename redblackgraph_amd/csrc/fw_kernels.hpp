@@ -283,8 +283,32 @@ __global__ __launch_bounds__(NT) void k_fw_panel(T* W, int64_t n, int64_t ld, in
   if ((tid & 63) == 0) record_diameter(st, dmax);
 }
 
+// 16-byte vector moves of four consecutive elements (addresses are multiples
+// of four elements: padded orders are multiples of 128).
+__device__ inline void ld4(const uint32_t* p, uint32_t (&o)[4]) {
+  const uint4 v = *reinterpret_cast<const uint4*>(p);
+  o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+}
+__device__ inline void ld4(const uint64_t* p, uint64_t (&o)[4]) {
+  const ulonglong2 a = *reinterpret_cast<const ulonglong2*>(p);
+  const ulonglong2 b = *reinterpret_cast<const ulonglong2*>(p + 2);
+  o[0] = a.x; o[1] = a.y; o[2] = b.x; o[3] = b.y;
+}
+__device__ inline void st4(uint32_t* p, const uint32_t (&v)[4]) {
+  *reinterpret_cast<uint4*>(p) = make_uint4(v[0], v[1], v[2], v[3]);
+}
+__device__ inline void st4(uint64_t* p, const uint64_t (&v)[4]) {
+  *reinterpret_cast<ulonglong2*>(p) = make_ulonglong2(v[0], v[1]);
+  *reinterpret_cast<ulonglong2*>(p + 2) = make_ulonglong2(v[2], v[3]);
+}
+
 // ---------------------------------------------------------------------------
-// Panel kernel, fast mode: same schedule as k_fw_panel, E-domain arithmetic.
+// Panel kernel, fast mode, for one matrix (k_fw_panel_fast): same schedule as
+// k_fw_panel, E-domain arithmetic.  2 (n/32 - 1) workgroups each walk the
+// diagonal block together with one 32 x 32 tile of the row or column panel,
+// which spreads a block's short dependent chain over the whole chip.  The
+// panels also leave, per k, the largest MSB of the column and row snapshots
+// in the guard words that phase 3 reads.
 // At step k the pivot row and column do not change (they are skipped), so the
 // step reads them while it updates every other cell and needs one barrier.
 // Operands off the pivot row/column are never +-1 (precheck), so the product
@@ -296,7 +320,7 @@ __global__ __launch_bounds__(NT) void k_fw_panel(T* W, int64_t n, int64_t ld, in
 // first canonicalised to all ones, and so is every snapshot phase 3 reads.
 // ---------------------------------------------------------------------------
 template <typename T>
-__device__ inline void fw_panel_fast_body(T* W, int64_t n, int64_t ld, int64_t K, FwWork ws, int width,
+__device__ inline void fw_panel_tiles_body(T* W, int64_t n, int64_t ld, int64_t K, FwWork ws, int width,
                                           rbg_status* st, int bid) {
   using U = typename unsigned_of<T>::type;
   constexpr U TOP = (U)1 << (sizeof(U) * 8 - 1);
@@ -464,23 +488,219 @@ __device__ inline void fw_panel_fast_body(T* W, int64_t n, int64_t ld, int64_t K
   if ((tid & 63) == 0) record_diameter(st, dm);
 }
 
-// 16-byte vector moves of four consecutive elements (addresses are multiples
-// of four elements: padded orders are multiples of 128).
-__device__ inline void ld4(const uint32_t* p, uint32_t (&o)[4]) {
-  const uint4 v = *reinterpret_cast<const uint4*>(p);
-  o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
-}
-__device__ inline void ld4(const uint64_t* p, uint64_t (&o)[4]) {
-  const ulonglong2 a = *reinterpret_cast<const ulonglong2*>(p);
-  const ulonglong2 b = *reinterpret_cast<const ulonglong2*>(p + 2);
-  o[0] = a.x; o[1] = a.y; o[2] = b.x; o[3] = b.y;
-}
-__device__ inline void st4(uint32_t* p, const uint32_t (&v)[4]) {
-  *reinterpret_cast<uint4*>(p) = make_uint4(v[0], v[1], v[2], v[3]);
-}
-__device__ inline void st4(uint64_t* p, const uint64_t (&v)[4]) {
-  *reinterpret_cast<ulonglong2*>(p) = make_ulonglong2(v[0], v[1]);
-  *reinterpret_cast<ulonglong2*>(p + 2) = make_ulonglong2(v[2], v[3]);
+// ---------------------------------------------------------------------------
+// Fast mode, phases 1 and 2 for the batched launcher (fw_panel_lines_body:
+// throughput over thousands of components).  Each workgroup first walks
+// the 32 x 32 diagonal block of pivot block K through its 32 steps in LDS
+// (four cells per thread, one barrier per step); workgroup 0 owns the result
+// and its events.  Then each of its 256 threads owns one panel line: a column
+// j of the row panel (rows of the block, j outside it) or a row i of the
+// column panel.  Given the diagonal block's snapshots a line evolves on its
+// own:
+//     W[r][j] = W[r][j] (+) colsnap_k[r] (x) W[k][j]   (r != k)
+// with W[k][j] invariant at step k, so the thread walks its 32 cells through
+// the 32 steps in registers -- 31 shift-or + min per step, the shift and low
+// bits of the line's own operand computed once per step -- and writes the
+// line's snapshots (W[k][j] at step k) for phase 3.  The MSB guard is checked
+// per product on the diagonal block and per step, against the largest
+// operand on the block's side, on the lines.
+//
+// Operands are canonicalised (absent = all ones) before use: a product with
+// an absent operand is some value with the top bit set, which the min may
+// store, and which must not be shifted again.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ inline void fw_panel_lines_body(T* W, int64_t n, int64_t ld, int64_t K, FwWork ws, int width,
+                                          rbg_status* st, int bid) {
+  using U = typename unsigned_of<T>::type;
+  constexpr U TOP = (U)1 << (sizeof(U) * 8 - 1);
+  __shared__ U D[BLK][BLK + 1];
+  __shared__ U sC[BLK][BLK + 1];  // [kk][r] = W[r][kk] at step kk (diagonal block)
+  __shared__ U sR[BLK][BLK + 1];  // [kk][c] = W[kk][c] at step kk (diagonal block)
+  __shared__ U Mb[BLK][BLK];      // shift and low bits of sR
+  __shared__ U Yb[BLK][BLK];
+  __shared__ int maxA[BLK], maxB[BLK], sbad;
+  const int tid = threadIdx.x;
+  const int64_t k0 = K * BLK;
+  const int par = (int)(K & 1);
+  const bool owner = bid == 0;
+  T* colT = ws.col<T>(par);
+  T* rowS = ws.row<T>(par);
+  auto canon = [](U x) -> U { return (T)x < 0 ? ~(U)0 : x; };
+  if (tid == 0) sbad = (st->flags & (RBG_F_NEED_GENERAL | RBG_F_PRECOND_FAIL)) ? 2 : 0;
+  __syncthreads();
+  if (sbad) return;
+  if (owner && tid == 0 && K > 0) capture_block<T, true>(st, ws.col<T>(par ^ 1), ws.row<T>(par ^ 1), k0 - BLK, n);
+  for (int e = tid; e < BLK * BLK; e += NT) D[e / BLK][e % BLK] = canon((U)W[(k0 + e / BLK) * ld + k0 + e % BLK]);
+  // this thread's panel line, loaded while the diagonal block is walked
+  const bool rowp = (int64_t)bid * NT + tid < n;
+  const int64_t line = rowp ? (int64_t)bid * NT + tid : (int64_t)bid * NT + tid - n;
+  const bool active = (int64_t)bid * NT + tid < 2 * n && !(line >= k0 && line < k0 + BLK);
+  U c[BLK];
+  U* Wu = (U*)W;
+  if (active) {
+    if (rowp) {
+#pragma unroll
+      for (int r = 0; r < BLK; r++) c[r] = Wu[(k0 + r) * ld + line];
+    } else {
+#pragma unroll
+      for (int h = 0; h < BLK / 4; h++) {
+        U q[4];
+        ld4(Wu + line * ld + k0 + 4 * h, q);
+#pragma unroll
+        for (int x = 0; x < 4; x++) c[4 * h + x] = q[x];
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- the diagonal block ----
+  const int i = tid >> 3;
+  const int j0 = (tid & 7) * 4;
+  const int lim = width - 2;
+  U dv[4];
+#pragma unroll
+  for (int x = 0; x < 4; x++) dv[x] = D[i][j0 + x];
+  T dmax = 0;
+  bool bad = false;
+  for (int kk = 0; kk < BLK; kk++) {
+    const U aD = canon(D[i][kk]);
+    U bD[4];
+#pragma unroll
+    for (int x = 0; x < 4; x++) bD[x] = canon(D[kk][j0 + x]);
+    if ((tid & 7) == 0) sC[kk][i] = aD;
+    if (i == 0) {
+#pragma unroll
+      for (int x = 0; x < 4; x++) sR[kk][j0 + x] = bD[x];
+    }
+    const int maD = (T)aD > 1 ? msb_u(aD) : 0;
+#pragma unroll
+    for (int x = 0; x < 4; x++) {
+      const int j = j0 + x;
+      if (i != kk && j != kk) {
+        U m, y;
+        if ((T)bD[x] < 0) {
+          m = 0;
+          y = TOP;
+        } else {
+          m = (U)msb_u(bD[x]);
+          y = bD[x] & (((U)1 << m) - 1);
+        }
+        if ((T)bD[x] > 1 && maD + (int)m > lim) bad = true;
+        const U p = (aD << m) | y;
+        if (owner && i == j && (T)p >= 0) record_cycle(st, naive_key(k0 + kk, k0 + i, k0 + i, n));
+        dv[x] = dv[x] < p ? dv[x] : p;
+        D[i][j] = dv[x];
+      }
+      if (owner && (T)dv[x] > dmax) dmax = (T)dv[x];
+    }
+    __syncthreads();
+  }
+  // shift / low bits of the block's rows, and the per-step operand maxima
+  for (int e = tid; e < BLK * BLK; e += NT) {
+    const int kk = e / BLK, x = e % BLK;
+    const U b = sR[kk][x];
+    if ((T)b < 0) {
+      Mb[kk][x] = 0;
+      Yb[kk][x] = TOP;
+    } else {
+      const U m = (U)msb_u(b);
+      Mb[kk][x] = m;
+      Yb[kk][x] = b & (((U)1 << m) - 1);
+    }
+  }
+  if (bad) sbad = 1;
+  __syncthreads();
+  if (sbad) {
+    if (tid == 0) atomicOr(&st->flags, (uint32_t)RBG_F_NEED_GENERAL);
+    return;
+  }
+  if (tid < 2 * BLK) {
+    const int kk = tid & (BLK - 1);
+    int mx = 0;
+    for (int x = 0; x < BLK; x++) {
+      int m;
+      if (tid < BLK) {
+        const U a = sC[kk][x];
+        m = (T)a > 1 ? msb_u(a) : 0;
+      } else {
+        m = (T)Yb[kk][x] < 0 ? 0 : (int)Mb[kk][x];
+      }
+      mx = m > mx ? m : mx;
+    }
+    if (tid < BLK) maxA[kk] = mx;
+    else maxB[kk] = mx;
+  }
+  __syncthreads();
+  if (owner) {
+    for (int e = tid; e < BLK * BLK; e += NT) {
+      const int r = e / BLK, cc = e % BLK;
+      W[(k0 + r) * ld + k0 + cc] = (T)D[r][cc];
+      colT[r * n + k0 + cc] = (T)sC[r][cc];
+      rowS[r * n + k0 + cc] = (T)sR[r][cc];
+    }
+  }
+
+  // ---- the panel lines ----
+  T d0 = dmax, d1 = 0;
+  bool lbad = false;
+  if (active) {
+#pragma unroll
+    for (int kk = 0; kk < BLK; kk++) {
+      const U pk = canon(c[kk]);  // invariant at step kk
+      if (rowp) rowS[kk * n + line] = (T)pk;
+      else colT[kk * n + line] = (T)pk;
+      d0 = (T)pk > d0 ? (T)pk : d0;
+      if (rowp) {
+        U m, y;
+        if ((T)pk < 0) {
+          m = 0;
+          y = TOP;
+        } else {
+          m = (U)msb_u(pk);
+          y = pk & (((U)1 << m) - 1);
+        }
+        if ((T)pk > 1 && maxA[kk] + (int)m > lim) lbad = true;
+#pragma unroll
+        for (int r = 0; r < BLK; r++) {
+          if (r != kk) {
+            const U p = (sC[kk][r] << m) | y;
+            const U v = c[r] < p ? c[r] : p;
+            c[r] = v;
+            if (r & 1) d1 = (T)v > d1 ? (T)v : d1;
+            else d0 = (T)v > d0 ? (T)v : d0;
+          }
+        }
+      } else {
+        if ((T)pk > 1 && (int)msb_u(pk) + maxB[kk] > lim) lbad = true;
+#pragma unroll
+        for (int x = 0; x < BLK; x++) {
+          if (x != kk) {
+            const U p = (pk << Mb[kk][x]) | Yb[kk][x];
+            const U v = c[x] < p ? c[x] : p;
+            c[x] = v;
+            if (x & 1) d1 = (T)v > d1 ? (T)v : d1;
+            else d0 = (T)v > d0 ? (T)v : d0;
+          }
+        }
+      }
+    }
+    if (lbad) atomicOr(&st->flags, (uint32_t)RBG_F_NEED_GENERAL);
+    if (rowp) {
+#pragma unroll
+      for (int r = 0; r < BLK; r++) Wu[(k0 + r) * ld + line] = c[r];
+    } else {
+#pragma unroll
+      for (int h = 0; h < BLK / 4; h++) {
+        U q[4];
+#pragma unroll
+        for (int x = 0; x < 4; x++) q[x] = c[4 * h + x];
+        st4(Wu + line * ld + k0 + 4 * h, q);
+      }
+    }
+  }
+  const int64_t dm = wave_max_i64((int64_t)(d0 > d1 ? d0 : d1));
+  if ((tid & 63) == 0) record_diameter(st, dm);
 }
 
 // ---------------------------------------------------------------------------
@@ -495,7 +715,10 @@ __device__ inline void st4(uint64_t* p, const uint64_t (&v)[4]) {
 // are relaxed again with the same snapshot products, which leaves them and
 // the trajectory maximum unchanged.
 // ---------------------------------------------------------------------------
-template <typename T, int RPT, int CPT>
+// GLOBAL_GUARD: the per-k MSB guard comes from the guard words the panel
+// kernel left (one matrix); otherwise the tile computes it from the
+// snapshots it stages (the batched launcher's panels leave none).
+template <typename T, int RPT, int CPT, bool GLOBAL_GUARD>
 __device__ inline void fw_tile_fast_body(T* W, int64_t n, int64_t ld, int64_t K, FwWork ws, int width,
                                          rbg_status* st, int bid) {
   using U = typename unsigned_of<T>::type;
@@ -506,7 +729,7 @@ __device__ inline void fw_tile_fast_body(T* W, int64_t n, int64_t ld, int64_t K,
   __shared__ U As[BLK][TL];
   __shared__ U Ms[BLK][TL];
   __shared__ U Ys[BLK][TL];
-  __shared__ int sbad;
+  __shared__ int sbad, sGa[BLK], sGb[BLK];
   const int tid = threadIdx.x;
   const int nt = (int)(n / TL);
   const int I = bid / nt, J = bid % nt;
@@ -514,7 +737,6 @@ __device__ inline void fw_tile_fast_body(T* W, int64_t n, int64_t ld, int64_t K,
   const int par = (int)(K & 1);
   const U* colT = (const U*)ws.col<T>(par);
   const U* rowS = (const U*)ws.row<T>(par);
-  const int* guard = ws.guard(par);
 
   const int tr = tid >> 4, tc = tid & 15;
   const int64_t r0 = (int64_t)I * TL + tr * RPT;
@@ -538,15 +760,20 @@ __device__ inline void fw_tile_fast_body(T* W, int64_t n, int64_t ld, int64_t K,
 
   // a workgroup-uniform exit when an earlier kernel already asked for general mode
   if (tid == 0) sbad = (st->flags & (RBG_F_NEED_GENERAL | RBG_F_PRECOND_FAIL)) ? 2 : 0;
+  if (tid < BLK) sGa[tid] = sGb[tid] = 0;
   __syncthreads();
-  if (tid < BLK) {
+  if (GLOBAL_GUARD && tid < BLK) {
+    const int* guard = ws.guard(par);
     if (guard[tid] + guard[BLK + tid] > width - 2) atomicOr(&sbad, 1);
   }
+  // stage the snapshots; per k, the largest MSB among this tile's left
+  // operands and among its right operands (the MSB guard of this tile)
   for (int e = tid; e < BLK * TL / 4; e += NT) {
     const int kk = e / (TL / 4), x = (e % (TL / 4)) * 4;
     U a[4], b[4], m[4], y[4];
     ld4(colT + kk * n + (int64_t)I * TL + x, a);
     ld4(rowS + kk * n + (int64_t)J * TL + x, b);
+    int ga = 0, gb = 0;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       if ((T)b[q] < 0) {  // absent
@@ -556,12 +783,23 @@ __device__ inline void fw_tile_fast_body(T* W, int64_t n, int64_t ld, int64_t K,
         m[q] = (U)msb_u(b[q]);
         y[q] = b[q] & (((U)1 << m[q]) - 1);
       }
+      const int mq = (int)m[q], aq = (T)a[q] > 1 ? msb_u(a[q]) : 0;
+      gb = mq > gb ? mq : gb;
+      ga = aq > ga ? aq : ga;
+    }
+    if (!GLOBAL_GUARD) {
+      if (ga) atomicMax(&sGa[kk], ga);
+      if (gb) atomicMax(&sGb[kk], gb);
     }
     st4(&As[kk][x], a);
     st4(&Ms[kk][x], m);
     st4(&Ys[kk][x], y);
   }
   __syncthreads();
+  if (!GLOBAL_GUARD) {
+    if (tid < BLK && sGa[tid] + sGb[tid] > width - 2) atomicOr(&sbad, 1);
+    __syncthreads();
+  }
   if (sbad) {
     if (tid == 0 && (sbad & 1)) atomicOr(&st->flags, (uint32_t)RBG_F_NEED_GENERAL);
     return;
@@ -645,13 +883,13 @@ __device__ inline void fw_tile_fast_body(T* W, int64_t n, int64_t ld, int64_t K,
 template <typename T>
 __global__ __launch_bounds__(NT) void k_fw_panel_fast(T* W, int64_t n, int64_t ld, int64_t K, FwWork ws,
                                                       int width, rbg_status* st) {
-  fw_panel_fast_body<T>(W, n, ld, K, ws, width, st, blockIdx.x);
+  fw_panel_tiles_body<T>(W, n, ld, K, ws, width, st, blockIdx.x);
 }
 
 template <typename T>
 __global__ __launch_bounds__(NT) void k_fw_tile_fast(T* W, int64_t n, int64_t ld, int64_t K, FwWork ws,
                                                      int width, rbg_status* st) {
-  fw_tile_fast_body<T, RPT, CPT>(W, n, ld, K, ws, width, st, blockIdx.x);
+  fw_tile_fast_body<T, RPT, CPT, true>(W, n, ld, K, ws, width, st, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------

@@ -7,9 +7,10 @@
 // every component is padded to a multiple of 128 and stored contiguously in a
 // pool; components are ordered by padded order, largest first, so that at
 // pivot-block step K the components still running (npad / 32 > K) form a
-// prefix of every work list.  Step K is one panel launch and one tile launch
-// per tile shape, each over the work items of all running components, so a
-// launch carries the panels (or tiles) of thousands of components at once.
+// prefix of every work list.  Step K is one panel launch (groups of 256
+// panel lines) and one tile launch per tile shape, each over the work of all
+// running components, so a launch carries the panels (or tiles) of thousands
+// of components at once.
 // Arithmetic, snapshots, event keys and the MSB guard are the single-matrix
 // fast kernels' (fw_kernels.hpp), per component.  A component whose input is
 // outside the fast domain, or that needs general arithmetic or saw a cycle,
@@ -36,6 +37,10 @@ struct Item {
 };
 
 constexpr int CHUNK = 128;  // precheck / domain conversion granule
+
+// workgroups of 256 panel lines per component (2 npad lines: its row-panel
+// columns and column-panel rows)
+static inline int lines_groups(int npad) { return (2 * npad + NT - 1) / NT; }
 
 static __global__ void k_bfw_init(const BComp* comps, int32_t count) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -91,11 +96,12 @@ static __global__ __launch_bounds__(NT) void k_bfw_convert(const BComp* comps, c
   }
 }
 
+// phases 1 and 2: item = (component, group of 256 panel lines)
 static __global__ __launch_bounds__(NT) void k_bfw_panel(const BComp* comps, const Item* items, int64_t K) {
   const Item it = items[blockIdx.x];
   const BComp c = comps[it.comp];
   FwWork ws{c.snap, c.npad, 4};
-  fw_panel_fast_body<int32_t>(c.W, c.npad, c.npad, K, ws, 32, c.st, it.bid);
+  fw_panel_lines_body<int32_t>(c.W, c.npad, c.npad, K, ws, 32, c.st, it.bid);
 }
 
 template <int RC>
@@ -103,7 +109,7 @@ static __global__ __launch_bounds__(NT) void k_bfw_tile(const BComp* comps, cons
   const Item it = items[blockIdx.x];
   const BComp c = comps[it.comp];
   FwWork ws{c.snap, c.npad, 4};
-  fw_tile_fast_body<int32_t, RC, RC>(c.W, c.npad, c.npad, K, ws, 32, c.st, it.bid);
+  fw_tile_fast_body<int32_t, RC, RC, false>(c.W, c.npad, c.npad, K, ws, 32, c.st, it.bid);
 }
 
 // Operands of the last block's detections (the panel of the next block would
@@ -187,9 +193,9 @@ int rbg_batch_create(const rbg_batch_comp* comps, int32_t count, rbg_batch** out
   b->maxnb = c[0].npad / BLK;
   std::vector<Item> chunks, panels, t128, t64;
   for (int i = 0; i < count; i++) {
-    const int nc = c[i].npad / CHUNK, nb = c[i].npad / BLK, nt = c[i].npad / c[i].tile;
+    const int nc = c[i].npad / CHUNK, nt = c[i].npad / c[i].tile;
     for (int q = 0; q < nc * nc; q++) chunks.push_back({i, q});
-    for (int q = 0; q < 2 * (nb - 1); q++) panels.push_back({i, q});
+    for (int q = 0; q < lines_groups(c[i].npad); q++) panels.push_back({i, q});
     auto& tl = c[i].tile == 128 ? t128 : t64;
     for (int q = 0; q < nt * nt; q++) tl.push_back({i, q});
   }
@@ -203,7 +209,7 @@ int rbg_batch_create(const rbg_batch_comp* comps, int32_t count, rbg_batch** out
   for (int i = 0; i < count; i++) {
     const int nb = c[i].npad / BLK, nt = c[i].npad / c[i].tile;
     for (int K = 0; K < nb; K++) {
-      b->panel_cnt[K] += 2 * (nb - 1);
+      b->panel_cnt[K] += lines_groups(c[i].npad);
       (c[i].tile == 128 ? b->t128_cnt : b->t64_cnt)[K] += (int64_t)nt * nt;
     }
   }
