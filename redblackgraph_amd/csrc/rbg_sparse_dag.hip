@@ -480,14 +480,139 @@ __device__ inline void merge_lists(const RowJob<T>& J, const int32_t* __restrict
   }
 }
 
+// Per-row list metadata for the LDS merge: list 0 is the vertex's own entries
+// (columns in `indices`, values in `data` from e0), list k >= 1 is
+// A[v][w_k] (x) C[w_k] for its k-th non-self successor (arena row at ro[k],
+// multiplier av[k]); lst[k] is the list's first candidate index (lst[nl] =
+// the row's candidate count).
+constexpr int LV_MAXD = 7;  // direct entries a row may have for the LDS paths
+template <typename T>
+struct RowMeta {
+  int32_t nl;
+  int32_t lst[LV_MAXD + 2];
+  int64_t ro[LV_MAXD + 1];
+  T av[LV_MAXD + 1];
+};
+
+// Merge one row entirely in LDS: `lane` of `nl_lanes` cooperating threads
+// (a wave, or the workgroup), candidates at lcol[base ...], merged order to
+// mcol/mval[base ...].  Every access to the staged lists is an LDS access.
+template <typename T, int NL>
+__device__ inline void merge_row_lds(const int32_t* __restrict__ indices, const T* __restrict__ data,
+                                     const DagWork& ws, const DagArena<T>& ar, int32_t v, int64_t e0, int nd,
+                                     int64_t cnt, int base, int32_t* s_lcol, int32_t* s_mcol, T* s_mval,
+                                     RowMeta<T>& M, int32_t* s_len, int lane, int width, T& cmax, uint32_t& evs,
+                                     bool first_wave) {
+  // metadata: lanes < nd of the first wave read the direct entries
+  if (first_wave) {
+    int32_t w = v;
+    if (lane < nd) w = indices[e0 + lane];
+    const bool succ = lane < nd && w != v;
+    const uint64_t msk = __ballot(succ);
+    if (succ) {
+      const int k = 1 + __popcll(msk & ((1ull << lane) - 1ull));
+      M.ro[k] = ws.row_off[w];
+      M.av[k] = data[e0 + lane];
+      s_len[k] = (int32_t)ws.row_len[w];
+    }
+    if (lane == 0) {
+      M.nl = 1 + __popcll(msk);
+      M.ro[0] = e0;
+      M.lst[0] = 0;
+      M.lst[1] = nd;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    if (lane == 0)
+      for (int k = 1; k < M.nl; k++) M.lst[k + 1] = M.lst[k] + s_len[k];
+  }
+  if (NL == 64) {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+  } else {
+    __syncthreads();
+  }
+  const int nl = M.nl;
+  // stage every list's columns
+  for (int q = lane; q < nd; q += NL) s_lcol[base + q] = indices[e0 + q];
+  for (int k = 1; k < nl; k++) {
+    const int lo = M.lst[k], len = M.lst[k + 1] - lo;
+    const int64_t ro = M.ro[k];
+    for (int q = lane; q < len; q += NL) s_lcol[base + lo + q] = ar.cols[ro + q];
+  }
+  if (NL == 64) {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+  } else {
+    __syncthreads();
+  }
+  for (int64_t idx = lane; idx < cnt; idx += NL) {
+    int k = 0;
+    while (k + 1 < nl && M.lst[k + 1] <= idx) k++;
+    const int p = (int)idx - M.lst[k];
+    const int32_t c = s_lcol[base + idx];
+    T x;
+    if (k == 0) {
+      x = data[e0 + p];
+    } else {
+      uint32_t ev = 0;
+      x = avos_product<T>(M.av[k], ar.vals[M.ro[k] + p], width, false, ev);
+      evs |= ev;
+    }
+    {
+      const T a = x < 0 ? (T)-x : x;
+      cmax = a > cmax ? a : cmax;
+    }
+    int pos = p;
+    bool head = true;
+    T comb = x;
+    for (int k2 = 0; k2 < nl; k2++) {
+      if (k2 == k) continue;
+      const int lo = base + M.lst[k2];
+      const int len = M.lst[k2 + 1] - M.lst[k2];
+      int a = 0, b = len;  // first index with column > c (k2 < k) or >= c (k2 > k)
+      if (k2 < k) {
+        while (a < b) {
+          const int mid = (a + b) >> 1;
+          if (s_lcol[lo + mid] <= c) a = mid + 1;
+          else b = mid;
+        }
+        pos += a;
+        if (a > 0 && s_lcol[lo + a - 1] == c) head = false;
+      } else {
+        while (a < b) {
+          const int mid = (a + b) >> 1;
+          if (s_lcol[lo + mid] < c) a = mid + 1;
+          else b = mid;
+        }
+        pos += a;
+        if (head && a < len && s_lcol[lo + a] == c) {
+          T y;
+          if (k2 == 0) {
+            y = data[e0 + a];
+          } else {
+            uint32_t ev = 0;
+            y = avos_product<T>(M.av[k2], ar.vals[M.ro[k2] + a], width, false, ev);
+            evs |= ev;
+          }
+          comb = avos_sum<T>(comb, y);
+        }
+      }
+    }
+    s_mcol[base + pos] = (head && comb != 0) ? c : -1;
+    s_mval[base + pos] = comb;
+  }
+}
+
 // Round r, step 3: closure rows of the level.  Each workgroup takes the level's
-// vertices four at a time: a vertex with at most LV_SMALL candidates is merged
-// by one wave in that wave's LDS share; a longer one by the whole workgroup,
-// in LDS up to LV_BIG candidates, else in its arena slot.  The compaction
-// writes the row to the front of the slot [abase + seg, abase + seg + cnt)
-// that the count scan reserved for the vertex; when the row was merged in the
-// slot itself every store of a pass lands at or below the entries the pass
-// has already loaded.
+// vertices four at a time.  A row with at most LV_SMALL candidates and at
+// most LV_MAXD direct entries is merged by one wave in that wave's LDS share;
+// a longer one (up to LV_BIG) by the whole workgroup in LDS; anything else by
+// the workgroup in its arena slot, reading the lists from global memory.  The
+// compaction writes the row to the front of the slot [abase + seg, abase +
+// seg + cnt) that the count scan reserved for the vertex; when the row was
+// merged in the slot itself every store of a pass lands at or below the
+// entries the pass has already loaded.
 template <typename T>
 __global__ __launch_bounds__(LV_NT) void k_lvl_merge(const int64_t* __restrict__ indptr,
                                                      const int32_t* __restrict__ indices,
@@ -496,7 +621,12 @@ __global__ __launch_bounds__(LV_NT) void k_lvl_merge(const int64_t* __restrict__
   __shared__ int32_t s_lcol[LV_BIG];
   __shared__ int32_t s_mcol[LV_BIG];
   __shared__ T s_mval[LV_BIG];
+  __shared__ RowMeta<T> s_meta[LV_WAVES];
+  __shared__ int32_t s_len[LV_WAVES][LV_MAXD + 2];
   __shared__ int64_t s_seg[LV_WAVES + 1];
+  __shared__ int32_t s_v[LV_WAVES];
+  __shared__ int64_t s_e0[LV_WAVES];
+  __shared__ int s_nd[LV_WAVES];
   __shared__ int s_wsum[LV_WAVES];
   __shared__ int64_t s_cmax, s_fmax;
   int64_t* ctr = ws.counters;
@@ -515,35 +645,35 @@ __global__ __launch_bounds__(LV_NT) void k_lvl_merge(const int64_t* __restrict__
     if (threadIdx.x <= LV_WAVES) {
       const int64_t t = t0 + threadIdx.x < Ls ? t0 + threadIdx.x : Ls;
       s_seg[threadIdx.x] = scan_at(ws.cand_loc, ws.sums_a, t);
+      if (threadIdx.x < LV_WAVES && t < Ls) {
+        const int32_t v = ws.order[L0 + t];
+        const int64_t e0 = indptr[v];
+        s_v[threadIdx.x] = v;
+        s_e0[threadIdx.x] = e0;
+        s_nd[threadIdx.x] = (int)(indptr[v + 1] - e0);
+      }
     }
     __syncthreads();
-    // vertices merged by one wave
+    // rows merged by one wave
     {
       const int64_t t = t0 + wv;
       const int64_t cnt = s_seg[wv + 1] - s_seg[wv];
-      if (t < Ls && cnt <= LV_SMALL) {
-        RowJob<T> J;
-        J.v = ws.order[L0 + t];
-        J.e0 = indptr[J.v];
-        J.e1 = indptr[J.v + 1];
-        J.nd = J.e1 - J.e0;
-        J.cnt = cnt;
-        J.dst = abase + s_seg[wv];
-        int32_t* lcol = s_lcol + wv * LV_SMALL;
-        int32_t* mcol = s_mcol + wv * LV_SMALL;
-        T* mval = s_mval + wv * LV_SMALL;
-        stage_lists<T>(J, indices, ws, ar, lcol, lane, 64);
-        wave_sync();
-        merge_lists<T>(J, indices, data, ws, ar, lcol, mcol, mval, lane, 64, width, cmax, evs);
-        wave_sync();
+      if (t < Ls && cnt <= LV_SMALL && s_nd[wv] <= LV_MAXD) {
+        const int32_t v = s_v[wv];
+        const int64_t dst = abase + s_seg[wv];
+        const int base = wv * LV_SMALL;
+        merge_row_lds<T, 64>(indices, data, ws, ar, v, s_e0[wv], s_nd[wv], cnt, base, s_lcol, s_mcol, s_mval,
+                             s_meta[wv], s_len[wv], lane, width, cmax, evs, true);
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+        __builtin_amdgcn_wave_barrier();
         int64_t out = 0;
         for (int64_t b0 = 0; b0 < cnt; b0 += 64) {
           const int64_t i = b0 + lane;
-          const int32_t c = i < cnt ? mcol[i] : -1;
-          const T val = i < cnt ? mval[i] : (T)0;
+          const int32_t c = i < cnt ? s_mcol[base + i] : -1;
+          const T val = i < cnt ? s_mval[base + i] : (T)0;
           const uint64_t msk = __ballot(c >= 0);
           if (c >= 0) {
-            const int64_t o = J.dst + out + __popcll(msk & ((1ull << lane) - 1ull));
+            const int64_t o = dst + out + __popcll(msk & ((1ull << lane) - 1ull));
             ar.cols[o] = c;
             ar.vals[o] = val;
             const T a = val < 0 ? (T)-val : val;
@@ -552,33 +682,35 @@ __global__ __launch_bounds__(LV_NT) void k_lvl_merge(const int64_t* __restrict__
           out += __popcll(msk);
         }
         if (lane == 0) {
-          ws.row_off[J.v] = J.dst;
-          ws.row_len[J.v] = out;
+          ws.row_off[v] = dst;
+          ws.row_len[v] = out;
         }
       }
     }
     __syncthreads();
-    // vertices merged by the workgroup
+    // rows merged by the workgroup
     for (int j = 0; j < LV_WAVES; j++) {
       const int64_t t = t0 + j;
       const int64_t cnt = s_seg[j + 1] - s_seg[j];
-      if (t >= Ls || cnt <= LV_SMALL) continue;
-      RowJob<T> J;
-      J.v = ws.order[L0 + t];
-      J.e0 = indptr[J.v];
-      J.e1 = indptr[J.v + 1];
-      J.nd = J.e1 - J.e0;
-      J.cnt = cnt;
-      J.dst = abase + s_seg[j];
-      const bool in_lds = cnt <= LV_BIG;
-      int32_t* mcol = in_lds ? s_mcol : ar.cols + J.dst;
-      T* mval = in_lds ? s_mval : ar.vals + J.dst;
+      if (t >= Ls || (cnt <= LV_SMALL && s_nd[j] <= LV_MAXD)) continue;
+      const int32_t v = s_v[j];
+      const int64_t dst = abase + s_seg[j];
+      const bool in_lds = cnt <= LV_BIG && s_nd[j] <= LV_MAXD;
+      int32_t* mcol = in_lds ? s_mcol : ar.cols + dst;
+      T* mval = in_lds ? s_mval : ar.vals + dst;
       if (in_lds) {
-        stage_lists<T>(J, indices, ws, ar, s_lcol, threadIdx.x, LV_NT);
-        __syncthreads();
+        merge_row_lds<T, LV_NT>(indices, data, ws, ar, v, s_e0[j], s_nd[j], cnt, 0, s_lcol, s_mcol, s_mval,
+                                s_meta[0], s_len[0], threadIdx.x, width, cmax, evs, wv == 0);
+      } else {
+        RowJob<T> J;
+        J.v = v;
+        J.e0 = s_e0[j];
+        J.e1 = s_e0[j] + s_nd[j];
+        J.nd = s_nd[j];
+        J.cnt = cnt;
+        J.dst = dst;
+        merge_lists<T>(J, indices, data, ws, ar, nullptr, mcol, mval, threadIdx.x, LV_NT, width, cmax, evs);
       }
-      merge_lists<T>(J, indices, data, ws, ar, in_lds ? s_lcol : nullptr, mcol, mval, threadIdx.x, LV_NT, width,
-                     cmax, evs);
       __syncthreads();
       int64_t out = 0;
       for (int64_t b0 = 0; b0 < cnt; b0 += LV_NT) {
@@ -594,7 +726,7 @@ __global__ __launch_bounds__(LV_NT) void k_lvl_merge(const int64_t* __restrict__
           total += s_wsum[q];
         }
         if (c >= 0) {
-          const int64_t o = J.dst + out + before + __popcll(msk & ((1ull << lane) - 1ull));
+          const int64_t o = dst + out + before + __popcll(msk & ((1ull << lane) - 1ull));
           ar.cols[o] = c;
           ar.vals[o] = val;
           const T a = val < 0 ? (T)-val : val;
@@ -604,11 +736,11 @@ __global__ __launch_bounds__(LV_NT) void k_lvl_merge(const int64_t* __restrict__
         __syncthreads();
       }
       if (threadIdx.x == 0) {
-        ws.row_off[J.v] = J.dst;
-        ws.row_len[J.v] = out;
+        ws.row_off[v] = dst;
+        ws.row_len[v] = out;
       }
     }
-    __syncthreads();  // s_seg is rewritten by the next chunk
+    __syncthreads();  // s_seg and the row metadata are rewritten by the next chunk
   }
   if (cmax) atomicMax((long long*)&s_cmax, (long long)cmax);
   if (fmax) atomicMax((long long*)&s_fmax, (long long)fmax);
