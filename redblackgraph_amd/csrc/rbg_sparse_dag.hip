@@ -221,9 +221,21 @@ struct DagWork {
   int64_t* cand_loc;   // [n+1] block-local scan of cnt
   int64_t* sums_a;     // scan block sums (vertex scans)
   int64_t* counters;   // [C_NCTR]
+  // the current level's vertices by slot (written by the count step): vertex,
+  // first entry, direct entries, lists; and for rows with at most SLOT_D
+  // direct entries, each non-self successor's arena row, length and edge value
+  int32_t* slot_v;     // [n]
+  int64_t* slot_e0;    // [n]
+  int32_t* slot_nd;    // [n]
+  int32_t* slot_nl;    // [n]
+  int64_t* slot_ro;    // [n * SLOT_D]
+  int32_t* slot_len;   // [n * SLOT_D]
+  int64_t* slot_av;    // [n * SLOT_D] (edge values, stored as int64)
+  static constexpr int SLOT_D = 7;
   static int64_t bytes(int64_t n) {
     int64_t b = 4 * n * 4 + 8 * n + 8 * (n + 1) * 3 + 8 * (n / SCAN_CHUNK + 8) + 8 * C_NCTR;
-    return b + 16 * 256;
+    b += n * (4 + 8 + 4 + 4) + n * SLOT_D * (8 + 4 + 8);
+    return b + 24 * 256;
   }
   static DagWork carve(void* base, int64_t n) {
     DagWork w;
@@ -243,6 +255,13 @@ struct DagWork {
     w.cand_loc = (int64_t*)take(8 * (n + 1));
     w.sums_a = (int64_t*)take(8 * (n / SCAN_CHUNK + 8));
     w.counters = (int64_t*)take(8 * C_NCTR);
+    w.slot_v = (int32_t*)take(4 * n);
+    w.slot_e0 = (int64_t*)take(8 * n);
+    w.slot_nd = (int32_t*)take(4 * n);
+    w.slot_nl = (int32_t*)take(4 * n);
+    w.slot_ro = (int64_t*)take(8 * n * SLOT_D);
+    w.slot_len = (int32_t*)take(4 * n * SLOT_D);
+    w.slot_av = (int64_t*)take(8 * n * SLOT_D);
     return w;
   }
 };
@@ -315,7 +334,10 @@ __global__ __launch_bounds__(256) void k_round_place(const int64_t* __restrict__
 
 // Round r, step 2: level size, and the candidate count of every vertex of the
 // level: its direct entries plus the closure rows of its non-self successors.
-__global__ void k_lvl_count(const int64_t* __restrict__ indptr, const int32_t* __restrict__ indices, DagWork ws) {
+// The slot metadata the merge step needs is recorded on the way.
+template <typename T>
+__global__ void k_lvl_count(const int64_t* __restrict__ indptr, const int32_t* __restrict__ indices,
+                            const T* __restrict__ data, DagWork ws) {
   int64_t* ctr = ws.counters;
   if (dag_halted(ctr)) return;
   const int64_t L0 = ctr[C_LSTART], Ls = ctr[C_PLACED] - L0;
@@ -326,12 +348,30 @@ __global__ void k_lvl_count(const int64_t* __restrict__ indptr, const int32_t* _
       continue;
     }
     const int32_t v = ws.order[L0 + t];
+    const int64_t e0 = indptr[v], e1 = indptr[v + 1];
+    const bool keep = e1 - e0 <= DagWork::SLOT_D;
     int64_t c = 0;
-    for (int64_t e = indptr[v]; e < indptr[v + 1]; e++) {
-      int32_t w = indices[e];
-      c += (w == v) ? 1 : 1 + ws.row_len[w];
+    int k = 0;
+    for (int64_t e = e0; e < e1; e++) {
+      const int32_t w = indices[e];
+      if (w == v) {
+        c += 1;
+        continue;
+      }
+      const int64_t len = ws.row_len[w];
+      c += 1 + len;
+      if (keep) {
+        ws.slot_ro[t * DagWork::SLOT_D + k] = ws.row_off[w];
+        ws.slot_len[t * DagWork::SLOT_D + k] = (int32_t)len;
+        ws.slot_av[t * DagWork::SLOT_D + k] = (int64_t)data[e];
+      }
+      k++;
     }
     ws.cnt[t] = c;
+    ws.slot_v[t] = v;
+    ws.slot_e0[t] = e0;
+    ws.slot_nd[t] = (int32_t)(e1 - e0);
+    ws.slot_nl[t] = 1 + k;
   }
 }
 
@@ -485,7 +525,7 @@ __device__ inline void merge_lists(const RowJob<T>& J, const int32_t* __restrict
 // A[v][w_k] (x) C[w_k] for its k-th non-self successor (arena row at ro[k],
 // multiplier av[k]); lst[k] is the list's first candidate index (lst[nl] =
 // the row's candidate count).
-constexpr int LV_MAXD = 7;  // direct entries a row may have for the LDS paths
+constexpr int LV_MAXD = DagWork::SLOT_D;  // direct entries a row may have for the LDS paths
 template <typename T>
 struct RowMeta {
   int32_t nl;
@@ -499,24 +539,21 @@ struct RowMeta {
 // mcol/mval[base ...].  Every access to the staged lists is an LDS access.
 template <typename T, int NL>
 __device__ inline void merge_row_lds(const int32_t* __restrict__ indices, const T* __restrict__ data,
-                                     const DagWork& ws, const DagArena<T>& ar, int32_t v, int64_t e0, int nd,
+                                     const DagWork& ws, const DagArena<T>& ar, int64_t slot, int64_t e0, int nd,
                                      int64_t cnt, int base, int32_t* s_lcol, int32_t* s_mcol, T* s_mval,
                                      RowMeta<T>& M, int32_t* s_len, int lane, int width, T& cmax, uint32_t& evs,
                                      bool first_wave) {
-  // metadata: lanes < nd of the first wave read the direct entries
+  // metadata from the slot arrays the count step wrote
   if (first_wave) {
-    int32_t w = v;
-    if (lane < nd) w = indices[e0 + lane];
-    const bool succ = lane < nd && w != v;
-    const uint64_t msk = __ballot(succ);
-    if (succ) {
-      const int k = 1 + __popcll(msk & ((1ull << lane) - 1ull));
-      M.ro[k] = ws.row_off[w];
-      M.av[k] = data[e0 + lane];
-      s_len[k] = (int32_t)ws.row_len[w];
+    const int nl0 = ws.slot_nl[slot];
+    if (lane + 1 < nl0) {
+      const int64_t q = slot * DagWork::SLOT_D + lane;
+      M.ro[lane + 1] = ws.slot_ro[q];
+      M.av[lane + 1] = (T)ws.slot_av[q];
+      s_len[lane + 1] = ws.slot_len[q];
     }
     if (lane == 0) {
-      M.nl = 1 + __popcll(msk);
+      M.nl = nl0;
       M.ro[0] = e0;
       M.lst[0] = 0;
       M.lst[1] = nd;
@@ -524,7 +561,7 @@ __device__ inline void merge_row_lds(const int32_t* __restrict__ indices, const 
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
     if (lane == 0)
-      for (int k = 1; k < M.nl; k++) M.lst[k + 1] = M.lst[k] + s_len[k];
+      for (int k = 1; k < nl0; k++) M.lst[k + 1] = M.lst[k] + s_len[k];
   }
   if (NL == 64) {
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
@@ -641,19 +678,32 @@ __global__ __launch_bounds__(LV_NT) void k_lvl_merge(const int64_t* __restrict__
   if (threadIdx.x == 0) s_cmax = s_fmax = 0;
   T cmax = 0, fmax = 0;
   uint32_t evs = 0;
+  // threads 0..LV_WAVES load a chunk's slot data one chunk ahead
+  int64_t p_seg = 0, p_e0 = 0;
+  int32_t p_v = 0, p_nd = 0;
+  auto fetch = [&](int64_t t0) {
+    if (threadIdx.x <= LV_WAVES && t0 < Ls) {
+      const int64_t t = t0 + threadIdx.x < Ls ? t0 + threadIdx.x : Ls;
+      p_seg = scan_at(ws.cand_loc, ws.sums_a, t);
+      if (threadIdx.x < LV_WAVES && t < Ls) {
+        p_v = ws.slot_v[t];
+        p_e0 = ws.slot_e0[t];
+        p_nd = ws.slot_nd[t];
+      }
+    }
+  };
+  fetch((int64_t)blockIdx.x * LV_WAVES);
   for (int64_t t0 = (int64_t)blockIdx.x * LV_WAVES; t0 < Ls; t0 += (int64_t)gridDim.x * LV_WAVES) {
     if (threadIdx.x <= LV_WAVES) {
-      const int64_t t = t0 + threadIdx.x < Ls ? t0 + threadIdx.x : Ls;
-      s_seg[threadIdx.x] = scan_at(ws.cand_loc, ws.sums_a, t);
-      if (threadIdx.x < LV_WAVES && t < Ls) {
-        const int32_t v = ws.order[L0 + t];
-        const int64_t e0 = indptr[v];
-        s_v[threadIdx.x] = v;
-        s_e0[threadIdx.x] = e0;
-        s_nd[threadIdx.x] = (int)(indptr[v + 1] - e0);
+      s_seg[threadIdx.x] = p_seg;
+      if (threadIdx.x < LV_WAVES) {
+        s_v[threadIdx.x] = p_v;
+        s_e0[threadIdx.x] = p_e0;
+        s_nd[threadIdx.x] = p_nd;
       }
     }
     __syncthreads();
+    fetch(t0 + (int64_t)gridDim.x * LV_WAVES);
     // rows merged by one wave
     {
       const int64_t t = t0 + wv;
@@ -662,7 +712,7 @@ __global__ __launch_bounds__(LV_NT) void k_lvl_merge(const int64_t* __restrict__
         const int32_t v = s_v[wv];
         const int64_t dst = abase + s_seg[wv];
         const int base = wv * LV_SMALL;
-        merge_row_lds<T, 64>(indices, data, ws, ar, v, s_e0[wv], s_nd[wv], cnt, base, s_lcol, s_mcol, s_mval,
+        merge_row_lds<T, 64>(indices, data, ws, ar, t, s_e0[wv], s_nd[wv], cnt, base, s_lcol, s_mcol, s_mval,
                              s_meta[wv], s_len[wv], lane, width, cmax, evs, true);
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
         __builtin_amdgcn_wave_barrier();
@@ -699,7 +749,7 @@ __global__ __launch_bounds__(LV_NT) void k_lvl_merge(const int64_t* __restrict__
       int32_t* mcol = in_lds ? s_mcol : ar.cols + dst;
       T* mval = in_lds ? s_mval : ar.vals + dst;
       if (in_lds) {
-        merge_row_lds<T, LV_NT>(indices, data, ws, ar, v, s_e0[j], s_nd[j], cnt, 0, s_lcol, s_mcol, s_mval,
+        merge_row_lds<T, LV_NT>(indices, data, ws, ar, t, s_e0[j], s_nd[j], cnt, 0, s_lcol, s_mcol, s_mval,
                                 s_meta[0], s_len[0], threadIdx.x, width, cmax, evs, wv == 0);
       } else {
         RowJob<T> J;
@@ -1186,7 +1236,7 @@ int64_t rbg_dag_workspace_bytes(int32_t n) { return DagWork::bytes(n > 0 ? n : 1
     for (int32_t r = 0; r <= n; r++) {                                                                          \
       hipLaunchKernelGGL(k_round_place, dim3(gv), dim3(256), 0, s, indptr, indices, ws, r);                     \
       RBG_CHECK_LAUNCH();                                                                                       \
-      hipLaunchKernelGGL(k_lvl_count, dim3(gv), dim3(256), 0, s, indptr, indices, ws);                          \
+      hipLaunchKernelGGL((k_lvl_count<T>), dim3(gv), dim3(256), 0, s, indptr, indices, data, ws);                \
       RBG_CHECK_LAUNCH();                                                                                       \
       int rc = scan_dev(ws.cnt, ws.cand_loc, ws.sums_a, ws.counters + C_LSIZE, 1, n, ws.counters + C_M, s);    \
       if (rc) return rc;                                                                                        \
