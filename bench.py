@@ -317,7 +317,28 @@ def bench_batched(args):
     t1 = time.perf_counter()
     ms = (t1 - t0) * 1e3 / args.steps
     identical_graph = bool(t.equal(b.pool, eager_result))
-    # plain path: one closure per component through the single-matrix entry point
+    # plain path: the same kernels launched one graph at a time on one stream
+    # (a one-component schedule per graph, built before the clock starts)
+    handles = []
+    for i in range(b.count):
+        one = (_lib.RbgBatchComp * 1)(b.descs[i])
+        h = ctypes.c_void_p()
+        _lib.call("rbg_batch_create", one, 1, ctypes.byref(h))
+        handles.append(h)
+    sh = _lib.stream_handle(b.stream)
+    with t.cuda.stream(b.stream):
+        b.pool.copy_(pristine)
+    b.stream.synchronize()
+    tp0 = time.perf_counter()
+    for h in handles:
+        _lib.call("rbg_batch_run", h, sh)
+    b.stream.synchronize()
+    plain_ms = (time.perf_counter() - tp0) * 1e3
+    identical_plain = bool(t.equal(b.pool, eager_result))
+    for h in handles:
+        L.rbg_batch_destroy(h)
+    # the single-matrix entry point, one closure per component (its own panel
+    # kernel, a status read per closure)
     et = elem_type(np.int32)
     views = []
     for i, n in enumerate(sizes):
@@ -325,11 +346,11 @@ def bench_batched(args):
         views.append(pristine[o:o + p * p].view(p, p)[:int(n), :int(n)])
     t.cuda.synchronize()
     tp0 = time.perf_counter()
-    plain = [dense.fw_closure(V, et)[0] for V in views]
+    single = [dense.fw_closure(V, et)[0] for V in views]
     t.cuda.synchronize()
-    plain_ms = (time.perf_counter() - tp0) * 1e3
-    identical_plain = all(bool(t.equal(plain[i], b.result(i))) for i in range(len(sizes)))
-    del plain
+    single_ms = (time.perf_counter() - tp0) * 1e3
+    identical_single = all(bool(t.equal(single[i], b.result(i))) for i in range(len(sizes)))
+    del single
     P = b.layout.npad.astype(np.float64)
     relax_exec = float(np.sum((P - FW_BLK) ** 2 * P))
     tile_s = ms_.value / 1e3
@@ -353,9 +374,12 @@ def bench_batched(args):
                                 f"hipGraph replay per step (input reloaded by one device copy inside the step)"),
                    "components": len(sizes), "sum_n": int(sizes.sum()), "sum_n2": entries,
                    "closures_per_s": len(sizes) / (ms / 1e3),
+                   "plain": "the same kernels launched one graph at a time on one stream",
                    "plain_ms": plain_ms, "plain_closures_per_s": len(sizes) / (plain_ms / 1e3),
                    "batched_vs_plain": plain_ms / ms,
                    "identical_to_plain": identical_plain, "graph_replay_identical_to_eager": identical_graph,
+                   "single_matrix_path_ms": single_ms, "batched_vs_single_matrix_path": single_ms / ms,
+                   "identical_to_single_matrix_path": identical_single,
                    "launches_per_batch": info["launches"], "pivot_steps": info["steps"],
                    "tiles": {str(k): v for k, v in sorted(b.tiles.items())}, "autotune_s": tune_s},
         "roofline": roof,

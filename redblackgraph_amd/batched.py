@@ -98,6 +98,7 @@ class BatchedClosure:
         h = ctypes.c_void_p()
         _lib.call("rbg_batch_create", descs, self.count, ctypes.byref(h))
         self._h = h
+        self.descs = descs
         self.captured = False
         self.stream.wait_stream(t.cuda.current_stream())  # the pool's zero fill
 

@@ -38,7 +38,10 @@ constexpr int CPT = 8;     // phase-3 cols per thread
 
 struct FwWork {
   // two snapshot buffers (block parity), each BLK x n for columns and rows,
-  // then 2 x 2 x BLK guard words
+  // then the guard words of the current pivot block: per 32-block b and k,
+  // gblk[b][0][k] = the largest MSB among the left operands of rows of block b
+  // (column snapshot k) and gblk[b][1][k] = among the right operands of
+  // columns of block b (row snapshot k)
   void* base;
   int64_t n;
   int elem;
@@ -50,10 +53,8 @@ struct FwWork {
   __host__ __device__ T* row(int parity) const {
     return (T*)base + (size_t)parity * 2 * BLK * n + (size_t)BLK * n;
   }
-  __host__ __device__ int* guard(int parity) const {  // [0..BLK) col msb, [BLK..2BLK) row msb
-    return (int*)((char*)base + (size_t)4 * BLK * n * elem) + parity * 2 * BLK;
-  }
-  static int64_t bytes(int64_t n, int elem) { return (int64_t)4 * BLK * n * elem + 4 * BLK * 4 + 256; }
+  __host__ __device__ int* gblk() const { return (int*)((char*)base + (size_t)4 * BLK * n * elem); }
+  static int64_t bytes(int64_t n, int elem) { return (int64_t)4 * BLK * n * elem + (n / BLK) * 2 * BLK * 4 + 256; }
 };
 
 __device__ inline uint64_t naive_key(int64_t k, int64_t i, int64_t j, int64_t n) {
@@ -113,10 +114,6 @@ static __global__ void k_status_init(rbg_status* st) {
   for (int i = 0; i < 8; i++) st->aux[i] = 0;
 }
 
-static __global__ void k_guard_init(int* g) {
-  if (threadIdx.x < 4 * BLK) g[threadIdx.x] = 0;
-}
-
 // Precondition of the fast kernels: every value >= -1, no off-diagonal +-1,
 // every diagonal value in {-1, 0, 1}.  With such diagonals the products that
 // involve a pivot cell (k, k) can neither overflow nor reveal a cycle, which
@@ -158,14 +155,12 @@ __global__ __launch_bounds__(NT) void k_fw_panel(T* W, int64_t n, int64_t ld, in
                                                  int width, int is_uns, rbg_status* st) {
   __shared__ T Dg[BLK][BLK + 1];
   __shared__ T Pn[BLK][BLK + 1];
-  __shared__ int sguard[BLK];
   const int tid = threadIdx.x;
   const int nb = (int)(n / BLK);
   const int64_t k0 = K * BLK;
   const int par = (int)(K & 1);
   T* colT = ws.col<T>(par);
   T* rowS = ws.row<T>(par);
-  int* guard = ws.guard(par);
 
   // panel role: 0 = diagonal only, 1 = row panel (K, J), 2 = column panel (J, K)
   int role = 0, J = 0;
@@ -177,12 +172,7 @@ __global__ __launch_bounds__(NT) void k_fw_panel(T* W, int64_t n, int64_t ld, in
   }
   const bool owner = blockIdx.x == 0;
 
-  if (owner && tid == 0) {
-    if (K > 0) capture_block<T, FAST>(st, ws.col<T>(par ^ 1), ws.row<T>(par ^ 1), k0 - BLK, n);
-    int* gnext = ws.guard(par ^ 1);
-    for (int q = 0; q < 2 * BLK; q++) gnext[q] = 0;
-  }
-  if (tid < BLK) sguard[tid] = 0;
+  if (owner && tid == 0 && K > 0) capture_block<T, FAST>(st, ws.col<T>(par ^ 1), ws.row<T>(par ^ 1), k0 - BLK, n);
 
   // load tiles (natural domain)
   const int64_t pr0 = role == 2 ? (int64_t)J * BLK : k0;  // panel row origin
@@ -223,19 +213,10 @@ __global__ __launch_bounds__(NT) void k_fw_panel(T* W, int64_t n, int64_t ld, in
       }
     }
     if (role == 1 && i == 0) {
-      int mm = 0;
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
-        rowS[kk * n + pc0 + j0 + q] = snap_enc<T, FAST>(bP[q]);
-        int m = bP[q] > 1 ? msb<T>(bP[q]) : 0;
-        mm = m > mm ? m : mm;
-      }
-      atomicMax(&sguard[kk], mm);
+      for (int q = 0; q < 4; q++) rowS[kk * n + pc0 + j0 + q] = snap_enc<T, FAST>(bP[q]);
     }
-    if (role == 2 && (tid & 7) == 0) {
-      colT[kk * n + pr0 + i] = snap_enc<T, FAST>(aP);
-      atomicMax(&sguard[kk], aP > 1 ? msb<T>(aP) : 0);
-    }
+    if (role == 2 && (tid & 7) == 0) colT[kk * n + pr0 + i] = snap_enc<T, FAST>(aP);
     __syncthreads();
     // diagonal tile
 #pragma unroll
@@ -275,10 +256,6 @@ __global__ __launch_bounds__(NT) void k_fw_panel(T* W, int64_t n, int64_t ld, in
     if (owner) W[(k0 + r) * ld + k0 + c] = snap_enc<T, FAST>(Dg[r][c]);
     if (role) W[(pr0 + r) * ld + pc0 + c] = snap_enc<T, FAST>(Pn[r][c]);
   }
-  if (tid < BLK) {
-    int g = sguard[tid];
-    if (g) atomicMax(&guard[(role == 1 ? BLK : 0) + tid], g);
-  }
   dmax = wave_max_i64(dmax);
   if ((tid & 63) == 0) record_diameter(st, dmax);
 }
@@ -303,189 +280,209 @@ __device__ inline void st4(uint64_t* p, const uint64_t (&v)[4]) {
 }
 
 // ---------------------------------------------------------------------------
-// Panel kernel, fast mode, for one matrix (k_fw_panel_fast): same schedule as
-// k_fw_panel, E-domain arithmetic.  2 (n/32 - 1) workgroups each walk the
-// diagonal block together with one 32 x 32 tile of the row or column panel,
-// which spreads a block's short dependent chain over the whole chip.  The
-// panels also leave, per k, the largest MSB of the column and row snapshots
-// in the guard words that phase 3 reads.
-// At step k the pivot row and column do not change (they are skipped), so the
-// step reads them while it updates every other cell and needs one barrier.
-// Operands off the pivot row/column are never +-1 (precheck), so the product
-// is (x << MSB(y)) | low(y) there.  A per-product MSB check raises
-// RBG_F_NEED_GENERAL before any value could overflow.
+// Panel kernel, fast mode, for one matrix (k_fw_panel_fast): phases 1 and 2
+// of pivot block K in the E domain.  Workgroup P (PANEL_NT threads, P = 0 ..
+// n/32 - 2) takes panel J = P < K ? P : P + 1, that is the row-panel tile
+// (block K, block J) and the column-panel tile (block J, block K), and walks
+// both through the 32 steps of block K together with the diagonal block
+// (redundantly; workgroup 0 owns the diagonal block's results).  Thread
+// (i, j) keeps cell (i, j) of each of the three tiles in registers:
+//     diagonal   D[i][j]  = D[i][j]  (+) D[i][k]  (x) D[k][j]
+//     row panel  R[i][j]  = R[i][j]  (+) D[i][k]  (x) R[k][j]
+//     col panel  C[i][j]  = C[i][j]  (+) C[i][k]  (x) D[k][j]
+// Right operands are used as (shift, low bits) pairs: y -> (MSB(y), y minus
+// its top bit), so a product is one shift-or.  The threads that hold the
+// pivot row of step k+1 compute its pairs, and the threads of row and column
+// k+1 write the step-(k+1) snapshots, as they finish step k; one barrier
+// separates the steps.
+//
+// Pivot exclusions without tests: while the block is walked its diagonal
+// cells hold "absent" in LDS (their values are kept aside), so at step k the
+// products of the pivot row and pivot column have an absent operand and leave
+// those cells as they are -- what the natural-domain rules give for a
+// product with a +-1 diagonal entry.  A diagonal cell could only receive a
+// present product through a cycle, and cycles are found after the walk from
+// the snapshots (step k, vertex v: W[v][k] and W[k][v] both present); a cycle
+// sets RBG_F_NEED_GENERAL, since the caller reruns general mode for its
+// exact detections.
+//
 // A product with an absent operand is some value with the top bit set; the
 // min can store it into an absent cell.  Such a value must not be shifted
-// again (the shift could clear its top bit), so every operand read here is
-// first canonicalised to all ones, and so is every snapshot phase 3 reads.
+// again (the shift could clear its top bit), so left operands are
+// canonicalised to all ones when read, and so is every snapshot phase 3
+// reads.  Per product, MSB(x) + MSB(y) > width - 2 raises RBG_F_NEED_GENERAL
+// before any value could overflow.  Each workgroup also leaves, per k, the
+// largest MSB of its column-panel and row-panel snapshots in block J's guard
+// words (plain stores; phase 3 combines the blocks of its own tile).
 // ---------------------------------------------------------------------------
+constexpr int PANEL_NT = BLK * BLK;
+
 template <typename T>
-__device__ inline void fw_panel_tiles_body(T* W, int64_t n, int64_t ld, int64_t K, FwWork ws, int width,
+__device__ inline void fw_panel_pair_body(T* W, int64_t n, int64_t ld, int64_t K, FwWork ws, int width,
                                           rbg_status* st, int bid) {
   using U = typename unsigned_of<T>::type;
   constexpr U TOP = (U)1 << (sizeof(U) * 8 - 1);
+  constexpr U ALL = ~(U)0;
   __shared__ U Dg[BLK][BLK + 1];
-  __shared__ U Pn[BLK][BLK + 1];
-  // snapshots of step kk, staged in LDS and written out after the loop so the
-  // per-step barrier never waits on global stores
-  __shared__ U sDc[BLK][BLK + 1];  // [kk][i] = diag W[i][kk]
-  __shared__ U sDr[BLK][BLK + 1];  // [kk][j] = diag W[kk][j]
-  __shared__ U sP[BLK][BLK + 1];   // row panel: [kk][j] = W[kk][j]; column panel: [kk][i] = W[i][kk]
+  __shared__ U Pr[BLK][BLK + 1];
+  __shared__ U Pc[BLK][BLK + 1];
+  // snapshots of step kk: [kk][x]
+  __shared__ U sDc[BLK][BLK + 1];  // diagonal column kk, row x
+  __shared__ U sDr[BLK][BLK + 1];  // diagonal row kk, column x
+  __shared__ U sPr[BLK][BLK + 1];  // row-panel row kk, column x
+  __shared__ U sPc[BLK][BLK + 1];  // column-panel column kk, row x
+  // (shift, low bits) of the pivot rows, by step parity
+  __shared__ U MYd[2][BLK][2];
+  __shared__ U MYr[2][BLK][2];
+  __shared__ U sDiag[BLK];
+  __shared__ T sMax[PANEL_NT / 64];
   __shared__ int sbad;
   const int tid = threadIdx.x;
-  const int nb = (int)(n / BLK);
+  const int i = tid >> 5, j = tid & (BLK - 1);
   const int64_t k0 = K * BLK;
-  const int par = (int)(K & 1);
-  T* colT = ws.col<T>(par);
-  T* rowS = ws.row<T>(par);
-  int* guard = ws.guard(par);
-
-  int role = 0, J = 0;
-  if (nb > 1) {
-    int t = bid;
-    role = t < nb - 1 ? 1 : 2;
-    int idx = role == 1 ? t : t - (nb - 1);
-    J = idx < K ? idx : idx + 1;
-  }
+  const int J = bid < K ? bid : bid + 1;
+  const int64_t j0 = (int64_t)J * BLK;
   const bool owner = bid == 0;
-  // a workgroup-uniform exit once the matrix is known to need general mode
+  const int par = (int)(K & 1);
+  const int lim = width - 2;
+  auto canon = [](U x) -> U { return (T)x < 0 ? ALL : x; };
+  auto pair = [](U b, U& m, U& y) {
+    if ((T)b < 0) {
+      m = 0;
+      y = TOP;
+    } else {
+      m = (U)msb_u(b);
+      y = b ^ ((U)1 << m);
+    }
+  };
+  auto amsb = [](U a) -> int { return (T)a > 1 ? msb_u(a) : 0; };
+
+  // the three cells of this thread, loaded before anything waits
+  U d = canon((U)W[(k0 + i) * ld + k0 + j]);
+  U r = canon((U)W[(k0 + i) * ld + j0 + j]);
+  U c = canon((U)W[(j0 + i) * ld + k0 + j]);
   if (tid == 0) sbad = (st->flags & (RBG_F_NEED_GENERAL | RBG_F_PRECOND_FAIL)) ? 2 : 0;
+  if (i == j) {
+    sDiag[i] = d;
+    d = ALL;
+  }
+  Dg[i][j] = d;
+  Pr[i][j] = r;
+  Pc[i][j] = c;
+  // step 0: pivot row 0's pairs and the step-0 snapshots
+  if (i == 0) {
+    pair(d, MYd[0][j][0], MYd[0][j][1]);
+    pair(r, MYr[0][j][0], MYr[0][j][1]);
+    sDr[0][j] = d;
+    sPr[0][j] = r;
+  }
+  if (j == 0) {
+    sDc[0][i] = d;
+    sPc[0][i] = c;
+  }
   __syncthreads();
   if (sbad) return;
-  if (owner && tid == 0) {
-    if (K > 0) capture_block<T, true>(st, ws.col<T>(par ^ 1), ws.row<T>(par ^ 1), k0 - BLK, n);
-    int* gnext = ws.guard(par ^ 1);
-    for (int q = 0; q < 2 * BLK; q++) gnext[q] = 0;
-  }
+  if (owner && tid == 0 && K > 0) capture_block<T, true>(st, ws.col<T>(par ^ 1), ws.row<T>(par ^ 1), k0 - BLK, n);
+  const int64_t seen = st->diameter_max;  // a lower bound of the value at exit
 
-  const int64_t pr0 = role == 2 ? (int64_t)J * BLK : k0;
-  const int64_t pc0 = role == 1 ? (int64_t)J * BLK : k0;
-  auto canon = [](U x) -> U { return (T)x < 0 ? ~(U)0 : x; };
-  for (int e = tid; e < BLK * BLK; e += NT) {
-    int r = e / BLK, c = e % BLK;
-    Dg[r][c] = canon((U)W[(k0 + r) * ld + k0 + c]);
-    if (role) Pn[r][c] = canon((U)W[(pr0 + r) * ld + pc0 + c]);
-  }
-  __syncthreads();
-
-  const int i = tid >> 3;
-  const int j0 = (tid & 7) * 4;
-  const int lim = width - 2;
-  U dv[4], pv[4];
-#pragma unroll
-  for (int q = 0; q < 4; q++) {
-    dv[q] = Dg[i][j0 + q];
-    pv[q] = role ? Pn[i][j0 + q] : (U)0;
-  }
-  T dmax = 0;
-  bool bad = false;
+  T dd = 0, dr = 0, dc = 0;
+  int g = 0;
+#pragma unroll 2
   for (int kk = 0; kk < BLK; kk++) {
-    const int64_t k = k0 + kk;
-    const U aD = canon(Dg[i][kk]);
-    U bD[4], bP[4], aP = 0;
-#pragma unroll
-    for (int q = 0; q < 4; q++) bD[q] = canon(Dg[kk][j0 + q]);
-    if (role == 1) {
-      aP = aD;
-#pragma unroll
-      for (int q = 0; q < 4; q++) bP[q] = canon(Pn[kk][j0 + q]);
-    } else if (role == 2) {
-      aP = canon(Pn[i][kk]);
-#pragma unroll
-      for (int q = 0; q < 4; q++) bP[q] = bD[q];
-    }
-    if ((tid & 7) == 0) {
-      sDc[kk][i] = aD;
-      if (role == 2) sP[kk][i] = aP;
-    }
-    if (i == 0) {
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        sDr[kk][j0 + q] = bD[q];
-        if (role == 1) sP[kk][j0 + q] = bP[q];
+    const int q = kk & 1;
+    const U ad = canon(Dg[i][kk]);
+    const U ac = canon(Pc[i][kk]);
+    const U md = MYd[q][j][0], yd = MYd[q][j][1];
+    const U mr = MYr[q][j][0], yr = MYr[q][j][1];
+    const int gd = amsb(ad) + (int)(md > mr ? md : mr);
+    const int gc = amsb(ac) + (int)md;
+    g = g > gd ? g : gd;
+    g = g > gc ? g : gc;
+    const U pd = (ad << md) | yd;
+    const U pr = (ad << mr) | yr;
+    const U pc = (ac << md) | yd;
+    d = d < pd ? d : pd;
+    r = r < pr ? r : pr;
+    c = c < pc ? c : pc;
+    dd = dd > (T)d ? dd : (T)d;
+    dr = dr > (T)r ? dr : (T)r;
+    dc = dc > (T)c ? dc : (T)c;
+    Dg[i][j] = d;
+    Pr[i][j] = r;
+    Pc[i][j] = c;
+    if (kk + 1 < BLK) {
+      if (i == kk + 1) {
+        const U cd = canon(d), cr = canon(r);
+        pair(cd, MYd[q ^ 1][j][0], MYd[q ^ 1][j][1]);
+        pair(cr, MYr[q ^ 1][j][0], MYr[q ^ 1][j][1]);
+        sDr[kk + 1][j] = cd;
+        sPr[kk + 1][j] = cr;
       }
-    }
-    // diagonal tile: row kk and column kk are invariant at step kk
-    const int maD = (T)aD > 1 ? msb_u(aD) : 0;
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const int j = j0 + q;
-      if (i != kk && j != kk) {
-        U m, y;
-        if ((T)bD[q] < 0) {
-          m = 0;
-          y = TOP;
-        } else {
-          m = (U)msb_u(bD[q]);
-          y = bD[q] & (((U)1 << m) - 1);
-        }
-        if ((T)bD[q] > 1 && maD + (int)m > lim) bad = true;
-        U p = (aD << m) | y;
-        if (owner && i == j && (T)p >= 0) record_cycle(st, naive_key(k, k0 + i, k0 + i, n));
-        dv[q] = dv[q] < p ? dv[q] : p;
-        Dg[i][j] = dv[q];
-      }
-      if (owner && (T)dv[q] > dmax) dmax = (T)dv[q];
-    }
-    if (role) {
-      const int maP = (T)aP > 1 ? msb_u(aP) : 0;
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const int j = j0 + q;
-        const bool pivot = role == 1 ? (i == kk) : (j == kk);
-        if (!pivot) {
-          U m, y;
-          if ((T)bP[q] < 0) {
-            m = 0;
-            y = TOP;
-          } else {
-            m = (U)msb_u(bP[q]);
-            y = bP[q] & (((U)1 << m) - 1);
-          }
-          if ((T)bP[q] > 1 && maP + (int)m > lim) bad = true;
-          U p = (aP << m) | y;
-          pv[q] = pv[q] < p ? pv[q] : p;
-          Pn[i][j] = pv[q];
-        }
-        if ((T)pv[q] > dmax) dmax = (T)pv[q];
+      if (j == kk + 1) {
+        sDc[kk + 1][i] = canon(d);
+        sPc[kk + 1][i] = canon(c);
       }
     }
     __syncthreads();
   }
-  if (bad) sbad = 1;
+  if (g > lim) sbad = 1;
+  // cycles: step kk, vertex x with W[x][kk] and W[kk][x] both present
+  if (owner && i != j && (T)sDc[i][j] >= 0 && (T)sDr[i][j] >= 0) {
+    record_cycle(st, naive_key(k0 + i, k0 + j, k0 + j, n));
+    sbad = 1;
+  }
   __syncthreads();
   if (sbad) {
     if (tid == 0) atomicOr(&st->flags, (uint32_t)RBG_F_NEED_GENERAL);
     return;
   }
-  // tiles and snapshots out, coalesced
-  for (int e = tid; e < BLK * BLK; e += NT) {
-    int r = e / BLK, c = e % BLK;
+  // tiles and snapshots out, coalesced (thread (i, j): snapshot step i, index j)
+  T* colT = ws.col<T>(par);
+  T* rowS = ws.row<T>(par);
+  if (owner) {
+    W[(k0 + i) * ld + k0 + j] = (T)(i == j ? sDiag[i] : d);
+    colT[i * n + k0 + j] = (T)sDc[i][j];
+    rowS[i * n + k0 + j] = (T)sDr[i][j];
+  }
+  W[(k0 + i) * ld + j0 + j] = (T)r;
+  rowS[i * n + j0 + j] = (T)sPr[i][j];
+  W[(j0 + i) * ld + k0 + j] = (T)c;
+  colT[i * n + j0 + j] = (T)sPc[i][j];
+  // guard words of block J: the largest MSB of column-panel snapshot kk (left
+  // operands of rows in block J) and of row-panel snapshot kk (right operands
+  // of columns in block J).  Block K's are zero: phase 3 skips its rows, and
+  // its columns' products were checked here.
+  int ga = amsb(sPc[i][j]);
+  int gb = (T)sPr[i][j] >= 0 ? msb_u(sPr[i][j]) : 0;
+#pragma unroll
+  for (int o = 16; o > 0; o >>= 1) {
+    const int ta = __shfl_xor(ga, o), tb = __shfl_xor(gb, o);
+    ga = ga > ta ? ga : ta;
+    gb = gb > tb ? gb : tb;
+  }
+  if (j == 0) {
+    int* gw = ws.gblk();
+    gw[(J * 2 + 0) * BLK + i] = ga;
+    gw[(J * 2 + 1) * BLK + i] = gb;
     if (owner) {
-      W[(k0 + r) * ld + k0 + c] = (T)Dg[r][c];
-      colT[r * n + k0 + c] = (T)sDc[r][c];
-      rowS[r * n + k0 + c] = (T)sDr[r][c];
-    }
-    if (role == 1) {
-      W[(pr0 + r) * ld + pc0 + c] = (T)Pn[r][c];
-      rowS[r * n + pc0 + c] = (T)sP[r][c];
-    } else if (role == 2) {
-      W[(pr0 + r) * ld + pc0 + c] = (T)Pn[r][c];
-      colT[r * n + pr0 + c] = (T)sP[r][c];
+      gw[(K * 2 + 0) * BLK + i] = 0;
+      gw[(K * 2 + 1) * BLK + i] = 0;
     }
   }
-  // per-k MSB guard for phase 3 from this panel's snapshots
-  if (role && tid < BLK) {
-    int mm = 0;
-    for (int c = 0; c < BLK; c++) {
-      U v = sP[tid][c];
-      int m = (T)v > 1 ? msb_u(v) : 0;
-      mm = m > mm ? m : mm;
-    }
-    if (mm) atomicMax(&guard[(role == 1 ? BLK : 0) + tid], mm);
+  // trajectory maximum: the workgroup's, then one atomic
+  T m = dr > dc ? dr : dc;
+  if (owner) {
+    m = m > dd ? m : dd;
+    if (i == j && (T)sDiag[i] > m) m = (T)sDiag[i];
   }
-  int64_t dm = wave_max_i64((int64_t)dmax);
-  if ((tid & 63) == 0) record_diameter(st, dm);
+  const int64_t wm = wave_max_i64((int64_t)m);
+  if ((tid & 63) == 0) sMax[tid >> 6] = (T)wm;
+  __syncthreads();
+  if (tid == 0) {
+    T x = 0;
+    for (int w = 0; w < PANEL_NT / 64; w++) x = x > sMax[w] ? x : sMax[w];
+    if ((int64_t)x > seen) record_diameter(st, (int64_t)x);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -531,6 +528,7 @@ __device__ inline void fw_panel_lines_body(T* W, int64_t n, int64_t ld, int64_t 
   __syncthreads();
   if (sbad) return;
   if (owner && tid == 0 && K > 0) capture_block<T, true>(st, ws.col<T>(par ^ 1), ws.row<T>(par ^ 1), k0 - BLK, n);
+  const int64_t seen = st->diameter_max;  // a lower bound of the value at exit
   for (int e = tid; e < BLK * BLK; e += NT) D[e / BLK][e % BLK] = canon((U)W[(k0 + e / BLK) * ld + k0 + e % BLK]);
   // this thread's panel line, loaded while the diagonal block is walked
   const bool rowp = (int64_t)bid * NT + tid < n;
@@ -700,7 +698,7 @@ __device__ inline void fw_panel_lines_body(T* W, int64_t n, int64_t ld, int64_t 
     }
   }
   const int64_t dm = wave_max_i64((int64_t)(d0 > d1 ? d0 : d1));
-  if ((tid & 63) == 0) record_diameter(st, dm);
+  if ((tid & 63) == 0 && dm > seen) record_diameter(st, dm);
 }
 
 // ---------------------------------------------------------------------------
@@ -715,10 +713,12 @@ __device__ inline void fw_panel_lines_body(T* W, int64_t n, int64_t ld, int64_t 
 // are relaxed again with the same snapshot products, which leaves them and
 // the trajectory maximum unchanged.
 // ---------------------------------------------------------------------------
-// GLOBAL_GUARD: the per-k MSB guard comes from the guard words the panel
-// kernel left (one matrix); otherwise the tile computes it from the
-// snapshots it stages (the batched launcher's panels leave none).
-template <typename T, int RPT, int CPT, bool GLOBAL_GUARD>
+// PANEL_GUARD: the per-k MSB guard comes from the per-block guard words the
+// panel kernel left (one matrix): the largest left-operand MSB over the
+// tile's row blocks plus the largest right-operand MSB over its column
+// blocks.  Otherwise the tile computes the same bound from the snapshots it
+// stages (the batched launcher's panels leave no guard words).
+template <typename T, int RPT, int CPT, bool PANEL_GUARD>
 __device__ inline void fw_tile_fast_body(T* W, int64_t n, int64_t ld, int64_t K, FwWork ws, int width,
                                          rbg_status* st, int bid) {
   using U = typename unsigned_of<T>::type;
@@ -743,7 +743,8 @@ __device__ inline void fw_tile_fast_body(T* W, int64_t n, int64_t ld, int64_t K,
   const int64_t cb = (int64_t)J * TL + tc * 4;  // column of chunk 0; chunk h adds 64 h
   const bool skip = r0 >= k0 && r0 < k0 + BLK;
 
-  // issue the tile loads first
+  // every global load first: the tile, this thread's share of the
+  // snapshots, the status flags and (one matrix) the guard words
   U w[RPT][CPT];
   U* Wu = (U*)W;
   if (!skip) {
@@ -757,53 +758,73 @@ __device__ inline void fw_tile_fast_body(T* W, int64_t n, int64_t ld, int64_t K,
         for (int x = 0; x < 4; x++) w[r][4 * h + x] = q[x];
       }
   }
-
-  // a workgroup-uniform exit when an earlier kernel already asked for general mode
-  if (tid == 0) sbad = (st->flags & (RBG_F_NEED_GENERAL | RBG_F_PRECOND_FAIL)) ? 2 : 0;
-  if (tid < BLK) sGa[tid] = sGb[tid] = 0;
-  __syncthreads();
-  if (GLOBAL_GUARD && tid < BLK) {
-    const int* guard = ws.guard(par);
-    if (guard[tid] + guard[BLK + tid] > width - 2) atomicOr(&sbad, 1);
+  constexpr int GL = TL / 4;            // threads staging one k (a lane group)
+  constexpr int SPT = BLK * GL / NT;    // staged chunks per thread
+  U sa[SPT][4], sb[SPT][4];
+#pragma unroll
+  for (int q = 0; q < SPT; q++) {
+    const int e = tid + q * NT, kk = e / GL, x = (e % GL) * 4;
+    ld4(colT + kk * n + (int64_t)I * TL + x, sa[q]);
+    ld4(rowS + kk * n + (int64_t)J * TL + x, sb[q]);
   }
-  // stage the snapshots; per k, the largest MSB among this tile's left
-  // operands and among its right operands (the MSB guard of this tile)
-  for (int e = tid; e < BLK * TL / 4; e += NT) {
-    const int kk = e / (TL / 4), x = (e % (TL / 4)) * 4;
-    U a[4], b[4], m[4], y[4];
-    ld4(colT + kk * n + (int64_t)I * TL + x, a);
-    ld4(rowS + kk * n + (int64_t)J * TL + x, b);
+  const uint32_t flags = st->flags;
+  const int64_t seen = st->diameter_max;  // a lower bound of the value at exit
+  int pga = 0, pgb = 0;
+  if (PANEL_GUARD && tid < BLK) {
+    constexpr int NBT = TL / BLK;  // 32-blocks per tile edge
+    const int* gw = ws.gblk();
+#pragma unroll
+    for (int q = 0; q < NBT; q++) {
+      const int a = gw[((I * NBT + q) * 2 + 0) * BLK + tid];
+      const int b = gw[((J * NBT + q) * 2 + 1) * BLK + tid];
+      pga = a > pga ? a : pga;
+      pgb = b > pgb ? b : pgb;
+    }
+  }
+  // (shift, low bits) pairs into LDS; without guard words, per k the largest
+  // MSB among this tile's left operands and among its right operands.  One
+  // lane group (within one wave) stages each k: its first lane clears the two
+  // words and the group's LDS max operations follow it in the wave's order.
+#pragma unroll
+  for (int q = 0; q < SPT; q++) {
+    const int e = tid + q * NT, kk = e / GL, x = (e % GL) * 4;
+    U m[4], y[4];
     int ga = 0, gb = 0;
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      if ((T)b[q] < 0) {  // absent
-        m[q] = 0;
-        y[q] = TOP;
+    for (int c = 0; c < 4; c++) {
+      const U b = sb[q][c];
+      if ((T)b < 0) {  // absent
+        m[c] = 0;
+        y[c] = TOP;
       } else {
-        m[q] = (U)msb_u(b[q]);
-        y[q] = b[q] & (((U)1 << m[q]) - 1);
+        m[c] = (U)msb_u(b);
+        y[c] = b & (((U)1 << m[c]) - 1);
       }
-      const int mq = (int)m[q], aq = (T)a[q] > 1 ? msb_u(a[q]) : 0;
-      gb = mq > gb ? mq : gb;
-      ga = aq > ga ? aq : ga;
+      const int mc = (int)m[c], ac = (T)sa[q][c] > 1 ? msb_u(sa[q][c]) : 0;
+      gb = mc > gb ? mc : gb;
+      ga = ac > ga ? ac : ga;
     }
-    if (!GLOBAL_GUARD) {
+    if (!PANEL_GUARD) {
+      if (e % GL == 0) {
+        sGa[kk] = 0;
+        sGb[kk] = 0;
+      }
       if (ga) atomicMax(&sGa[kk], ga);
       if (gb) atomicMax(&sGb[kk], gb);
     }
-    st4(&As[kk][x], a);
+    st4(&As[kk][x], sa[q]);
     st4(&Ms[kk][x], m);
     st4(&Ys[kk][x], y);
   }
+  // a workgroup-uniform exit when an earlier kernel already asked for general
+  // mode; a failed MSB guard only asks for it (the values this launch then
+  // computes are discarded with the rest of the fast run)
+  if (tid == 0) sbad = (flags & (RBG_F_NEED_GENERAL | RBG_F_PRECOND_FAIL)) ? 2 : 0;
+  if (PANEL_GUARD && tid < BLK && pga + pgb > width - 2) atomicOr(&st->flags, (uint32_t)RBG_F_NEED_GENERAL);
   __syncthreads();
-  if (!GLOBAL_GUARD) {
-    if (tid < BLK && sGa[tid] + sGb[tid] > width - 2) atomicOr(&sbad, 1);
-    __syncthreads();
-  }
-  if (sbad) {
-    if (tid == 0 && (sbad & 1)) atomicOr(&st->flags, (uint32_t)RBG_F_NEED_GENERAL);
-    return;
-  }
+  if (!PANEL_GUARD && tid < BLK && sGa[tid] + sGb[tid] > width - 2)
+    atomicOr(&st->flags, (uint32_t)RBG_F_NEED_GENERAL);
+  if (sbad) return;
   if (skip) return;
 
   // diagonal cells of this thread (tiles with I == J): four at most, rows
@@ -877,17 +898,17 @@ __device__ inline void fw_tile_fast_body(T* W, int64_t n, int64_t ld, int64_t K,
     }
   int64_t dm = (int64_t)(d0 > d1 ? d0 : d1);
   dm = wave_max_i64(dm);
-  if ((tid & 63) == 0) record_diameter(st, dm);
+  if ((tid & 63) == 0 && dm > seen) record_diameter(st, dm);
 }
 
 template <typename T>
-__global__ __launch_bounds__(NT) void k_fw_panel_fast(T* W, int64_t n, int64_t ld, int64_t K, FwWork ws,
-                                                      int width, rbg_status* st) {
-  fw_panel_tiles_body<T>(W, n, ld, K, ws, width, st, blockIdx.x);
+__global__ __launch_bounds__(PANEL_NT) void k_fw_panel_fast(T* W, int64_t n, int64_t ld, int64_t K, FwWork ws,
+                                                            int width, rbg_status* st) {
+  fw_panel_pair_body<T>(W, n, ld, K, ws, width, st, blockIdx.x);
 }
 
 template <typename T>
-__global__ __launch_bounds__(NT) void k_fw_tile_fast(T* W, int64_t n, int64_t ld, int64_t K, FwWork ws,
+__global__ __launch_bounds__(NT, sizeof(T) == 4 ? 3 : 1) void k_fw_tile_fast(T* W, int64_t n, int64_t ld, int64_t K, FwWork ws,
                                                      int width, rbg_status* st) {
   fw_tile_fast_body<T, RPT, CPT, true>(W, n, ld, K, ws, width, st, blockIdx.x);
 }

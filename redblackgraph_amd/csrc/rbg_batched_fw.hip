@@ -57,9 +57,6 @@ static __global__ void k_bfw_init(const BComp* comps, int32_t count) {
   st->ov_last_a = st->ov_last_b = 0;
   st->diameter_max = 0;
   for (int i = 0; i < 8; i++) st->aux[i] = 0;
-  FwWork ws{comps[c].snap, comps[c].npad, 4};
-  int* g = ws.guard(0);
-  for (int q = 0; q < 4 * BLK; q++) g[q] = 0;
 }
 
 // Fast-domain precheck of one 128 x 128 chunk (rules of k_fw_precheck).
@@ -105,7 +102,7 @@ static __global__ __launch_bounds__(NT) void k_bfw_panel(const BComp* comps, con
 }
 
 template <int RC>
-static __global__ __launch_bounds__(NT) void k_bfw_tile(const BComp* comps, const Item* items, int64_t K) {
+static __global__ __launch_bounds__(NT, 3) void k_bfw_tile(const BComp* comps, const Item* items, int64_t K) {
   const Item it = items[blockIdx.x];
   const BComp c = comps[it.comp];
   FwWork ws{c.snap, c.npad, 4};

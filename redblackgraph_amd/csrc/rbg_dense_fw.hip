@@ -15,8 +15,6 @@ static int fw_dense(T* W, int64_t n, int64_t ld, int width, int is_uns, int mode
   const int nb = (int)(n / BLK);
   const int nt = (int)(n / TILE);
   const int conv_grid = (int)((n * n + NT * 8 - 1) / (NT * 8)) < 4096 ? (int)((n * n + NT * 8 - 1) / (NT * 8)) : 4096;
-  hipLaunchKernelGGL(k_guard_init, dim3(1), dim3(4 * BLK), 0, s, ws.guard(0));
-  RBG_CHECK_LAUNCH();
   if (fast) {
     hipLaunchKernelGGL((k_e_convert<T, true>), dim3(conv_grid), dim3(NT), 0, s, W, n, ld);
     RBG_CHECK_LAUNCH();
@@ -25,7 +23,7 @@ static int fw_dense(T* W, int64_t n, int64_t ld, int width, int is_uns, int mode
   for (int K = 0; K < nb; K++) {
     int p0 = prof_start(s);
     if (fast) {
-      hipLaunchKernelGGL((k_fw_panel_fast<T>), dim3(panel_grid), dim3(NT), 0, s, W, n, ld, (int64_t)K, ws, width,
+      hipLaunchKernelGGL((k_fw_panel_fast<T>), dim3(nb - 1), dim3(PANEL_NT), 0, s, W, n, ld, (int64_t)K, ws, width,
                          st);
     } else {
       hipLaunchKernelGGL((k_fw_panel<T, false>), dim3(panel_grid), dim3(NT), 0, s, W, n, ld, (int64_t)K, ws,

@@ -20,9 +20,9 @@ def test_layout_offsets_and_padding():
 
 
 def test_tuning_batch_sizes():
-    assert autotune.tuning_components(128) == 256
-    assert autotune.tuning_components(4096) == 1
-    assert autotune.tuning_components(1024) == 16
+    assert autotune.tuning_components(128) == 4096
+    assert autotune.tuning_components(4096) == 12
+    assert autotune.tuning_components(1024) == 192
 
 
 def test_tile_plan_reads_cache(tmp_path, monkeypatch):
