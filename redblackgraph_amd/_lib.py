@@ -111,6 +111,9 @@ def load():
                 raise LibraryMissing(
                     f"{LIB_PATH} is not built; run `python -m redblackgraph_amd.build` "
                     "(the AVOS path has no CPU fallback)")
+            # torch first, so that the library binds to the HIP runtime torch
+            # has loaded rather than loading a second one
+            torch()
             L = ctypes.CDLL(LIB_PATH)
             for name, (res, args) in SIGNATURES.items():
                 f = getattr(L, name)

@@ -534,118 +534,109 @@ struct RowMeta {
   T av[LV_MAXD + 1];
 };
 
-// Merge one row entirely in LDS: `lane` of `nl_lanes` cooperating threads
-// (a wave, or the workgroup), candidates at lcol[base ...], merged order to
-// mcol/mval[base ...].  Every access to the staged lists is an LDS access.
+constexpr int LV_R = LV_SMALL / 64;  // candidates per thread on both LDS paths
+static_assert(LV_BIG / LV_NT == LV_R, "a wave's and the workgroup's LDS paths hold as many candidates per thread");
+
+// Merge one row entirely in LDS: `lane` of NL cooperating threads (a wave, or
+// the workgroup), list metadata in M, merged order to mcol/mval[base ...].
+// One round of global loads stages every candidate: its column to LDS, its
+// value (A[v][w_k] (x) C[w_k][c] for lists k >= 1) to a register of the
+// thread that owns it (candidates lane + NL r).  A candidate's place in the
+// merged order is the number of entries before it over all lists: in its own
+// list its index, in a lower list the entries with column <= c, in a higher
+// list those with column < c.  Equal columns therefore sit next to each other
+// in list order, and the compaction sums each such run (lds_run_head).
 template <typename T, int NL>
 __device__ inline void merge_row_lds(const int32_t* __restrict__ indices, const T* __restrict__ data,
-                                     const DagWork& ws, const DagArena<T>& ar, int64_t slot, int64_t e0, int nd,
-                                     int64_t cnt, int base, int32_t* s_lcol, int32_t* s_mcol, T* s_mval,
-                                     RowMeta<T>& M, int32_t* s_len, int lane, int width, T& cmax, uint32_t& evs,
-                                     bool first_wave) {
-  // metadata from the slot arrays the count step wrote
-  if (first_wave) {
-    const int nl0 = ws.slot_nl[slot];
-    if (lane + 1 < nl0) {
-      const int64_t q = slot * DagWork::SLOT_D + lane;
-      M.ro[lane + 1] = ws.slot_ro[q];
-      M.av[lane + 1] = (T)ws.slot_av[q];
-      s_len[lane + 1] = ws.slot_len[q];
-    }
-    if (lane == 0) {
-      M.nl = nl0;
-      M.ro[0] = e0;
-      M.lst[0] = 0;
-      M.lst[1] = nd;
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-    if (lane == 0)
-      for (int k = 1; k < nl0; k++) M.lst[k + 1] = M.lst[k] + s_len[k];
-  }
-  if (NL == 64) {
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-  } else {
-    __syncthreads();
-  }
+                                     const DagArena<T>& ar, const RowMeta<T>& M, int64_t e0, int cnt, int base,
+                                     int32_t* s_lcol, int32_t* s_mcol, T* s_mval, int lane, int width, T& cmax,
+                                     uint32_t& evs) {
   const int nl = M.nl;
-  // stage every list's columns
-  for (int q = lane; q < nd; q += NL) s_lcol[base + q] = indices[e0 + q];
-  for (int k = 1; k < nl; k++) {
-    const int lo = M.lst[k], len = M.lst[k + 1] - lo;
-    const int64_t ro = M.ro[k];
-    for (int q = lane; q < len; q += NL) s_lcol[base + lo + q] = ar.cols[ro + q];
-  }
-  if (NL == 64) {
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-  } else {
-    __syncthreads();
-  }
-  for (int64_t idx = lane; idx < cnt; idx += NL) {
-    int k = 0;
-    while (k + 1 < nl && M.lst[k + 1] <= idx) k++;
-    const int p = (int)idx - M.lst[k];
-    const int32_t c = s_lcol[base + idx];
-    T x;
-    if (k == 0) {
-      x = data[e0 + p];
-    } else {
-      uint32_t ev = 0;
-      x = avos_product<T>(M.av[k], ar.vals[M.ro[k] + p], width, false, ev);
-      evs |= ev;
-    }
-    {
-      const T a = x < 0 ? (T)-x : x;
+  int lst[LV_MAXD + 2];
+#pragma unroll
+  for (int q = 0; q < LV_MAXD + 2; q++) lst[q] = q <= nl ? M.lst[q] : cnt;
+  T x[LV_R];
+  int32_t cr[LV_R];
+#pragma unroll
+  for (int r = 0; r < LV_R; r++) {
+    const int idx = lane + NL * r;
+    x[r] = 0;
+    cr[r] = 0;
+    if (idx < cnt) {
+      int k = 0;
+#pragma unroll
+      for (int q = 1; q <= LV_MAXD; q++) k += (q < nl && idx >= lst[q]) ? 1 : 0;
+      const int p = idx - lst[k];
+      int32_t c;
+      T v;
+      if (k == 0) {
+        c = indices[e0 + p];
+        v = data[e0 + p];
+      } else {
+        const int64_t o = M.ro[k] + p;
+        c = ar.cols[o];
+        uint32_t ev = 0;
+        v = avos_product<T>(M.av[k], ar.vals[o], width, false, ev);
+        evs |= ev;
+      }
+      s_lcol[base + idx] = c;
+      x[r] = v;
+      cr[r] = c;
+      const T a = v < 0 ? (T)-v : v;
       cmax = a > cmax ? a : cmax;
     }
-    int pos = p;
-    bool head = true;
-    T comb = x;
+  }
+  if (NL == 64) {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+  } else {
+    __syncthreads();
+  }
+  for (int r = 0; r < LV_R; r++) {
+    const int idx = lane + NL * r;
+    if (idx >= cnt) break;
+    const int32_t c = cr[r];
+    int pos = 0;
     for (int k2 = 0; k2 < nl; k2++) {
-      if (k2 == k) continue;
-      const int lo = base + M.lst[k2];
-      const int len = M.lst[k2 + 1] - M.lst[k2];
-      int a = 0, b = len;  // first index with column > c (k2 < k) or >= c (k2 > k)
-      if (k2 < k) {
-        while (a < b) {
-          const int mid = (a + b) >> 1;
-          if (s_lcol[lo + mid] <= c) a = mid + 1;
-          else b = mid;
-        }
-        pos += a;
-        if (a > 0 && s_lcol[lo + a - 1] == c) head = false;
-      } else {
-        while (a < b) {
-          const int mid = (a + b) >> 1;
-          if (s_lcol[lo + mid] < c) a = mid + 1;
-          else b = mid;
-        }
-        pos += a;
-        if (head && a < len && s_lcol[lo + a] == c) {
-          T y;
-          if (k2 == 0) {
-            y = data[e0 + a];
-          } else {
-            uint32_t ev = 0;
-            y = avos_product<T>(M.av[k2], ar.vals[M.ro[k2] + a], width, false, ev);
-            evs |= ev;
-          }
-          comb = avos_sum<T>(comb, y);
-        }
+      const int lo = base + lst[k2];
+      const int hi = lst[k2 + 1];
+      if (idx >= lst[k2] && idx < hi) {  // its own list: its index
+        pos += idx - lst[k2];
+        continue;
       }
+      // entries before it: column <= c in a lower list, < c in a higher one
+      const int32_t key = idx >= hi ? c + 1 : c;
+      int a = 0, b = hi - lst[k2];
+      while (a < b) {
+        const int mid = (a + b) >> 1;
+        if (s_lcol[lo + mid] < key) a = mid + 1;
+        else b = mid;
+      }
+      pos += a;
     }
-    s_mcol[base + pos] = (head && comb != 0) ? c : -1;
-    s_mval[base + pos] = comb;
+    s_mcol[base + pos] = c;
+    s_mval[base + pos] = x[r];
   }
 }
 
+// Merged position i of an LDS-merged row: whether it starts a run of equal
+// columns, and the AVOS sum of the run's values (the closure entry).
+template <typename T>
+__device__ inline bool lds_run_head(const int32_t* mcol, const T* mval, int i, int cnt, int32_t& c, T& sum) {
+  c = mcol[i];
+  if (i > 0 && mcol[i - 1] == c) return false;
+  sum = mval[i];
+  for (int j = i + 1; j < cnt && mcol[j] == c; j++) sum = avos_sum<T>(sum, mval[j]);
+  return true;
+}
+
 // Round r, step 3: closure rows of the level.  Each workgroup takes the level's
-// vertices four at a time.  A row with at most LV_SMALL candidates and at
-// most LV_MAXD direct entries is merged by one wave in that wave's LDS share;
-// a longer one (up to LV_BIG) by the whole workgroup in LDS; anything else by
-// the workgroup in its arena slot, reading the lists from global memory.  The
+// vertices four at a time, one per wave.  A row with at most LV_SMALL
+// candidates and at most LV_MAXD direct entries is merged by its wave in that
+// wave's LDS share; a longer one (up to LV_BIG) by the whole workgroup in LDS;
+// anything else by the workgroup in its arena slot, reading the lists from
+// global memory.  Each wave loads the next chunk's slot data for its row (list
+// rows, lengths, edge values) while the current chunk is merged.  The
 // compaction writes the row to the front of the slot [abase + seg, abase +
 // seg + cnt) that the count scan reserved for the vertex; when the row was
 // merged in the slot itself every store of a pass lands at or below the
@@ -659,7 +650,6 @@ __global__ __launch_bounds__(LV_NT) void k_lvl_merge(const int64_t* __restrict__
   __shared__ int32_t s_mcol[LV_BIG];
   __shared__ T s_mval[LV_BIG];
   __shared__ RowMeta<T> s_meta[LV_WAVES];
-  __shared__ int32_t s_len[LV_WAVES][LV_MAXD + 2];
   __shared__ int64_t s_seg[LV_WAVES + 1];
   __shared__ int32_t s_v[LV_WAVES];
   __shared__ int64_t s_e0[LV_WAVES];
@@ -668,7 +658,7 @@ __global__ __launch_bounds__(LV_NT) void k_lvl_merge(const int64_t* __restrict__
   __shared__ int64_t s_cmax, s_fmax;
   int64_t* ctr = ws.counters;
   if (dag_halted(ctr)) return;
-  const int64_t L0 = ctr[C_LSTART], Ls = ctr[C_LSIZE], M = ctr[C_M], abase = ctr[C_ARENA];
+  const int64_t Ls = ctr[C_LSIZE], M = ctr[C_M], abase = ctr[C_ARENA];
   if (abase + M > ar.cap) {
     if (blockIdx.x == 0 && threadIdx.x == 0) ctr[C_CAP] = 1;
     return;
@@ -678,9 +668,10 @@ __global__ __launch_bounds__(LV_NT) void k_lvl_merge(const int64_t* __restrict__
   if (threadIdx.x == 0) s_cmax = s_fmax = 0;
   T cmax = 0, fmax = 0;
   uint32_t evs = 0;
-  // threads 0..LV_WAVES load a chunk's slot data one chunk ahead
-  int64_t p_seg = 0, p_e0 = 0;
-  int32_t p_v = 0, p_nd = 0;
+  // one chunk ahead: threads 0..LV_WAVES load the candidate-scan bounds and
+  // slot data, and lanes 0..SLOT_D-1 of wave w the list rows of slot t0 + w
+  int64_t p_seg = 0, p_e0 = 0, p_ro = 0, p_av = 0, q_e0 = 0;
+  int32_t p_v = 0, p_nd = 0, p_len = 0, q_nl = 0, q_nd = 0;
   auto fetch = [&](int64_t t0) {
     if (threadIdx.x <= LV_WAVES && t0 < Ls) {
       const int64_t t = t0 + threadIdx.x < Ls ? t0 + threadIdx.x : Ls;
@@ -689,6 +680,18 @@ __global__ __launch_bounds__(LV_NT) void k_lvl_merge(const int64_t* __restrict__
         p_v = ws.slot_v[t];
         p_e0 = ws.slot_e0[t];
         p_nd = ws.slot_nd[t];
+      }
+    }
+    const int64_t t = t0 + wv;
+    if (t < Ls) {
+      q_nl = ws.slot_nl[t];
+      q_nd = ws.slot_nd[t];
+      q_e0 = ws.slot_e0[t];
+      if (lane < DagWork::SLOT_D) {
+        const int64_t q = t * DagWork::SLOT_D + lane;
+        p_ro = ws.slot_ro[q];
+        p_len = ws.slot_len[q];
+        p_av = ws.slot_av[q];
       }
     }
   };
@@ -702,6 +705,30 @@ __global__ __launch_bounds__(LV_NT) void k_lvl_merge(const int64_t* __restrict__
         s_nd[threadIdx.x] = p_nd;
       }
     }
+    // list metadata of this wave's row (only rows with at most SLOT_D direct
+    // entries have it, and only those take an LDS path)
+    if (t0 + wv < Ls && q_nd <= LV_MAXD) {
+      RowMeta<T>& Mw = s_meta[wv];
+      const int len = lane + 1 < q_nl ? p_len : 0;
+      int incl = len;  // inclusive prefix of the list lengths over lanes 0..SLOT_D-1
+#pragma unroll
+      for (int o = 1; o < 8; o <<= 1) {
+        const int u = __shfl_up(incl, o);
+        if (lane >= o) incl += u;
+      }
+      if (lane + 1 < q_nl) {
+        Mw.ro[lane + 1] = p_ro;
+        Mw.av[lane + 1] = (T)p_av;
+        Mw.lst[lane + 2] = q_nd + incl;
+      }
+      if (lane == 0) {
+        Mw.nl = q_nl;
+        Mw.ro[0] = q_e0;
+        Mw.av[0] = 0;
+        Mw.lst[0] = 0;
+        Mw.lst[1] = q_nd;
+      }
+    }
     __syncthreads();
     fetch(t0 + (int64_t)gridDim.x * LV_WAVES);
     // rows merged by one wave
@@ -712,17 +739,19 @@ __global__ __launch_bounds__(LV_NT) void k_lvl_merge(const int64_t* __restrict__
         const int32_t v = s_v[wv];
         const int64_t dst = abase + s_seg[wv];
         const int base = wv * LV_SMALL;
-        merge_row_lds<T, 64>(indices, data, ws, ar, t, s_e0[wv], s_nd[wv], cnt, base, s_lcol, s_mcol, s_mval,
-                             s_meta[wv], s_len[wv], lane, width, cmax, evs, true);
+        merge_row_lds<T, 64>(indices, data, ar, s_meta[wv], s_e0[wv], (int)cnt, base, s_lcol, s_mcol, s_mval,
+                             lane, width, cmax, evs);
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
         __builtin_amdgcn_wave_barrier();
         int64_t out = 0;
-        for (int64_t b0 = 0; b0 < cnt; b0 += 64) {
-          const int64_t i = b0 + lane;
-          const int32_t c = i < cnt ? s_mcol[base + i] : -1;
-          const T val = i < cnt ? s_mval[base + i] : (T)0;
-          const uint64_t msk = __ballot(c >= 0);
-          if (c >= 0) {
+        for (int b0 = 0; b0 < cnt; b0 += 64) {
+          const int i = b0 + lane;
+          int32_t c = -1;
+          T val = 0;
+          const bool keep = i < cnt && lds_run_head<T>(s_mcol + base, s_mval + base, i, (int)cnt, c, val) &&
+                            val != 0;
+          const uint64_t msk = __ballot(keep);
+          if (keep) {
             const int64_t o = dst + out + __popcll(msk & ((1ull << lane) - 1ull));
             ar.cols[o] = c;
             ar.vals[o] = val;
@@ -749,8 +778,8 @@ __global__ __launch_bounds__(LV_NT) void k_lvl_merge(const int64_t* __restrict__
       int32_t* mcol = in_lds ? s_mcol : ar.cols + dst;
       T* mval = in_lds ? s_mval : ar.vals + dst;
       if (in_lds) {
-        merge_row_lds<T, LV_NT>(indices, data, ws, ar, t, s_e0[j], s_nd[j], cnt, 0, s_lcol, s_mcol, s_mval,
-                                s_meta[0], s_len[0], threadIdx.x, width, cmax, evs, wv == 0);
+        merge_row_lds<T, LV_NT>(indices, data, ar, s_meta[j], s_e0[j], (int)cnt, 0, s_lcol, s_mcol, s_mval,
+                                threadIdx.x, width, cmax, evs);
       } else {
         RowJob<T> J;
         J.v = v;
@@ -765,9 +794,19 @@ __global__ __launch_bounds__(LV_NT) void k_lvl_merge(const int64_t* __restrict__
       int64_t out = 0;
       for (int64_t b0 = 0; b0 < cnt; b0 += LV_NT) {
         const int64_t i = b0 + threadIdx.x;
-        const int32_t c = i < cnt ? mcol[i] : -1;
-        const T val = i < cnt ? mval[i] : (T)0;
-        const uint64_t msk = __ballot(c >= 0);
+        int32_t c = -1;
+        T val = 0;
+        bool keep = false;
+        if (i < cnt) {
+          if (in_lds) {
+            keep = lds_run_head<T>(s_mcol, s_mval, (int)i, (int)cnt, c, val) && val != 0;
+          } else {
+            c = mcol[i];
+            val = mval[i];
+            keep = c >= 0;
+          }
+        }
+        const uint64_t msk = __ballot(keep);
         if (lane == 0) s_wsum[wv] = __popcll(msk);
         __syncthreads();
         int64_t before = 0, total = 0;
@@ -775,7 +814,7 @@ __global__ __launch_bounds__(LV_NT) void k_lvl_merge(const int64_t* __restrict__
           before += q < wv ? s_wsum[q] : 0;
           total += s_wsum[q];
         }
-        if (c >= 0) {
+        if (keep) {
           const int64_t o = dst + out + before + __popcll(msk & ((1ull << lane) - 1ull));
           ar.cols[o] = c;
           ar.vals[o] = val;
