@@ -215,7 +215,7 @@ def bench_sparse(args):
     kernel_s = ms_.value / 1e3 / args.steps
     achieved = bytes_alg / kernel_s
     roof = {
-        "bound": "hbm", "kernel": "k_lvl_merge<long>",
+        "bound": "hbm", "kernel": "k_lvl_units<long> + k_lvl_rows<long> + k_lvl_long<long> (level merge)",
         "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s", "frac": achieved / PEAK_HBM,
         "traffic": _read_traffic("lvl_merge_sparse1m_pmc.txt"),
         "algorithmic_bytes_per_closure": bytes_alg,

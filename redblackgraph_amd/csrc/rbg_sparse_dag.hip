@@ -196,7 +196,7 @@ enum {
   C_LSTART = 3,   // first index in `order` of the current level
   C_LSIZE = 4,    // vertices in the current level
   C_M = 5,        // candidate entries of the current level
-  C_UNUSED6 = 6,
+  C_LONG = 6,     // rows of the current level left to the long-row step
   C_ARENA = 7,    // arena entries used
   C_CMAX = 8,     // max |candidate|
   C_FMAX = 9,     // max |final value| > 1
@@ -231,11 +231,12 @@ struct DagWork {
   int64_t* slot_ro;    // [n * SLOT_D]
   int32_t* slot_len;   // [n * SLOT_D]
   int64_t* slot_av;    // [n * SLOT_D] (edge values, stored as int64)
+  int32_t* long_list;  // [n] slots of the current level left to the long-row step
   static constexpr int SLOT_D = 7;
   static int64_t bytes(int64_t n) {
     int64_t b = 4 * n * 4 + 8 * n + 8 * (n + 1) * 3 + 8 * (n / SCAN_CHUNK + 8) + 8 * C_NCTR;
-    b += n * (4 + 8 + 4 + 4) + n * SLOT_D * (8 + 4 + 8);
-    return b + 24 * 256;
+    b += n * (4 + 8 + 4 + 4 + 4) + n * SLOT_D * (8 + 4 + 8);
+    return b + 25 * 256;
   }
   static DagWork carve(void* base, int64_t n) {
     DagWork w;
@@ -262,6 +263,7 @@ struct DagWork {
     w.slot_ro = (int64_t*)take(8 * n * SLOT_D);
     w.slot_len = (int32_t*)take(4 * n * SLOT_D);
     w.slot_av = (int64_t*)take(8 * n * SLOT_D);
+    w.long_list = (int32_t*)take(4 * n);
     return w;
   }
 };
@@ -341,7 +343,10 @@ __global__ void k_lvl_count(const int64_t* __restrict__ indptr, const int32_t* _
   int64_t* ctr = ws.counters;
   if (dag_halted(ctr)) return;
   const int64_t L0 = ctr[C_LSTART], Ls = ctr[C_PLACED] - L0;
-  if (blockIdx.x == 0 && threadIdx.x == 0) ctr[C_LSIZE] = Ls;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ctr[C_LSIZE] = Ls;
+    ctr[C_LONG] = 0;
+  }
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t <= Ls; t += (int64_t)gridDim.x * blockDim.x) {
     if (t == Ls) {
       ws.cnt[t] = 0;
@@ -630,32 +635,305 @@ __device__ inline bool lds_run_head(const int32_t* mcol, const T* mval, int i, i
   return true;
 }
 
-// Round r, step 3: closure rows of the level.  Each workgroup takes the level's
-// vertices four at a time, one per wave.  A row with at most LV_SMALL
-// candidates and at most LV_MAXD direct entries is merged by its wave in that
-// wave's LDS share; a longer one (up to LV_BIG) by the whole workgroup in LDS;
-// anything else by the workgroup in its arena slot, reading the lists from
-// global memory.  Each wave loads the next chunk's slot data for its row (list
-// rows, lengths, edge values) while the current chunk is merged.  The
-// compaction writes the row to the front of the slot [abase + seg, abase +
-// seg + cnt) that the count scan reserved for the vertex; when the row was
-// merged in the slot itself every store of a pass lands at or below the
-// entries the pass has already loaded.
+// A row's slot data in registers (wave-uniform), and in lane k < SLOT_D the
+// arena row, length and edge value of its list k + 1.
+struct SlotRegs {
+  int32_t v, nd, nl, len;
+  int64_t e0, seg, cnt, ro, av;
+};
+
+__device__ inline int32_t uni(int32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ inline int64_t uni(int64_t x) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)x >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+// Issues a row's slot loads (t is wave-uniform; S.cnt holds the end of the
+// row's candidate range until settle_slot).
+__device__ inline void load_slot(const DagWork& ws, int64_t t, int64_t Ls, int lane, SlotRegs& S) {
+  S.cnt = -1;
+  S.seg = 0;
+  if (t >= Ls) return;
+  S.v = ws.slot_v[t];
+  S.e0 = ws.slot_e0[t];
+  S.nd = ws.slot_nd[t];
+  S.nl = ws.slot_nl[t];
+  S.seg = scan_at(ws.cand_loc, ws.sums_a, t);
+  S.cnt = scan_at(ws.cand_loc, ws.sums_a, t + 1);
+  if (lane < DagWork::SLOT_D) {
+    const int64_t q = t * DagWork::SLOT_D + lane;
+    S.ro = ws.slot_ro[q];
+    S.len = ws.slot_len[q];
+    S.av = ws.slot_av[q];
+  }
+}
+
+// Once a row's slot loads are needed: its scalars as wave-uniform values, so
+// that the row's loops branch on scalar conditions.
+__device__ inline void settle_slot(SlotRegs& S) {
+  if (S.cnt < 0) return;
+  S.v = uni(S.v);
+  S.e0 = uni(S.e0);
+  S.nd = uni(S.nd);
+  S.nl = uni(S.nl);
+  S.seg = uni(S.seg);
+  S.cnt = uni(S.cnt) - S.seg;
+}
+
+
+// The list metadata of a row that one wave merges (lanes 0..SLOT_D-1 hold its
+// lists; list starts are a prefix sum over those lanes).
 template <typename T>
-__global__ __launch_bounds__(LV_NT) void k_lvl_merge(const int64_t* __restrict__ indptr,
-                                                     const int32_t* __restrict__ indices,
-                                                     const T* __restrict__ data, DagWork ws, DagArena<T> ar,
-                                                     int width, rbg_status* st) {
-  __shared__ int32_t s_lcol[LV_BIG];
-  __shared__ int32_t s_mcol[LV_BIG];
-  __shared__ T s_mval[LV_BIG];
-  __shared__ RowMeta<T> s_meta[LV_WAVES];
-  __shared__ int64_t s_seg[LV_WAVES + 1];
-  __shared__ int32_t s_v[LV_WAVES];
-  __shared__ int64_t s_e0[LV_WAVES];
-  __shared__ int s_nd[LV_WAVES];
-  __shared__ int s_wsum[LV_WAVES];
-  __shared__ int64_t s_cmax, s_fmax;
+__device__ inline void put_meta(const SlotRegs& S, RowMeta<T>& M, int lane) {
+  const int len = lane + 1 < S.nl ? S.len : 0;
+  int incl = len;
+#pragma unroll
+  for (int o = 1; o < 8; o <<= 1) {
+    const int u = __shfl_up(incl, o);
+    if (lane >= o) incl += u;
+  }
+  if (lane + 1 < S.nl) {
+    M.ro[lane + 1] = S.ro;
+    M.av[lane + 1] = (T)S.av;
+    M.lst[lane + 2] = S.nd + incl;
+  }
+  if (lane == 0) {
+    M.nl = S.nl;
+    M.lst[0] = 0;
+    M.lst[1] = S.nd;
+  }
+}
+
+// Round r, step 3a: the level's rows merged by single waves, several at a
+// time.  A unit is a run of consecutive slots (rows) whose candidates (at most
+// LU_CAND, in at most LU_LISTS lists) one wave stages, merges and compacts
+// together.  Chunks of C consecutive slots are the work items, C from the
+// level's mean candidate count (about 256 candidates per chunk); a chunk that
+// does not fit one unit is taken row by row.  Rows with more than LU_CAND
+// candidates or more than LV_MAXD direct entries go to the long-row list.
+//
+// In a unit, a candidate of row j's list g is placed at: the candidates of the
+// unit's rows before j, plus its index in g, plus for every other list of row
+// j the number of entries before it (column <= c in a lower list, < c in a
+// higher one).  Equal columns of a row sit next to each other in list order;
+// the compaction keeps each run's head with the run's AVOS sum (zero sums
+// dropped) and writes the unit's rows one after another from the unit's first
+// reserved arena entry; each row lands at or below its own reserved range.
+constexpr int LU_GRID = 4096;
+constexpr int LU_ROWS_MEAN = 64;  // levels with a larger mean candidate count use k_lvl_rows
+constexpr int LU_CAND = 512;
+constexpr int LU_ROWS = 64;
+constexpr int LU_LISTS = 128;
+static_assert(LU_CAND / 64 == LV_R, "candidates per lane");
+
+template <typename T>
+struct UnitLds {
+  int32_t lcol[LU_CAND];          // staged columns, list after list
+  int32_t mcol[LU_CAND];          // merged columns
+  T mval[LU_CAND];                // merged values (run heads: the run's sum)
+  int32_t pref[LU_CAND + 1];      // kept entries before each merged position
+  uint8_t mrow[LU_CAND];          // merged position -> row
+  int32_t lstart[LU_LISTS + 1];   // list starts in the unit; lstart[nlists] = candidates
+  int64_t lsrc[LU_LISTS];         // a row's own entries: its first entry in A; else the arena row
+  T lav[LU_LISTS];                // edge value multiplying an arena list
+  uint8_t lrow[LU_LISTS];         // list -> row
+  uint8_t lown[LU_LISTS];         // list holds the row's own entries
+  int32_t rlist[LU_ROWS + 1];     // row -> first list
+  int32_t rbase[LU_ROWS + 1];     // row -> first candidate; rbase[rows] = candidates
+  int32_t rv[LU_ROWS];            // row -> vertex
+};
+
+// Wave-wide exclusive prefix sum (64 lanes) and total.
+__device__ inline int wave_excl_scan(int x, int lane, int& total) {
+  int incl = x;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(incl, o);
+    if (lane >= o) incl += u;
+  }
+  total = __shfl(incl, 63);
+  return incl - x;
+}
+
+// Merge the unit's rows j < m; lane j < m holds row j's slot t, its scalars
+// (v, e0, nd, nl, candidate count) and, through slot_* arrays, its lists.  A
+// row with nl = 0 is a place holder (a row left to the long-row step): no
+// lists, no candidates, no output.  U and NL: the unit's candidates and lists
+// (uniform); dst0: the unit's first reserved arena entry.
+template <typename T>
+__device__ inline void merge_unit(const int32_t* __restrict__ indices, const T* __restrict__ data,
+                                  const DagWork& ws, const DagArena<T>& ar, UnitLds<T>& L, int lane, int m,
+                                  int64_t t, int32_t v, int64_t e0, int32_t nd, int32_t nl, int64_t cntj, int U,
+                                  int NL, int64_t dst0, int width, T& cmax, T& fmax, uint32_t& evs) {
+  auto wsync = []() {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+  };
+  // row and list tables
+  {
+    int tot;
+    const int rb = wave_excl_scan(lane < m ? (int)cntj : 0, lane, tot);
+    const int rl = wave_excl_scan(lane < m ? nl : 0, lane, tot);
+    if (lane < m) {
+      L.rbase[lane] = rb;
+      L.rlist[lane] = rl;
+      L.rv[lane] = v;
+    }
+    if (lane < m && nl > 0) {
+      L.lstart[rl] = rb;
+      L.lsrc[rl] = e0;
+      L.lav[rl] = 0;
+      L.lrow[rl] = (uint8_t)lane;
+      L.lown[rl] = 1;
+      int64_t ro[DagWork::SLOT_D], av[DagWork::SLOT_D];
+      int32_t len[DagWork::SLOT_D];
+#pragma unroll
+      for (int k = 0; k < DagWork::SLOT_D; k++) {
+        if (k + 1 < nl) {
+          const int64_t q = t * DagWork::SLOT_D + k;
+          ro[k] = ws.slot_ro[q];
+          len[k] = ws.slot_len[q];
+          av[k] = ws.slot_av[q];
+        }
+      }
+      int start = rb + nd;
+#pragma unroll
+      for (int k = 0; k < DagWork::SLOT_D; k++) {
+        if (k + 1 < nl) {
+          const int g = rl + 1 + k;
+          L.lstart[g] = start;
+          L.lsrc[g] = ro[k];
+          L.lav[g] = (T)av[k];
+          L.lrow[g] = (uint8_t)lane;
+          L.lown[g] = 0;
+          start += len[k];
+        }
+      }
+    }
+    if (lane == 0) {
+      L.rbase[m] = U;
+      L.rlist[m] = NL;
+      L.lstart[NL] = U;
+    }
+  }
+  wsync();
+  // stage: columns to LDS, values to registers
+  int32_t c[LV_R];
+  T x[LV_R];
+  int gq[LV_R];
+#pragma unroll
+  for (int r = 0; r < LV_R; r++) {
+    if (64 * r >= U) break;
+    const int u = lane + 64 * r;
+    gq[r] = 0;
+    if (u < U) {
+      int lo = 0, hi = NL;  // the last list start <= u
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (L.lstart[mid] <= u) lo = mid;
+        else hi = mid;
+      }
+      gq[r] = lo;
+      const int64_t o = L.lsrc[lo] + (u - L.lstart[lo]);
+      if (L.lown[lo]) {
+        c[r] = indices[o];
+        x[r] = data[o];
+      } else {
+        c[r] = ar.cols[o];
+        x[r] = ar.vals[o];
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < LV_R; r++) {
+    if (64 * r >= U) break;
+    const int u = lane + 64 * r;
+    if (u < U) {
+      const int g = gq[r];
+      if (!L.lown[g]) {
+        uint32_t ev = 0;
+        x[r] = avos_product<T>(L.lav[g], x[r], width, false, ev);
+        evs |= ev;
+      }
+      L.lcol[u] = c[r];
+      const T a = x[r] < 0 ? (T)-x[r] : x[r];
+      cmax = a > cmax ? a : cmax;
+    }
+  }
+  wsync();
+  // place
+  for (int r = 0; r < LV_R; r++) {
+    if (64 * r >= U) break;
+    const int u = lane + 64 * r;
+    if (u >= U) continue;
+    const int g = gq[r];
+    const int j = L.lrow[g];
+    const int32_t cc = c[r];
+    int pos = L.rbase[j] + (u - L.lstart[g]);
+    const int g1 = L.rlist[j + 1];
+    for (int g2 = L.rlist[j]; g2 < g1; g2++) {
+      if (g2 == g) continue;
+      const int lo = L.lstart[g2];
+      const int32_t key = g2 < g ? cc + 1 : cc;  // entries before it: <= c below, < c above
+      int a = 0, b = L.lstart[g2 + 1] - lo;
+      while (a < b) {
+        const int mid = (a + b) >> 1;
+        if (L.lcol[lo + mid] < key) a = mid + 1;
+        else b = mid;
+      }
+      pos += a;
+    }
+    L.mcol[pos] = cc;
+    L.mval[pos] = x[r];
+    L.mrow[pos] = (uint8_t)j;
+  }
+  wsync();
+  // compaction: run heads and their sums, then the kept-entry prefix
+  int run = 0;
+  for (int b0 = 0; b0 < U; b0 += 64) {
+    const int i = b0 + lane;
+    bool keep = false;
+    if (i < U) {
+      const int32_t cc = L.mcol[i];
+      const int j = L.mrow[i];
+      const int end = L.rbase[j + 1];
+      if (i == L.rbase[j] || L.mcol[i - 1] != cc) {
+        T sum = L.mval[i];
+        for (int i2 = i + 1; i2 < end && L.mcol[i2] == cc; i2++) sum = avos_sum<T>(sum, L.mval[i2]);
+        keep = sum != 0;
+        if (keep) L.mval[i] = sum;
+      }
+    }
+    const uint64_t msk = __ballot(keep);
+    if (i < U) L.pref[i] = run + __popcll(msk & ((1ull << lane) - 1ull));
+    run += __popcll(msk);
+  }
+  if (lane == 0) L.pref[U] = run;
+  wsync();
+  for (int b0 = 0; b0 < U; b0 += 64) {
+    const int i = b0 + lane;
+    if (i < U && L.pref[i + 1] != L.pref[i]) {
+      const int64_t o = dst0 + L.pref[i];
+      const T val = L.mval[i];
+      ar.cols[o] = L.mcol[i];
+      ar.vals[o] = val;
+      const T a = val < 0 ? (T)-val : val;
+      if (a > 1 && a > fmax) fmax = a;
+    }
+  }
+  if (lane < m && nl > 0) {
+    const int p0 = L.pref[L.rbase[lane]], p1 = L.pref[L.rbase[lane + 1]];
+    ws.row_off[L.rv[lane]] = dst0 + p0;
+    ws.row_len[L.rv[lane]] = p1 - p0;
+  }
+  wsync();  // the tables are rewritten by the next unit
+}
+
+template <typename T>
+__global__ __launch_bounds__(64) void k_lvl_units(const int32_t* __restrict__ indices, const T* __restrict__ data,
+                                                  DagWork ws, DagArena<T> ar, int width, rbg_status* st) {
+  __shared__ UnitLds<T> L;
   int64_t* ctr = ws.counters;
   if (dag_halted(ctr)) return;
   const int64_t Ls = ctr[C_LSIZE], M = ctr[C_M], abase = ctr[C_ARENA];
@@ -663,173 +941,325 @@ __global__ __launch_bounds__(LV_NT) void k_lvl_merge(const int64_t* __restrict__
     if (blockIdx.x == 0 && threadIdx.x == 0) ctr[C_CAP] = 1;
     return;
   }
-  const int wv = threadIdx.x >> 6;
-  const int lane = threadIdx.x & 63;
-  if (threadIdx.x == 0) s_cmax = s_fmax = 0;
+  const int lane = threadIdx.x;
+  const T seen_c = (T)ctr[C_CMAX], seen_f = (T)ctr[C_FMAX];  // lower bounds of the values at exit
   T cmax = 0, fmax = 0;
   uint32_t evs = 0;
-  // one chunk ahead: threads 0..LV_WAVES load the candidate-scan bounds and
-  // slot data, and lanes 0..SLOT_D-1 of wave w the list rows of slot t0 + w
-  int64_t p_seg = 0, p_e0 = 0, p_ro = 0, p_av = 0, q_e0 = 0;
-  int32_t p_v = 0, p_nd = 0, p_len = 0, q_nl = 0, q_nd = 0;
-  auto fetch = [&](int64_t t0) {
-    if (threadIdx.x <= LV_WAVES && t0 < Ls) {
-      const int64_t t = t0 + threadIdx.x < Ls ? t0 + threadIdx.x : Ls;
-      p_seg = scan_at(ws.cand_loc, ws.sums_a, t);
-      if (threadIdx.x < LV_WAVES && t < Ls) {
-        p_v = ws.slot_v[t];
-        p_e0 = ws.slot_e0[t];
-        p_nd = ws.slot_nd[t];
-      }
+  // rows per chunk: a power of two near 256 / (mean candidates per row)
+  const int64_t mean = Ls > 0 ? (M + Ls - 1) / Ls : 1;
+  if (mean > LU_ROWS_MEAN) return;  // a level of long rows: k_lvl_rows takes it
+  int C = 1;
+  while (C < LU_ROWS && (int64_t)(2 * C) * mean <= 256) C <<= 1;
+  const int64_t nch = (Ls + C - 1) / C;
+  for (int64_t ch = blockIdx.x; ch < nch; ch += gridDim.x) {
+    const int64_t t0 = ch * C;
+    const int m = (int)(Ls - t0 < C ? Ls - t0 : C);
+    // lane j < m: row t0 + j
+    const int64_t t = t0 + (lane < m ? lane : 0);
+    const int32_t v = ws.slot_v[t], nd = ws.slot_nd[t], nl = ws.slot_nl[t];
+    const int64_t e0 = ws.slot_e0[t];
+    const int64_t seg = scan_at(ws.cand_loc, ws.sums_a, t);
+    const int64_t cntj = scan_at(ws.cand_loc, ws.sums_a, t + 1) - seg;
+    // rows no unit takes go to the long-row list
+    const bool lng = lane < m && (nd > LV_MAXD || cntj > LU_CAND);
+    if (lng) ws.long_list[atomicAdd((unsigned long long*)&ctr[C_LONG], 1ull)] = (int32_t)t;
+    // greedy sub-units of consecutive rows within LU_CAND candidates and
+    // LU_LISTS lists (a long row counts as none)
+    const int cw = lane < m && !lng ? (int)cntj : 0, lw = lane < m && !lng ? nl : 0;
+    int totc, totl;
+    const int pc = wave_excl_scan(cw, lane, totc) + cw;  // inclusive prefixes
+    const int pl = wave_excl_scan(lw, lane, totl) + lw;
+    int s0 = 0;
+    while (s0 < m) {
+      const int bc = s0 > 0 ? __shfl(pc, s0 - 1) : 0, bl = s0 > 0 ? __shfl(pl, s0 - 1) : 0;
+      const uint64_t fit = __ballot(lane >= s0 && lane < m && pc - bc <= LU_CAND && pl - bl <= LU_LISTS);
+      const int e = fit ? 63 - __builtin_clzll(fit) : s0;  // last row of the sub-unit (a row always fits)
+      const int mu = e - s0 + 1;
+      // rows s0..e to lanes 0..mu-1
+      const int src = s0 + lane < 64 ? s0 + lane : 63;
+      const int64_t tu = __shfl(t, src), e0u = __shfl(e0, src), cu = __shfl((int64_t)cw, src);
+      const int32_t vu = __shfl(v, src), ndu = __shfl(nd, src), nlu = __shfl(lw, src);
+      const int64_t segu = __shfl(seg, s0);
+      merge_unit<T>(indices, data, ws, ar, L, lane, mu, tu, vu, e0u, ndu, nlu, cu, __shfl(pc, e) - bc,
+                    __shfl(pl, e) - bl, abase + segu, width, cmax, fmax, evs);
+      s0 = e + 1;
     }
-    const int64_t t = t0 + wv;
-    if (t < Ls) {
-      q_nl = ws.slot_nl[t];
-      q_nd = ws.slot_nd[t];
-      q_e0 = ws.slot_e0[t];
-      if (lane < DagWork::SLOT_D) {
-        const int64_t q = t * DagWork::SLOT_D + lane;
-        p_ro = ws.slot_ro[q];
-        p_len = ws.slot_len[q];
-        p_av = ws.slot_av[q];
-      }
-    }
-  };
-  fetch((int64_t)blockIdx.x * LV_WAVES);
-  for (int64_t t0 = (int64_t)blockIdx.x * LV_WAVES; t0 < Ls; t0 += (int64_t)gridDim.x * LV_WAVES) {
-    if (threadIdx.x <= LV_WAVES) {
-      s_seg[threadIdx.x] = p_seg;
-      if (threadIdx.x < LV_WAVES) {
-        s_v[threadIdx.x] = p_v;
-        s_e0[threadIdx.x] = p_e0;
-        s_nd[threadIdx.x] = p_nd;
-      }
-    }
-    // list metadata of this wave's row (only rows with at most SLOT_D direct
-    // entries have it, and only those take an LDS path)
-    if (t0 + wv < Ls && q_nd <= LV_MAXD) {
-      RowMeta<T>& Mw = s_meta[wv];
-      const int len = lane + 1 < q_nl ? p_len : 0;
-      int incl = len;  // inclusive prefix of the list lengths over lanes 0..SLOT_D-1
+  }
+  const int64_t wc = wave_max_i64((int64_t)cmax), wf = wave_max_i64((int64_t)fmax);
+  if (lane == 0) {
+    if (wc > (int64_t)seen_c) atomicMax((long long*)&ctr[C_CMAX], (long long)wc);
+    if (wf > (int64_t)seen_f) atomicMax((long long*)&ctr[C_FMAX], (long long)wf);
+  }
+  if (evs) atomicOr(&st->events, evs);
+}
+
+__device__ inline bool lds_row(const SlotRegs& S) { return S.cnt >= 0 && S.cnt <= LV_SMALL && S.nd <= LV_MAXD; }
+
+__device__ inline int list_of(const int (&lst)[LV_MAXD + 2], int nl, int idx) {
+  int k = 0;
 #pragma unroll
-      for (int o = 1; o < 8; o <<= 1) {
-        const int u = __shfl_up(incl, o);
-        if (lane >= o) incl += u;
-      }
-      if (lane + 1 < q_nl) {
-        Mw.ro[lane + 1] = p_ro;
-        Mw.av[lane + 1] = (T)p_av;
-        Mw.lst[lane + 2] = q_nd + incl;
-      }
-      if (lane == 0) {
-        Mw.nl = q_nl;
-        Mw.ro[0] = q_e0;
-        Mw.av[0] = 0;
-        Mw.lst[0] = 0;
-        Mw.lst[1] = q_nd;
-      }
+  for (int q = 1; q <= LV_MAXD; q++) k += (q < nl && idx >= lst[q]) ? 1 : 0;
+  return k;
+}
+
+// Issue the loads of a row's candidates (lane + 64 r): columns and raw values
+// (an arena value, or the vertex's own entry) into registers.
+template <typename T>
+__device__ inline void issue_lists(const int32_t* __restrict__ indices, const T* __restrict__ data,
+                                   const DagArena<T>& ar, const RowMeta<T>& M, int64_t e0, int cnt, int lane,
+                                   int32_t (&c)[LV_R], T (&raw)[LV_R]) {
+  const int nl = uni(M.nl);
+  int lst[LV_MAXD + 2];
+#pragma unroll
+  for (int q = 0; q < LV_MAXD + 2; q++) lst[q] = q <= nl ? uni(M.lst[q]) : cnt;
+#pragma unroll
+  for (int r = 0; r < LV_R; r++) {
+    if (64 * r >= cnt) break;
+    const int idx = lane + 64 * r;
+    if (idx < cnt) {
+      const int k = list_of(lst, nl, idx);
+      const int64_t p = idx - lst[k];
+      const int32_t* cb = k == 0 ? indices + e0 : ar.cols + M.ro[k];
+      const T* vb = k == 0 ? data + e0 : ar.vals + M.ro[k];
+      c[r] = cb[p];
+      raw[r] = vb[p];
     }
-    __syncthreads();
-    fetch(t0 + (int64_t)gridDim.x * LV_WAVES);
-    // rows merged by one wave
-    {
-      const int64_t t = t0 + wv;
-      const int64_t cnt = s_seg[wv + 1] - s_seg[wv];
-      if (t < Ls && cnt <= LV_SMALL && s_nd[wv] <= LV_MAXD) {
-        const int32_t v = s_v[wv];
-        const int64_t dst = abase + s_seg[wv];
-        const int base = wv * LV_SMALL;
-        merge_row_lds<T, 64>(indices, data, ar, s_meta[wv], s_e0[wv], (int)cnt, base, s_lcol, s_mcol, s_mval,
-                             lane, width, cmax, evs);
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
-        __builtin_amdgcn_wave_barrier();
-        int64_t out = 0;
-        for (int b0 = 0; b0 < cnt; b0 += 64) {
-          const int i = b0 + lane;
-          int32_t c = -1;
-          T val = 0;
-          const bool keep = i < cnt && lds_run_head<T>(s_mcol + base, s_mval + base, i, (int)cnt, c, val) &&
-                            val != 0;
-          const uint64_t msk = __ballot(keep);
-          if (keep) {
-            const int64_t o = dst + out + __popcll(msk & ((1ull << lane) - 1ull));
-            ar.cols[o] = c;
-            ar.vals[o] = val;
-            const T a = val < 0 ? (T)-val : val;
-            if (a > 1 && a > fmax) fmax = a;
+  }
+}
+
+// Round r, step 3a for levels of long rows (mean candidate count above
+// LU_ROWS_MEAN): one row per wave at a time, each workgroup one wave walking
+// slots b, b + W, ... two deep: while a row is placed and compacted, the lists
+// of the next row and the slot data of the one after are in flight.  Rows with
+// more than LV_SMALL candidates or LV_MAXD direct entries go to the long-row
+// list.  Placement and compaction as in merge_row_lds / lds_run_head.
+constexpr int LR_GRID = 4096;
+
+template <typename T>
+__global__ __launch_bounds__(64) void k_lvl_rows(const int32_t* __restrict__ indices, const T* __restrict__ data,
+                                                 DagWork ws, DagArena<T> ar, int width, rbg_status* st) {
+  __shared__ int32_t s_lcol[LV_SMALL];
+  __shared__ int32_t s_mcol[LV_SMALL];
+  __shared__ T s_mval[LV_SMALL];
+  __shared__ RowMeta<T> s_meta[2];
+  int64_t* ctr = ws.counters;
+  if (dag_halted(ctr)) return;
+  const int64_t Ls = ctr[C_LSIZE], M = ctr[C_M], abase = ctr[C_ARENA];
+  if (abase + M > ar.cap) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctr[C_CAP] = 1;
+    return;
+  }
+  if ((Ls > 0 ? (M + Ls - 1) / Ls : 1) <= LU_ROWS_MEAN) return;  // k_lvl_units takes the level
+  const int lane = threadIdx.x;
+  const int64_t W = gridDim.x;
+  const T seen_c = (T)ctr[C_CMAX], seen_f = (T)ctr[C_FMAX];  // lower bounds of the values at exit
+  T cmax = 0, fmax = 0;
+  uint32_t evs = 0;
+  auto wsync = []() {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+  };
+
+  SlotRegs A, B;
+  int32_t c0[LV_R], c1[LV_R];
+  T raw0[LV_R], raw1[LV_R];
+#pragma unroll
+  for (int r = 0; r < LV_R; r++) {
+    c0[r] = c1[r] = 0;
+    raw0[r] = raw1[r] = 0;
+  }
+  load_slot(ws, blockIdx.x, Ls, lane, A);
+  settle_slot(A);
+  if (lds_row(A)) put_meta<T>(A, s_meta[0], lane);
+  wsync();
+  issue_lists<T>(indices, data, ar, s_meta[0], A.e0, lds_row(A) ? (int)A.cnt : 0, lane, c0, raw0);
+  load_slot(ws, blockIdx.x + W, Ls, lane, B);
+  int cur = 0;
+  for (int64_t t = blockIdx.x; t < Ls; t += W) {
+    // the next row: metadata out of B, its lists in flight, B refilled two ahead
+    SlotRegs N = B;
+    settle_slot(N);
+    if (lds_row(N)) put_meta<T>(N, s_meta[cur ^ 1], lane);
+    wsync();
+    load_slot(ws, t + 2 * W, Ls, lane, B);
+    issue_lists<T>(indices, data, ar, s_meta[cur ^ 1], N.e0, lds_row(N) ? (int)N.cnt : 0, lane, c1, raw1);
+
+    // this row
+    if (lds_row(A)) {
+      const RowMeta<T>& Mc = s_meta[cur];
+      const int cnt = (int)A.cnt, nl = uni(Mc.nl);
+      int lst[LV_MAXD + 2];
+#pragma unroll
+      for (int q = 0; q < LV_MAXD + 2; q++) lst[q] = q <= nl ? uni(Mc.lst[q]) : cnt;
+#pragma unroll
+      for (int r = 0; r < LV_R; r++) {
+        if (64 * r >= cnt) break;
+        const int idx = lane + 64 * r;
+        if (idx < cnt) {
+          const int k = list_of(lst, nl, idx);
+          T x = raw0[r];
+          if (k != 0) {
+            uint32_t ev = 0;
+            x = avos_product<T>(Mc.av[k], x, width, false, ev);
+            evs |= ev;
           }
-          out += __popcll(msk);
-        }
-        if (lane == 0) {
-          ws.row_off[v] = dst;
-          ws.row_len[v] = out;
+          raw0[r] = x;
+          s_lcol[idx] = c0[r];
+          const T a = x < 0 ? (T)-x : x;
+          cmax = a > cmax ? a : cmax;
         }
       }
-    }
-    __syncthreads();
-    // rows merged by the workgroup
-    for (int j = 0; j < LV_WAVES; j++) {
-      const int64_t t = t0 + j;
-      const int64_t cnt = s_seg[j + 1] - s_seg[j];
-      if (t >= Ls || (cnt <= LV_SMALL && s_nd[j] <= LV_MAXD)) continue;
-      const int32_t v = s_v[j];
-      const int64_t dst = abase + s_seg[j];
-      const bool in_lds = cnt <= LV_BIG && s_nd[j] <= LV_MAXD;
-      int32_t* mcol = in_lds ? s_mcol : ar.cols + dst;
-      T* mval = in_lds ? s_mval : ar.vals + dst;
-      if (in_lds) {
-        merge_row_lds<T, LV_NT>(indices, data, ar, s_meta[j], s_e0[j], (int)cnt, 0, s_lcol, s_mcol, s_mval,
-                                threadIdx.x, width, cmax, evs);
-      } else {
-        RowJob<T> J;
-        J.v = v;
-        J.e0 = s_e0[j];
-        J.e1 = s_e0[j] + s_nd[j];
-        J.nd = s_nd[j];
-        J.cnt = cnt;
-        J.dst = dst;
-        merge_lists<T>(J, indices, data, ws, ar, nullptr, mcol, mval, threadIdx.x, LV_NT, width, cmax, evs);
+      wsync();
+      for (int r = 0; r < LV_R; r++) {
+        if (64 * r >= cnt) break;
+        const int idx = lane + 64 * r;
+        if (idx >= cnt) continue;
+        const int32_t c = c0[r];
+        int pos = 0;
+        for (int k2 = 0; k2 < nl; k2++) {
+          const int lo = lst[k2], hi = lst[k2 + 1];
+          if (idx >= lo && idx < hi) {
+            pos += idx - lo;
+            continue;
+          }
+          const int32_t key = idx >= hi ? c + 1 : c;  // entries before it: <= c below, < c above
+          int a = 0, b = hi - lo;
+          while (a < b) {
+            const int mid = (a + b) >> 1;
+            if (s_lcol[lo + mid] < key) a = mid + 1;
+            else b = mid;
+          }
+          pos += a;
+        }
+        s_mcol[pos] = c;
+        s_mval[pos] = raw0[r];
       }
-      __syncthreads();
+      wsync();
+      const int64_t dst = abase + A.seg;
       int64_t out = 0;
-      for (int64_t b0 = 0; b0 < cnt; b0 += LV_NT) {
-        const int64_t i = b0 + threadIdx.x;
+      for (int b0 = 0; b0 < cnt; b0 += 64) {
+        const int i = b0 + lane;
         int32_t c = -1;
         T val = 0;
-        bool keep = false;
-        if (i < cnt) {
-          if (in_lds) {
-            keep = lds_run_head<T>(s_mcol, s_mval, (int)i, (int)cnt, c, val) && val != 0;
-          } else {
-            c = mcol[i];
-            val = mval[i];
-            keep = c >= 0;
-          }
-        }
+        const bool keep = i < cnt && lds_run_head<T>(s_mcol, s_mval, i, cnt, c, val) && val != 0;
         const uint64_t msk = __ballot(keep);
-        if (lane == 0) s_wsum[wv] = __popcll(msk);
-        __syncthreads();
-        int64_t before = 0, total = 0;
-        for (int q = 0; q < LV_WAVES; q++) {
-          before += q < wv ? s_wsum[q] : 0;
-          total += s_wsum[q];
-        }
         if (keep) {
-          const int64_t o = dst + out + before + __popcll(msk & ((1ull << lane) - 1ull));
+          const int64_t o = dst + out + __popcll(msk & ((1ull << lane) - 1ull));
           ar.cols[o] = c;
           ar.vals[o] = val;
           const T a = val < 0 ? (T)-val : val;
           if (a > 1 && a > fmax) fmax = a;
         }
-        out += total;
-        __syncthreads();
+        out += __popcll(msk);
       }
-      if (threadIdx.x == 0) {
-        ws.row_off[v] = dst;
-        ws.row_len[v] = out;
+      if (lane == 0) {
+        ws.row_off[A.v] = dst;
+        ws.row_len[A.v] = out;
       }
+    } else if (A.cnt >= 0 && lane == 0) {
+      ws.long_list[atomicAdd((unsigned long long*)&ctr[C_LONG], 1ull)] = (int32_t)t;
     }
-    __syncthreads();  // s_seg and the row metadata are rewritten by the next chunk
+    // advance
+    A = N;
+#pragma unroll
+    for (int r = 0; r < LV_R; r++) {
+      c0[r] = c1[r];
+      raw0[r] = raw1[r];
+    }
+    cur ^= 1;
+  }
+  const int64_t wc = wave_max_i64((int64_t)cmax), wf = wave_max_i64((int64_t)fmax);
+  if (lane == 0) {
+    if (wc > (int64_t)seen_c) atomicMax((long long*)&ctr[C_CMAX], (long long)wc);
+    if (wf > (int64_t)seen_f) atomicMax((long long*)&ctr[C_FMAX], (long long)wf);
+  }
+  if (evs) atomicOr(&st->events, evs);
+}
+
+// Round r, step 3b: the long rows step 3a left, one per workgroup at a time:
+// merged in LDS when they fit (LV_BIG candidates, LV_MAXD direct entries),
+// otherwise in their arena slot from the lists in global memory.
+template <typename T>
+__global__ __launch_bounds__(LV_NT) void k_lvl_long(const int32_t* __restrict__ indices, const T* __restrict__ data,
+                                                    DagWork ws, DagArena<T> ar, int width, rbg_status* st) {
+  __shared__ int32_t s_lcol[LV_BIG];
+  __shared__ int32_t s_mcol[LV_BIG];
+  __shared__ T s_mval[LV_BIG];
+  __shared__ RowMeta<T> s_meta;
+  __shared__ int s_wsum[LV_WAVES];
+  __shared__ int64_t s_cmax, s_fmax;
+  int64_t* ctr = ws.counters;
+  if (dag_halted(ctr) || ctr[C_CAP]) return;
+  const int64_t nlong = ctr[C_LONG], Ls = ctr[C_LSIZE], abase = ctr[C_ARENA];
+  if (nlong == 0) return;
+  const int wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  if (threadIdx.x == 0) s_cmax = s_fmax = 0;
+  T cmax = 0, fmax = 0;
+  uint32_t evs = 0;
+  for (int64_t i = blockIdx.x; i < nlong; i += gridDim.x) {
+    const int64_t t = ws.long_list[i];
+    SlotRegs S;
+    load_slot(ws, t, Ls, lane, S);
+    settle_slot(S);
+    const int64_t cnt = S.cnt;
+    const int64_t dst = abase + S.seg;
+    const bool in_lds = cnt <= LV_BIG && S.nd <= LV_MAXD;
+    if (in_lds && wv == 0) put_meta<T>(S, s_meta, lane);
+    __syncthreads();
+    int32_t* mcol = in_lds ? s_mcol : ar.cols + dst;
+    T* mval = in_lds ? s_mval : ar.vals + dst;
+    if (in_lds) {
+      merge_row_lds<T, LV_NT>(indices, data, ar, s_meta, S.e0, (int)cnt, 0, s_lcol, s_mcol, s_mval,
+                              threadIdx.x, width, cmax, evs);
+    } else {
+      RowJob<T> J;
+      J.v = S.v;
+      J.e0 = S.e0;
+      J.e1 = S.e0 + S.nd;
+      J.nd = S.nd;
+      J.cnt = cnt;
+      J.dst = dst;
+      merge_lists<T>(J, indices, data, ws, ar, nullptr, mcol, mval, threadIdx.x, LV_NT, width, cmax, evs);
+    }
+    __syncthreads();
+    int64_t out = 0;
+    for (int64_t b0 = 0; b0 < cnt; b0 += LV_NT) {
+      const int64_t i2 = b0 + threadIdx.x;
+      int32_t c = -1;
+      T val = 0;
+      bool keep = false;
+      if (i2 < cnt) {
+        if (in_lds) {
+          keep = lds_run_head<T>(s_mcol, s_mval, (int)i2, (int)cnt, c, val) && val != 0;
+        } else {
+          c = mcol[i2];
+          val = mval[i2];
+          keep = c >= 0;
+        }
+      }
+      const uint64_t msk = __ballot(keep);
+      if (lane == 0) s_wsum[wv] = __popcll(msk);
+      __syncthreads();
+      int64_t before = 0, total = 0;
+      for (int q = 0; q < LV_WAVES; q++) {
+        before += q < wv ? s_wsum[q] : 0;
+        total += s_wsum[q];
+      }
+      if (keep) {
+        const int64_t o = dst + out + before + __popcll(msk & ((1ull << lane) - 1ull));
+        ar.cols[o] = c;
+        ar.vals[o] = val;
+        const T a = val < 0 ? (T)-val : val;
+        if (a > 1 && a > fmax) fmax = a;
+      }
+      out += total;
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      ws.row_off[S.v] = dst;
+      ws.row_len[S.v] = out;
+    }
   }
   if (cmax) atomicMax((long long*)&s_cmax, (long long)cmax);
   if (fmax) atomicMax((long long*)&s_fmax, (long long)fmax);
@@ -1270,7 +1700,8 @@ int64_t rbg_dag_workspace_bytes(int32_t n) { return DagWork::bytes(n > 0 ? n : 1
                        ws.counters, n);                                                                         \
     RBG_CHECK_LAUNCH();                                                                                         \
     const int gv = (int)((n + 255) / 256 < 2048 ? (n + 255) / 256 : 2048);                                      \
-    const int gw = (int)((n + LV_WAVES - 1) / LV_WAVES < 4096 ? (n + LV_WAVES - 1) / LV_WAVES : 4096);          \
+    const int gr = (int)(n < LU_GRID ? n : LU_GRID);                                                          \
+    const int gl = (int)(n < 1024 ? n : 1024);                                                                \
     int64_t host[C_NCTR];                                                                                       \
     for (int32_t r = 0; r <= n; r++) {                                                                          \
       hipLaunchKernelGGL(k_round_place, dim3(gv), dim3(256), 0, s, indptr, indices, ws, r);                     \
@@ -1280,8 +1711,11 @@ int64_t rbg_dag_workspace_bytes(int32_t n) { return DagWork::bytes(n > 0 ? n : 1
       int rc = scan_dev(ws.cnt, ws.cand_loc, ws.sums_a, ws.counters + C_LSIZE, 1, n, ws.counters + C_M, s);    \
       if (rc) return rc;                                                                                        \
       int pi = prof_start(s);                                                                                   \
-      hipLaunchKernelGGL((k_lvl_merge<T>), dim3(gw), dim3(LV_NT), 0, s, indptr, indices, data, ws, ar,          \
-                         width, st);                                                                            \
+      hipLaunchKernelGGL((k_lvl_units<T>), dim3(gr), dim3(64), 0, s, indices, data, ws, ar, width, st);         \
+      RBG_CHECK_LAUNCH();                                                                                       \
+      hipLaunchKernelGGL((k_lvl_rows<T>), dim3(gr), dim3(64), 0, s, indices, data, ws, ar, width, st);          \
+      RBG_CHECK_LAUNCH();                                                                                       \
+      hipLaunchKernelGGL((k_lvl_long<T>), dim3(gl), dim3(LV_NT), 0, s, indices, data, ws, ar, width, st);        \
       RBG_CHECK_LAUNCH();                                                                                       \
       prof_stop(2, pi, s);                                                                                      \
       hipLaunchKernelGGL(k_round_end, dim3(1), dim3(1), 0, s, ws, r);                                           \
