@@ -186,7 +186,7 @@ def bench_sparse(args):
     L = _lib.load()
     for _ in range(max(1, args.warmup)):
         C, info = sd.dag_closure(sd.canonicalize(Ad), plan=plan)
-    plan = sd.DagPlan(A.shape[0], t.int64, info.arena + 4096)
+    plan = sd.DagPlan(A.shape[0], t.int64, info.arena + 4096, A.nnz)
     C, info = sd.dag_closure(sd.canonicalize(Ad), plan=plan)
     t.cuda.synchronize()
     L.rbg_prof_enable(1)

@@ -162,7 +162,9 @@ int rbg_csr_validate(const int64_t *indptr, const int32_t *idx, int32_t n, int32
 
 /* ---- sparse DAG closure (_transitive_closure_dag.pyx:168-351,
  *      gpu/transitive_closure.py:474-652) --------------------------------------- */
-int64_t rbg_dag_workspace_bytes(int32_t n);
+/* Workspace for n vertices and a matrix of nnz entries (nnz = 0 for the
+ * SpGEMM entry points, which need no predecessor lists). */
+int64_t rbg_dag_workspace_bytes(int32_t n, int64_t nnz);
 /* Closure of a DAG (canonical CSR: int64 row pointers, sorted rows, no explicit
  * zeros).  Rounds r = 0, 1, ...: the vertices whose non-self successors all sit
  * in levels < r form level r, and each of their closure rows is merged by one
@@ -171,7 +173,10 @@ int64_t rbg_dag_workspace_bytes(int32_t n);
  * into the front of their slot), so the arena needs the total candidate count
  * sum_v (nnz(A[v]) + sum_w nnz(C[w])), not nnz(C).  All sizes live on the
  * device; the host synchronises once every `batch` rounds to test completion.
- * Results: closure rows in the arena (see rbg_dag_gather_*); st->aux[2] max
+ * Levels come from the reversed edges (a vertex joins level r + 1 when its last
+ * unplaced successor is placed in level r), whose lists live in the workspace:
+ * size it with rbg_dag_workspace_bytes(n, nnz(A)) (a smaller one sets
+ * RBG_F_CAPACITY).  Results: closure rows in the arena (see rbg_dag_gather_*); st->aux[2] max
  * |candidate| (Cython diameter rule), [3] max |final value| > 1 (GPU diameter
  * rule), [4] arena entries used, [6] levels.  A cycle sets RBG_F_CYCLE with
  * st->aux[0] = a vertex on the cycle; a full arena sets RBG_F_CAPACITY (grow

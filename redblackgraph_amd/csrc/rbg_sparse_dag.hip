@@ -191,8 +191,8 @@ static int scan_dev(const int64_t* in, int64_t* local, int64_t* sums, const int6
 // ---------------------------------------------------------------------------
 enum {
   C_PLACED = 0,   // vertices placed so far
-  C_M0 = 1,       // unplaced count, even rounds read C_M0 and write C_M1
-  C_M1 = 2,
+  C_UNUSED1 = 1,
+  C_UNUSED2 = 2,
   C_LSTART = 3,   // first index in `order` of the current level
   C_LSIZE = 4,    // vertices in the current level
   C_M = 5,        // candidate entries of the current level
@@ -214,7 +214,7 @@ enum {
 struct DagWork {
   int32_t* level;      // [n] level of each vertex (INT32_MAX = unplaced)
   int32_t* order;      // [n] vertices grouped by level, sinks first
-  int32_t* rem[2];     // [n] unplaced vertex lists (ping-pong)
+  int32_t* rem[2];     // [n] unplaced successors of each vertex; predecessor fill cursors
   int64_t* row_off;    // [n] arena offset of each closure row
   int64_t* row_len;    // [n+1] closure row lengths
   int64_t* cnt;        // [n+1] candidate counts of the current level's vertices
@@ -232,12 +232,17 @@ struct DagWork {
   int32_t* slot_len;   // [n * SLOT_D]
   int64_t* slot_av;    // [n * SLOT_D] (edge values, stored as int64)
   int32_t* long_list;  // [n] slots of the current level left to the long-row step
+  int64_t* pptr;       // [n+1] predecessor lists (the reversed non-self edges) ...
+  int32_t* preds;      // [ecap] ... carved from the workspace's tail by the closure
+  int64_t ecap;
   static constexpr int SLOT_D = 7;
   static int64_t bytes(int64_t n) {
-    int64_t b = 4 * n * 4 + 8 * n + 8 * (n + 1) * 3 + 8 * (n / SCAN_CHUNK + 8) + 8 * C_NCTR;
+    int64_t b = 4 * n * 4 + 8 * n + 8 * (n + 1) * 4 + 8 * (n / SCAN_CHUNK + 8) + 8 * C_NCTR;
     b += n * (4 + 8 + 4 + 4 + 4) + n * SLOT_D * (8 + 4 + 8);
-    return b + 25 * 256;
+    return b + 26 * 256;
   }
+  // with room for the predecessor lists of a matrix with nnz entries
+  static int64_t bytes(int64_t n, int64_t nnz) { return bytes(n) + 4 * (nnz > 0 ? nnz : 0) + 256; }
   static DagWork carve(void* base, int64_t n) {
     DagWork w;
     char* p = (char*)(((uintptr_t)base + 255) & ~(uintptr_t)255);
@@ -264,73 +269,122 @@ struct DagWork {
     w.slot_len = (int32_t*)take(4 * n * SLOT_D);
     w.slot_av = (int64_t*)take(8 * n * SLOT_D);
     w.long_list = (int32_t*)take(4 * n);
+    w.pptr = (int64_t*)take(8 * (n + 1));
+    w.preds = (int32_t*)p;
+    w.ecap = 0;
     return w;
   }
 };
 
-__global__ void k_dag_init(int32_t* level, int32_t* rem, int64_t* row_len, int64_t* counters, int32_t n) {
+__global__ void k_dag_init(DagWork ws, int32_t n) {
   int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (v < n) {
-    level[v] = INT32_MAX;
-    rem[v] = (int32_t)v;
-    row_len[v] = 0;
+    ws.level[v] = INT32_MAX;
+    ws.rem[1][v] = 0;
+    ws.row_len[v] = 0;
+    ws.cnt[v] = 0;
   }
   if (v == 0) {
-    for (int q = 0; q < C_NCTR; q++) counters[q] = 0;
-    counters[C_M0] = n;
-    counters[C_CYCV] = -1;
+    for (int q = 0; q < C_NCTR; q++) ws.counters[q] = 0;
+    ws.counters[C_CYCV] = -1;
   }
 }
 
 __device__ inline bool dag_halted(const int64_t* c) { return c[C_STUCK] | c[C_CAP] | c[C_DONE]; }
 
-// Round r, step 1: every unplaced vertex whose non-self successors all sit in
-// levels < r joins level r.  Placement offsets come from one atomic per
-// workgroup (a workgroup-local count first), not one per vertex.
-__global__ __launch_bounds__(256) void k_round_place(const int64_t* __restrict__ indptr,
-                                                     const int32_t* __restrict__ indices, DagWork ws, int32_t r) {
-  __shared__ int s_ready, s_wait;
-  __shared__ int64_t s_base_ready, s_base_wait;
-  int64_t* ctr = ws.counters;
-  if (dag_halted(ctr)) return;
-  const int64_t m = ctr[(r & 1) ? C_M1 : C_M0];
-  const int32_t* rem_in = ws.rem[r & 1];
-  int32_t* rem_out = ws.rem[(r & 1) ^ 1];
-  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < m; base += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t t = base + threadIdx.x;
-    if (threadIdx.x == 0) s_ready = s_wait = 0;
-    __syncthreads();
-    int32_t v = -1;
-    bool ready = false;
-    int slot = 0;
-    if (t < m) {
-      v = rem_in[t];
-      ready = true;
+// Levels by Kahn's method on the reversed edges: level 0 is every vertex
+// without non-self successors, and a vertex joins level r + 1 when its last
+// unplaced non-self successor is placed in level r, so its level is the
+// longest path to a sink.  Placement offsets in `order` come from one atomic
+// per workgroup (a workgroup-local count first).
+__device__ inline void place_ready(DagWork& ws, int32_t v, bool ready, int32_t r) {
+  __shared__ int s_n;
+  __shared__ int64_t s_base;
+  if (threadIdx.x == 0) s_n = 0;
+  __syncthreads();
+  const int slot = ready ? atomicAdd(&s_n, 1) : 0;
+  __syncthreads();
+  if (threadIdx.x == 0 && s_n) s_base = (int64_t)atomicAdd((unsigned long long*)&ws.counters[C_PLACED], (unsigned long long)s_n);
+  __syncthreads();
+  if (ready) {
+    ws.level[v] = r;
+    ws.order[s_base + slot] = v;
+  }
+  __syncthreads();
+}
+
+// Setup, step 1: each vertex's unplaced-successor count, each vertex's
+// predecessor count, and level 0.
+__global__ __launch_bounds__(256) void k_kahn_count(const int64_t* __restrict__ indptr,
+                                                    const int32_t* __restrict__ indices, DagWork ws, int32_t n) {
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t v = base + threadIdx.x;
+    int32_t c = 0;
+    if (v < n) {
       for (int64_t e = indptr[v]; e < indptr[v + 1]; e++) {
-        int32_t w = indices[e];
-        if (w != v && ws.level[w] >= r) {
-          ready = false;
-          break;
+        const int32_t w = indices[e];
+        if (w != v) {
+          c++;
+          atomicAdd((unsigned long long*)&ws.cnt[w], 1ull);
         }
       }
-      slot = ready ? atomicAdd(&s_ready, 1) : atomicAdd(&s_wait, 1);
+      ws.rem[0][v] = c;
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      s_base_ready = s_ready ? (int64_t)atomicAdd((unsigned long long*)&ctr[C_PLACED], (unsigned long long)s_ready) : 0;
-      s_base_wait = s_wait ? (int64_t)atomicAdd((unsigned long long*)&ctr[(r & 1) ? C_M0 : C_M1],
-                                                (unsigned long long)s_wait) : 0;
-    }
-    __syncthreads();
-    if (t < m) {
-      if (ready) {
-        ws.level[v] = r;
-        ws.order[s_base_ready + slot] = v;
-      } else {
-        rem_out[s_base_wait + slot] = v;
+    place_ready(ws, (int32_t)v, v < n && c == 0, 0);
+  }
+}
+
+// Setup, step 2 (after the predecessor counts are scanned into pptr): the
+// predecessor lists.
+__global__ __launch_bounds__(256) void k_kahn_fill(const int64_t* __restrict__ indptr,
+                                                   const int32_t* __restrict__ indices, DagWork ws, int32_t n) {
+  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t e = indptr[v]; e < indptr[v + 1]; e++) {
+      const int32_t w = indices[e];
+      if (w != v) {
+        const int64_t q = ws.pptr[w] + atomicAdd(&ws.rem[1][w], 1);
+        if (q < ws.ecap) ws.preds[q] = (int32_t)v;
+        else ws.counters[C_CAP] = 1;  // a workspace sized without the matrix's nnz
       }
     }
+  }
+}
+
+// Round r >= 1, step 1: level r from level r - 1 (the slots [C_LSTART -
+// C_LSIZE, C_LSTART) of `order`): each placed vertex releases its
+// predecessors, and a predecessor whose count reaches zero joins level r.
+__global__ __launch_bounds__(256) void k_kahn_round(DagWork ws, int32_t r) {
+  const int64_t* ctr = ws.counters;
+  if (r == 0 || dag_halted(ctr)) return;
+  const int64_t f1 = ctr[C_LSTART], f0 = f1 - ctr[C_LSIZE];
+  for (int64_t base = f0 + (int64_t)blockIdx.x * blockDim.x; base < f1; base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = base + threadIdx.x;
+    // each thread releases one placed vertex's predecessors; those it frees
+    // are placed in rounds of the workgroup
+    int64_t q0 = 0, q1 = 0;
+    if (i < f1) {
+      const int32_t u = ws.order[i];
+      q0 = ws.pptr[u];
+      q1 = ws.pptr[u + 1];
+    }
+    int64_t kmax = q1 - q0;
+    // the workgroup's longest predecessor list bounds its placement rounds
+    __shared__ int64_t s_k;
+    if (threadIdx.x == 0) s_k = 0;
     __syncthreads();
+    if (kmax) atomicMax((unsigned long long*)&s_k, (unsigned long long)kmax);
+    __syncthreads();
+    kmax = s_k;
+    for (int64_t k = 0; k < kmax; k++) {
+      int32_t p = -1;
+      bool ready = false;
+      if (q0 + k < q1) {
+        p = ws.preds[q0 + k];
+        ready = atomicSub(&ws.rem[0][p], 1) == 1;
+      }
+      place_ready(ws, p, ready, r);
+    }
+    __syncthreads();  // s_k is reset by the next pass
   }
 }
 
@@ -1274,20 +1328,18 @@ __global__ __launch_bounds__(LV_NT) void k_lvl_long(const int32_t* __restrict__ 
 // Round r, step 5 (single lane): advance the arena, count the level, detect a
 // cycle (nothing placed while vertices remain) and completion, and set up
 // round r + 1.
-__global__ void k_round_end(DagWork ws, int32_t r) {
+__global__ void k_round_end(DagWork ws, int32_t r, int32_t n) {
   int64_t* ctr = ws.counters;
   if (ctr[C_STUCK] | ctr[C_CAP] | ctr[C_DONE]) return;
   const int64_t placed_now = ctr[C_PLACED] - ctr[C_LSTART];
-  const int64_t m_next = ctr[(r & 1) ? C_M0 : C_M1];
   ctr[C_ARENA] += ctr[C_M];  // the level's reserved slots
   if (placed_now > 0) ctr[C_NLEV] = r + 1;
-  if (m_next == 0) {
+  if (ctr[C_PLACED] == n) {
     ctr[C_DONE] = 1;
   } else if (placed_now == 0) {
-    ctr[C_STUCK] = 1;
+    ctr[C_STUCK] = 1;  // nothing placed while vertices remain: a cycle
   }
   ctr[C_LSTART] = ctr[C_PLACED];
-  ctr[(r & 1) ? C_M1 : C_M0] = 0;  // becomes the output count of round r + 1
   ctr[C_ROUND] = r + 1;
 }
 
@@ -1683,7 +1735,7 @@ using namespace rbg;
 
 extern "C" {
 
-int64_t rbg_dag_workspace_bytes(int32_t n) { return DagWork::bytes(n > 0 ? n : 1); }
+int64_t rbg_dag_workspace_bytes(int32_t n, int64_t nnz) { return DagWork::bytes(n > 0 ? n : 1, nnz); }
 
 #define RBG_DAG_ENTRY(SUF, T)                                                                                   \
   int rbg_dag_closure_##SUF(const int64_t* indptr, const int32_t* indices, const T* data, int32_t n,            \
@@ -1695,16 +1747,27 @@ int64_t rbg_dag_workspace_bytes(int32_t n) { return DagWork::bytes(n > 0 ? n : 1
     if (ws_bytes < DagWork::bytes(n)) return RBG_E_WORKSPACE;                                                   \
     if (batch < 1) batch = 8;                                                                                   \
     DagWork ws = DagWork::carve(workspace, n);                                                                  \
+    ws.ecap = ((char*)workspace + ws_bytes - (char*)ws.preds) / 4;                                              \
     DagArena<T> ar{arena_cols, arena_vals, arena_cap};                                                          \
-    hipLaunchKernelGGL(k_dag_init, dim3((n + 255) / 256), dim3(256), 0, s, ws.level, ws.rem[0], ws.row_len,     \
-                       ws.counters, n);                                                                         \
+    hipLaunchKernelGGL(k_dag_init, dim3((n + 255) / 256), dim3(256), 0, s, ws, n);                              \
     RBG_CHECK_LAUNCH();                                                                                         \
     const int gv = (int)((n + 255) / 256 < 2048 ? (n + 255) / 256 : 2048);                                      \
     const int gr = (int)(n < LU_GRID ? n : LU_GRID);                                                          \
     const int gl = (int)(n < 1024 ? n : 1024);                                                                \
     int64_t host[C_NCTR];                                                                                       \
+    hipLaunchKernelGGL(k_kahn_count, dim3(gv), dim3(256), 0, s, indptr, indices, ws, n);                        \
+    RBG_CHECK_LAUNCH();                                                                                         \
+    {                                                                                                           \
+      int rc = scan_exclusive(ws.cnt, ws.pptr, n, ws.sums_a, ws.counters + C_NNZ, s);                           \
+      if (rc) return rc;                                                                                        \
+      hipLaunchKernelGGL(k_set_last, dim3(1), dim3(1), 0, s, ws.pptr, (int64_t)n,                               \
+                         (const int64_t*)(ws.counters + C_NNZ));                                                \
+      RBG_CHECK_LAUNCH();                                                                                       \
+    }                                                                                                           \
+    hipLaunchKernelGGL(k_kahn_fill, dim3(gv), dim3(256), 0, s, indptr, indices, ws, n);                         \
+    RBG_CHECK_LAUNCH();                                                                                         \
     for (int32_t r = 0; r <= n; r++) {                                                                          \
-      hipLaunchKernelGGL(k_round_place, dim3(gv), dim3(256), 0, s, indptr, indices, ws, r);                     \
+      hipLaunchKernelGGL(k_kahn_round, dim3(gv), dim3(256), 0, s, ws, r);                                       \
       RBG_CHECK_LAUNCH();                                                                                       \
       hipLaunchKernelGGL((k_lvl_count<T>), dim3(gv), dim3(256), 0, s, indptr, indices, data, ws);                \
       RBG_CHECK_LAUNCH();                                                                                       \
@@ -1718,7 +1781,7 @@ int64_t rbg_dag_workspace_bytes(int32_t n) { return DagWork::bytes(n > 0 ? n : 1
       hipLaunchKernelGGL((k_lvl_long<T>), dim3(gl), dim3(LV_NT), 0, s, indices, data, ws, ar, width, st);        \
       RBG_CHECK_LAUNCH();                                                                                       \
       prof_stop(2, pi, s);                                                                                      \
-      hipLaunchKernelGGL(k_round_end, dim3(1), dim3(1), 0, s, ws, r);                                           \
+      hipLaunchKernelGGL(k_round_end, dim3(1), dim3(1), 0, s, ws, r, n);                                        \
       RBG_CHECK_LAUNCH();                                                                                       \
       if ((r + 1) % batch == 0) {                                                                               \
         hipError_t e = hipMemcpyAsync(host, ws.counters, sizeof(host), hipMemcpyDeviceToHost, s);              \

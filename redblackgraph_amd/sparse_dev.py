@@ -109,13 +109,14 @@ class DagPlan:
     so it needs sum_v (nnz(A[v]) + sum_w nnz(C[w])) entries (DagInfo.arena
     after a run), somewhat more than nnz(C)."""
 
-    def __init__(self, n: int, value_dtype, arena_entries: int):
+    def __init__(self, n: int, value_dtype, arena_entries: int, nnz: int = 0):
         t = _lib.torch()
         dev = _lib.device()
         L = _lib.load()
         self.n = n
         self.vdt = value_dtype
-        self.ws = t.empty(int(L.rbg_dag_workspace_bytes(max(n, 1))), dtype=t.uint8, device=dev)
+        self.nnz = int(nnz)
+        self.ws = t.empty(int(L.rbg_dag_workspace_bytes(max(n, 1), self.nnz)), dtype=t.uint8, device=dev)
         self.arena_cols = t.empty(arena_entries, dtype=t.int32, device=dev)
         self.arena_vals = t.empty(arena_entries, dtype=value_dtype, device=dev)
 
@@ -138,8 +139,8 @@ def dag_closure(A: DevCSR, width: int | None = None, stream=None, plan: DagPlan 
         return A, DagInfo(0, 0, 0, -1, 0, 0)
     st = status()
     sh = _lib.stream_handle(stream)
-    if plan is None or plan.n != n or plan.vdt != vdt:
-        plan = DagPlan(n, vdt, int(max(4096, arena_factor * (A.nnz + n))))
+    if plan is None or plan.n != n or plan.vdt != vdt or plan.nnz < A.nnz:
+        plan = DagPlan(n, vdt, int(max(4096, arena_factor * (A.nnz + n))), A.nnz)
     sfx = "i32" if vdt == t.int32 else "i64"
     for _attempt in range(8):
         st.reset(stream)
@@ -150,7 +151,7 @@ def dag_closure(A: DevCSR, width: int | None = None, stream=None, plan: DagPlan 
         if s.flags & _lib.F_CYCLE:
             return None, DagInfo(0, 0, 0, int(s.aux[0]), int(s.aux[6]), 0)
         if s.flags & _lib.F_CAPACITY:
-            plan = DagPlan(n, vdt, 2 * plan.arena_cols.numel())
+            plan = DagPlan(n, vdt, 2 * plan.arena_cols.numel(), A.nnz)
             continue
         info = DagInfo(0, int(s.aux[2]), int(s.aux[3]), -1, int(s.aux[6]), int(s.events))
         info.arena = int(s.aux[4])
@@ -190,7 +191,7 @@ def spgemm(A: DevCSR, B: DevCSR, mask_upper: bool = False, width: int | None = N
     st = status()
     sh = _lib.stream_handle(stream)
     sfx = "i32" if vdt == t.int32 else "i64"
-    ws = t.empty(int(_lib.load().rbg_dag_workspace_bytes(m)), dtype=t.uint8, device=dev)
+    ws = t.empty(int(_lib.load().rbg_dag_workspace_bytes(m, 0)), dtype=t.uint8, device=dev)
     cap = max(1024, expand)
     tcap = max(1024, 3 * expand)
     ac = t.empty(cap, dtype=t.int32, device=dev)
