@@ -273,6 +273,53 @@ def cpu_baseline_batched(sizes, diag, edges, budget_s: float = 15.0):
     }
 
 
+def _batched_plain_legs(b, sizes, pristine, eager_result):
+    """The batch's components closed one graph at a time: by the same kernels
+    (a one-component schedule each, on one stream), and by the single-matrix
+    entry point.  Returns (plain_ms, identical, single_ms, identical)."""
+    import ctypes
+    import numpy as np
+    from redblackgraph_amd import _lib, dense
+    from redblackgraph_amd._dtypes import elem_type
+    t = _lib.torch()
+    L = _lib.load()
+    # plain path: the same kernels launched one graph at a time on one stream
+    # (a one-component schedule per graph, built before the clock starts)
+    handles = []
+    for i in range(b.count):
+        one = (_lib.RbgBatchComp * 1)(b.descs[i])
+        h = ctypes.c_void_p()
+        _lib.call("rbg_batch_create", one, 1, ctypes.byref(h))
+        handles.append(h)
+    sh = _lib.stream_handle(b.stream)
+    with t.cuda.stream(b.stream):
+        b.pool.copy_(pristine)
+    b.stream.synchronize()
+    tp0 = time.perf_counter()
+    for h in handles:
+        _lib.call("rbg_batch_run", h, sh)
+    b.stream.synchronize()
+    plain_ms = (time.perf_counter() - tp0) * 1e3
+    identical_plain = bool(t.equal(b.pool, eager_result))
+    for h in handles:
+        L.rbg_batch_destroy(h)
+    # the single-matrix entry point, one closure per component (its own panel
+    # kernel, a status read per closure)
+    et = elem_type(np.int32)
+    views = []
+    for i, n in enumerate(sizes):
+        p, o = int(b.layout.npad[i]), int(b.layout.offset[i])
+        views.append(pristine[o:o + p * p].view(p, p)[:int(n), :int(n)])
+    t.cuda.synchronize()
+    tp0 = time.perf_counter()
+    single = [dense.fw_closure(V, et)[0] for V in views]
+    t.cuda.synchronize()
+    single_ms = (time.perf_counter() - tp0) * 1e3
+    identical_single = all(bool(t.equal(single[i], b.result(i))) for i in range(len(sizes)))
+    del single
+    return plain_ms, identical_plain, single_ms, identical_single
+
+
 def bench_batched(args):
     import ctypes
     import numpy as np
@@ -317,40 +364,11 @@ def bench_batched(args):
     t1 = time.perf_counter()
     ms = (t1 - t0) * 1e3 / args.steps
     identical_graph = bool(t.equal(b.pool, eager_result))
-    # plain path: the same kernels launched one graph at a time on one stream
-    # (a one-component schedule per graph, built before the clock starts)
-    handles = []
-    for i in range(b.count):
-        one = (_lib.RbgBatchComp * 1)(b.descs[i])
-        h = ctypes.c_void_p()
-        _lib.call("rbg_batch_create", one, 1, ctypes.byref(h))
-        handles.append(h)
-    sh = _lib.stream_handle(b.stream)
-    with t.cuda.stream(b.stream):
-        b.pool.copy_(pristine)
-    b.stream.synchronize()
-    tp0 = time.perf_counter()
-    for h in handles:
-        _lib.call("rbg_batch_run", h, sh)
-    b.stream.synchronize()
-    plain_ms = (time.perf_counter() - tp0) * 1e3
-    identical_plain = bool(t.equal(b.pool, eager_result))
-    for h in handles:
-        L.rbg_batch_destroy(h)
-    # the single-matrix entry point, one closure per component (its own panel
-    # kernel, a status read per closure)
-    et = elem_type(np.int32)
-    views = []
-    for i, n in enumerate(sizes):
-        p, o = int(b.layout.npad[i]), int(b.layout.offset[i])
-        views.append(pristine[o:o + p * p].view(p, p)[:int(n), :int(n)])
-    t.cuda.synchronize()
-    tp0 = time.perf_counter()
-    single = [dense.fw_closure(V, et)[0] for V in views]
-    t.cuda.synchronize()
-    single_ms = (time.perf_counter() - tp0) * 1e3
-    identical_single = all(bool(t.equal(single[i], b.result(i))) for i in range(len(sizes)))
-    del single
+    if args.no_plain:
+        plain_ms = single_ms = None
+        identical_plain = identical_single = None
+    else:
+        plain_ms, identical_plain, single_ms, identical_single = _batched_plain_legs(b, sizes, pristine, eager_result)
     P = b.layout.npad.astype(np.float64)
     relax_exec = float(np.sum((P - FW_BLK) ** 2 * P))
     tile_s = ms_.value / 1e3
@@ -375,10 +393,12 @@ def bench_batched(args):
                    "components": len(sizes), "sum_n": int(sizes.sum()), "sum_n2": entries,
                    "closures_per_s": len(sizes) / (ms / 1e3),
                    "plain": "the same kernels launched one graph at a time on one stream",
-                   "plain_ms": plain_ms, "plain_closures_per_s": len(sizes) / (plain_ms / 1e3),
-                   "batched_vs_plain": plain_ms / ms,
+                   "plain_ms": plain_ms,
+                   "plain_closures_per_s": len(sizes) / (plain_ms / 1e3) if plain_ms else None,
+                   "batched_vs_plain": plain_ms / ms if plain_ms else None,
                    "identical_to_plain": identical_plain, "graph_replay_identical_to_eager": identical_graph,
-                   "single_matrix_path_ms": single_ms, "batched_vs_single_matrix_path": single_ms / ms,
+                   "single_matrix_path_ms": single_ms,
+                   "batched_vs_single_matrix_path": single_ms / ms if single_ms else None,
                    "identical_to_single_matrix_path": identical_single,
                    "launches_per_batch": info["launches"], "pivot_steps": info["steps"],
                    "tiles": {str(k): v for k, v in sorted(b.tiles.items())}, "autotune_s": tune_s},
@@ -502,6 +522,8 @@ def main():
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--cpu-sample", type=int, default=1024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="batched workload: skip the one-graph-at-a-time comparison legs (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU box has one GPU); "

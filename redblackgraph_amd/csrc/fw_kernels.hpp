@@ -907,10 +907,19 @@ __global__ __launch_bounds__(PANEL_NT) void k_fw_panel_fast(T* W, int64_t n, int
   fw_panel_pair_body<T>(W, n, ld, K, ws, width, st, blockIdx.x);
 }
 
+// Phase-3 tile shape per element type: 8 x 8 int32 cells per thread (128 x
+// 128 tiles), 4 x 4 int64 cells (64 x 64 tiles: an 8 x 8 int64 block needs more
+// registers than leave room for a second wave per SIMD).
 template <typename T>
-__global__ __launch_bounds__(NT, sizeof(T) == 4 ? 3 : 1) void k_fw_tile_fast(T* W, int64_t n, int64_t ld, int64_t K, FwWork ws,
-                                                     int width, rbg_status* st) {
-  fw_tile_fast_body<T, RPT, CPT, true>(W, n, ld, K, ws, width, st, blockIdx.x);
+struct FastTile {
+  static constexpr int RC = sizeof(T) == 4 ? 8 : 4;
+  static constexpr int EDGE = 16 * RC;
+};
+
+template <typename T>
+__global__ __launch_bounds__(NT, 3) void k_fw_tile_fast(T* W, int64_t n, int64_t ld, int64_t K, FwWork ws,
+                                                       int width, rbg_status* st) {
+  fw_tile_fast_body<T, FastTile<T>::RC, FastTile<T>::RC, true>(W, n, ld, K, ws, width, st, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------

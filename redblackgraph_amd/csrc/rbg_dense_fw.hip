@@ -33,7 +33,8 @@ static int fw_dense(T* W, int64_t n, int64_t ld, int width, int is_uns, int mode
     prof_stop(0, p0, s);
     int p1 = prof_start(s);
     if (fast) {
-      hipLaunchKernelGGL((k_fw_tile_fast<T>), dim3(nt * nt), dim3(NT), 0, s, W, n, ld, (int64_t)K, ws, width, st);
+      const int ft = (int)(n / FastTile<T>::EDGE);
+      hipLaunchKernelGGL((k_fw_tile_fast<T>), dim3(ft * ft), dim3(NT), 0, s, W, n, ld, (int64_t)K, ws, width, st);
     } else {
       hipLaunchKernelGGL((k_fw_tile_general<T>), dim3(nt * nt), dim3(NT), 0, s, W, n, ld, (int64_t)K, ws,
                          width, is_uns, st);
