@@ -193,8 +193,7 @@ class ClosureServer:
             if st.events:
                 raise OverflowError("Avos product results in an overflow during edge composition")
         else:
-            for (a, b, rel) in req.edges:
-                value = sd.edge_insert(value, int(a), int(b), int(rel), stream=stream)
+            value = sd.edge_insert_many(value, req.edges, stream=stream)
         res.entries = resident.n ** 2 if resident.kind == "dense" else value.nnz
         if self.digests:
             res.digest = _digest(value, resident.kind)

@@ -6,6 +6,7 @@ sparse.rb_matrix) wrap these.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes
 from dataclasses import dataclass
 
@@ -259,6 +260,64 @@ def csr_equal(A: DevCSR, B: DevCSR) -> bool:
         return True
     return bool(t.equal(A.indptr, B.indptr)) and bool(t.equal(A.indices, B.indices)) and \
         bool(t.equal(A.data, B.data.to(A.data.dtype)))
+
+
+# Largest dense form of a closure that edge_insert_many builds (bytes).
+DENSE_INSERT_MAX_BYTES = 4 << 30
+
+
+def to_dense(C: DevCSR):
+    """The n x n device matrix of a CSR (absent entries 0)."""
+    t = _lib.torch()
+    n, m = C.shape
+    D = t.zeros((n, m), dtype=C.data.dtype, device=C.indices.device)
+    if C.nnz:
+        rows = t.repeat_interleave(t.arange(n, device=D.device), C.indptr[1:] - C.indptr[:-1])
+        D[rows, C.indices.long()] = C.data
+    return D
+
+
+def from_dense(D) -> DevCSR:
+    """Canonical CSR of a device matrix (row-major order, zeros dropped)."""
+    t = _lib.torch()
+    mask = D != 0
+    indptr = t.zeros(D.shape[0] + 1, dtype=t.int64, device=D.device)
+    t.cumsum(mask.sum(1), 0, out=indptr[1:])
+    nz = mask.nonzero()
+    return DevCSR(indptr, nz[:, 1].to(t.int32).contiguous(), D[mask].contiguous(), tuple(D.shape))
+
+
+def edge_insert_many(C: DevCSR, edges, stream=None) -> DevCSR:
+    """C with the parent edges (alpha, beta, relationship) inserted one after
+    another, each as edge_insert.  While C's dense form fits
+    DENSE_INSERT_MAX_BYTES the edges are applied to it by the dense rank-1
+    kernels back to back (rbg_edge_compose_*, one status read for all of
+    them) and the result goes back to CSR; otherwise edge by edge.  Raises
+    ValueError when an edge would close a cycle, OverflowError on an
+    overflowing product."""
+    from . import dense
+    from ._dtypes import elem_type
+    t = _lib.torch()
+    n = C.n_rows
+    edges = [(int(a), int(b), int(r)) for (a, b, r) in edges]
+    for a, b, _ in edges:
+        if not (0 <= a < n and 0 <= b < n):
+            raise ValueError("vertex index out of range")
+    if not edges:
+        return C
+    if n * n * C.data.element_size() > DENSE_INSERT_MAX_BYTES:
+        for a, b, r in edges:
+            C = edge_insert(C, a, b, r, stream=stream)
+        return C
+    et = elem_type(np.int32 if C.data.dtype == t.int32 else np.int64)
+    with t.cuda.stream(stream) if stream is not None else contextlib.nullcontext():
+        D = to_dense(C)
+        W, st = dense.edge_compose_many(D, edges, et, stream=stream)
+        if st.flags & _lib.F_CYCLE:
+            raise ValueError("Relational composition would result in a cycle.")
+        if st.events:
+            raise OverflowError("Avos product results in an overflow during edge composition")
+        return from_dense(W)
 
 
 def edge_insert(C: DevCSR, alpha: int, beta: int, relationship: int, stream=None) -> DevCSR:

@@ -346,3 +346,38 @@ def test_sparse_edge_insert_equals_reclosure(orc, seed):
     k = int(np.nonzero(ref.indices < rows)[0][0])
     with pytest.raises(ValueError, match="cycle"):
         sd.edge_insert(C, int(ref.indices[k]), int(rows[k]), 2)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed", [1, 4])
+def test_sparse_edge_insert_many_dense_and_sparse_forms(orc, seed, monkeypatch):
+    """Several parent edges into a resident sparse closure: the dense-form
+    batch (edge_insert_many) and the edge-by-edge sparse form both equal the
+    oracle's closure of the edited graph; an edge closing a cycle raises."""
+    spec = synth.PedigreeSpec(2500, 11, 0.8, 3)
+    A = synth.csr_pedigree(spec, seed)
+    C0, _ = sd.dag_closure(sd.canonicalize(sd.from_scipy(A, np.int64)))
+    rng = np.random.default_rng(100 + seed)
+    diag = A.diagonal()
+    B = A.tolil()
+    edges = []
+    for _ in range(12):
+        a = int(rng.integers(1200, 2500))
+        rel = int(rng.choice([2, 3]))
+        cand = np.nonzero(diag[:a] == (-1 if rel == 2 else 1))[0]
+        b = int(rng.choice(cand))
+        edges.append((a, b, rel))
+        B[a, b] = rel
+    ref = orc.dag_closure(B.tocsr(), 64)
+    got = {"dense": sd.edge_insert_many(C0, edges)}
+    monkeypatch.setattr(sd, "DENSE_INSERT_MAX_BYTES", 0)
+    got["sparse"] = sd.edge_insert_many(C0, edges)
+    for form, C in got.items():
+        np.testing.assert_array_equal(C.indptr.cpu().numpy(), ref.indptr, err_msg=form)
+        np.testing.assert_array_equal(C.indices.cpu().numpy(), ref.indices, err_msg=form)
+        np.testing.assert_array_equal(C.data.cpu().numpy(), ref.data, err_msg=form)
+    monkeypatch.setattr(sd, "DENSE_INSERT_MAX_BYTES", 4 << 30)
+    rows = np.repeat(np.arange(2500), np.diff(ref.indptr))
+    k = int(np.nonzero(ref.indices < rows)[0][0])
+    with pytest.raises(ValueError, match="cycle"):
+        sd.edge_insert_many(got["dense"], [edges[0], (int(ref.indices[k]), int(rows[k]), 2)])
