@@ -12,6 +12,8 @@ Prints one JSON line (the bench contract) with:
   roofline       the dominant kernel (k_fw_tile_fast<int>, phase 3): integer
                  VALU ops it must execute (2.5 per relaxation) over its average
                  launch duration, measured with hipEvents on its launch stream
+                 (recorded on every --prof-every-th timed step, 4 by default,
+                 so that the events' own stalls stay out of most steps)
   cpu_baseline   the oracle's single-core naive loop (a C restatement of the
                  reference's warshall.c.in) on a bounded sample, scaled to the
                  workload by the N^3 law
@@ -90,6 +92,12 @@ def cpu_baseline_dense(n_sample: int, n_target: int, dtype):
     }
 
 
+def _sampled_steps(args) -> int:
+    """Timed steps whose kernels were bracketed by timing events (every
+    prof_every-th step, starting with the first)."""
+    return (args.steps + args.prof_every - 1) // args.prof_every
+
+
 def bench_dense(args, n: int, dtype_name: str):
     import numpy as np
     from redblackgraph_amd import _lib, dense, synth
@@ -107,10 +115,12 @@ def bench_dense(args, n: int, dtype_name: str):
     L.rbg_fw_prof_enable(1)
     t.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for i in range(args.steps):
+        L.rbg_prof_enable(2 if i % args.prof_every == 0 else 0)
         W, out = dense.fw_closure(Ad, et)
     t.cuda.synchronize()
     t1 = time.perf_counter()
+    sampled = _sampled_steps(args)
     import ctypes
     pm, pl, tm, tl = ctypes.c_double(), ctypes.c_int64(), ctypes.c_double(), ctypes.c_int64()
     _lib.call("rbg_fw_prof_read", ctypes.byref(pm), ctypes.byref(pl), ctypes.byref(tm), ctypes.byref(tl))
@@ -139,7 +149,8 @@ def bench_dense(args, n: int, dtype_name: str):
         "relaxations_per_launch": relax_per_launch,
         "avg_launch_ms": tile_avg_s * 1e3, "launches": tl.value,
         "panel_avg_ms": pm.value / max(pl.value, 1),
-        "kernel_share_of_step": (tm.value + pm.value) / (ms * args.steps),
+        "kernel_share_of_step": (tm.value + pm.value) / (ms * sampled),
+        "timed_steps_sampled": sampled,
     }
     res = {
         "metric": METRIC, "value": n * n / (ms / 1e3), "unit": "entries/s",
@@ -191,10 +202,12 @@ def bench_sparse(args):
     t.cuda.synchronize()
     L.rbg_prof_enable(1)
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for i in range(args.steps):
+        L.rbg_prof_enable(2 if i % args.prof_every == 0 else 0)
         C, info = sd.dag_closure(sd.canonicalize(Ad), plan=plan)
     t.cuda.synchronize()
     t1 = time.perf_counter()
+    sampled = _sampled_steps(args)
     ms_, nl = ctypes.c_double(), ctypes.c_int64()
     _lib.call("rbg_prof_read", 2, ctypes.byref(ms_), ctypes.byref(nl))
     L.rbg_prof_enable(0)
@@ -212,16 +225,17 @@ def bench_sparse(args):
     off = A.indices != rows
     reads = int(clen[A.indices[off]].sum())
     bytes_alg = 12 * A.nnz + 12 * reads + 12 * info.nnz + 8 * (n + 1)
-    kernel_s = ms_.value / 1e3 / args.steps
+    kernel_s = ms_.value / 1e3 / sampled
     achieved = bytes_alg / kernel_s
     roof = {
         "bound": "hbm", "kernel": "k_lvl_units<long> + k_lvl_rows<long> + k_lvl_long<long> (level merge)",
         "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s", "frac": achieved / PEAK_HBM,
         "traffic": _read_traffic("lvl_merge_sparse1m_pmc.txt"),
         "algorithmic_bytes_per_closure": bytes_alg,
-        "launches_per_closure": nl.value / args.steps, "avg_launch_ms": ms_.value / max(nl.value, 1),
+        "launches_per_closure": nl.value / sampled, "avg_launch_ms": ms_.value / max(nl.value, 1),
         "kernel_ms_per_closure": kernel_s * 1e3,
         "kernel_share_of_step": kernel_s * 1e3 / ms,
+        "timed_steps_sampled": sampled,
     }
     res = {
         "metric": METRIC, "value": info.nnz / (ms / 1e3), "unit": "entries/s",
@@ -522,9 +536,12 @@ def main():
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--cpu-sample", type=int, default=1024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--prof-every", type=int, default=4,
+                    help="bracket the kernels of every N-th timed step with timing events (1: every step)")
     ap.add_argument("--no-plain", action="store_true",
                     help="batched workload: skip the one-graph-at-a-time comparison legs (profiling runs)")
     args = ap.parse_args()
+    args.prof_every = max(1, args.prof_every)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU box has one GPU); "
               "run with --gpus 1", file=sys.stderr)

@@ -105,8 +105,10 @@ int64_t rbg_fw_workspace_bytes(int64_t n, int32_t elem_bytes);
  * durations (ms) and launch counts since the last read, then clears them. */
 int rbg_fw_prof_enable(int32_t on);
 /* Same facility for every timed launch kind: 0 FW panel, 1 FW tile, 2 DAG
- * level, 3 SpGEMM rows, 4 batched FW.  rbg_prof_read returns (and clears) the
- * summed duration and count of one kind. */
+ * level, 3 SpGEMM rows, 4 batched FW.  on = 1 starts recording afresh, 2
+ * resumes and 0 pauses, both keeping what was recorded (so a caller can time
+ * a sample of its launches).  rbg_prof_read returns (and clears) the summed
+ * duration and count of one kind. */
 int rbg_prof_enable(int32_t on);
 int rbg_prof_read(int32_t kind, double *total_ms, int64_t *launches);
 int rbg_fw_prof_read(double *panel_ms, int64_t *panel_launches, double *tile_ms, int64_t *tile_launches);
@@ -172,7 +174,10 @@ int64_t rbg_dag_workspace_bytes(int32_t n, int64_t nnz);
  * Every level reserves its candidate count in the arena (rows are compacted
  * into the front of their slot), so the arena needs the total candidate count
  * sum_v (nnz(A[v]) + sum_w nnz(C[w])), not nnz(C).  All sizes live on the
- * device; the host synchronises once every `batch` rounds to test completion.
+ * device; the host synchronises once every `batch` rounds to test completion
+ * (a graph of L levels takes L rounds, so batch = L, known from an earlier
+ * closure of the same graph, costs one synchronisation).  The call returns
+ * with the report still queued on `stream`: read st after the stream.
  * Levels come from the reversed edges (a vertex joins level r + 1 when its last
  * unplaced successor is placed in level r), whose lists live in the workspace:
  * size it with rbg_dag_workspace_bytes(n, nnz(A)) (a smaller one sets

@@ -26,6 +26,8 @@
 //            two steps.  Valid when no off-diagonal entry is +-1 or < -1 (checked
 //            by rbg_fw_precheck) and the per-block MSB guard rules out overflow;
 //            otherwise the kernels raise RBG_F_NEED_GENERAL.
+#include <type_traits>
+
 #include "rbg_common.hpp"
 
 namespace rbg {
@@ -718,7 +720,10 @@ __device__ inline void fw_panel_lines_body(T* W, int64_t n, int64_t ld, int64_t 
 // tile's row blocks plus the largest right-operand MSB over its column
 // blocks.  Otherwise the tile computes the same bound from the snapshots it
 // stages (the batched launcher's panels leave no guard words).
-template <typename T, int RPT, int CPT, bool PANEL_GUARD>
+// SHIFT_BYTES: the shifts are staged as bytes (36 KB of LDS for a 128-tile
+// instead of 48 KB, so four workgroups fit a CU when the registers allow it),
+// at 12 more VALU instructions per 128 relaxations to take them apart.
+template <typename T, int RPT, int CPT, bool PANEL_GUARD, bool SHIFT_BYTES>
 __device__ inline void fw_tile_fast_body(T* W, int64_t n, int64_t ld, int64_t K, FwWork ws, int width,
                                          rbg_status* st, int bid) {
   using U = typename unsigned_of<T>::type;
@@ -727,9 +732,10 @@ __device__ inline void fw_tile_fast_body(T* W, int64_t n, int64_t ld, int64_t K,
   constexpr int NCH = CPT / 4;  // column chunks per thread
   static_assert(RPT == CPT && CPT % 4 == 0, "square thread blocks of whole chunks");
   __shared__ U As[BLK][TL];
-  __shared__ U Ms[BLK][TL];
+  using MsT = typename std::conditional<SHIFT_BYTES, uint8_t, U>::type;
+  __shared__ MsT Ms[BLK][TL];
   __shared__ U Ys[BLK][TL];
-  __shared__ int sbad, sGa[BLK], sGb[BLK];
+  __shared__ int sGa[BLK], sGb[BLK];
   const int tid = threadIdx.x;
   const int nt = (int)(n / TL);
   const int I = bid / nt, J = bid % nt;
@@ -813,18 +819,21 @@ __device__ inline void fw_tile_fast_body(T* W, int64_t n, int64_t ld, int64_t K,
       if (gb) atomicMax(&sGb[kk], gb);
     }
     st4(&As[kk][x], sa[q]);
-    st4(&Ms[kk][x], m);
+    if (SHIFT_BYTES)
+      *(uint32_t*)&Ms[kk][x] = (uint32_t)m[0] | ((uint32_t)m[1] << 8) | ((uint32_t)m[2] << 16) | ((uint32_t)m[3] << 24);
+    else
+      st4((U*)&Ms[kk][x], m);
     st4(&Ys[kk][x], y);
   }
-  // a workgroup-uniform exit when an earlier kernel already asked for general
-  // mode; a failed MSB guard only asks for it (the values this launch then
-  // computes are discarded with the rest of the fast run)
-  if (tid == 0) sbad = (flags & (RBG_F_NEED_GENERAL | RBG_F_PRECOND_FAIL)) ? 2 : 0;
+  // exit when an earlier kernel already asked for general mode (no barrier
+  // follows, so threads may leave on their own reads of the flags); a failed
+  // MSB guard only asks for it (the values this launch then computes are
+  // discarded with the rest of the fast run)
   if (PANEL_GUARD && tid < BLK && pga + pgb > width - 2) atomicOr(&st->flags, (uint32_t)RBG_F_NEED_GENERAL);
   __syncthreads();
   if (!PANEL_GUARD && tid < BLK && sGa[tid] + sGb[tid] > width - 2)
     atomicOr(&st->flags, (uint32_t)RBG_F_NEED_GENERAL);
-  if (sbad) return;
+  if (flags & (RBG_F_NEED_GENERAL | RBG_F_PRECOND_FAIL)) return;
   if (skip) return;
 
   // diagonal cells of this thread (tiles with I == J): four at most, rows
@@ -839,19 +848,41 @@ __device__ inline void fw_tile_fast_body(T* W, int64_t n, int64_t ld, int64_t K,
     }
   }
   T d0 = 0, d1 = 0;
+  // a shift is used modulo the operand width, which is what the shift
+  // instructions do with their amount: a chunk's four shift bytes come in one
+  // LDS word and byte q is that word moved right by 8 q, no masking needed
+  constexpr uint32_t SHM = 8 * sizeof(U) - 1;
   for (int kk = 0; kk < BLK; kk += 2) {
-    U a0[RPT], a1[RPT], m0[CPT], m1[CPT], y0[CPT], y1[CPT];
+    U a0[RPT], a1[RPT], y0[CPT], y1[CPT];
+    uint32_t m0[CPT], m1[CPT];
 #pragma unroll
     for (int r = 0; r < RPT; r++) {
       a0[r] = As[kk][tr * RPT + r];
       a1[r] = As[kk + 1][tr * RPT + r];
     }
+    if (SHIFT_BYTES) {
+#pragma unroll
+      for (int h = 0; h < NCH; h++) {
+        const uint32_t p0 = *(const uint32_t*)&Ms[kk][64 * h + tc * 4];
+        const uint32_t p1 = *(const uint32_t*)&Ms[kk + 1][64 * h + tc * 4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          m0[4 * h + q] = p0 >> (8 * q);
+          m1[4 * h + q] = p1 >> (8 * q);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < CPT; c++) {
+        const int x = 64 * (c >> 2) + tc * 4 + (c & 3);
+        m0[c] = (uint32_t)Ms[kk][x];
+        m1[c] = (uint32_t)Ms[kk + 1][x];
+      }
+    }
 #pragma unroll
     for (int c = 0; c < CPT; c++) {
       const int x = 64 * (c >> 2) + tc * 4 + (c & 3);
-      m0[c] = Ms[kk][x];
       y0[c] = Ys[kk][x];
-      m1[c] = Ms[kk + 1][x];
       y1[c] = Ys[kk + 1][x];
     }
     if (dmode >= 0) {
@@ -862,8 +893,8 @@ __device__ inline void fw_tile_fast_body(T* W, int64_t n, int64_t ld, int64_t K,
 #pragma unroll
           for (int q = 0; q < 4; q++) {
             if (dr + q < RPT) {
-              const U p0 = (a0[dr + q] << m0[dc + q]) | y0[dc + q];
-              const U p1 = (a1[dr + q] << m1[dc + q]) | y1[dc + q];
+              const U p0 = (a0[dr + q] << (m0[dc + q] & SHM)) | y0[dc + q];
+              const U p1 = (a1[dr + q] << (m1[dc + q] & SHM)) | y1[dc + q];
               if ((T)p0 >= 0) record_cycle(st, naive_key(k0 + kk, r0 + dr + q, r0 + dr + q, n));
               if ((T)p1 >= 0) record_cycle(st, naive_key(k0 + kk + 1, r0 + dr + q, r0 + dr + q, n));
             }
@@ -875,8 +906,8 @@ __device__ inline void fw_tile_fast_body(T* W, int64_t n, int64_t ld, int64_t K,
     for (int r = 0; r < RPT; r++) {
 #pragma unroll
       for (int c = 0; c < CPT; c++) {
-        U p0 = (a0[r] << m0[c]) | y0[c];
-        U p1 = (a1[r] << m1[c]) | y1[c];
+        U p0 = (a0[r] << (m0[c] & SHM)) | y0[c];
+        U p1 = (a1[r] << (m1[c] & SHM)) | y1[c];
         U v1 = w[r][c] < p0 ? w[r][c] : p0;
         U v2 = v1 < p1 ? v1 : p1;
         w[r][c] = v2;
@@ -909,17 +940,25 @@ __global__ __launch_bounds__(PANEL_NT) void k_fw_panel_fast(T* W, int64_t n, int
 
 // Phase-3 tile shape per element type: 8 x 8 int32 cells per thread (128 x
 // 128 tiles), 4 x 4 int64 cells (64 x 64 tiles: an 8 x 8 int64 block needs more
-// registers than leave room for a second wave per SIMD).
+// registers than leave room for a second wave per SIMD).  The int64 tile
+// stages its shifts as bytes and fits four workgroups per CU (126 VGPRs, 34 KB
+// of LDS); the int32 tile keeps word shifts and three per CU: at four it
+// measured slower (182 us against 175 us per N = 8192 step), the byte
+// unpacking and a 128-register schedule costing more than the partial last
+// round of 768-workgroup rounds that four per CU removes.
 template <typename T>
 struct FastTile {
   static constexpr int RC = sizeof(T) == 4 ? 8 : 4;
   static constexpr int EDGE = 16 * RC;
+  static constexpr bool SHIFT_BYTES = sizeof(T) == 8;
+  static constexpr int PER_CU = SHIFT_BYTES ? 4 : 3;  // workgroups per CU the registers are sized for
 };
 
-template <typename T>
-__global__ __launch_bounds__(NT, 3) void k_fw_tile_fast(T* W, int64_t n, int64_t ld, int64_t K, FwWork ws,
-                                                       int width, rbg_status* st) {
-  fw_tile_fast_body<T, FastTile<T>::RC, FastTile<T>::RC, true>(W, n, ld, K, ws, width, st, blockIdx.x);
+template <typename T, int PER_CU = FastTile<T>::PER_CU>
+__global__ __launch_bounds__(NT, PER_CU) void k_fw_tile_fast(T* W, int64_t n, int64_t ld, int64_t K, FwWork ws,
+                                                            int width, rbg_status* st) {
+  fw_tile_fast_body<T, FastTile<T>::RC, FastTile<T>::RC, true, FastTile<T>::SHIFT_BYTES>(W, n, ld, K, ws, width,
+                                                                                        st, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------

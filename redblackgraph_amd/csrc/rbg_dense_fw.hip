@@ -81,9 +81,14 @@ int64_t rbg_fw_workspace_bytes(int64_t n, int32_t elem_bytes) { return FwWork::b
 
 int rbg_prof_enable(int32_t on) {
   std::lock_guard<std::mutex> g(g_prof.mu);
-  g_prof.on = on != 0;
-  g_prof.used = 0;
-  for (int k = 0; k < PROF_KINDS; k++) g_prof.pairs[k].clear();
+  if (on == 1) {  // start afresh
+    g_prof.used = 0;
+    for (int k = 0; k < PROF_KINDS; k++) g_prof.pairs[k].clear();
+  }
+  g_prof.on = on != 0;  // 2 resumes and 0 pauses, keeping what was recorded
+  bool empty = true;
+  for (int k = 0; k < PROF_KINDS; k++) empty = empty && g_prof.pairs[k].empty();
+  if (empty) g_prof.used = 0;
   return 0;
 }
 

@@ -353,38 +353,75 @@ __global__ __launch_bounds__(256) void k_kahn_fill(const int64_t* __restrict__ i
 // Round r >= 1, step 1: level r from level r - 1 (the slots [C_LSTART -
 // C_LSIZE, C_LSTART) of `order`): each placed vertex releases its
 // predecessors, and a predecessor whose count reaches zero joins level r.
-__global__ __launch_bounds__(256) void k_kahn_round(DagWork ws, int32_t r) {
-  const int64_t* ctr = ws.counters;
+// A workgroup takes 256 placed vertices and walks their predecessor lists as
+// one flattened range (a prefix of the list lengths in LDS), so every thread
+// issues about (edges / 256) decrements regardless of how the lists are
+// spread; vertices it frees are queued in LDS and placed with one atomic.
+constexpr int KR_NT = 256;
+constexpr int KR_QCAP = 4096;
+__global__ __launch_bounds__(KR_NT) void k_kahn_round(DagWork ws, int32_t r) {
+  int64_t* ctr = ws.counters;
   if (r == 0 || dag_halted(ctr)) return;
   const int64_t f1 = ctr[C_LSTART], f0 = f1 - ctr[C_LSIZE];
-  for (int64_t base = f0 + (int64_t)blockIdx.x * blockDim.x; base < f1; base += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t i = base + threadIdx.x;
-    // each thread releases one placed vertex's predecessors; those it frees
-    // are placed in rounds of the workgroup
-    int64_t q0 = 0, q1 = 0;
+  __shared__ int64_t s_q0[KR_NT];
+  __shared__ int64_t s_pre[KR_NT + 1];
+  __shared__ int64_t s_wsum[KR_NT / 64];
+  __shared__ int32_t s_q[KR_QCAP];
+  __shared__ int s_n;
+  __shared__ int64_t s_base;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  for (int64_t base = f0 + (int64_t)blockIdx.x * KR_NT; base < f1; base += (int64_t)gridDim.x * KR_NT) {
+    const int64_t i = base + tid;
+    int64_t q0 = 0, cnt = 0;
     if (i < f1) {
       const int32_t u = ws.order[i];
       q0 = ws.pptr[u];
-      q1 = ws.pptr[u + 1];
+      cnt = ws.pptr[u + 1] - q0;
     }
-    int64_t kmax = q1 - q0;
-    // the workgroup's longest predecessor list bounds its placement rounds
-    __shared__ int64_t s_k;
-    if (threadIdx.x == 0) s_k = 0;
+    int64_t incl = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int64_t t = __shfl_up(incl, o);
+      if (lane >= o) incl += t;
+    }
+    if (lane == 63) s_wsum[w] = incl;
+    if (tid == 0) s_n = 0;
     __syncthreads();
-    if (kmax) atomicMax((unsigned long long*)&s_k, (unsigned long long)kmax);
+    int64_t off = 0, total = 0;
+#pragma unroll
+    for (int k = 0; k < KR_NT / 64; k++) {
+      off += k < w ? s_wsum[k] : 0;
+      total += s_wsum[k];
+    }
+    s_pre[tid] = off + incl - cnt;
+    s_q0[tid] = q0;
+    if (tid == 0) s_pre[KR_NT] = total;
     __syncthreads();
-    kmax = s_k;
-    for (int64_t k = 0; k < kmax; k++) {
-      int32_t p = -1;
-      bool ready = false;
-      if (q0 + k < q1) {
-        p = ws.preds[q0 + k];
-        ready = atomicSub(&ws.rem[0][p], 1) == 1;
+    int k = 0;
+    for (int64_t e = tid; e < total; e += KR_NT) {
+      // owner of flattened edge e: the last k with s_pre[k] <= e (owners of
+      // empty lists share their successor's prefix and are passed over)
+      int lo = k, hi = KR_NT;
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (s_pre[mid] <= e) lo = mid;
+        else hi = mid;
       }
-      place_ready(ws, p, ready, r);
+      k = lo;
+      const int32_t p = ws.preds[s_q0[k] + (e - s_pre[k])];
+      if (atomicSub(&ws.rem[0][p], 1) == 1) {
+        ws.level[p] = r;
+        const int slot = atomicAdd(&s_n, 1);
+        if (slot < KR_QCAP) s_q[slot] = p;
+        else ws.order[atomicAdd((unsigned long long*)&ctr[C_PLACED], 1ull)] = p;
+      }
     }
-    __syncthreads();  // s_k is reset by the next pass
+    __syncthreads();
+    const int m = s_n < KR_QCAP ? s_n : KR_QCAP;
+    if (tid == 0 && m) s_base = (int64_t)atomicAdd((unsigned long long*)&ctr[C_PLACED], (unsigned long long)m);
+    __syncthreads();
+    for (int j = tid; j < m; j += KR_NT) ws.order[s_base + j] = s_q[j];
+    __syncthreads();  // s_n, s_pre and s_q are reused by the next pass
   }
 }
 
@@ -1345,9 +1382,11 @@ __global__ void k_round_end(DagWork ws, int32_t r, int32_t n) {
 
 // A vertex on a cycle among the unplaced ones: start at the smallest unplaced
 // vertex and keep following its first unplaced successor; the first vertex
-// met twice lies on a cycle.  Single lane; error path only.
+// met twice lies on a cycle.  Single lane; launched after every closure and
+// returns at once unless a round found no vertex to place.
 __global__ void k_dag_cycle_vertex(const int64_t* indptr, const int32_t* indices, DagWork ws, int32_t n,
                                    int32_t* stamp, rbg_status* st) {
+  if (!ws.counters[C_STUCK]) return;
   int32_t v = -1;
   for (int32_t u = 0; u < n; u++)
     if (ws.level[u] == INT32_MAX) {
@@ -1594,17 +1633,83 @@ __global__ void k_rows_init(int64_t* row_len, int64_t* counters, int32_t n) {
     for (int q = 0; q < C_NCTR; q++) counters[q] = 0;
 }
 
-template <typename T>
-__global__ void k_dag_gather(const DagWork ws, const DagArena<T> ar, const int64_t* __restrict__ out_ptr, int32_t n,
-                             int32_t* __restrict__ out_idx, T* __restrict__ out_val) {
-  const int lane = wave_lane();
-  const int64_t v = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (v >= n) return;
-  const int64_t src = ws.row_off[v], len = ws.row_len[v], dst = out_ptr[v];
-  for (int64_t q = lane; q < len; q += 64) {
-    out_idx[dst + q] = ar.cols[src + q];
-    out_val[dst + q] = ar.vals[src + q];
+// Segmented copy into CSR order: one wave moves 64 consecutive output rows,
+// which are contiguous in the output from out_ptr[r0].  Lane j holds row
+// r0 + j's source offset and length; the prefix of the lengths (in LDS) maps
+// every output entry of the group to its row, so stores are fully coalesced
+// and each lane keeps GC entries in flight.  src_of(r) / len_of(r) give a
+// row's source offset and length.
+constexpr int GC = 8;
+template <typename T, typename SrcF, typename LenF>
+__device__ inline void seg_copy_rows(int64_t r0, int32_t n, const int64_t* __restrict__ out_ptr, SrcF src_of,
+                                     LenF len_of, const int32_t* __restrict__ in_idx, const T* __restrict__ in_val,
+                                     int32_t* __restrict__ out_idx, T* __restrict__ out_val, int64_t* s_pre,
+                                     int64_t* s_src) {
+  const int lane = threadIdx.x & 63;
+  const int64_t r = r0 + lane;
+  int64_t len = 0, src = 0;
+  if (r < n) {
+    len = len_of(r);
+    src = src_of(r);
   }
+  int64_t incl = len;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int64_t t = __shfl_up(incl, o);
+    if (lane >= o) incl += t;
+  }
+  s_pre[lane + 1] = incl;
+  if (lane == 0) s_pre[0] = 0;
+  s_src[lane] = src;
+  const int64_t total = __shfl(incl, 63);
+  const int64_t dst0 = out_ptr[r0];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  int k = 0;
+  for (int64_t e0 = 0; e0 < total; e0 += 64 * GC) {
+    int64_t sp[GC];
+#pragma unroll
+    for (int u = 0; u < GC; u++) {
+      const int64_t e = e0 + u * 64 + lane;
+      sp[u] = -1;
+      if (e < total) {
+        while (s_pre[k + 1] <= e) k++;
+        sp[u] = s_src[k] + (e - s_pre[k]);
+      }
+    }
+    int32_t c[GC];
+    T v[GC];
+#pragma unroll
+    for (int u = 0; u < GC; u++)
+      if (sp[u] >= 0) {
+        c[u] = in_idx[sp[u]];
+        v[u] = in_val[sp[u]];
+      }
+#pragma unroll
+    for (int u = 0; u < GC; u++)
+      if (sp[u] >= 0) {
+        const int64_t e = e0 + u * 64 + lane;
+        out_idx[dst0 + e] = c[u];
+        out_val[dst0 + e] = v[u];
+      }
+  }
+  __builtin_amdgcn_wave_barrier();  // s_pre / s_src are rewritten by the next group
+}
+
+constexpr int GATHER_NT = 256;
+template <typename T>
+__global__ __launch_bounds__(GATHER_NT) void k_dag_gather(const DagWork ws, const DagArena<T> ar,
+                                                          const int64_t* __restrict__ out_ptr, int32_t n,
+                                                          int32_t* __restrict__ out_idx, T* __restrict__ out_val) {
+  __shared__ int64_t s_pre[GATHER_NT / 64][65];
+  __shared__ int64_t s_src[GATHER_NT / 64][64];
+  const int w = threadIdx.x >> 6;
+  const int64_t r0 = ((int64_t)blockIdx.x * (GATHER_NT / 64) + w) * 64;
+  if (r0 >= n) return;
+  seg_copy_rows<T>(
+      r0, n, out_ptr, [&](int64_t r) { return ws.row_off[r]; }, [&](int64_t r) { return ws.row_len[r]; }, ar.cols,
+      ar.vals, out_idx, out_val, s_pre[w], s_src[w]);
 }
 
 __global__ void k_dag_finish_status(const DagWork ws, rbg_status* st) {
@@ -1665,17 +1770,19 @@ __global__ void k_csr_sort_rows(const int64_t* __restrict__ indptr, int32_t* __r
 }
 
 template <typename T>
-__global__ void k_csr_compact(const int64_t* __restrict__ indptr, const int32_t* __restrict__ idx,
-                              const T* __restrict__ val, const int64_t* __restrict__ len, const int64_t* __restrict__ optr,
-                              int32_t n, int32_t* __restrict__ oidx, T* __restrict__ oval) {
-  const int lane = threadIdx.x & 63;
-  const int64_t r = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (r >= n) return;
-  const int64_t a = indptr[r], l = len[r], o = optr[r];
-  for (int64_t q = lane; q < l; q += 64) {
-    oidx[o + q] = idx[a + q];
-    oval[o + q] = val[a + q];
-  }
+__global__ __launch_bounds__(GATHER_NT) void k_csr_compact(const int64_t* __restrict__ indptr,
+                                                           const int32_t* __restrict__ idx, const T* __restrict__ val,
+                                                           const int64_t* __restrict__ len,
+                                                           const int64_t* __restrict__ optr, int32_t n,
+                                                           int32_t* __restrict__ oidx, T* __restrict__ oval) {
+  __shared__ int64_t s_pre[GATHER_NT / 64][65];
+  __shared__ int64_t s_src[GATHER_NT / 64][64];
+  const int w = threadIdx.x >> 6;
+  const int64_t r0 = ((int64_t)blockIdx.x * (GATHER_NT / 64) + w) * 64;
+  if (r0 >= n) return;
+  seg_copy_rows<T>(
+      r0, n, optr, [&](int64_t r) { return indptr[r]; }, [&](int64_t r) { return len[r]; }, idx, val, oidx, oval,
+      s_pre[w], s_src[w]);
 }
 
 __global__ void k_csr_check(const int64_t* __restrict__ indptr, const int32_t* __restrict__ idx, int32_t n,
@@ -1793,14 +1900,8 @@ int64_t rbg_dag_workspace_bytes(int32_t n, int64_t nnz) { return DagWork::bytes(
     }                                                                                                           \
     hipLaunchKernelGGL(k_dag_report, dim3(1), dim3(1), 0, s, ws, st);                                           \
     RBG_CHECK_LAUNCH();                                                                                         \
-    hipError_t e = hipMemcpyAsync(host, ws.counters, sizeof(host), hipMemcpyDeviceToHost, s);                  \
-    if (e != hipSuccess) return (int)e;                                                                         \
-    e = hipStreamSynchronize(s);                                                                                \
-    if (e != hipSuccess) return (int)e;                                                                         \
-    if (host[C_STUCK]) {                                                                                        \
-      hipLaunchKernelGGL(k_dag_cycle_vertex, dim3(1), dim3(1), 0, s, indptr, indices, ws, n, ws.rem[0], st);    \
-      RBG_CHECK_LAUNCH();                                                                                       \
-    }                                                                                                           \
+    hipLaunchKernelGGL(k_dag_cycle_vertex, dim3(1), dim3(1), 0, s, indptr, indices, ws, n, ws.rem[0], st);      \
+    RBG_CHECK_LAUNCH();                                                                                         \
     return 0;                                                                                                   \
   }                                                                                                             \
   int rbg_dag_gather_##SUF(void* workspace, int32_t n, const int32_t* arena_cols, const T* arena_vals,          \
@@ -1816,7 +1917,7 @@ int64_t rbg_dag_workspace_bytes(int32_t n, int64_t nnz) { return DagWork::bytes(
                        (const int64_t*)(ws.counters + C_NNZ));                                                  \
     RBG_CHECK_LAUNCH();                                                                                         \
     if (out_indices && out_data) {                                                                              \
-      hipLaunchKernelGGL((k_dag_gather<T>), dim3((n + 3) / 4), dim3(256), 0, s, ws, ar,                         \
+      hipLaunchKernelGGL((k_dag_gather<T>), dim3((n + GATHER_NT - 1) / GATHER_NT), dim3(GATHER_NT), 0, s, ws, ar,                         \
                          (const int64_t*)out_indptr, n, out_indices, out_data);                                 \
       RBG_CHECK_LAUNCH();                                                                                       \
     }                                                                                                           \
@@ -1852,7 +1953,7 @@ int64_t rbg_csr_tmp_bytes(int32_t n) { return 8 * ((int64_t)n + 1) + 8 * ((int64
     hipLaunchKernelGGL(k_set_last, dim3(1), dim3(1), 0, s, out_indptr, (int64_t)n, (const int64_t*)total);      \
     RBG_CHECK_LAUNCH();                                                                                         \
     if (out_idx && out_val) {                                                                                   \
-      hipLaunchKernelGGL((k_csr_compact<T>), dim3((n + 3) / 4), dim3(256), 0, s, indptr, (const int32_t*)idx,     \
+      hipLaunchKernelGGL((k_csr_compact<T>), dim3((n + GATHER_NT - 1) / GATHER_NT), dim3(GATHER_NT), 0, s, indptr, (const int32_t*)idx,     \
                          (const T*)val, (const int64_t*)len, (const int64_t*)out_indptr, n, out_idx, out_val);   \
       RBG_CHECK_LAUNCH();                                                                                       \
     }                                                                                                           \
@@ -1918,7 +2019,8 @@ extern "C" {
     hipLaunchKernelGGL(k_set_last, dim3(1), dim3(1), 0, s, out_indptr, (int64_t)n, (const int64_t*)total);      \
     RBG_CHECK_LAUNCH();                                                                                         \
     if (out_idx && out_val) {                                                                                   \
-      hipLaunchKernelGGL((k_csr_compact<T>), dim3((n + 3) / 4), dim3(256), 0, s, cat_ptr, (const int32_t*)cat_idx, \
+      hipLaunchKernelGGL((k_csr_compact<T>), dim3((n + GATHER_NT - 1) / GATHER_NT), dim3(GATHER_NT), 0, s,          \
+                         cat_ptr, (const int32_t*)cat_idx,                                                      \
                          (const T*)cat_val, (const int64_t*)len, (const int64_t*)out_indptr, n, out_idx, out_val); \
       RBG_CHECK_LAUNCH();                                                                                       \
     }                                                                                                           \

@@ -120,15 +120,19 @@ class DagPlan:
         self.ws = t.empty(int(L.rbg_dag_workspace_bytes(max(n, 1), self.nnz)), dtype=t.uint8, device=dev)
         self.arena_cols = t.empty(arena_entries, dtype=t.int32, device=dev)
         self.arena_vals = t.empty(arena_entries, dtype=value_dtype, device=dev)
+        self.levels = 0     # levels of the last graph closed with this plan (0: unknown)
 
 
 def dag_closure(A: DevCSR, width: int | None = None, stream=None, plan: DagPlan | None = None,
-                arena_factor: float = 48.0, batch: int = 8):
+                arena_factor: float = 48.0, batch: int = 0):
     """Closure of a DAG held as a canonical device CSR.
 
     Returns (C: DevCSR, DagInfo).  On a cycle returns (None, info) with
     info.cycle_vertex >= 0.  When a capacity flag comes back the buffers grow
-    and the closure runs again.
+    and the closure runs again.  `batch` is the number of level rounds between
+    completion tests (0: the level count of the plan's last graph, else 8);
+    with the right count the host waits twice per closure, once for the
+    levels and once for the status and output size.
     """
     from .dense import status
     t = _lib.torch()
@@ -145,9 +149,15 @@ def dag_closure(A: DevCSR, width: int | None = None, stream=None, plan: DagPlan 
     sfx = "i32" if vdt == t.int32 else "i64"
     for _attempt in range(8):
         st.reset(stream)
+        rounds = batch if batch > 0 else (plan.levels if plan.levels > 0 else 8)
         _lib.call(f"rbg_dag_closure_{sfx}", _lib.ptr(A.indptr), _lib.ptr(A.indices), _lib.ptr(A.data), n,
                   _lib.ptr(plan.ws), plan.ws.numel(), _lib.ptr(plan.arena_cols), _lib.ptr(plan.arena_vals),
-                  plan.arena_cols.numel(), batch, width, st.p, sh)
+                  plan.arena_cols.numel(), rounds, width, st.p, sh)
+        # row pointers of the result are scanned before the status is read, so
+        # one wait gives the flags and the output size together
+        optr = t.empty(n + 1, dtype=t.int64, device=A.indices.device)
+        _lib.call(f"rbg_dag_gather_{sfx}", _lib.ptr(plan.ws), n, _lib.ptr(plan.arena_cols),
+                  _lib.ptr(plan.arena_vals), _lib.ptr(optr), None, None, st.p, sh)
         s = st.read(stream)
         if s.flags & _lib.F_CYCLE:
             return None, DagInfo(0, 0, 0, int(s.aux[0]), int(s.aux[6]), 0)
@@ -156,10 +166,8 @@ def dag_closure(A: DevCSR, width: int | None = None, stream=None, plan: DagPlan 
             continue
         info = DagInfo(0, int(s.aux[2]), int(s.aux[3]), -1, int(s.aux[6]), int(s.events))
         info.arena = int(s.aux[4])
-        optr = t.empty(n + 1, dtype=t.int64, device=A.indices.device)
-        _lib.call(f"rbg_dag_gather_{sfx}", _lib.ptr(plan.ws), n, _lib.ptr(plan.arena_cols),
-                  _lib.ptr(plan.arena_vals), _lib.ptr(optr), None, None, st.p, sh)
-        nnz = int(optr[-1].item())
+        plan.levels = info.nlevels
+        nnz = int(s.aux[1])
         oi = t.empty(nnz, dtype=t.int32, device=A.indices.device)
         ov = t.empty(nnz, dtype=vdt, device=A.indices.device)
         _lib.call(f"rbg_dag_gather_{sfx}", _lib.ptr(plan.ws), n, _lib.ptr(plan.arena_cols),
