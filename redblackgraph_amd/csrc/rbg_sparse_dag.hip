@@ -111,10 +111,17 @@ static int scan_exclusive(const int64_t* in, int64_t* out, int64_t n, int64_t* t
 constexpr int SCAN_PER = 8;
 constexpr int64_t SCAN_CHUNK = (int64_t)SCAN_NT * SCAN_PER;
 
+// The scanned length is *n_ptr - *n_sub (n_sub may be null) plus `extra`
+// trailing zeros, at most max_n.
+__device__ inline int64_t scan_len(const int64_t* n_ptr, const int64_t* n_sub) {
+  return *n_ptr - (n_sub ? *n_sub : 0);
+}
+
 __global__ void k_scan_local(const int64_t* __restrict__ in, int64_t* __restrict__ local, const int64_t* n_ptr,
-                             int64_t extra, int64_t max_n, int64_t* __restrict__ sums) {
+                             const int64_t* n_sub, int64_t extra, int64_t max_n, int64_t* __restrict__ sums) {
   __shared__ int64_t sh[SCAN_NT];
-  int64_t n = *n_ptr + extra;
+  const int64_t nr = scan_len(n_ptr, n_sub);
+  int64_t n = nr + extra;
   n = n < max_n ? n : max_n;
   const int tid = threadIdx.x;
   for (int64_t blk = blockIdx.x; blk * SCAN_CHUNK < n; blk += gridDim.x) {
@@ -123,7 +130,7 @@ __global__ void k_scan_local(const int64_t* __restrict__ in, int64_t* __restrict
     int64_t acc = 0;
 #pragma unroll
     for (int q = 0; q < SCAN_PER; q++) {
-      v[q] = (base + q < n) ? in[base + q] : 0;
+      v[q] = (base + q < nr) ? in[base + q] : 0;
       acc += v[q];
     }
     sh[tid] = acc;
@@ -146,10 +153,11 @@ __global__ void k_scan_local(const int64_t* __restrict__ in, int64_t* __restrict
 }
 
 // scan of the block sums in place (single workgroup); *total = sum of all
-__global__ void k_scan_blocksums(int64_t* sums, const int64_t* n_ptr, int64_t extra, int64_t max_n, int64_t* total) {
+__global__ void k_scan_blocksums(int64_t* sums, const int64_t* n_ptr, const int64_t* n_sub, int64_t extra,
+                                 int64_t max_n, int64_t* total) {
   __shared__ int64_t sh[SCAN_NT];
   const int tid = threadIdx.x;
-  int64_t n = *n_ptr + extra;
+  int64_t n = scan_len(n_ptr, n_sub) + extra;
   n = n < max_n ? n : max_n;
   const int64_t nb = (n + SCAN_CHUNK - 1) / SCAN_CHUNK;
   int64_t carry = 0;
@@ -175,13 +183,15 @@ __device__ inline int64_t scan_at(const int64_t* local, const int64_t* sums, int
   return local[i] + sums[i / SCAN_CHUNK];
 }
 
-static int scan_dev(const int64_t* in, int64_t* local, int64_t* sums, const int64_t* n_ptr, int64_t extra,
-                    int64_t max_n, int64_t* total, hipStream_t s) {
+static int scan_dev(const int64_t* in, int64_t* local, int64_t* sums, const int64_t* n_ptr, const int64_t* n_sub,
+                    int64_t extra, int64_t max_n, int64_t* total, hipStream_t s) {
   int64_t nb = (max_n + extra + SCAN_CHUNK - 1) / SCAN_CHUNK;
   int grid = (int)(nb < 1 ? 1 : (nb > 4096 ? 4096 : nb));
-  hipLaunchKernelGGL(k_scan_local, dim3(grid), dim3(SCAN_NT), 0, s, in, local, n_ptr, extra, max_n + extra, sums);
+  hipLaunchKernelGGL(k_scan_local, dim3(grid), dim3(SCAN_NT), 0, s, in, local, n_ptr, n_sub, extra, max_n + extra,
+                     sums);
   RBG_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_scan_blocksums, dim3(1), dim3(SCAN_NT), 0, s, sums, n_ptr, extra, max_n + extra, total);
+  hipLaunchKernelGGL(k_scan_blocksums, dim3(1), dim3(SCAN_NT), 0, s, sums, n_ptr, n_sub, extra, max_n + extra,
+                     total);
   RBG_CHECK_LAUNCH();
   return 0;
 }
@@ -193,8 +203,8 @@ enum {
   C_PLACED = 0,   // vertices placed so far
   C_UNUSED1 = 1,
   C_UNUSED2 = 2,
-  C_LSTART = 3,   // first index in `order` of the current level
-  C_LSIZE = 4,    // vertices in the current level
+  C_LSTART = 3,   // first index in `order` of the current level (its size: C_PLACED - C_LSTART)
+  C_LPREV = 4,    // first index in `order` of the previous level
   C_M = 5,        // candidate entries of the current level
   C_LONG = 6,     // rows of the current level left to the long-row step
   C_ARENA = 7,    // arena entries used
@@ -221,7 +231,7 @@ struct DagWork {
   int64_t* cand_loc;   // [n+1] block-local scan of cnt
   int64_t* sums_a;     // scan block sums (vertex scans)
   int64_t* counters;   // [C_NCTR]
-  // the current level's vertices by slot (written by the count step): vertex,
+  // the current level's vertices by slot (written by the Kahn round): vertex,
   // first entry, direct entries, lists; and for rows with at most SLOT_D
   // direct entries, each non-self successor's arena row, length and edge value
   int32_t* slot_v;     // [n]
@@ -350,26 +360,70 @@ __global__ __launch_bounds__(256) void k_kahn_fill(const int64_t* __restrict__ i
   }
 }
 
-// Round r >= 1, step 1: level r from level r - 1 (the slots [C_LSTART -
-// C_LSIZE, C_LSTART) of `order`): each placed vertex releases its
-// predecessors, and a predecessor whose count reaches zero joins level r.
-// A workgroup takes 256 placed vertices and walks their predecessor lists as
-// one flattened range (a prefix of the list lengths in LDS), so every thread
-// issues about (edges / 256) decrements regardless of how the lists are
-// spread; vertices it frees are queued in LDS and placed with one atomic.
+// Vertex v's slot t of the current level: its candidate count (its direct
+// entries plus the closure rows of its non-self successors, all placed in
+// earlier levels) and the slot data the merge step reads.
+template <typename T>
+__device__ inline void level_slot(const int64_t* __restrict__ indptr, const int32_t* __restrict__ indices,
+                                  const T* __restrict__ data, const DagWork& ws, int64_t t, int32_t v) {
+  const int64_t e0 = indptr[v], e1 = indptr[v + 1];
+  const bool keep = e1 - e0 <= DagWork::SLOT_D;
+  int64_t c = 0;
+  int k = 0;
+  for (int64_t e = e0; e < e1; e++) {
+    const int32_t w = indices[e];
+    if (w == v) {
+      c += 1;
+      continue;
+    }
+    const int64_t len = ws.row_len[w];
+    c += 1 + len;
+    if (keep) {
+      ws.slot_ro[t * DagWork::SLOT_D + k] = ws.row_off[w];
+      ws.slot_len[t * DagWork::SLOT_D + k] = (int32_t)len;
+      ws.slot_av[t * DagWork::SLOT_D + k] = (int64_t)data[e];
+    }
+    k++;
+  }
+  ws.cnt[t] = c;
+  ws.slot_v[t] = v;
+  ws.slot_e0[t] = e0;
+  ws.slot_nd[t] = (int32_t)(e1 - e0);
+  ws.slot_nl[t] = 1 + k;
+}
+
+// Round r, step 1: level r and its slot data.  Round 0 only records the slot
+// data of level 0 (placed with the predecessor counts).  Round r >= 1 builds
+// level r from level r - 1 (the slots [C_LPREV, C_LSTART) of `order`): each
+// vertex of level r - 1 releases its predecessors, and a predecessor whose
+// count reaches zero joins level r.  A workgroup takes 256 vertices of level
+// r - 1 and walks their predecessor lists as one flattened range (a prefix of
+// the list lengths in LDS), so every thread issues about (edges / 256)
+// decrements however the lists are spread; the vertices it frees are queued
+// in LDS, placed with one atomic, and given their slot data.
 constexpr int KR_NT = 256;
 constexpr int KR_QCAP = 4096;
-__global__ __launch_bounds__(KR_NT) void k_kahn_round(DagWork ws, int32_t r) {
+template <typename T>
+__global__ __launch_bounds__(KR_NT) void k_kahn_round(const int64_t* __restrict__ indptr,
+                                                      const int32_t* __restrict__ indices, const T* __restrict__ data,
+                                                      DagWork ws, int32_t r) {
   int64_t* ctr = ws.counters;
-  if (r == 0 || dag_halted(ctr)) return;
-  const int64_t f1 = ctr[C_LSTART], f0 = f1 - ctr[C_LSIZE];
+  if (dag_halted(ctr)) return;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int64_t L0 = ctr[C_LSTART];
+  if (r == 0) {
+    const int64_t Ls = ctr[C_PLACED] - L0;
+    for (int64_t t = (int64_t)blockIdx.x * KR_NT + tid; t < Ls; t += (int64_t)gridDim.x * KR_NT)
+      level_slot<T>(indptr, indices, data, ws, t, ws.order[L0 + t]);
+    return;
+  }
+  const int64_t f0 = ctr[C_LPREV], f1 = L0;
   __shared__ int64_t s_q0[KR_NT];
   __shared__ int64_t s_pre[KR_NT + 1];
   __shared__ int64_t s_wsum[KR_NT / 64];
   __shared__ int32_t s_q[KR_QCAP];
   __shared__ int s_n;
   __shared__ int64_t s_base;
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   for (int64_t base = f0 + (int64_t)blockIdx.x * KR_NT; base < f1; base += (int64_t)gridDim.x * KR_NT) {
     const int64_t i = base + tid;
     int64_t q0 = 0, cnt = 0;
@@ -412,62 +466,25 @@ __global__ __launch_bounds__(KR_NT) void k_kahn_round(DagWork ws, int32_t r) {
       if (atomicSub(&ws.rem[0][p], 1) == 1) {
         ws.level[p] = r;
         const int slot = atomicAdd(&s_n, 1);
-        if (slot < KR_QCAP) s_q[slot] = p;
-        else ws.order[atomicAdd((unsigned long long*)&ctr[C_PLACED], 1ull)] = p;
+        if (slot < KR_QCAP) {
+          s_q[slot] = p;
+        } else {
+          const int64_t g = (int64_t)atomicAdd((unsigned long long*)&ctr[C_PLACED], 1ull);
+          ws.order[g] = p;
+          level_slot<T>(indptr, indices, data, ws, g - L0, p);
+        }
       }
     }
     __syncthreads();
     const int m = s_n < KR_QCAP ? s_n : KR_QCAP;
     if (tid == 0 && m) s_base = (int64_t)atomicAdd((unsigned long long*)&ctr[C_PLACED], (unsigned long long)m);
     __syncthreads();
-    for (int j = tid; j < m; j += KR_NT) ws.order[s_base + j] = s_q[j];
+    for (int j = tid; j < m; j += KR_NT) {
+      const int32_t p = s_q[j];
+      ws.order[s_base + j] = p;
+      level_slot<T>(indptr, indices, data, ws, s_base + j - L0, p);
+    }
     __syncthreads();  // s_n, s_pre and s_q are reused by the next pass
-  }
-}
-
-// Round r, step 2: level size, and the candidate count of every vertex of the
-// level: its direct entries plus the closure rows of its non-self successors.
-// The slot metadata the merge step needs is recorded on the way.
-template <typename T>
-__global__ void k_lvl_count(const int64_t* __restrict__ indptr, const int32_t* __restrict__ indices,
-                            const T* __restrict__ data, DagWork ws) {
-  int64_t* ctr = ws.counters;
-  if (dag_halted(ctr)) return;
-  const int64_t L0 = ctr[C_LSTART], Ls = ctr[C_PLACED] - L0;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    ctr[C_LSIZE] = Ls;
-    ctr[C_LONG] = 0;
-  }
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t <= Ls; t += (int64_t)gridDim.x * blockDim.x) {
-    if (t == Ls) {
-      ws.cnt[t] = 0;
-      continue;
-    }
-    const int32_t v = ws.order[L0 + t];
-    const int64_t e0 = indptr[v], e1 = indptr[v + 1];
-    const bool keep = e1 - e0 <= DagWork::SLOT_D;
-    int64_t c = 0;
-    int k = 0;
-    for (int64_t e = e0; e < e1; e++) {
-      const int32_t w = indices[e];
-      if (w == v) {
-        c += 1;
-        continue;
-      }
-      const int64_t len = ws.row_len[w];
-      c += 1 + len;
-      if (keep) {
-        ws.slot_ro[t * DagWork::SLOT_D + k] = ws.row_off[w];
-        ws.slot_len[t * DagWork::SLOT_D + k] = (int32_t)len;
-        ws.slot_av[t * DagWork::SLOT_D + k] = (int64_t)data[e];
-      }
-      k++;
-    }
-    ws.cnt[t] = c;
-    ws.slot_v[t] = v;
-    ws.slot_e0[t] = e0;
-    ws.slot_nd[t] = (int32_t)(e1 - e0);
-    ws.slot_nl[t] = 1 + k;
   }
 }
 
@@ -796,8 +813,8 @@ __device__ inline void put_meta(const SlotRegs& S, RowMeta<T>& M, int lane) {
   }
 }
 
-// Round r, step 3a: the level's rows merged by single waves, several at a
-// time.  A unit is a run of consecutive slots (rows) whose candidates (at most
+// Round r, step 3 (after the scan of the candidate counts): the level's rows
+// merged by single waves, several at a time.  A unit is a run of consecutive slots (rows) whose candidates (at most
 // LU_CAND, in at most LU_LISTS lists) one wave stages, merges and compacts
 // together.  Chunks of C consecutive slots are the work items, C from the
 // level's mean candidate count (about 256 candidates per chunk); a chunk that
@@ -1027,7 +1044,7 @@ __global__ __launch_bounds__(64) void k_lvl_units(const int32_t* __restrict__ in
   __shared__ UnitLds<T> L;
   int64_t* ctr = ws.counters;
   if (dag_halted(ctr)) return;
-  const int64_t Ls = ctr[C_LSIZE], M = ctr[C_M], abase = ctr[C_ARENA];
+  const int64_t Ls = ctr[C_PLACED] - ctr[C_LSTART], M = ctr[C_M], abase = ctr[C_ARENA];
   if (abase + M > ar.cap) {
     if (blockIdx.x == 0 && threadIdx.x == 0) ctr[C_CAP] = 1;
     return;
@@ -1118,7 +1135,7 @@ __device__ inline void issue_lists(const int32_t* __restrict__ indices, const T*
   }
 }
 
-// Round r, step 3a for levels of long rows (mean candidate count above
+// Round r, step 3 for levels of long rows (mean candidate count above
 // LU_ROWS_MEAN): one row per wave at a time, each workgroup one wave walking
 // slots b, b + W, ... two deep: while a row is placed and compacted, the lists
 // of the next row and the slot data of the one after are in flight.  Rows with
@@ -1135,7 +1152,7 @@ __global__ __launch_bounds__(64) void k_lvl_rows(const int32_t* __restrict__ ind
   __shared__ RowMeta<T> s_meta[2];
   int64_t* ctr = ws.counters;
   if (dag_halted(ctr)) return;
-  const int64_t Ls = ctr[C_LSIZE], M = ctr[C_M], abase = ctr[C_ARENA];
+  const int64_t Ls = ctr[C_PLACED] - ctr[C_LSTART], M = ctr[C_M], abase = ctr[C_ARENA];
   if (abase + M > ar.cap) {
     if (blockIdx.x == 0 && threadIdx.x == 0) ctr[C_CAP] = 1;
     return;
@@ -1267,7 +1284,7 @@ __global__ __launch_bounds__(64) void k_lvl_rows(const int32_t* __restrict__ ind
   if (evs) atomicOr(&st->events, evs);
 }
 
-// Round r, step 3b: the long rows step 3a left, one per workgroup at a time:
+// Round r, step 4: the long rows step 3 left, one per workgroup at a time:
 // merged in LDS when they fit (LV_BIG candidates, LV_MAXD direct entries),
 // otherwise in their arena slot from the lists in global memory.
 template <typename T>
@@ -1281,7 +1298,7 @@ __global__ __launch_bounds__(LV_NT) void k_lvl_long(const int32_t* __restrict__ 
   __shared__ int64_t s_cmax, s_fmax;
   int64_t* ctr = ws.counters;
   if (dag_halted(ctr) || ctr[C_CAP]) return;
-  const int64_t nlong = ctr[C_LONG], Ls = ctr[C_LSIZE], abase = ctr[C_ARENA];
+  const int64_t nlong = ctr[C_LONG], Ls = ctr[C_PLACED] - ctr[C_LSTART], abase = ctr[C_ARENA];
   if (nlong == 0) return;
   const int wv = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
@@ -1376,7 +1393,9 @@ __global__ void k_round_end(DagWork ws, int32_t r, int32_t n) {
   } else if (placed_now == 0) {
     ctr[C_STUCK] = 1;  // nothing placed while vertices remain: a cycle
   }
+  ctr[C_LPREV] = ctr[C_LSTART];
   ctr[C_LSTART] = ctr[C_PLACED];
+  ctr[C_LONG] = 0;
   ctr[C_ROUND] = r + 1;
 }
 
@@ -1874,11 +1893,10 @@ int64_t rbg_dag_workspace_bytes(int32_t n, int64_t nnz) { return DagWork::bytes(
     hipLaunchKernelGGL(k_kahn_fill, dim3(gv), dim3(256), 0, s, indptr, indices, ws, n);                         \
     RBG_CHECK_LAUNCH();                                                                                         \
     for (int32_t r = 0; r <= n; r++) {                                                                          \
-      hipLaunchKernelGGL(k_kahn_round, dim3(gv), dim3(256), 0, s, ws, r);                                       \
+      hipLaunchKernelGGL((k_kahn_round<T>), dim3(gv), dim3(KR_NT), 0, s, indptr, indices, data, ws, r);        \
       RBG_CHECK_LAUNCH();                                                                                       \
-      hipLaunchKernelGGL((k_lvl_count<T>), dim3(gv), dim3(256), 0, s, indptr, indices, data, ws);                \
-      RBG_CHECK_LAUNCH();                                                                                       \
-      int rc = scan_dev(ws.cnt, ws.cand_loc, ws.sums_a, ws.counters + C_LSIZE, 1, n, ws.counters + C_M, s);    \
+      int rc = scan_dev(ws.cnt, ws.cand_loc, ws.sums_a, ws.counters + C_PLACED, ws.counters + C_LSTART, 1, n,   \
+                        ws.counters + C_M, s);                                                                  \
       if (rc) return rc;                                                                                        \
       int pi = prof_start(s);                                                                                   \
       hipLaunchKernelGGL((k_lvl_units<T>), dim3(gr), dim3(64), 0, s, indices, data, ws, ar, width, st);         \
