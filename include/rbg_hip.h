@@ -157,6 +157,14 @@ int rbg_csr_canonicalize_i32(const int64_t *indptr, int32_t *idx, int32_t *val, 
                              int64_t *out_indptr, int32_t *out_idx, int32_t *out_val, rbg_stream_t stream);
 int rbg_csr_canonicalize_i64(const int64_t *indptr, int32_t *idx, int64_t *val, int32_t n, void *tmp,
                              int64_t *out_indptr, int32_t *out_idx, int64_t *out_val, rbg_stream_t stream);
+/* Canonical-form test (no reference counterpart: it lets a caller skip
+ * rbg_csr_canonicalize_* for a matrix that already is canonical): st->aux[0]
+ * is increased by the number of rows whose columns are not strictly
+ * increasing or that hold an explicit zero. */
+int rbg_csr_check_canonical_i32(const int64_t *indptr, const int32_t *idx, const int32_t *val, int32_t n,
+                                rbg_status *st, rbg_stream_t stream);
+int rbg_csr_check_canonical_i64(const int64_t *indptr, const int32_t *idx, const int64_t *val, int32_t n,
+                                rbg_status *st, rbg_stream_t stream);
 /* CSRMatrixGPU._validate_csr (gpu/csr_gpu.py:93-147): monotone indptr, columns in
  * range; a violation sets RBG_F_CAPACITY in st->flags. */
 int rbg_csr_validate(const int64_t *indptr, const int32_t *idx, int32_t n, int32_t ncols, rbg_status *st,
@@ -175,15 +183,19 @@ int64_t rbg_dag_workspace_bytes(int32_t n, int64_t nnz);
  * into the front of their slot), so the arena needs the total candidate count
  * sum_v (nnz(A[v]) + sum_w nnz(C[w])), not nnz(C).  All sizes live on the
  * device; the host synchronises once every `batch` rounds to test completion
- * (a graph of L levels takes L rounds, so batch = L, known from an earlier
- * closure of the same graph, costs one synchronisation).  The call returns
- * with the report still queued on `stream`: read st after the stream.
+ * (a graph of L levels takes L rounds).  batch < 0 runs exactly -batch rounds
+ * and returns without synchronising: with -batch = L, known from an earlier
+ * closure of the same graph, the whole closure is queued at once, and
+ * st->aux[7] = 0 with neither RBG_F_CYCLE nor RBG_F_CAPACITY set says that
+ * the rounds did not suffice (call again with batch > 0).  Either way the
+ * call returns with the report still queued on `stream`: read st after it.
  * Levels come from the reversed edges (a vertex joins level r + 1 when its last
  * unplaced successor is placed in level r), whose lists live in the workspace:
  * size it with rbg_dag_workspace_bytes(n, nnz(A)) (a smaller one sets
  * RBG_F_CAPACITY).  Results: closure rows in the arena (see rbg_dag_gather_*); st->aux[2] max
  * |candidate| (Cython diameter rule), [3] max |final value| > 1 (GPU diameter
- * rule), [4] arena entries used, [6] levels.  A cycle sets RBG_F_CYCLE with
+ * rule), [4] arena entries used, [6] levels, [7] 1 when every vertex was
+ * placed.  A cycle sets RBG_F_CYCLE with
  * st->aux[0] = a vertex on the cycle; a full arena sets RBG_F_CAPACITY (grow
  * it and call again). */
 int rbg_dag_closure_i32(const int64_t *indptr, const int32_t *indices, const int32_t *data, int32_t n,

@@ -70,6 +70,8 @@ SIGNATURES = {
     "rbg_csr_canonicalize_i32": (I32, [P, P, P, I32, P, P, P, P, P]),
     "rbg_csr_canonicalize_i64": (I32, [P, P, P, I32, P, P, P, P, P]),
     "rbg_csr_validate": (I32, [P, P, I32, I32, P, P]),
+    "rbg_csr_check_canonical_i32": (I32, [P, P, P, I32, P, P]),
+    "rbg_csr_check_canonical_i64": (I32, [P, P, P, I32, P, P]),
     "rbg_dag_workspace_bytes": (I64, [I32, I64]),
     "rbg_dag_closure_i32": (I32, [P, P, P, I32, P, I64, P, P, I64, I32, I32, P, P]),
     "rbg_dag_closure_i64": (I32, [P, P, P, I32, P, I64, P, P, I64, I32, I32, P, P]),

@@ -1804,6 +1804,21 @@ __global__ __launch_bounds__(GATHER_NT) void k_csr_compact(const int64_t* __rest
       s_pre[w], s_src[w]);
 }
 
+// Rows that are not canonical (columns not strictly increasing, or an
+// explicit zero), counted into st->aux[0] with one atomic per wave.
+template <typename T>
+__global__ void k_csr_check_canonical(const int64_t* __restrict__ indptr, const int32_t* __restrict__ idx,
+                                      const T* __restrict__ val, int32_t n, rbg_status* st) {
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  bool bad = false;
+  if (r < n) {
+    const int64_t a = indptr[r], b = indptr[r + 1];
+    for (int64_t x = a; x < b && !bad; x++) bad = val[x] == 0 || (x > a && idx[x] <= idx[x - 1]);
+  }
+  const uint64_t msk = __ballot(bad);
+  if ((threadIdx.x & 63) == 0 && msk) atomicAdd((unsigned long long*)&st->aux[0], (unsigned long long)__popcll(msk));
+}
+
 __global__ void k_csr_check(const int64_t* __restrict__ indptr, const int32_t* __restrict__ idx, int32_t n,
                             int32_t ncols, rbg_status* st) {
   int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -1871,7 +1886,8 @@ int64_t rbg_dag_workspace_bytes(int32_t n, int64_t nnz) { return DagWork::bytes(
     hipStream_t s = (hipStream_t)stream;                                                                        \
     if (n <= 0 || !workspace || !arena_cols || !arena_vals || arena_cap < 0) return RBG_E_ARG;                  \
     if (ws_bytes < DagWork::bytes(n)) return RBG_E_WORKSPACE;                                                   \
-    if (batch < 1) batch = 8;                                                                                   \
+    const bool exact = batch < 0; /* -batch rounds and no synchronisation */                                    \
+    if (batch == 0) batch = 8;                                                                                  \
     DagWork ws = DagWork::carve(workspace, n);                                                                  \
     ws.ecap = ((char*)workspace + ws_bytes - (char*)ws.preds) / 4;                                              \
     DagArena<T> ar{arena_cols, arena_vals, arena_cap};                                                          \
@@ -1908,7 +1924,9 @@ int64_t rbg_dag_workspace_bytes(int32_t n, int64_t nnz) { return DagWork::bytes(
       prof_stop(2, pi, s);                                                                                      \
       hipLaunchKernelGGL(k_round_end, dim3(1), dim3(1), 0, s, ws, r, n);                                        \
       RBG_CHECK_LAUNCH();                                                                                       \
-      if ((r + 1) % batch == 0) {                                                                               \
+      if (exact) {                                                                                              \
+        if (r + 1 == -batch) break;                                                                             \
+      } else if ((r + 1) % batch == 0) {                                                                        \
         hipError_t e = hipMemcpyAsync(host, ws.counters, sizeof(host), hipMemcpyDeviceToHost, s);              \
         if (e != hipSuccess) return (int)e;                                                                     \
         e = hipStreamSynchronize(s);                                                                            \
@@ -1979,6 +1997,19 @@ int64_t rbg_csr_tmp_bytes(int32_t n) { return 8 * ((int64_t)n + 1) + 8 * ((int64
   }
 RBG_CSR_ENTRY(i32, int32_t)
 RBG_CSR_ENTRY(i64, int64_t)
+
+#define RBG_CSR_CANON_ENTRY(SUF, T)                                                                             \
+  int rbg_csr_check_canonical_##SUF(const int64_t* indptr, const int32_t* idx, const T* val, int32_t n,        \
+                                    rbg_status* st, rbg_stream_t stream) {                                      \
+    if (n < 0 || !indptr || !st) return RBG_E_ARG;                                                             \
+    if (n == 0) return 0;                                                                                       \
+    hipLaunchKernelGGL((k_csr_check_canonical<T>), dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream,   \
+                       indptr, idx, val, n, st);                                                                \
+    RBG_CHECK_LAUNCH();                                                                                         \
+    return 0;                                                                                                   \
+  }
+RBG_CSR_CANON_ENTRY(i32, int32_t)
+RBG_CSR_CANON_ENTRY(i64, int64_t)
 
 // Validate a CSR structure: monotone indptr, column indices in [0, ncols).
 // Sets RBG_F_CAPACITY in st->flags on a violation.

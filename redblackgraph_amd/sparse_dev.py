@@ -52,17 +52,28 @@ def to_scipy(c: DevCSR):
 
 
 def canonicalize(c: DevCSR, stream=None) -> DevCSR:
-    """Sorted rows, duplicates combined with the AVOS sum, explicit zeros dropped."""
+    """Sorted rows, duplicates combined with the AVOS sum, explicit zeros dropped.
+
+    A matrix that already is canonical is returned as it is (one test kernel
+    and one wait instead of a sorted copy)."""
+    from .dense import status
     t = _lib.torch()
     n = c.n_rows
     if n == 0:
+        return c
+    sfx = "i32" if c.data.dtype == t.int32 else "i64"
+    sh = _lib.stream_handle(stream)
+    st = status()
+    st.reset(stream)
+    _lib.call(f"rbg_csr_check_canonical_{sfx}", _lib.ptr(c.indptr), _lib.ptr(c.indices), _lib.ptr(c.data), n, st.p,
+              sh)
+    if st.read(stream).aux[0] == 0:
         return c
     idx = c.indices.clone()
     val = c.data.clone()
     tmp = t.empty(int(_lib.load().rbg_csr_tmp_bytes(n)), dtype=t.uint8, device=idx.device)
     optr = t.empty(n + 1, dtype=t.int64, device=idx.device)
     fn = "rbg_csr_canonicalize_i32" if val.dtype == t.int32 else "rbg_csr_canonicalize_i64"
-    sh = _lib.stream_handle(stream)
     _lib.call(fn, _lib.ptr(c.indptr), _lib.ptr(idx), _lib.ptr(val), n, _lib.ptr(tmp), _lib.ptr(optr), None, None, sh)
     nnz = int(optr[-1].item())
     oi = t.empty(nnz, dtype=t.int32, device=idx.device)
@@ -130,9 +141,9 @@ def dag_closure(A: DevCSR, width: int | None = None, stream=None, plan: DagPlan 
     Returns (C: DevCSR, DagInfo).  On a cycle returns (None, info) with
     info.cycle_vertex >= 0.  When a capacity flag comes back the buffers grow
     and the closure runs again.  `batch` is the number of level rounds between
-    completion tests (0: the level count of the plan's last graph, else 8);
-    with the right count the host waits twice per closure, once for the
-    levels and once for the status and output size.
+    completion tests (0: the level count of the plan's last graph, queued
+    without any test, else 8); with the right count the host waits once per
+    closure, for the status and output size together.
     """
     from .dense import status
     t = _lib.torch()
@@ -147,9 +158,10 @@ def dag_closure(A: DevCSR, width: int | None = None, stream=None, plan: DagPlan 
     if plan is None or plan.n != n or plan.vdt != vdt or plan.nnz < A.nnz:
         plan = DagPlan(n, vdt, int(max(4096, arena_factor * (A.nnz + n))), A.nnz)
     sfx = "i32" if vdt == t.int32 else "i64"
+    exact = batch == 0 and plan.levels > 0
     for _attempt in range(8):
         st.reset(stream)
-        rounds = batch if batch > 0 else (plan.levels if plan.levels > 0 else 8)
+        rounds = -plan.levels if exact else (batch if batch > 0 else 8)
         _lib.call(f"rbg_dag_closure_{sfx}", _lib.ptr(A.indptr), _lib.ptr(A.indices), _lib.ptr(A.data), n,
                   _lib.ptr(plan.ws), plan.ws.numel(), _lib.ptr(plan.arena_cols), _lib.ptr(plan.arena_vals),
                   plan.arena_cols.numel(), rounds, width, st.p, sh)
@@ -163,6 +175,10 @@ def dag_closure(A: DevCSR, width: int | None = None, stream=None, plan: DagPlan 
             return None, DagInfo(0, 0, 0, int(s.aux[0]), int(s.aux[6]), 0)
         if s.flags & _lib.F_CAPACITY:
             plan = DagPlan(n, vdt, 2 * plan.arena_cols.numel(), A.nnz)
+            exact = False
+            continue
+        if not s.aux[7]:  # the plan's level count was too small for this graph
+            exact = False
             continue
         info = DagInfo(0, int(s.aux[2]), int(s.aux[3]), -1, int(s.aux[6]), int(s.events))
         info.arena = int(s.aux[4])

@@ -86,6 +86,76 @@ def test_dag_ahnentafel_and_diameters(orc, seed):
     assert info.diameter_final == ref.diameter_final
 
 
+def test_canonicalize_fast_path_and_sorting(orc):
+    """A canonical matrix comes back as it is; one with unsorted rows,
+    duplicate columns (combined with the AVOS sum) and explicit zeros is
+    sorted into a canonical copy."""
+    import torch
+    from redblackgraph_amd import _lib
+    A = synth.csr_pedigree(synth.PedigreeSpec(5000, 6, 0.7, 1), 2)
+    Ad = sd.from_scipy(A, np.int64)
+    assert sd.canonicalize(Ad) is Ad
+    rng = np.random.default_rng(11)
+    n = 400
+    rows, cols, vals = [], [], []
+    for i in range(n):
+        k = int(rng.integers(0, 9))
+        for c in rng.integers(0, n, k):
+            rows.append(i)
+            cols.append(int(c))
+            vals.append(int(rng.choice([0, -1, 1, 2, 3, 5, 9])))
+    indptr = np.zeros(n + 1, np.int64)
+    np.add.at(indptr, np.asarray(rows, np.int64) + 1, 1)
+    indptr = np.cumsum(indptr)
+    order = np.argsort(np.asarray(rows), kind="stable")
+    idx = np.asarray(cols, np.int32)[order]
+    val = np.asarray(vals, np.int64)[order]
+    dev = _lib.device()
+    M = sd.DevCSR(torch.from_numpy(indptr).to(dev), torch.from_numpy(idx).to(dev), torch.from_numpy(val).to(dev),
+                  (n, n))
+    Mc = sd.canonicalize(M)
+    assert Mc is not M
+    exp_ptr, exp_idx, exp_val = [0], [], []
+    for i in range(n):
+        acc = {}
+        for c, v in zip(idx[indptr[i]:indptr[i + 1]], val[indptr[i]:indptr[i + 1]]):
+            acc[int(c)] = orc.avos_sum(acc[int(c)], int(v)) if int(c) in acc else int(v)
+        for c in sorted(acc):
+            if acc[c] != 0:
+                exp_idx.append(c)
+                exp_val.append(acc[c])
+        exp_ptr.append(len(exp_idx))
+    np.testing.assert_array_equal(Mc.indptr.cpu().numpy(), exp_ptr)
+    np.testing.assert_array_equal(Mc.indices.cpu().numpy(), exp_idx)
+    np.testing.assert_array_equal(Mc.data.cpu().numpy(), exp_val)
+    assert sd.canonicalize(Mc) is Mc
+
+
+@pytest.mark.parametrize("hint", [1, 3, 40])
+def test_dag_plan_level_hint(orc, hint):
+    """A plan's level count queues the rounds without completion tests; a
+    count too small for the graph falls back to tested rounds, one too large
+    only adds idle rounds.  Results match the oracle either way."""
+    import torch
+    A = synth.csr_pedigree(synth.PedigreeSpec(20_000, 12, 0.7, 1), 5)
+    Ad = sd.canonicalize(sd.from_scipy(A, np.int64))
+    ref = orc.dag_closure(A, 64)
+    plan = sd.DagPlan(A.shape[0], torch.int64, 64 * (A.nnz + A.shape[0]), A.nnz)
+    plan.levels = hint
+    for _ in range(2):
+        C, info = sd.dag_closure(Ad, plan=plan)
+        np.testing.assert_array_equal(C.indptr.cpu().numpy(), ref.indptr)
+        np.testing.assert_array_equal(C.indices.cpu().numpy(), ref.indices)
+        np.testing.assert_array_equal(C.data.cpu().numpy(), ref.data)
+        assert plan.levels == info.nlevels
+    # a cycle under a hint still reports a vertex on it
+    Cy = sp.csr_matrix(np.array([[1, 2, 0], [0, 1, 3], [2, 0, 1]], np.int64))
+    cplan = sd.DagPlan(3, torch.int64, 4096, Cy.nnz)
+    cplan.levels = hint
+    C, info = sd.dag_closure(sd.canonicalize(sd.from_scipy(Cy, np.int64)), plan=cplan)
+    assert C is None and info.cycle_vertex in (0, 1, 2)
+
+
 def test_dag_pedigree_100k_vs_oracle(orc):
     spec = synth.PedigreeSpec(100_000, 15, 0.7, 1)
     A = synth.csr_pedigree(spec, 3)
