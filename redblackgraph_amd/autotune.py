@@ -35,10 +35,12 @@ def cache_path() -> str:
 
 
 def device_key() -> str:
+    """Architecture and CU count (the marketing name is left out: some
+    runtime configurations report it empty)."""
     t = _lib.torch()
     props = t.cuda.get_device_properties(_lib.device())
     arch = getattr(props, "gcnArchName", "") or ""
-    return f"{props.name}|{arch.split(':')[0]}|{props.multi_processor_count}cu"
+    return f"{arch.split(':')[0]}|{props.multi_processor_count}cu"
 
 
 def _load(path: str) -> dict:

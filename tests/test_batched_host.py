@@ -32,3 +32,17 @@ def test_tile_plan_reads_cache(tmp_path, monkeypatch):
     monkeypatch.setenv("RBG_AUTOTUNE_CACHE", str(cache))
     plan = autotune.tile_plan([256, 512, 256], measure=False, dev=dev)
     assert plan == {256: 64, 512: 128}
+
+
+def test_shipped_tile_table(tmp_path, monkeypatch):
+    """The table shipped for MI355X (gfx950, 256 CUs) covers every padded
+    order up to 4096 and seeds the plan without measuring."""
+    table = json.load(open(autotune.DEFAULTS_PATH))
+    dev = "gfx950|256cu"
+    for npad in range(128, 4097, 128):
+        e = table[f"{dev}|batched_fw_tile|{npad}"]
+        assert e["tile"] in autotune.TILES
+        assert e["components"] == autotune.tuning_components(npad)
+    monkeypatch.setenv("RBG_AUTOTUNE_CACHE", str(tmp_path / "empty.json"))
+    plan = autotune.tile_plan(range(128, 4097, 128), measure=False, dev=dev)
+    assert set(plan.values()) <= set(autotune.TILES) and len(plan) == 32
