@@ -228,7 +228,7 @@ def bench_sparse(args):
     kernel_s = ms_.value / 1e3 / sampled
     achieved = bytes_alg / kernel_s
     roof = {
-        "bound": "hbm", "kernel": "k_lvl_units<long> + k_lvl_rows<long> + k_lvl_long<long> (level merge)",
+        "bound": "hbm", "kernel": "k_lvl_units<long> + k_lvl_pairs<long> + k_lvl_long<long> (level merge)",
         "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s", "frac": achieved / PEAK_HBM,
         "traffic": _read_traffic("lvl_merge_sparse1m_pmc.txt"),
         "algorithmic_bytes_per_closure": bytes_alg,

@@ -829,7 +829,7 @@ __device__ inline void put_meta(const SlotRegs& S, RowMeta<T>& M, int lane) {
 // dropped) and writes the unit's rows one after another from the unit's first
 // reserved arena entry; each row lands at or below its own reserved range.
 constexpr int LU_GRID = 4096;
-constexpr int LU_ROWS_MEAN = 64;  // levels with a larger mean candidate count use k_lvl_rows
+constexpr int LU_ROWS_MEAN = 64;  // levels with a larger mean candidate count use k_lvl_pairs
 constexpr int LU_CAND = 512;
 constexpr int LU_ROWS = 64;
 constexpr int LU_LISTS = 128;
@@ -1055,7 +1055,7 @@ __global__ __launch_bounds__(64) void k_lvl_units(const int32_t* __restrict__ in
   uint32_t evs = 0;
   // rows per chunk: a power of two near 256 / (mean candidates per row)
   const int64_t mean = Ls > 0 ? (M + Ls - 1) / Ls : 1;
-  if (mean > LU_ROWS_MEAN) return;  // a level of long rows: k_lvl_rows takes it
+  if (mean > LU_ROWS_MEAN) return;  // a level of long rows: k_lvl_pairs takes it
   int C = 1;
   while (C < LU_ROWS && (int64_t)(2 * C) * mean <= 256) C <<= 1;
   const int64_t nch = (Ls + C - 1) / C;
@@ -1101,55 +1101,24 @@ __global__ __launch_bounds__(64) void k_lvl_units(const int32_t* __restrict__ in
   if (evs) atomicOr(&st->events, evs);
 }
 
-__device__ inline bool lds_row(const SlotRegs& S) { return S.cnt >= 0 && S.cnt <= LV_SMALL && S.nd <= LV_MAXD; }
-
-__device__ inline int list_of(const int (&lst)[LV_MAXD + 2], int nl, int idx) {
-  int k = 0;
-#pragma unroll
-  for (int q = 1; q <= LV_MAXD; q++) k += (q < nl && idx >= lst[q]) ? 1 : 0;
-  return k;
-}
-
-// Issue the loads of a row's candidates (lane + 64 r): columns and raw values
-// (an arena value, or the vertex's own entry) into registers.
-template <typename T>
-__device__ inline void issue_lists(const int32_t* __restrict__ indices, const T* __restrict__ data,
-                                   const DagArena<T>& ar, const RowMeta<T>& M, int64_t e0, int cnt, int lane,
-                                   int32_t (&c)[LV_R], T (&raw)[LV_R]) {
-  const int nl = uni(M.nl);
-  int lst[LV_MAXD + 2];
-#pragma unroll
-  for (int q = 0; q < LV_MAXD + 2; q++) lst[q] = q <= nl ? uni(M.lst[q]) : cnt;
-#pragma unroll
-  for (int r = 0; r < LV_R; r++) {
-    if (64 * r >= cnt) break;
-    const int idx = lane + 64 * r;
-    if (idx < cnt) {
-      const int k = list_of(lst, nl, idx);
-      const int64_t p = idx - lst[k];
-      const int32_t* cb = k == 0 ? indices + e0 : ar.cols + M.ro[k];
-      const T* vb = k == 0 ? data + e0 : ar.vals + M.ro[k];
-      c[r] = cb[p];
-      raw[r] = vb[p];
-    }
-  }
-}
-
 // Round r, step 3 for levels of long rows (mean candidate count above
-// LU_ROWS_MEAN): one row per wave at a time, each workgroup one wave walking
-// slots b, b + W, ... two deep: while a row is placed and compacted, the lists
-// of the next row and the slot data of the one after are in flight.  Rows with
-// more than LV_SMALL candidates or LV_MAXD direct entries go to the long-row
-// list.  Placement and compaction as in merge_row_lds / lds_run_head.
-constexpr int LR_GRID = 4096;
-
+// LU_ROWS_MEAN), the rows with one or two successor lists (at most LV_SMALL
+// candidates, LV_MAXD direct entries): one row per wave, no state carried
+// between rows, so that many waves fit a CU.  The two closure lists A and B
+// (sorted columns, and their products with the row's edge values) are staged
+// in LDS and merged by a merge path: lane j produces merged positions j R ..
+// j R + R - 1 (R = (|A| + |B|) / 64 rounded up) after one binary search for
+// its start on the merge diagonal.  A goes first on equal columns, so a column
+// both lists hold sits in adjacent positions and the first of the pair takes
+// the AVOS sum.  The direct entries (sorted, in lanes 0..nd-1) are matched
+// against the merged heads and placed by counting (a direct entry goes before
+// a merged candidate of the same column).  The row is written through LDS so
+// that its stores are coalesced.  Other rows go to the long-row list.
 template <typename T>
-__global__ __launch_bounds__(64) void k_lvl_rows(const int32_t* __restrict__ indices, const T* __restrict__ data,
-                                                 DagWork ws, DagArena<T> ar, int width, rbg_status* st) {
-  __shared__ int32_t s_lcol[LV_SMALL];
-  __shared__ int32_t s_mcol[LV_SMALL];
-  __shared__ T s_mval[LV_SMALL];
-  __shared__ RowMeta<T> s_meta[2];
+__global__ __launch_bounds__(64) void k_lvl_pairs(const int32_t* __restrict__ indices, const T* __restrict__ data,
+                                                  DagWork ws, DagArena<T> ar, int width, rbg_status* st) {
+  __shared__ int32_t s_c[LV_SMALL];
+  __shared__ T s_v[LV_SMALL];
   int64_t* ctr = ws.counters;
   if (dag_halted(ctr)) return;
   const int64_t Ls = ctr[C_PLACED] - ctr[C_LSTART], M = ctr[C_M], abase = ctr[C_ARENA];
@@ -1159,122 +1128,174 @@ __global__ __launch_bounds__(64) void k_lvl_rows(const int32_t* __restrict__ ind
   }
   if ((Ls > 0 ? (M + Ls - 1) / Ls : 1) <= LU_ROWS_MEAN) return;  // k_lvl_units takes the level
   const int lane = threadIdx.x;
-  const int64_t W = gridDim.x;
   const T seen_c = (T)ctr[C_CMAX], seen_f = (T)ctr[C_FMAX];  // lower bounds of the values at exit
   T cmax = 0, fmax = 0;
   uint32_t evs = 0;
   auto wsync = []() {
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   };
-
-  SlotRegs A, B;
-  int32_t c0[LV_R], c1[LV_R];
-  T raw0[LV_R], raw1[LV_R];
-#pragma unroll
-  for (int r = 0; r < LV_R; r++) {
-    c0[r] = c1[r] = 0;
-    raw0[r] = raw1[r] = 0;
-  }
-  load_slot(ws, blockIdx.x, Ls, lane, A);
-  settle_slot(A);
-  if (lds_row(A)) put_meta<T>(A, s_meta[0], lane);
-  wsync();
-  issue_lists<T>(indices, data, ar, s_meta[0], A.e0, lds_row(A) ? (int)A.cnt : 0, lane, c0, raw0);
-  load_slot(ws, blockIdx.x + W, Ls, lane, B);
-  int cur = 0;
-  for (int64_t t = blockIdx.x; t < Ls; t += W) {
-    // the next row: metadata out of B, its lists in flight, B refilled two ahead
-    SlotRegs N = B;
-    settle_slot(N);
-    if (lds_row(N)) put_meta<T>(N, s_meta[cur ^ 1], lane);
+  for (int64_t t = blockIdx.x; t < Ls; t += gridDim.x) {
+    const int nd = ws.slot_nd[t], nl = ws.slot_nl[t];
+    const int64_t seg = scan_at(ws.cand_loc, ws.sums_a, t);
+    const int cnt = (int)(scan_at(ws.cand_loc, ws.sums_a, t + 1) - seg);
+    if (nl < 2 || nl > 3 || cnt > LV_SMALL || nd > LV_MAXD) {
+      if (lane == 0) ws.long_list[atomicAdd((unsigned long long*)&ctr[C_LONG], 1ull)] = (int32_t)t;
+      continue;
+    }
+    const int32_t v = ws.slot_v[t];
+    const int64_t e0 = ws.slot_e0[t];
+    const int64_t q0 = t * DagWork::SLOT_D;
+    const int64_t ro1 = ws.slot_ro[q0];
+    const int a_len = ws.slot_len[q0];
+    const T av1 = (T)ws.slot_av[q0];
+    const int64_t ro2 = nl > 2 ? ws.slot_ro[q0 + 1] : 0;
+    const int b_len = nl > 2 ? ws.slot_len[q0 + 1] : 0;
+    const T av2 = nl > 2 ? (T)ws.slot_av[q0 + 1] : (T)0;
+    // stage both lists, each with its own (uniform) edge value
+    for (int q = lane; q < a_len; q += 64) {
+      uint32_t ev = 0;
+      const T x = avos_product<T>(av1, ar.vals[ro1 + q], width, false, ev);
+      evs |= ev;
+      s_c[q] = ar.cols[ro1 + q];
+      s_v[q] = x;
+      const T ax = x < 0 ? (T)-x : x;
+      cmax = ax > cmax ? ax : cmax;
+    }
+    for (int q = lane; q < b_len; q += 64) {
+      uint32_t ev = 0;
+      const T x = avos_product<T>(av2, ar.vals[ro2 + q], width, false, ev);
+      evs |= ev;
+      s_c[a_len + q] = ar.cols[ro2 + q];
+      s_v[a_len + q] = x;
+      const T ax = x < 0 ? (T)-x : x;
+      cmax = ax > cmax ? ax : cmax;
+    }
+    int32_t l0c = INT32_MAX;
+    T l0v = 0;
+    if (lane < nd) {
+      l0c = indices[e0 + lane];
+      l0v = data[e0 + lane];
+      const T ax = l0v < 0 ? (T)-l0v : l0v;
+      cmax = ax > cmax ? ax : cmax;
+    }
     wsync();
-    load_slot(ws, t + 2 * W, Ls, lane, B);
-    issue_lists<T>(indices, data, ar, s_meta[cur ^ 1], N.e0, lds_row(N) ? (int)N.cnt : 0, lane, c1, raw1);
-
-    // this row
-    if (lds_row(A)) {
-      const RowMeta<T>& Mc = s_meta[cur];
-      const int cnt = (int)A.cnt, nl = uni(Mc.nl);
-      int lst[LV_MAXD + 2];
-#pragma unroll
-      for (int q = 0; q < LV_MAXD + 2; q++) lst[q] = q <= nl ? uni(Mc.lst[q]) : cnt;
-#pragma unroll
-      for (int r = 0; r < LV_R; r++) {
-        if (64 * r >= cnt) break;
-        const int idx = lane + 64 * r;
-        if (idx < cnt) {
-          const int k = list_of(lst, nl, idx);
-          T x = raw0[r];
-          if (k != 0) {
-            uint32_t ev = 0;
-            x = avos_product<T>(Mc.av[k], x, width, false, ev);
-            evs |= ev;
-          }
-          raw0[r] = x;
-          s_lcol[idx] = c0[r];
-          const T a = x < 0 ? (T)-x : x;
-          cmax = a > cmax ? a : cmax;
-        }
-      }
-      wsync();
-      for (int r = 0; r < LV_R; r++) {
-        if (64 * r >= cnt) break;
-        const int idx = lane + 64 * r;
-        if (idx >= cnt) continue;
-        const int32_t c = c0[r];
-        int pos = 0;
-        for (int k2 = 0; k2 < nl; k2++) {
-          const int lo = lst[k2], hi = lst[k2 + 1];
-          if (idx >= lo && idx < hi) {
-            pos += idx - lo;
-            continue;
-          }
-          const int32_t key = idx >= hi ? c + 1 : c;  // entries before it: <= c below, < c above
-          int a = 0, b = hi - lo;
-          while (a < b) {
-            const int mid = (a + b) >> 1;
-            if (s_lcol[lo + mid] < key) a = mid + 1;
-            else b = mid;
-          }
-          pos += a;
-        }
-        s_mcol[pos] = c;
-        s_mval[pos] = raw0[r];
-      }
-      wsync();
-      const int64_t dst = abase + A.seg;
-      int64_t out = 0;
-      for (int b0 = 0; b0 < cnt; b0 += 64) {
-        const int i = b0 + lane;
-        int32_t c = -1;
-        T val = 0;
-        const bool keep = i < cnt && lds_run_head<T>(s_mcol, s_mval, i, cnt, c, val) && val != 0;
-        const uint64_t msk = __ballot(keep);
-        if (keep) {
-          const int64_t o = dst + out + __popcll(msk & ((1ull << lane) - 1ull));
-          ar.cols[o] = c;
-          ar.vals[o] = val;
-          const T a = val < 0 ? (T)-val : val;
-          if (a > 1 && a > fmax) fmax = a;
-        }
-        out += __popcll(msk);
-      }
-      if (lane == 0) {
-        ws.row_off[A.v] = dst;
-        ws.row_len[A.v] = out;
-      }
-    } else if (A.cnt >= 0 && lane == 0) {
-      ws.long_list[atomicAdd((unsigned long long*)&ctr[C_LONG], 1ull)] = (int32_t)t;
+    const int m = a_len + b_len, R = (m + 63) >> 6;
+    const int32_t* Bc = s_c + a_len;
+    const T* Bv = s_v + a_len;
+    const int d = lane * R < m ? lane * R : m;
+    int lo = d > b_len ? d - b_len : 0, hi = d < a_len ? d : a_len;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (s_c[mid] <= Bc[d - mid - 1]) lo = mid + 1;
+      else hi = mid;
     }
-    // advance
-    A = N;
+    int i = lo, j = d - lo;
+    int32_t ha = i < a_len ? s_c[i] : INT32_MAX, hb = j < b_len ? Bc[j] : INT32_MAX;
+    const int nv = m - d < R ? m - d : R;  // this lane's merged positions
+    int32_t oc[LV_R];
+    T ov[LV_R];
 #pragma unroll
-    for (int r = 0; r < LV_R; r++) {
-      c0[r] = c1[r];
-      raw0[r] = raw1[r];
+    for (int q = 0; q < LV_R; q++) {
+      oc[q] = INT32_MAX;
+      ov[q] = 0;
+      if (q < nv) {
+        if (ha <= hb) {
+          oc[q] = ha;
+          ov[q] = s_v[i];
+          i++;
+          ha = i < a_len ? s_c[i] : INT32_MAX;
+        } else {
+          oc[q] = hb;
+          ov[q] = Bv[j];
+          j++;
+          hb = j < b_len ? Bc[j] : INT32_MAX;
+        }
+      }
     }
-    cur ^= 1;
+    // pairs of equal columns: the first takes the AVOS sum, the second drops
+    int32_t lastc = INT32_MIN;
+#pragma unroll
+    for (int q = 0; q < LV_R; q++)
+      if (q < nv) lastc = oc[q];
+    const int32_t prev0 = __shfl_up(lastc, 1);
+    const int32_t next0c = __shfl_down(oc[0], 1);
+    const T next0v = __shfl_down(ov[0], 1);
+    bool keep[LV_R];
+#pragma unroll
+    for (int q = 0; q < LV_R; q++) {
+      const int32_t pc = q == 0 ? (lane > 0 ? prev0 : INT32_MIN) : oc[q == 0 ? 0 : q - 1];
+      const bool nx_in = q + 1 < nv;
+      const int32_t nc = nx_in ? oc[q + 1 < LV_R ? q + 1 : q] : (lane < 63 ? next0c : INT32_MAX);
+      const T nvv = nx_in ? ov[q + 1 < LV_R ? q + 1 : q] : next0v;
+      keep[q] = q < nv && pc != oc[q];
+      if (keep[q] && nc == oc[q]) ov[q] = avos_sum<T>(ov[q], nvv);
+    }
+    // direct entries: absorbed by a merged head of the same column, else kept
+    bool l0keep = lane < nd;
+    for (int k = 0; k < nd; k++) {
+      const int32_t ck = __builtin_amdgcn_readlane(l0c, k);
+      const T vk = __shfl(l0v, k);
+      bool hit = false;
+#pragma unroll
+      for (int q = 0; q < LV_R; q++)
+        if (keep[q] && oc[q] == ck) {
+          ov[q] = avos_sum<T>(vk, ov[q]);
+          hit = true;
+        }
+      if (__ballot(hit) && lane == k) l0keep = false;
+    }
+#pragma unroll
+    for (int q = 0; q < LV_R; q++) keep[q] = keep[q] && ov[q] != 0;
+    l0keep = l0keep && l0v != 0;
+    const uint64_t l0mask = __ballot(l0keep);
+    int mine = 0;
+#pragma unroll
+    for (int q = 0; q < LV_R; q++) mine += keep[q] ? 1 : 0;
+    int totS;
+    const int baseS = wave_excl_scan(mine, lane, totS);
+    wsync();  // the staged lists are read: the buffers take the merged row
+    int run = baseS;
+#pragma unroll
+    for (int q = 0; q < LV_R; q++) {
+      if (keep[q]) {
+        int pos = run;
+        for (int k = 0; k < nd; k++)
+          if (((l0mask >> k) & 1ull) && __builtin_amdgcn_readlane(l0c, k) < oc[q]) pos++;
+        s_c[pos] = oc[q];
+        s_v[pos] = ov[q];
+        const T ax = ov[q] < 0 ? (T)-ov[q] : ov[q];
+        if (ax > 1 && ax > fmax) fmax = ax;
+        run++;
+      }
+    }
+    for (int k = 0; k < nd; k++) {
+      if (!((l0mask >> k) & 1ull)) continue;
+      const int32_t ck = __builtin_amdgcn_readlane(l0c, k);
+      int below = 0;
+#pragma unroll
+      for (int q = 0; q < LV_R; q++) below += __popcll(__ballot(keep[q] && oc[q] < ck));
+      if (lane == k) {
+        const int pos = __popcll(l0mask & ((1ull << k) - 1ull)) + below;
+        s_c[pos] = l0c;
+        s_v[pos] = l0v;
+        const T ax = l0v < 0 ? (T)-l0v : l0v;
+        if (ax > 1 && ax > fmax) fmax = ax;
+      }
+    }
+    const int out = totS + __popcll(l0mask);
+    wsync();
+    const int64_t dst = abase + seg;
+    for (int q = lane; q < out; q += 64) {
+      ar.cols[dst + q] = s_c[q];
+      ar.vals[dst + q] = s_v[q];
+    }
+    if (lane == 0) {
+      ws.row_off[v] = dst;
+      ws.row_len[v] = out;
+    }
+    wsync();  // the next row restages the buffers
   }
   const int64_t wc = wave_max_i64((int64_t)cmax), wf = wave_max_i64((int64_t)fmax);
   if (lane == 0) {
@@ -1917,7 +1938,7 @@ int64_t rbg_dag_workspace_bytes(int32_t n, int64_t nnz) { return DagWork::bytes(
       int pi = prof_start(s);                                                                                   \
       hipLaunchKernelGGL((k_lvl_units<T>), dim3(gr), dim3(64), 0, s, indices, data, ws, ar, width, st);         \
       RBG_CHECK_LAUNCH();                                                                                       \
-      hipLaunchKernelGGL((k_lvl_rows<T>), dim3(gr), dim3(64), 0, s, indices, data, ws, ar, width, st);          \
+      hipLaunchKernelGGL((k_lvl_pairs<T>), dim3(gr), dim3(64), 0, s, indices, data, ws, ar, width, st);         \
       RBG_CHECK_LAUNCH();                                                                                       \
       hipLaunchKernelGGL((k_lvl_long<T>), dim3(gl), dim3(LV_NT), 0, s, indices, data, ws, ar, width, st);        \
       RBG_CHECK_LAUNCH();                                                                                       \
