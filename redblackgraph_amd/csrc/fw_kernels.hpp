@@ -954,6 +954,16 @@ struct FastTile {
   static constexpr int PER_CU = SHIFT_BYTES ? 4 : 3;  // workgroups per CU the registers are sized for
 };
 
+// Small int32 matrices (at most SMALL_TILE_MAX_N): 64 x 64 tiles, four times
+// as many workgroups per pivot step, since a 128-tile grid leaves most CUs
+// idle there (N = 512: 16 workgroups for 256 CUs).
+constexpr int64_t SMALL_TILE_MAX_N = 2048;
+template <typename T>
+__global__ __launch_bounds__(NT, 4) void k_fw_tile_fast_small(T* W, int64_t n, int64_t ld, int64_t K, FwWork ws,
+                                                             int width, rbg_status* st) {
+  fw_tile_fast_body<T, 4, 4, true, false>(W, n, ld, K, ws, width, st, blockIdx.x);
+}
+
 template <typename T, int PER_CU = FastTile<T>::PER_CU>
 __global__ __launch_bounds__(NT, PER_CU) void k_fw_tile_fast(T* W, int64_t n, int64_t ld, int64_t K, FwWork ws,
                                                             int width, rbg_status* st) {

@@ -32,7 +32,17 @@ static int fw_dense(T* W, int64_t n, int64_t ld, int width, int is_uns, int mode
     RBG_CHECK_LAUNCH();
     prof_stop(0, p0, s);
     int p1 = prof_start(s);
-    if (fast) {
+    bool small = false;
+    if constexpr (sizeof(T) == 4) {
+      if (fast && n <= SMALL_TILE_MAX_N) {
+        small = true;
+        const int ft = (int)(n / 64);
+        hipLaunchKernelGGL((k_fw_tile_fast_small<T>), dim3(ft * ft), dim3(NT), 0, s, W, n, ld, (int64_t)K, ws,
+                           width, st);
+      }
+    }
+    if (small) {
+    } else if (fast) {
       const int ft = (int)(n / FastTile<T>::EDGE);
       hipLaunchKernelGGL((k_fw_tile_fast<T>), dim3(ft * ft), dim3(NT), 0, s, W, n, ld, (int64_t)K, ws, width, st);
     } else {
