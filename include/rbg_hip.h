@@ -137,6 +137,19 @@ int rbg_edge_compose_i32(const int32_t *R, int64_t n, int64_t alpha, int64_t bet
 int rbg_edge_compose_i64(const int64_t *R, int64_t n, int64_t alpha, int64_t beta, int64_t rel,
                          int64_t *out, int32_t width, int32_t is_unsigned, rbg_status *st,
                          rbg_stream_t stream);
+/* The same composition in place (no reference counterpart: it serves repeated
+ * insertions into a resident closure).  Only the rows that reach alpha (W[m][alpha]
+ * != 0) and, within them, the columns of the composed row change, so the call
+ * costs those rows instead of the whole matrix; the results are those of
+ * rbg_edge_compose_*.  scratch: rbg_edge_compose_scratch_bytes(n) bytes of device
+ * memory (the composed row and the list of affected rows).  When st->flags gets
+ * RBG_F_CYCLE, W no longer holds a meaningful closure: compose into a copy where W
+ * must survive that error. */
+int64_t rbg_edge_compose_scratch_bytes(int64_t n);
+int rbg_edge_compose_inplace_i32(int32_t *W, int64_t n, int64_t alpha, int64_t beta, int32_t rel, void *scratch,
+                                 int32_t width, int32_t is_unsigned, rbg_status *st, rbg_stream_t stream);
+int rbg_edge_compose_inplace_i64(int64_t *W, int64_t n, int64_t alpha, int64_t beta, int64_t rel, void *scratch,
+                                 int32_t width, int32_t is_unsigned, rbg_status *st, rbg_stream_t stream);
 /* Vertex composition: out is (n+1) x (n+1).  On a cycle violation st->flags |=
  * RBG_F_CYCLE and st->aux[0..2] = {idx, u_idx, v_idx} of the first violation. */
 int rbg_vertex_compose_i32(const int32_t *u, const int32_t *R, const int32_t *v, int32_t color,

@@ -64,6 +64,9 @@ SIGNATURES = {
     "rbg_matmul_i64": (I32, [P, I64, P, I64, P, I64, I64, I64, I64, I32, I32, P, P]),
     "rbg_edge_compose_i32": (I32, [P, I64, I64, I64, I32, P, I32, I32, P, P]),
     "rbg_edge_compose_i64": (I32, [P, I64, I64, I64, I64, P, I32, I32, P, P]),
+    "rbg_edge_compose_scratch_bytes": (I64, [I64]),
+    "rbg_edge_compose_inplace_i32": (I32, [P, I64, I64, I64, I32, P, I32, I32, P, P]),
+    "rbg_edge_compose_inplace_i64": (I32, [P, I64, I64, I64, I64, P, I32, I32, P, P]),
     "rbg_vertex_compose_i32": (I32, [P, P, P, I32, I64, P, I32, I32, P, P]),
     "rbg_vertex_compose_i64": (I32, [P, P, P, I64, I64, P, I32, I32, P, P]),
     "rbg_csr_tmp_bytes": (I64, [I32]),

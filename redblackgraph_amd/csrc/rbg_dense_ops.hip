@@ -259,6 +259,70 @@ __global__ __launch_bounds__(256) void k_edge_update(const T* __restrict__ R, in
   if (evs) atomicOr(&st->events, evs);
 }
 
+__global__ void k_zero_i64(int64_t* p, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+
+// In-place edge composition, step 2: the rows m with W[m][alpha] != 0 (the
+// vertices that reach alpha) are the only rows the update changes, and within
+// them only the columns where u is present.  The first kernel lists those
+// rows (one read of column alpha), the second updates them in place; every
+// other entry is left untouched, so a composition into a pedigree closure
+// costs its affected rows rather than the whole matrix.
+template <typename T>
+__global__ __launch_bounds__(256) void k_edge_rows_hit(const T* __restrict__ W, int64_t n, int64_t alpha,
+                                                       int64_t* __restrict__ hit, int64_t* __restrict__ nhit) {
+  __shared__ int s_n;
+  __shared__ int64_t s_base;
+  __shared__ int64_t s_m[256];
+  if (threadIdx.x == 0) s_n = 0;
+  __syncthreads();
+  const int64_t m = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (m < n && m != alpha && W[m * n + alpha] != 0) s_m[atomicAdd(&s_n, 1)] = m;
+  __syncthreads();
+  if (threadIdx.x == 0 && s_n) s_base = (int64_t)atomicAdd((unsigned long long*)nhit, (unsigned long long)s_n);
+  __syncthreads();
+  if (threadIdx.x < s_n) hit[s_base + threadIdx.x] = s_m[threadIdx.x];
+}
+
+// One workgroup per listed row, four consecutive columns per lane; row alpha
+// takes u from the scratch row once every product is formed from it.
+template <typename T>
+__global__ __launch_bounds__(256) void k_edge_update_rows(T* __restrict__ W, int64_t n, int64_t alpha,
+                                                          const T* __restrict__ u, const int64_t* __restrict__ hit,
+                                                          const int64_t* __restrict__ nhit, int width, int is_uns,
+                                                          rbg_status* st) {
+  uint32_t evs = 0;
+  const int64_t nh = *nhit;
+  for (int64_t h = blockIdx.x; h < nh; h += gridDim.x) {
+    const int64_t m = hit[h];
+    T* Wm = W + m * n;
+    const T rma = Wm[alpha];  // unchanged by the update: u[alpha] is alpha's own diagonal
+    for (int64_t p0 = (int64_t)threadIdx.x * 4; p0 < n; p0 += 256 * 4) {
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int64_t p = p0 + q;
+        if (p < n) {
+          const T up = u[p];
+          if (up != 0) {
+            uint32_t ev = 0;
+            Wm[p] = avos_sum<T>(Wm[p], avos_product<T>(rma, up, width, is_uns != 0, ev));
+            evs |= ev;
+          }
+        }
+      }
+    }
+  }
+  if (evs) atomicOr(&st->events, evs);
+}
+
+template <typename T>
+__global__ void k_edge_put_row(T* __restrict__ W, int64_t n, int64_t alpha, const T* __restrict__ u) {
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x)
+    W[alpha * n + p] = u[p];
+}
+
 // Vertex composition, steps 3-4 (relational_composition.c.in:223-266).
 template <typename T>
 __global__ void k_vertex_update(const T* __restrict__ R, int64_t n, T color, T* __restrict__ out, int width,
@@ -329,6 +393,38 @@ static int edge_compose(const T* R, int64_t n, int64_t alpha, int64_t beta, T re
   const unsigned gx = (unsigned)((n + 1023) / 1024);
   const unsigned gy = (unsigned)(n < 65535 ? n : 65535);
   hipLaunchKernelGGL((k_edge_update<T>), dim3(gx, gy), dim3(256), 0, s, R, n, alpha, out, width, is_uns, st);
+  RBG_CHECK_LAUNCH();
+  return 0;
+}
+
+// scratch: rbg_edge_compose_scratch_bytes(n) bytes (the composed row and the
+// list of affected rows)
+template <typename T>
+static int edge_compose_inplace(T* W, int64_t n, int64_t alpha, int64_t beta, T rel, void* scratch, int width,
+                                int is_uns, rbg_status* st, hipStream_t s) {
+  if (!W || !scratch || !st || n <= 0 || alpha < 0 || alpha >= n || beta < 0 || beta >= n) return RBG_E_ARG;
+  using U = typename unsigned_of<T>::type;
+  char* base = (char*)(((uintptr_t)scratch + 255) & ~(uintptr_t)255);
+  U* ukey = (U*)base;
+  int64_t* nhit = (int64_t*)(base + ((n * (int64_t)sizeof(T) + 255) & ~(int64_t)255));
+  int64_t* hit = nhit + 32;
+  const unsigned cols = (unsigned)((n + 255) / 256);
+  hipLaunchKernelGGL((k_edge_begin<T>), dim3(cols), dim3(256), 0, s, W, n, alpha, beta, ukey, st);
+  RBG_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_zero_i64, dim3(1), dim3(32), 0, s, nhit, (int64_t)1);
+  RBG_CHECK_LAUNCH();
+  hipLaunchKernelGGL((k_edge_row<T>), dim3(cols, (unsigned)((n + EC_ROWS - 1) / EC_ROWS)), dim3(256), 0, s, W, n,
+                     alpha, beta, rel, ukey, width, is_uns, st);
+  RBG_CHECK_LAUNCH();
+  hipLaunchKernelGGL((k_edge_row_end<T>), dim3(cols), dim3(256), 0, s, ukey, n);
+  RBG_CHECK_LAUNCH();
+  hipLaunchKernelGGL((k_edge_rows_hit<T>), dim3(cols), dim3(256), 0, s, W, n, alpha, hit, nhit);
+  RBG_CHECK_LAUNCH();
+  const unsigned grid = (unsigned)(n < 2048 ? n : 2048);
+  hipLaunchKernelGGL((k_edge_update_rows<T>), dim3(grid), dim3(256), 0, s, W, n, alpha, (const T*)ukey, hit, nhit,
+                     width, is_uns, st);
+  RBG_CHECK_LAUNCH();
+  hipLaunchKernelGGL((k_edge_put_row<T>), dim3(cols), dim3(256), 0, s, W, n, alpha, (const T*)ukey);
   RBG_CHECK_LAUNCH();
   return 0;
 }
@@ -438,6 +534,17 @@ int rbg_edge_compose_i32(const int32_t* R, int64_t n, int64_t alpha, int64_t bet
 int rbg_edge_compose_i64(const int64_t* R, int64_t n, int64_t alpha, int64_t beta, int64_t rel, int64_t* out,
                          int32_t width, int32_t is_unsigned, rbg_status* st, rbg_stream_t stream) {
   return edge_compose<int64_t>(R, n, alpha, beta, rel, out, width, is_unsigned, st, (hipStream_t)stream);
+}
+int64_t rbg_edge_compose_scratch_bytes(int64_t n) {
+  return n <= 0 ? 0 : 256 + ((n * 8 + 255) & ~(int64_t)255) + 8 * 32 + 8 * n;
+}
+int rbg_edge_compose_inplace_i32(int32_t* W, int64_t n, int64_t alpha, int64_t beta, int32_t rel, void* scratch,
+                                 int32_t width, int32_t is_unsigned, rbg_status* st, rbg_stream_t stream) {
+  return edge_compose_inplace<int32_t>(W, n, alpha, beta, rel, scratch, width, is_unsigned, st, (hipStream_t)stream);
+}
+int rbg_edge_compose_inplace_i64(int64_t* W, int64_t n, int64_t alpha, int64_t beta, int64_t rel, void* scratch,
+                                 int32_t width, int32_t is_unsigned, rbg_status* st, rbg_stream_t stream) {
+  return edge_compose_inplace<int64_t>(W, n, alpha, beta, rel, scratch, width, is_unsigned, st, (hipStream_t)stream);
 }
 int rbg_vertex_compose_i32(const int32_t* u, const int32_t* R, const int32_t* v, int32_t color, int64_t n,
                            int32_t* out, int32_t width, int32_t is_unsigned, rbg_status* st, rbg_stream_t stream) {

@@ -144,25 +144,24 @@ def edge_compose(R, alpha: int, beta: int, rel: int, et: ElemType, stream=None):
     return out, st.read(stream)
 
 
-def edge_compose_many(R, edges, et: ElemType, stream=None):
+def edge_compose_many(R, edges, et: ElemType, stream=None, inplace: bool = False):
     """Apply edge compositions one after another without host round trips:
-    each edge reads the previous result (two device buffers alternate; R is
-    not modified).  Returns (closure, status); status.flags has RBG_F_CYCLE
-    when some edge would have closed a cycle, and the result is then
-    meaningless."""
+    each edge reads the previous result.  The compositions run in place on
+    one copy of R (R itself too when ``inplace``), touching only the rows
+    that reach each edge's source.  Returns (closure, status); status.flags
+    has RBG_F_CYCLE when some edge would have closed a cycle, and the result
+    is then meaningless."""
     t = _lib.torch()
     n = R.shape[0]
     st = status()
     st.reset(stream)
-    fn = "rbg_edge_compose_i32" if R.dtype == t.int32 else "rbg_edge_compose_i64"
+    fn = "rbg_edge_compose_inplace_i32" if R.dtype == t.int32 else "rbg_edge_compose_inplace_i64"
     sh = _lib.stream_handle(stream)
-    cur = R.contiguous()
-    bufs = [t.empty_like(cur), t.empty_like(cur)]
-    for q, (a, b, rel) in enumerate(edges):
-        out = bufs[q & 1]
-        _lib.call(fn, _lib.ptr(cur), n, int(a), int(b), int(rel), _lib.ptr(out), et.width, int(et.unsigned),
+    cur = R if (inplace and R.is_contiguous()) else R.contiguous().clone()
+    scratch = t.empty(int(_lib.load().rbg_edge_compose_scratch_bytes(n)), dtype=t.uint8, device=R.device)
+    for a, b, rel in edges:
+        _lib.call(fn, _lib.ptr(cur), n, int(a), int(b), int(rel), _lib.ptr(scratch), et.width, int(et.unsigned),
                   st.p, sh)
-        cur = out
     return cur, st.read(stream)
 
 

@@ -364,6 +364,33 @@ def synthetic_trace(count: int = 200, rate: float = 50.0, p_insert: float = 0.35
     return reqs
 
 
+def warmup_trace(hi: int = 16384, max_edges: int = 64, seed: int = 0) -> List[Request]:
+    """Requests of the largest shapes a trace from synthetic_trace(hi=hi) can
+    hold: the largest dense graph and a graph of ``hi`` vertices, closed and
+    then given ``max_edges`` insertions each, all arriving at once.  Serving
+    them first leaves the device memory pools holding blocks of every size the
+    trace will ask for, so no measured request waits for a fresh allocation."""
+    from . import synth
+    rng = np.random.Generator(np.random.PCG64(seed))
+    reqs: List[Request] = []
+    for i, n in enumerate((DENSE_LIMIT - 1, hi)):
+        spec = synth.component_spec(n)
+        gseed = 7_000_003 + seed * 101 + i
+        is_red, _, _, _ = synth.pedigree_edges(spec, gseed)
+        g = synth.dense_pedigree(spec, gseed) if n < DENSE_LIMIT else synth.csr_pedigree(spec, gseed)
+        key = f"warm{i}"
+        reqs.append(Request(2 * i, "close", 0.0, key, graph=g))
+        edges = []
+        for _ in range(max_edges):
+            a = int(rng.integers(max(1, n // 2), n))
+            rel = 2 if rng.random() < 0.5 else 3
+            cand = np.nonzero(is_red[:a] == (rel == 2))[0]
+            if cand.size:
+                edges.append((a, int(cand[int(rng.integers(0, cand.size))]), rel))
+        reqs.append(Request(2 * i + 1, "insert", 0.0, key, edges=edges))
+    return reqs
+
+
 def summarize(results: Sequence[Result], makespan: Optional[float] = None) -> dict:
     ok = [r for r in results if r.error is None]
     lat = np.array([r.latency for r in ok]) * 1e3 if ok else np.zeros(1)

@@ -336,7 +336,7 @@ def edge_insert_many(C: DevCSR, edges, stream=None) -> DevCSR:
     et = elem_type(np.int32 if C.data.dtype == t.int32 else np.int64)
     with t.cuda.stream(stream) if stream is not None else contextlib.nullcontext():
         D = to_dense(C)
-        W, st = dense.edge_compose_many(D, edges, et, stream=stream)
+        W, st = dense.edge_compose_many(D, edges, et, stream=stream, inplace=True)
         if st.flags & _lib.F_CYCLE:
             raise ValueError("Relational composition would result in a cycle.")
         if st.events:

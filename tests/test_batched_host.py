@@ -46,3 +46,18 @@ def test_shipped_tile_table(tmp_path, monkeypatch):
     monkeypatch.setenv("RBG_AUTOTUNE_CACHE", str(tmp_path / "empty.json"))
     plan = autotune.tile_plan(range(128, 4097, 128), measure=False, dev=dev)
     assert set(plan.values()) <= set(autotune.TILES) and len(plan) == 32
+
+
+def test_serving_warmup_trace_shapes():
+    """The serving warm-up covers the largest dense and sparse shapes, each
+    closed and then given the maximum number of acyclic parent edges."""
+    import scipy.sparse as sp
+    from redblackgraph_amd.serving import DENSE_LIMIT, warmup_trace
+    reqs = warmup_trace(hi=5000, max_edges=8, seed=3)
+    assert [r.kind for r in reqs] == ["close", "insert", "close", "insert"]
+    dense_g, sparse_g = reqs[0].graph, reqs[2].graph
+    assert isinstance(dense_g, np.ndarray) and dense_g.shape == (DENSE_LIMIT - 1, DENSE_LIMIT - 1)
+    assert sp.issparse(sparse_g) and sparse_g.shape == (5000, 5000)
+    for r in (reqs[1], reqs[3]):
+        assert 1 <= len(r.edges) <= 8
+        assert all(b < a and rel in (2, 3) for a, b, rel in r.edges)

@@ -342,6 +342,39 @@ def test_edge_compose_many_equals_reclosure(orc, dt, n):
     assert st.flags & _lib.F_CYCLE
 
 
+@pytest.mark.parametrize("dt,n", [(np.int32, 600), (np.int64, 400)])
+def test_edge_compose_inplace_equals_out_of_place(dt, n):
+    """The in-place composition (affected rows only) writes exactly what the
+    reference-shaped out-of-place entry writes, edge after edge, including
+    the composed row itself and rows that do not reach the edge's source."""
+    import torch
+    A = synth.dense_pedigree(synth.component_spec(n), 6, dtype=dt)
+    et = elem_type(A.dtype)
+    W, _ = dense.fw_closure(_lib.to_device(A), et)
+    rng = np.random.default_rng(12)
+    diag = np.diagonal(A)
+    cur = W.contiguous()
+    inpl = W.contiguous().clone()
+    scratch = torch.empty(int(_lib.load().rbg_edge_compose_scratch_bytes(n)), dtype=torch.uint8, device=W.device)
+    sfx = "i32" if dt == np.int32 else "i64"
+    for _ in range(10):
+        a = int(rng.integers(n // 2, n))
+        rel = int(rng.choice([2, 3]))
+        b = int(rng.choice(np.nonzero(diag[:a] == (-1 if rel == 2 else 1))[0]))
+        st = dense.status()
+        st.reset()
+        out = torch.empty_like(cur)
+        _lib.call(f"rbg_edge_compose_{sfx}", _lib.ptr(cur), n, a, b, rel, _lib.ptr(out), et.width, 0, st.p, _lib.stream_handle(None))
+        s1 = st.read()
+        st.reset()
+        _lib.call(f"rbg_edge_compose_inplace_{sfx}", _lib.ptr(inpl), n, a, b, rel, _lib.ptr(scratch), et.width, 0,
+                  st.p, _lib.stream_handle(None))
+        s2 = st.read()
+        assert (s1.flags, s1.events) == (s2.flags, s2.events)
+        np.testing.assert_array_equal(inpl.cpu().numpy(), out.cpu().numpy())
+        cur = out
+
+
 @pytest.mark.parametrize("dt,n", [(np.int32, 33), (np.int8, 9), (np.int64, 65)])
 def test_fw_overflow_chain(orc, dt, n):
     A = overflow_chain(n, dt)
