@@ -817,8 +817,8 @@ __device__ inline void put_meta(const SlotRegs& S, RowMeta<T>& M, int lane) {
 // merged by single waves, several at a time.  A unit is a run of consecutive slots (rows) whose candidates (at most
 // LU_CAND, in at most LU_LISTS lists) one wave stages, merges and compacts
 // together.  Chunks of C consecutive slots are the work items, C from the
-// level's mean candidate count (about 256 candidates per chunk); a chunk that
-// does not fit one unit is taken row by row.  Rows with more than LU_CAND
+// level's mean candidate count; a chunk that does not fit one unit is split
+// into greedy sub-units of consecutive rows.  Rows with more than LU_CAND
 // candidates or more than LV_MAXD direct entries go to the long-row list.
 //
 // In a unit, a candidate of row j's list g is placed at: the candidates of the
@@ -828,7 +828,8 @@ __device__ inline void put_meta(const SlotRegs& S, RowMeta<T>& M, int lane) {
 // the compaction keeps each run's head with the run's AVOS sum (zero sums
 // dropped) and writes the unit's rows one after another from the unit's first
 // reserved arena entry; each row lands at or below its own reserved range.
-constexpr int LU_GRID = 4096;
+constexpr int LU_GRID = 4096;        // waves of k_lvl_pairs
+constexpr int LU_UNITS_GRID = 2048;  // waves of k_lvl_units
 constexpr int LU_ROWS_MEAN = 64;  // levels with a larger mean candidate count use k_lvl_pairs
 constexpr int LU_CAND = 512;
 constexpr int LU_ROWS = 64;
@@ -1053,11 +1054,15 @@ __global__ __launch_bounds__(64) void k_lvl_units(const int32_t* __restrict__ in
   const T seen_c = (T)ctr[C_CMAX], seen_f = (T)ctr[C_FMAX];  // lower bounds of the values at exit
   T cmax = 0, fmax = 0;
   uint32_t evs = 0;
-  // rows per chunk: a power of two near 256 / (mean candidates per row)
+  // rows per chunk: the largest power of two (at most LU_ROWS) whose mean
+  // candidate count is at most tgt / 2.  A unit's fixed costs (its tables,
+  // its dependent loads, the wait for its stores) dominate levels of short
+  // rows, so their chunks are larger.
   const int64_t mean = Ls > 0 ? (M + Ls - 1) / Ls : 1;
   if (mean > LU_ROWS_MEAN) return;  // a level of long rows: k_lvl_pairs takes it
+  const int64_t tgt = mean <= 16 ? 1024 : (mean <= 32 ? 512 : 256);
   int C = 1;
-  while (C < LU_ROWS && (int64_t)(2 * C) * mean <= 256) C <<= 1;
+  while (C < LU_ROWS && (int64_t)(2 * C) * mean <= tgt) C <<= 1;
   const int64_t nch = (Ls + C - 1) / C;
   for (int64_t ch = blockIdx.x; ch < nch; ch += gridDim.x) {
     const int64_t t0 = ch * C;
@@ -1916,6 +1921,7 @@ int64_t rbg_dag_workspace_bytes(int32_t n, int64_t nnz) { return DagWork::bytes(
     RBG_CHECK_LAUNCH();                                                                                         \
     const int gv = (int)((n + 255) / 256 < 2048 ? (n + 255) / 256 : 2048);                                      \
     const int gr = (int)(n < LU_GRID ? n : LU_GRID);                                                          \
+    const int gu = (int)(n < LU_UNITS_GRID ? n : LU_UNITS_GRID);                                              \
     const int gl = (int)(n < 1024 ? n : 1024);                                                                \
     int64_t host[C_NCTR];                                                                                       \
     hipLaunchKernelGGL(k_kahn_count, dim3(gv), dim3(256), 0, s, indptr, indices, ws, n);                        \
@@ -1936,7 +1942,7 @@ int64_t rbg_dag_workspace_bytes(int32_t n, int64_t nnz) { return DagWork::bytes(
                         ws.counters + C_M, s);                                                                  \
       if (rc) return rc;                                                                                        \
       int pi = prof_start(s);                                                                                   \
-      hipLaunchKernelGGL((k_lvl_units<T>), dim3(gr), dim3(64), 0, s, indices, data, ws, ar, width, st);         \
+      hipLaunchKernelGGL((k_lvl_units<T>), dim3(gu), dim3(64), 0, s, indices, data, ws, ar, width, st);         \
       RBG_CHECK_LAUNCH();                                                                                       \
       hipLaunchKernelGGL((k_lvl_pairs<T>), dim3(gr), dim3(64), 0, s, indices, data, ws, ar, width, st);         \
       RBG_CHECK_LAUNCH();                                                                                       \
