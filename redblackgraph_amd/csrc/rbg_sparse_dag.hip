@@ -836,6 +836,26 @@ constexpr int LU_ROWS = 64;
 constexpr int LU_LISTS = 128;
 static_assert(LU_CAND / 64 == LV_R, "candidates per lane");
 
+// Lane j's value of x, read by every lane (j uniform).
+__device__ __forceinline__ int32_t lane_i32(int32_t x, int j) { return __builtin_amdgcn_readlane(x, j); }
+__device__ __forceinline__ int64_t lane_i64(int64_t x, int j) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int32_t)(uint32_t)x, j);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int32_t)(uint32_t)((uint64_t)x >> 32), j);
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+// Appends the slots of the lanes with `mine` set to the long-row list: one
+// atomic for the wave, so that a level with many long rows does not queue
+// one same-address atomic per row.
+__device__ __forceinline__ void push_long(const DagWork& ws, bool mine, int64_t t, int lane) {
+  const uint64_t m = __ballot(mine);
+  if (!m) return;
+  unsigned long long base = 0;
+  if (lane == 0) base = atomicAdd((unsigned long long*)&ws.counters[C_LONG], (unsigned long long)__popcll(m));
+  base = __shfl(base, 0);
+  if (mine) ws.long_list[base + __popcll(m & ((1ull << lane) - 1ull))] = (int32_t)t;
+}
+
 template <typename T>
 struct UnitLds {
   int32_t lcol[LU_CAND];          // staged columns, list after list
@@ -1075,7 +1095,7 @@ __global__ __launch_bounds__(64) void k_lvl_units(const int32_t* __restrict__ in
     const int64_t cntj = scan_at(ws.cand_loc, ws.sums_a, t + 1) - seg;
     // rows no unit takes go to the long-row list
     const bool lng = lane < m && (nd > LV_MAXD || cntj > LU_CAND);
-    if (lng) ws.long_list[atomicAdd((unsigned long long*)&ctr[C_LONG], 1ull)] = (int32_t)t;
+    push_long(ws, lng, t, lane);
     // greedy sub-units of consecutive rows within LU_CAND candidates and
     // LU_LISTS lists (a long row counts as none)
     const int cw = lane < m && !lng ? (int)cntj : 0, lw = lane < m && !lng ? nl : 0;
@@ -1141,23 +1161,50 @@ __global__ __launch_bounds__(64) void k_lvl_pairs(const int32_t* __restrict__ in
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   };
-  for (int64_t t = blockIdx.x; t < Ls; t += gridDim.x) {
-    const int nd = ws.slot_nd[t], nl = ws.slot_nl[t];
-    const int64_t seg = scan_at(ws.cand_loc, ws.sums_a, t);
-    const int cnt = (int)(scan_at(ws.cand_loc, ws.sums_a, t + 1) - seg);
-    if (nl < 2 || nl > 3 || cnt > LV_SMALL || nd > LV_MAXD) {
-      if (lane == 0) ws.long_list[atomicAdd((unsigned long long*)&ctr[C_LONG], 1ull)] = (int32_t)t;
-      continue;
+  // The wave's rows are t = blockIdx.x + G j.  Each batch of 64 of them is
+  // read ahead, one row per lane, so that every short row's slot data is a
+  // lane read when its turn comes and a row's first dependent loads are its
+  // two lists.  The batch's long rows go to the long-row list after its short
+  // rows: waves finish at different times, so their atomics do not queue.
+  const int64_t G = gridDim.x;
+  for (int64_t tb = blockIdx.x; tb < Ls; tb += 64 * G) {
+    const int64_t tl = tb + G * lane;
+    int32_t p_nd = 0, p_nl = 0, p_cnt = 0, p_v = 0, p_alen = 0, p_blen = 0;
+    int64_t p_seg = 0, p_e0 = 0, p_ro1 = 0, p_ro2 = 0, p_av1 = 0, p_av2 = 0;
+    bool lng = false;
+    if (tl < Ls) {
+      p_nd = ws.slot_nd[tl];
+      p_nl = ws.slot_nl[tl];
+      p_seg = scan_at(ws.cand_loc, ws.sums_a, tl);
+      const int64_t c = scan_at(ws.cand_loc, ws.sums_a, tl + 1) - p_seg;
+      lng = p_nl < 2 || p_nl > 3 || c > LV_SMALL || p_nd > LV_MAXD;
+      p_cnt = (int32_t)(c < LV_SMALL ? c : LV_SMALL);
+      p_v = ws.slot_v[tl];
+      p_e0 = ws.slot_e0[tl];
+      const int64_t q0 = tl * DagWork::SLOT_D;  // the slot data of a long row is not used
+      p_ro1 = ws.slot_ro[q0];
+      p_alen = ws.slot_len[q0];
+      p_av1 = ws.slot_av[q0];
+      if (p_nl > 2) {
+        p_ro2 = ws.slot_ro[q0 + 1];
+        p_blen = ws.slot_len[q0 + 1];
+        p_av2 = ws.slot_av[q0 + 1];
+      }
     }
-    const int32_t v = ws.slot_v[t];
-    const int64_t e0 = ws.slot_e0[t];
-    const int64_t q0 = t * DagWork::SLOT_D;
-    const int64_t ro1 = ws.slot_ro[q0];
-    const int a_len = ws.slot_len[q0];
-    const T av1 = (T)ws.slot_av[q0];
-    const int64_t ro2 = nl > 2 ? ws.slot_ro[q0 + 1] : 0;
-    const int b_len = nl > 2 ? ws.slot_len[q0 + 1] : 0;
-    const T av2 = nl > 2 ? (T)ws.slot_av[q0 + 1] : (T)0;
+    const uint64_t short_rows = __ballot(tl < Ls && !lng);
+  for (int rj = 0; rj < 64; rj++) {
+    if (!((short_rows >> rj) & 1)) continue;
+    const int nd = lane_i32(p_nd, rj), nl = lane_i32(p_nl, rj);
+    const int64_t seg = lane_i64(p_seg, rj);
+    const int cnt = lane_i32(p_cnt, rj);
+    const int32_t v = lane_i32(p_v, rj);
+    const int64_t e0 = lane_i64(p_e0, rj);
+    const int64_t ro1 = lane_i64(p_ro1, rj);
+    const int a_len = lane_i32(p_alen, rj);
+    const T av1 = (T)lane_i64(p_av1, rj);
+    const int64_t ro2 = lane_i64(p_ro2, rj);
+    const int b_len = lane_i32(p_blen, rj);
+    const T av2 = (T)lane_i64(p_av2, rj);
     // stage both lists, each with its own (uniform) edge value
     for (int q = lane; q < a_len; q += 64) {
       uint32_t ev = 0;
@@ -1301,6 +1348,8 @@ __global__ __launch_bounds__(64) void k_lvl_pairs(const int32_t* __restrict__ in
       ws.row_len[v] = out;
     }
     wsync();  // the next row restages the buffers
+  }
+    push_long(ws, lng, tl, lane);
   }
   const int64_t wc = wave_max_i64((int64_t)cmax), wf = wave_max_i64((int64_t)fmax);
   if (lane == 0) {
