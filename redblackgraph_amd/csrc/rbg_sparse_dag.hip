@@ -1126,6 +1126,125 @@ __global__ __launch_bounds__(64) void k_lvl_units(const int32_t* __restrict__ in
   if (evs) atomicOr(&st->events, evs);
 }
 
+// The merge of a pair row (k_lvl_pairs): lists A = s_c/s_v[0, a_len) and B =
+// [a_len, a_len + b_len) staged, the direct entries in lanes 0..nd-1 (l0c,
+// l0v).  RQ bounds the merged positions per lane (R <= RQ), so that a short
+// row runs loops unrolled for its own size.  Leaves the row in s_c / s_v and
+// returns its length.
+template <typename T, int RQ>
+__device__ __forceinline__ int pair_merge(int32_t* s_c, T* s_v, int a_len, int b_len, int nd, int32_t l0c, T l0v,
+                                          int lane, T& fmax) {
+  auto wave_sync = []() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  };
+  const int m = a_len + b_len, R = (m + 63) >> 6;
+  const int32_t* Bc = s_c + a_len;
+  const T* Bv = s_v + a_len;
+  const int d = lane * R < m ? lane * R : m;
+  int lo = d > b_len ? d - b_len : 0, hi = d < a_len ? d : a_len;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (s_c[mid] <= Bc[d - mid - 1]) lo = mid + 1;
+    else hi = mid;
+  }
+  int i = lo, j = d - lo;
+  int32_t ha = i < a_len ? s_c[i] : INT32_MAX, hb = j < b_len ? Bc[j] : INT32_MAX;
+  const int nv = m - d < R ? m - d : R;  // this lane's merged positions
+  int32_t oc[RQ];
+  T ov[RQ];
+#pragma unroll
+  for (int q = 0; q < RQ; q++) {
+    oc[q] = INT32_MAX;
+    ov[q] = 0;
+    if (q < nv) {
+      if (ha <= hb) {
+        oc[q] = ha;
+        ov[q] = s_v[i];
+        i++;
+        ha = i < a_len ? s_c[i] : INT32_MAX;
+      } else {
+        oc[q] = hb;
+        ov[q] = Bv[j];
+        j++;
+        hb = j < b_len ? Bc[j] : INT32_MAX;
+      }
+    }
+  }
+  // pairs of equal columns: the first takes the AVOS sum, the second drops
+  int32_t lastc = INT32_MIN;
+#pragma unroll
+  for (int q = 0; q < RQ; q++)
+    if (q < nv) lastc = oc[q];
+  const int32_t prev0 = __shfl_up(lastc, 1);
+  const int32_t next0c = __shfl_down(oc[0], 1);
+  const T next0v = __shfl_down(ov[0], 1);
+  bool keep[RQ];
+#pragma unroll
+  for (int q = 0; q < RQ; q++) {
+    const int32_t pc = q == 0 ? (lane > 0 ? prev0 : INT32_MIN) : oc[q == 0 ? 0 : q - 1];
+    const bool nx_in = q + 1 < nv;
+    const int32_t nc = nx_in ? oc[q + 1 < RQ ? q + 1 : q] : (lane < 63 ? next0c : INT32_MAX);
+    const T nvv = nx_in ? ov[q + 1 < RQ ? q + 1 : q] : next0v;
+    keep[q] = q < nv && pc != oc[q];
+    if (keep[q] && nc == oc[q]) ov[q] = avos_sum<T>(ov[q], nvv);
+  }
+  // direct entries: absorbed by a merged head of the same column, else kept
+  bool l0keep = lane < nd;
+  for (int k = 0; k < nd; k++) {
+    const int32_t ck = __builtin_amdgcn_readlane(l0c, k);
+    const T vk = __shfl(l0v, k);
+    bool hit = false;
+#pragma unroll
+    for (int q = 0; q < RQ; q++)
+      if (keep[q] && oc[q] == ck) {
+        ov[q] = avos_sum<T>(vk, ov[q]);
+        hit = true;
+      }
+    if (__ballot(hit) && lane == k) l0keep = false;
+  }
+#pragma unroll
+  for (int q = 0; q < RQ; q++) keep[q] = keep[q] && ov[q] != 0;
+  l0keep = l0keep && l0v != 0;
+  const uint64_t l0mask = __ballot(l0keep);
+  int mine = 0;
+#pragma unroll
+  for (int q = 0; q < RQ; q++) mine += keep[q] ? 1 : 0;
+  int totS;
+  const int baseS = wave_excl_scan(mine, lane, totS);
+  wave_sync();  // the staged lists are read: the buffers take the merged row
+  int run = baseS;
+#pragma unroll
+  for (int q = 0; q < RQ; q++) {
+    if (keep[q]) {
+      int pos = run;
+      for (int k = 0; k < nd; k++)
+        if (((l0mask >> k) & 1ull) && __builtin_amdgcn_readlane(l0c, k) < oc[q]) pos++;
+      s_c[pos] = oc[q];
+      s_v[pos] = ov[q];
+      const T ax = ov[q] < 0 ? (T)-ov[q] : ov[q];
+      if (ax > 1 && ax > fmax) fmax = ax;
+      run++;
+    }
+  }
+  for (int k = 0; k < nd; k++) {
+    if (!((l0mask >> k) & 1ull)) continue;
+    const int32_t ck = __builtin_amdgcn_readlane(l0c, k);
+    int below = 0;
+#pragma unroll
+    for (int q = 0; q < RQ; q++) below += __popcll(__ballot(keep[q] && oc[q] < ck));
+    if (lane == k) {
+      const int pos = __popcll(l0mask & ((1ull << k) - 1ull)) + below;
+      s_c[pos] = l0c;
+      s_v[pos] = l0v;
+      const T ax = l0v < 0 ? (T)-l0v : l0v;
+      if (ax > 1 && ax > fmax) fmax = ax;
+    }
+  }
+  return totS + __popcll(l0mask);
+}
+
 // Round r, step 3 for levels of long rows (mean candidate count above
 // LU_ROWS_MEAN), the rows with one or two successor lists (at most LV_SMALL
 // candidates, LV_MAXD direct entries): one row per wave, no state carried
@@ -1233,110 +1352,12 @@ __global__ __launch_bounds__(64) void k_lvl_pairs(const int32_t* __restrict__ in
       cmax = ax > cmax ? ax : cmax;
     }
     wsync();
-    const int m = a_len + b_len, R = (m + 63) >> 6;
-    const int32_t* Bc = s_c + a_len;
-    const T* Bv = s_v + a_len;
-    const int d = lane * R < m ? lane * R : m;
-    int lo = d > b_len ? d - b_len : 0, hi = d < a_len ? d : a_len;
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (s_c[mid] <= Bc[d - mid - 1]) lo = mid + 1;
-      else hi = mid;
-    }
-    int i = lo, j = d - lo;
-    int32_t ha = i < a_len ? s_c[i] : INT32_MAX, hb = j < b_len ? Bc[j] : INT32_MAX;
-    const int nv = m - d < R ? m - d : R;  // this lane's merged positions
-    int32_t oc[LV_R];
-    T ov[LV_R];
-#pragma unroll
-    for (int q = 0; q < LV_R; q++) {
-      oc[q] = INT32_MAX;
-      ov[q] = 0;
-      if (q < nv) {
-        if (ha <= hb) {
-          oc[q] = ha;
-          ov[q] = s_v[i];
-          i++;
-          ha = i < a_len ? s_c[i] : INT32_MAX;
-        } else {
-          oc[q] = hb;
-          ov[q] = Bv[j];
-          j++;
-          hb = j < b_len ? Bc[j] : INT32_MAX;
-        }
-      }
-    }
-    // pairs of equal columns: the first takes the AVOS sum, the second drops
-    int32_t lastc = INT32_MIN;
-#pragma unroll
-    for (int q = 0; q < LV_R; q++)
-      if (q < nv) lastc = oc[q];
-    const int32_t prev0 = __shfl_up(lastc, 1);
-    const int32_t next0c = __shfl_down(oc[0], 1);
-    const T next0v = __shfl_down(ov[0], 1);
-    bool keep[LV_R];
-#pragma unroll
-    for (int q = 0; q < LV_R; q++) {
-      const int32_t pc = q == 0 ? (lane > 0 ? prev0 : INT32_MIN) : oc[q == 0 ? 0 : q - 1];
-      const bool nx_in = q + 1 < nv;
-      const int32_t nc = nx_in ? oc[q + 1 < LV_R ? q + 1 : q] : (lane < 63 ? next0c : INT32_MAX);
-      const T nvv = nx_in ? ov[q + 1 < LV_R ? q + 1 : q] : next0v;
-      keep[q] = q < nv && pc != oc[q];
-      if (keep[q] && nc == oc[q]) ov[q] = avos_sum<T>(ov[q], nvv);
-    }
-    // direct entries: absorbed by a merged head of the same column, else kept
-    bool l0keep = lane < nd;
-    for (int k = 0; k < nd; k++) {
-      const int32_t ck = __builtin_amdgcn_readlane(l0c, k);
-      const T vk = __shfl(l0v, k);
-      bool hit = false;
-#pragma unroll
-      for (int q = 0; q < LV_R; q++)
-        if (keep[q] && oc[q] == ck) {
-          ov[q] = avos_sum<T>(vk, ov[q]);
-          hit = true;
-        }
-      if (__ballot(hit) && lane == k) l0keep = false;
-    }
-#pragma unroll
-    for (int q = 0; q < LV_R; q++) keep[q] = keep[q] && ov[q] != 0;
-    l0keep = l0keep && l0v != 0;
-    const uint64_t l0mask = __ballot(l0keep);
-    int mine = 0;
-#pragma unroll
-    for (int q = 0; q < LV_R; q++) mine += keep[q] ? 1 : 0;
-    int totS;
-    const int baseS = wave_excl_scan(mine, lane, totS);
-    wsync();  // the staged lists are read: the buffers take the merged row
-    int run = baseS;
-#pragma unroll
-    for (int q = 0; q < LV_R; q++) {
-      if (keep[q]) {
-        int pos = run;
-        for (int k = 0; k < nd; k++)
-          if (((l0mask >> k) & 1ull) && __builtin_amdgcn_readlane(l0c, k) < oc[q]) pos++;
-        s_c[pos] = oc[q];
-        s_v[pos] = ov[q];
-        const T ax = ov[q] < 0 ? (T)-ov[q] : ov[q];
-        if (ax > 1 && ax > fmax) fmax = ax;
-        run++;
-      }
-    }
-    for (int k = 0; k < nd; k++) {
-      if (!((l0mask >> k) & 1ull)) continue;
-      const int32_t ck = __builtin_amdgcn_readlane(l0c, k);
-      int below = 0;
-#pragma unroll
-      for (int q = 0; q < LV_R; q++) below += __popcll(__ballot(keep[q] && oc[q] < ck));
-      if (lane == k) {
-        const int pos = __popcll(l0mask & ((1ull << k) - 1ull)) + below;
-        s_c[pos] = l0c;
-        s_v[pos] = l0v;
-        const T ax = l0v < 0 ? (T)-l0v : l0v;
-        if (ax > 1 && ax > fmax) fmax = ax;
-      }
-    }
-    const int out = totS + __popcll(l0mask);
+    const int R = (a_len + b_len + 63) >> 6;
+    int out;
+    if (R <= 1) out = pair_merge<T, 1>(s_c, s_v, a_len, b_len, nd, l0c, l0v, lane, fmax);
+    else if (R <= 2) out = pair_merge<T, 2>(s_c, s_v, a_len, b_len, nd, l0c, l0v, lane, fmax);
+    else if (R <= 4) out = pair_merge<T, 4>(s_c, s_v, a_len, b_len, nd, l0c, l0v, lane, fmax);
+    else out = pair_merge<T, LV_R>(s_c, s_v, a_len, b_len, nd, l0c, l0v, lane, fmax);
     wsync();
     const int64_t dst = abase + seg;
     for (int q = lane; q < out; q += 64) {
