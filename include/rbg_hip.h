@@ -218,14 +218,18 @@ int rbg_dag_closure_i64(const int64_t *indptr, const int32_t *indices, const int
                         void *workspace, int64_t ws_bytes, int32_t *arena_cols, int64_t *arena_vals,
                         int64_t arena_cap, int32_t batch, int32_t width, rbg_status *st, rbg_stream_t stream);
 /* Gather arena rows into CSR: out_indptr (n+1, int64) always; indices/data when
- * non-null.  st->aux: [1] nnz, [2] max |candidate|, [3] max |final value| > 1,
- * [4] arena entries used, [5] temp entries used. */
+ * non-null, into buffers of out_cap entries.  When nnz > out_cap nothing is
+ * written to them and st->flags gets RBG_F_CAPACITY (call again with buffers of
+ * st->aux[1] entries).  Rows a failed closure did not merge have length 0, so the
+ * call may be queued before the closure's status is read.  st->aux: [1] nnz,
+ * [2] max |candidate|, [3] max |final value| > 1, [4] arena entries used, [5] temp
+ * entries used. */
 int rbg_dag_gather_i32(void *workspace, int32_t n, const int32_t *arena_cols, const int32_t *arena_vals,
-                       int64_t *out_indptr, int32_t *out_indices, int32_t *out_data, rbg_status *st,
-                       rbg_stream_t stream);
+                       int64_t *out_indptr, int32_t *out_indices, int32_t *out_data, int64_t out_cap,
+                       rbg_status *st, rbg_stream_t stream);
 int rbg_dag_gather_i64(void *workspace, int32_t n, const int32_t *arena_cols, const int64_t *arena_vals,
-                       int64_t *out_indptr, int32_t *out_indices, int64_t *out_data, rbg_status *st,
-                       rbg_stream_t stream);
+                       int64_t *out_indptr, int32_t *out_indices, int64_t *out_data, int64_t out_cap,
+                       rbg_status *st, rbg_stream_t stream);
 
 /* ---- AVOS SpGEMM (sparse/sparsetools/rbm_math.h:116-222, gpu/spgemm*.py) ---------- */
 /* Rows of C = A (x) B into the row arena (rbg_dag_workspace_bytes(m) workspace,

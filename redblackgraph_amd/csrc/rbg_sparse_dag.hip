@@ -1816,9 +1816,14 @@ constexpr int GATHER_NT = 256;
 template <typename T>
 __global__ __launch_bounds__(GATHER_NT) void k_dag_gather(const DagWork ws, const DagArena<T> ar,
                                                           const int64_t* __restrict__ out_ptr, int32_t n,
-                                                          int32_t* __restrict__ out_idx, T* __restrict__ out_val) {
+                                                          int32_t* __restrict__ out_idx, T* __restrict__ out_val,
+                                                          int64_t out_cap, rbg_status* st) {
   __shared__ int64_t s_pre[GATHER_NT / 64][65];
   __shared__ int64_t s_src[GATHER_NT / 64][64];
+  if (ws.counters[C_NNZ] > out_cap) {  // the output buffers are too small: nothing is written
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&st->flags, (uint32_t)RBG_F_CAPACITY);
+    return;
+  }
   const int w = threadIdx.x >> 6;
   const int64_t r0 = ((int64_t)blockIdx.x * (GATHER_NT / 64) + w) * 64;
   if (r0 >= n) return;
@@ -2038,8 +2043,8 @@ int64_t rbg_dag_workspace_bytes(int32_t n, int64_t nnz) { return DagWork::bytes(
     return 0;                                                                                                   \
   }                                                                                                             \
   int rbg_dag_gather_##SUF(void* workspace, int32_t n, const int32_t* arena_cols, const T* arena_vals,          \
-                           int64_t* out_indptr, int32_t* out_indices, T* out_data, rbg_status* st,              \
-                           rbg_stream_t stream) {                                                               \
+                           int64_t* out_indptr, int32_t* out_indices, T* out_data, int64_t out_cap,             \
+                           rbg_status* st, rbg_stream_t stream) {                                               \
     hipStream_t s = (hipStream_t)stream;                                                                        \
     if (n <= 0 || !workspace || !out_indptr) return RBG_E_ARG;                                                  \
     DagWork ws = DagWork::carve(workspace, n);                                                                  \
@@ -2050,8 +2055,8 @@ int64_t rbg_dag_workspace_bytes(int32_t n, int64_t nnz) { return DagWork::bytes(
                        (const int64_t*)(ws.counters + C_NNZ));                                                  \
     RBG_CHECK_LAUNCH();                                                                                         \
     if (out_indices && out_data) {                                                                              \
-      hipLaunchKernelGGL((k_dag_gather<T>), dim3((n + GATHER_NT - 1) / GATHER_NT), dim3(GATHER_NT), 0, s, ws, ar,                         \
-                         (const int64_t*)out_indptr, n, out_indices, out_data);                                 \
+      hipLaunchKernelGGL((k_dag_gather<T>), dim3((n + GATHER_NT - 1) / GATHER_NT), dim3(GATHER_NT), 0, s, ws, ar, \
+                         (const int64_t*)out_indptr, n, out_indices, out_data, out_cap, st);                    \
       RBG_CHECK_LAUNCH();                                                                                       \
     }                                                                                                           \
     hipLaunchKernelGGL(k_dag_finish_status, dim3(1), dim3(1), 0, s, ws, st);                                     \

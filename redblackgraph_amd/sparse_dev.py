@@ -132,6 +132,7 @@ class DagPlan:
         self.arena_cols = t.empty(arena_entries, dtype=t.int32, device=dev)
         self.arena_vals = t.empty(arena_entries, dtype=value_dtype, device=dev)
         self.levels = 0     # levels of the last graph closed with this plan (0: unknown)
+        self.out_nnz = 0    # closure entries of that graph (0: unknown)
 
 
 def dag_closure(A: DevCSR, width: int | None = None, stream=None, plan: DagPlan | None = None,
@@ -158,6 +159,7 @@ def dag_closure(A: DevCSR, width: int | None = None, stream=None, plan: DagPlan 
     if plan is None or plan.n != n or plan.vdt != vdt or plan.nnz < A.nnz:
         plan = DagPlan(n, vdt, int(max(4096, arena_factor * (A.nnz + n))), A.nnz)
     sfx = "i32" if vdt == t.int32 else "i64"
+    dev = A.indices.device
     exact = batch == 0 and plan.levels > 0
     for _attempt in range(8):
         st.reset(stream)
@@ -165,15 +167,23 @@ def dag_closure(A: DevCSR, width: int | None = None, stream=None, plan: DagPlan 
         _lib.call(f"rbg_dag_closure_{sfx}", _lib.ptr(A.indptr), _lib.ptr(A.indices), _lib.ptr(A.data), n,
                   _lib.ptr(plan.ws), plan.ws.numel(), _lib.ptr(plan.arena_cols), _lib.ptr(plan.arena_vals),
                   plan.arena_cols.numel(), rounds, width, st.p, sh)
-        # row pointers of the result are scanned before the status is read, so
-        # one wait gives the flags and the output size together
-        optr = t.empty(n + 1, dtype=t.int64, device=A.indices.device)
+        # The row pointers are scanned before the status is read, so that one
+        # wait gives the flags and the output size together.  When the plan
+        # knows its last output size, the entries are gathered into buffers of
+        # that size in the same queue (a gather that does not fit writes nothing
+        # and says so); otherwise they are gathered after the wait.
+        optr = t.empty(n + 1, dtype=t.int64, device=dev)
+        cap = plan.out_nnz if exact else 0
+        oi = t.empty(cap, dtype=t.int32, device=dev) if cap else None
+        ov = t.empty(cap, dtype=vdt, device=dev) if cap else None
         _lib.call(f"rbg_dag_gather_{sfx}", _lib.ptr(plan.ws), n, _lib.ptr(plan.arena_cols),
-                  _lib.ptr(plan.arena_vals), _lib.ptr(optr), None, None, st.p, sh)
+                  _lib.ptr(plan.arena_vals), _lib.ptr(optr), _lib.ptr(oi) if cap else None,
+                  _lib.ptr(ov) if cap else None, cap, st.p, sh)
         s = st.read(stream)
         if s.flags & _lib.F_CYCLE:
             return None, DagInfo(0, 0, 0, int(s.aux[0]), int(s.aux[6]), 0)
-        if s.flags & _lib.F_CAPACITY:
+        nnz = int(s.aux[1])
+        if s.flags & _lib.F_CAPACITY and not (s.aux[7] and cap and nnz > cap):
             plan = DagPlan(n, vdt, 2 * plan.arena_cols.numel(), A.nnz)
             exact = False
             continue
@@ -183,11 +193,14 @@ def dag_closure(A: DevCSR, width: int | None = None, stream=None, plan: DagPlan 
         info = DagInfo(0, int(s.aux[2]), int(s.aux[3]), -1, int(s.aux[6]), int(s.events))
         info.arena = int(s.aux[4])
         plan.levels = info.nlevels
-        nnz = int(s.aux[1])
-        oi = t.empty(nnz, dtype=t.int32, device=A.indices.device)
-        ov = t.empty(nnz, dtype=vdt, device=A.indices.device)
-        _lib.call(f"rbg_dag_gather_{sfx}", _lib.ptr(plan.ws), n, _lib.ptr(plan.arena_cols),
-                  _lib.ptr(plan.arena_vals), _lib.ptr(optr), _lib.ptr(oi), _lib.ptr(ov), st.p, sh)
+        plan.out_nnz = nnz
+        if cap and nnz <= cap:
+            oi, ov = oi[:nnz], ov[:nnz]
+        else:
+            oi = t.empty(nnz, dtype=t.int32, device=dev)
+            ov = t.empty(nnz, dtype=vdt, device=dev)
+            _lib.call(f"rbg_dag_gather_{sfx}", _lib.ptr(plan.ws), n, _lib.ptr(plan.arena_cols),
+                      _lib.ptr(plan.arena_vals), _lib.ptr(optr), _lib.ptr(oi), _lib.ptr(ov), nnz, st.p, sh)
         info.nnz = nnz
         return DevCSR(optr, oi, ov, A.shape), info
     raise MemoryError("DAG closure buffers could not be sized")
@@ -228,7 +241,8 @@ def spgemm(A: DevCSR, B: DevCSR, mask_upper: bool = False, width: int | None = N
               _lib.ptr(B.indptr), _lib.ptr(B.indices), _lib.ptr(B.data), int(mask_upper), _lib.ptr(ws),
               _lib.ptr(ac), _lib.ptr(av), cap, _lib.ptr(tc), _lib.ptr(tv), tcap, width, int(is_unsigned), st.p, sh)
     optr = t.empty(m + 1, dtype=t.int64, device=dev)
-    _lib.call(f"rbg_dag_gather_{sfx}", _lib.ptr(ws), m, _lib.ptr(ac), _lib.ptr(av), _lib.ptr(optr), None, None, st.p, sh)
+    _lib.call(f"rbg_dag_gather_{sfx}", _lib.ptr(ws), m, _lib.ptr(ac), _lib.ptr(av), _lib.ptr(optr), None, None, 0,
+              st.p, sh)
     s = st.read(stream)
     if s.flags & _lib.F_CAPACITY:
         raise MemoryError("SpGEMM arena too small")  # bounds are exact; not expected
@@ -236,7 +250,7 @@ def spgemm(A: DevCSR, B: DevCSR, mask_upper: bool = False, width: int | None = N
     oi = t.empty(nnz, dtype=t.int32, device=dev)
     ov = t.empty(nnz, dtype=vdt, device=dev)
     _lib.call(f"rbg_dag_gather_{sfx}", _lib.ptr(ws), m, _lib.ptr(ac), _lib.ptr(av), _lib.ptr(optr), _lib.ptr(oi),
-              _lib.ptr(ov), st.p, sh)
+              _lib.ptr(ov), nnz, st.p, sh)
     s = st.read(stream)
     return DevCSR(optr, oi, ov, (m, B.shape[1])), int(s.events)
 

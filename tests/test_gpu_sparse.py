@@ -156,6 +156,41 @@ def test_dag_plan_level_hint(orc, hint):
     assert C is None and info.cycle_vertex in (0, 1, 2)
 
 
+@pytest.mark.parametrize("delta", [-1, 0, 5000])
+def test_dag_plan_output_size_hint(orc, delta):
+    """With a known output size the entries are gathered before the status is
+    read.  Buffers one entry short are not written and the gather is redone
+    after the wait; larger ones are trimmed.  A second graph of the same size
+    with fewer entries reuses the plan.  Results match the oracle throughout."""
+    import torch
+    spec = synth.PedigreeSpec(20_000, 12, 0.7, 1)
+    graphs = [synth.csr_pedigree(spec, seed) for seed in (5, 6)]
+    refs = [orc.dag_closure(A, 64) for A in graphs]
+    big = int(np.argmax([r.indices.size for r in refs]))
+    order = [big, 1 - big]
+    A0 = graphs[order[0]]
+    plan = sd.DagPlan(A0.shape[0], torch.int64, 64 * (A0.nnz + A0.shape[0]), max(A.nnz for A in graphs))
+    for k, g in enumerate(order):
+        Ad = sd.canonicalize(sd.from_scipy(graphs[g], np.int64))
+        if k == 0:
+            C, info = sd.dag_closure(Ad, plan=plan)  # sizes the plan
+            assert plan.out_nnz == info.nnz
+            plan.out_nnz = info.nnz + delta
+        C, info = sd.dag_closure(Ad, plan=plan)
+        ref = refs[g]
+        assert info.nnz == ref.indices.size == C.indices.numel() == C.data.numel()
+        np.testing.assert_array_equal(C.indptr.cpu().numpy(), ref.indptr)
+        np.testing.assert_array_equal(C.indices.cpu().numpy(), ref.indices)
+        np.testing.assert_array_equal(C.data.cpu().numpy(), ref.data)
+        assert plan.out_nnz == info.nnz
+    # a cycle under both hints still reports a vertex on it
+    Cy = sp.csr_matrix(np.array([[1, 2, 0], [0, 1, 3], [2, 0, 1]], np.int64))
+    cplan = sd.DagPlan(3, torch.int64, 4096, Cy.nnz)
+    cplan.levels, cplan.out_nnz = 2, 16
+    C, info = sd.dag_closure(sd.canonicalize(sd.from_scipy(Cy, np.int64)), plan=cplan)
+    assert C is None and info.cycle_vertex in (0, 1, 2)
+
+
 def test_dag_pedigree_100k_vs_oracle(orc):
     spec = synth.PedigreeSpec(100_000, 15, 0.7, 1)
     A = synth.csr_pedigree(spec, 3)
