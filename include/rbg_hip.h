@@ -202,6 +202,8 @@ int64_t rbg_dag_workspace_bytes(int32_t n, int64_t nnz);
  * st->aux[7] = 0 with neither RBG_F_CYCLE nor RBG_F_CAPACITY set says that
  * the rounds did not suffice (call again with batch > 0).  Either way the
  * call returns with the report still queued on `stream`: read st after it.
+ * level_kinds: NULL, or with batch < 0 one byte per round as rbg_dag_level_kinds
+ * returned it after an earlier closure of the same graph.
  * Levels come from the reversed edges (a vertex joins level r + 1 when its last
  * unplaced successor is placed in level r), whose lists live in the workspace:
  * size it with rbg_dag_workspace_bytes(n, nnz(A)) (a smaller one sets
@@ -213,10 +215,20 @@ int64_t rbg_dag_workspace_bytes(int32_t n, int64_t nnz);
  * it and call again). */
 int rbg_dag_closure_i32(const int64_t *indptr, const int32_t *indices, const int32_t *data, int32_t n,
                         void *workspace, int64_t ws_bytes, int32_t *arena_cols, int32_t *arena_vals,
-                        int64_t arena_cap, int32_t batch, int32_t width, rbg_status *st, rbg_stream_t stream);
+                        int64_t arena_cap, int32_t batch, const uint8_t *level_kinds, int32_t width,
+                        rbg_status *st, rbg_stream_t stream);
 int rbg_dag_closure_i64(const int64_t *indptr, const int32_t *indices, const int64_t *data, int32_t n,
                         void *workspace, int64_t ws_bytes, int32_t *arena_cols, int64_t *arena_vals,
-                        int64_t arena_cap, int32_t batch, int32_t width, rbg_status *st, rbg_stream_t stream);
+                        int64_t arena_cap, int32_t batch, const uint8_t *level_kinds, int32_t width,
+                        rbg_status *st, rbg_stream_t stream);
+/* No reference counterpart: the levels' kinds of the last closure in `workspace`
+ * (count <= n bytes, one per level: 1 = merged by the unit kernel, else by the
+ * pair kernel; | 2 = it had long rows), copied to host_dst (pinned) on `stream`.
+ * Passed back as level_kinds with batch = -levels for the same graph, they
+ * launch only the merge kernels each level needs.  A level that needs one the
+ * hint left out stops the rounds (st->aux[7] = 0 without RBG_F_CYCLE or
+ * RBG_F_CAPACITY: call again with batch > 0); a cycle is still reported. */
+int rbg_dag_level_kinds(void *workspace, int32_t n, uint8_t *host_dst, int32_t count, rbg_stream_t stream);
 /* Gather arena rows into CSR: out_indptr (n+1, int64) always; indices/data when
  * non-null, into buffers of out_cap entries.  When nnz > out_cap nothing is
  * written to them and st->flags gets RBG_F_CAPACITY (call again with buffers of

@@ -76,8 +76,9 @@ SIGNATURES = {
     "rbg_csr_check_canonical_i32": (I32, [P, P, P, I32, P, P]),
     "rbg_csr_check_canonical_i64": (I32, [P, P, P, I32, P, P]),
     "rbg_dag_workspace_bytes": (I64, [I32, I64]),
-    "rbg_dag_closure_i32": (I32, [P, P, P, I32, P, I64, P, P, I64, I32, I32, P, P]),
-    "rbg_dag_closure_i64": (I32, [P, P, P, I32, P, I64, P, P, I64, I32, I32, P, P]),
+    "rbg_dag_level_kinds": (I32, [P, I32, P, I32, P]),
+    "rbg_dag_closure_i32": (I32, [P, P, P, I32, P, I64, P, P, I64, I32, P, I32, P, P]),
+    "rbg_dag_closure_i64": (I32, [P, P, P, I32, P, I64, P, P, I64, I32, P, I32, P, P]),
     "rbg_dag_gather_i32": (I32, [P, I32, P, P, P, P, P, I64, P, P]),
     "rbg_dag_gather_i64": (I32, [P, I32, P, P, P, P, P, I64, P, P]),
     "rbg_spgemm_rows_i32": (I32, [P, P, P, I32, P, P, P, I32, P, P, P, I64, P, P, I64, I32, I32, P, P]),

@@ -218,6 +218,7 @@ enum {
   C_ROUND = 15,   // round counter
   C_CYCV = 16,    // a vertex on a cycle
   C_TEMP = 17,    // temp arena entries used (SpGEMM long rows)
+  C_HINT = 18,    // a level did not match its hint (the rounds stop; the caller reruns without hints)
   C_NCTR = 32,
 };
 
@@ -242,13 +243,14 @@ struct DagWork {
   int32_t* slot_len;   // [n * SLOT_D]
   int64_t* slot_av;    // [n * SLOT_D] (edge values, stored as int64)
   int32_t* long_list;  // [n] slots of the current level left to the long-row step
+  uint8_t* kinds;      // [n] per level: LK_UNITS (merged by k_lvl_units) | LK_LONG (it had long rows)
   int64_t* pptr;       // [n+1] predecessor lists (the reversed non-self edges) ...
   int32_t* preds;      // [ecap] ... carved from the workspace's tail by the closure
   int64_t ecap;
   static constexpr int SLOT_D = 7;
   static int64_t bytes(int64_t n) {
     int64_t b = 4 * n * 4 + 8 * n + 8 * (n + 1) * 4 + 8 * (n / SCAN_CHUNK + 8) + 8 * C_NCTR;
-    b += n * (4 + 8 + 4 + 4 + 4) + n * SLOT_D * (8 + 4 + 8);
+    b += n * (4 + 8 + 4 + 4 + 4 + 1) + n * SLOT_D * (8 + 4 + 8);
     return b + 26 * 256;
   }
   // with room for the predecessor lists of a matrix with nnz entries
@@ -279,6 +281,7 @@ struct DagWork {
     w.slot_len = (int32_t*)take(4 * n * SLOT_D);
     w.slot_av = (int64_t*)take(8 * n * SLOT_D);
     w.long_list = (int32_t*)take(4 * n);
+    w.kinds = (uint8_t*)take(n);
     w.pptr = (int64_t*)take(8 * (n + 1));
     w.preds = (int32_t*)p;
     w.ecap = 0;
@@ -300,7 +303,7 @@ __global__ void k_dag_init(DagWork ws, int32_t n) {
   }
 }
 
-__device__ inline bool dag_halted(const int64_t* c) { return c[C_STUCK] | c[C_CAP] | c[C_DONE]; }
+__device__ inline bool dag_halted(const int64_t* c) { return c[C_STUCK] | c[C_CAP] | c[C_DONE] | c[C_HINT]; }
 
 // Levels by Kahn's method on the reversed edges: level 0 is every vertex
 // without non-self successors, and a vertex joins level r + 1 when its last
@@ -828,6 +831,8 @@ __device__ inline void put_meta(const SlotRegs& S, RowMeta<T>& M, int lane) {
 // the compaction keeps each run's head with the run's AVOS sum (zero sums
 // dropped) and writes the unit's rows one after another from the unit's first
 // reserved arena entry; each row lands at or below its own reserved range.
+constexpr int LK_UNITS = 1;  // level kinds: merged by k_lvl_units (else k_lvl_pairs)
+constexpr int LK_LONG = 2;   // the level had rows for k_lvl_long
 constexpr int LU_GRID = 4096;        // waves of k_lvl_pairs
 constexpr int LU_UNITS_GRID = 2048;  // waves of k_lvl_units
 constexpr int LU_ROWS_MEAN = 64;  // levels with a larger mean candidate count use k_lvl_pairs
@@ -1478,10 +1483,19 @@ __global__ __launch_bounds__(LV_NT) void k_lvl_long(const int32_t* __restrict__ 
 // Round r, step 5 (single lane): advance the arena, count the level, detect a
 // cycle (nothing placed while vertices remain) and completion, and set up
 // round r + 1.
-__global__ void k_round_end(DagWork ws, int32_t r, int32_t n) {
+__global__ void k_round_end(DagWork ws, int32_t r, int32_t n, int32_t hint) {
   int64_t* ctr = ws.counters;
-  if (ctr[C_STUCK] | ctr[C_CAP] | ctr[C_DONE]) return;
+  if (dag_halted(ctr)) return;
   const int64_t placed_now = ctr[C_PLACED] - ctr[C_LSTART];
+  // the level's kind, recorded for the next closure of the same graph; a hint
+  // that left out a merge kernel the level needed stops the rounds
+  const int64_t mean = placed_now > 0 ? (ctr[C_M] + placed_now - 1) / placed_now : 1;
+  const int kind = (mean <= LU_ROWS_MEAN ? LK_UNITS : 0) | (ctr[C_LONG] > 0 ? LK_LONG : 0);
+  ws.kinds[r < n ? r : n - 1] = (uint8_t)kind;
+  if (hint >= 0 && placed_now > 0 && ((kind & LK_UNITS) != (hint & LK_UNITS) || (kind & ~hint & LK_LONG))) {
+    ctr[C_HINT] = 1;
+    return;
+  }
   ctr[C_ARENA] += ctr[C_M];  // the level's reserved slots
   if (placed_now > 0) ctr[C_NLEV] = r + 1;
   if (ctr[C_PLACED] == n) {
@@ -1982,8 +1996,8 @@ int64_t rbg_dag_workspace_bytes(int32_t n, int64_t nnz) { return DagWork::bytes(
 #define RBG_DAG_ENTRY(SUF, T)                                                                                   \
   int rbg_dag_closure_##SUF(const int64_t* indptr, const int32_t* indices, const T* data, int32_t n,            \
                             void* workspace, int64_t ws_bytes, int32_t* arena_cols, T* arena_vals,              \
-                            int64_t arena_cap, int32_t batch, int32_t width, rbg_status* st,                    \
-                            rbg_stream_t stream) {                                                              \
+                            int64_t arena_cap, int32_t batch, const uint8_t* level_kinds, int32_t width,        \
+                            rbg_status* st, rbg_stream_t stream) {                                              \
     hipStream_t s = (hipStream_t)stream;                                                                        \
     if (n <= 0 || !workspace || !arena_cols || !arena_vals || arena_cap < 0) return RBG_E_ARG;                  \
     if (ws_bytes < DagWork::bytes(n)) return RBG_E_WORKSPACE;                                                   \
@@ -2016,15 +2030,23 @@ int64_t rbg_dag_workspace_bytes(int32_t n, int64_t nnz) { return DagWork::bytes(
       int rc = scan_dev(ws.cnt, ws.cand_loc, ws.sums_a, ws.counters + C_PLACED, ws.counters + C_LSTART, 1, n,   \
                         ws.counters + C_M, s);                                                                  \
       if (rc) return rc;                                                                                        \
+      /* with a hint (exact rounds only), the merge kernels the level needs */                                  \
+      const int hint = exact && level_kinds ? (int)level_kinds[r] : -1;                                         \
       int pi = prof_start(s);                                                                                   \
-      hipLaunchKernelGGL((k_lvl_units<T>), dim3(gu), dim3(64), 0, s, indices, data, ws, ar, width, st);         \
-      RBG_CHECK_LAUNCH();                                                                                       \
-      hipLaunchKernelGGL((k_lvl_pairs<T>), dim3(gr), dim3(64), 0, s, indices, data, ws, ar, width, st);         \
-      RBG_CHECK_LAUNCH();                                                                                       \
-      hipLaunchKernelGGL((k_lvl_long<T>), dim3(gl), dim3(LV_NT), 0, s, indices, data, ws, ar, width, st);        \
-      RBG_CHECK_LAUNCH();                                                                                       \
+      if (hint < 0 || (hint & LK_UNITS)) {                                                                      \
+        hipLaunchKernelGGL((k_lvl_units<T>), dim3(gu), dim3(64), 0, s, indices, data, ws, ar, width, st);       \
+        RBG_CHECK_LAUNCH();                                                                                     \
+      }                                                                                                         \
+      if (hint < 0 || !(hint & LK_UNITS)) {                                                                     \
+        hipLaunchKernelGGL((k_lvl_pairs<T>), dim3(gr), dim3(64), 0, s, indices, data, ws, ar, width, st);       \
+        RBG_CHECK_LAUNCH();                                                                                     \
+      }                                                                                                         \
+      if (hint < 0 || (hint & LK_LONG)) {                                                                       \
+        hipLaunchKernelGGL((k_lvl_long<T>), dim3(gl), dim3(LV_NT), 0, s, indices, data, ws, ar, width, st);      \
+        RBG_CHECK_LAUNCH();                                                                                     \
+      }                                                                                                         \
       prof_stop(2, pi, s);                                                                                      \
-      hipLaunchKernelGGL(k_round_end, dim3(1), dim3(1), 0, s, ws, r, n);                                        \
+      hipLaunchKernelGGL(k_round_end, dim3(1), dim3(1), 0, s, ws, r, n, hint);                                  \
       RBG_CHECK_LAUNCH();                                                                                       \
       if (exact) {                                                                                              \
         if (r + 1 == -batch) break;                                                                             \
@@ -2066,6 +2088,14 @@ int64_t rbg_dag_workspace_bytes(int32_t n, int64_t nnz) { return DagWork::bytes(
 
 RBG_DAG_ENTRY(i32, int32_t)
 RBG_DAG_ENTRY(i64, int64_t)
+
+int rbg_dag_level_kinds(void* workspace, int32_t n, uint8_t* host_dst, int32_t count, rbg_stream_t stream) {
+  if (n <= 0 || !workspace || !host_dst || count < 0 || count > n) return RBG_E_ARG;
+  if (count == 0) return 0;
+  DagWork ws = DagWork::carve(workspace, n);
+  hipError_t e = hipMemcpyAsync(host_dst, ws.kinds, (size_t)count, hipMemcpyDeviceToHost, (hipStream_t)stream);
+  return e == hipSuccess ? 0 : (int)e;
+}
 
 }  // extern "C"
 

@@ -115,6 +115,9 @@ class DagInfo:
         return int(self.final_max).bit_length() - 1 if self.final_max > 1 else 0
 
 
+_KINDS_MAX = 4096  # levels whose kinds a plan keeps (deeper graphs run every merge kernel each round)
+
+
 class DagPlan:
     """Reusable device buffers for closing graphs of one size: the workspace
     and the row arena.  Every level reserves its candidate count in the arena,
@@ -133,6 +136,8 @@ class DagPlan:
         self.arena_vals = t.empty(arena_entries, dtype=value_dtype, device=dev)
         self.levels = 0     # levels of the last graph closed with this plan (0: unknown)
         self.out_nnz = 0    # closure entries of that graph (0: unknown)
+        self.kinds = None   # one byte per level of that graph: which merge kernels it needed
+        self.kinds_host = None
 
 
 def dag_closure(A: DevCSR, width: int | None = None, stream=None, plan: DagPlan | None = None,
@@ -164,9 +169,15 @@ def dag_closure(A: DevCSR, width: int | None = None, stream=None, plan: DagPlan 
     for _attempt in range(8):
         st.reset(stream)
         rounds = -plan.levels if exact else (batch if batch > 0 else 8)
+        # the levels' kinds from the last closure with this plan launch only the
+        # merge kernels each level needs; a level that needs another one stops
+        # the rounds, and the closure runs again with tested rounds
+        hinted = exact and plan.kinds is not None and len(plan.kinds) == plan.levels
+        kinds = (ctypes.c_uint8 * len(plan.kinds)).from_buffer_copy(plan.kinds) if hinted else None
         _lib.call(f"rbg_dag_closure_{sfx}", _lib.ptr(A.indptr), _lib.ptr(A.indices), _lib.ptr(A.data), n,
                   _lib.ptr(plan.ws), plan.ws.numel(), _lib.ptr(plan.arena_cols), _lib.ptr(plan.arena_vals),
-                  plan.arena_cols.numel(), rounds, width, st.p, sh)
+                  plan.arena_cols.numel(), rounds, ctypes.cast(kinds, ctypes.c_void_p) if hinted else None, width,
+                  st.p, sh)
         # The row pointers are scanned before the status is read, so that one
         # wait gives the flags and the output size together.  When the plan
         # knows its last output size, the entries are gathered into buffers of
@@ -179,6 +190,11 @@ def dag_closure(A: DevCSR, width: int | None = None, stream=None, plan: DagPlan 
         _lib.call(f"rbg_dag_gather_{sfx}", _lib.ptr(plan.ws), n, _lib.ptr(plan.arena_cols),
                   _lib.ptr(plan.arena_vals), _lib.ptr(optr), _lib.ptr(oi) if cap else None,
                   _lib.ptr(ov) if cap else None, cap, st.p, sh)
+        if not hinted:
+            kcount = min(n, _KINDS_MAX)
+            if plan.kinds_host is None or plan.kinds_host.numel() < kcount:
+                plan.kinds_host = t.empty(kcount, dtype=t.uint8, pin_memory=True)
+            _lib.call("rbg_dag_level_kinds", _lib.ptr(plan.ws), n, _lib.ptr(plan.kinds_host), kcount, sh)
         s = st.read(stream)
         if s.flags & _lib.F_CYCLE:
             return None, DagInfo(0, 0, 0, int(s.aux[0]), int(s.aux[6]), 0)
@@ -194,6 +210,8 @@ def dag_closure(A: DevCSR, width: int | None = None, stream=None, plan: DagPlan 
         info.arena = int(s.aux[4])
         plan.levels = info.nlevels
         plan.out_nnz = nnz
+        if not hinted:
+            plan.kinds = bytes(plan.kinds_host[:info.nlevels].numpy()) if info.nlevels <= _KINDS_MAX else None
         if cap and nnz <= cap:
             oi, ov = oi[:nnz], ov[:nnz]
         else:

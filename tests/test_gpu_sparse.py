@@ -191,6 +191,38 @@ def test_dag_plan_output_size_hint(orc, delta):
     assert C is None and info.cycle_vertex in (0, 1, 2)
 
 
+def test_dag_plan_level_kinds(orc):
+    """A plan records which merge kernels each level needed and the next
+    closure of the graph launches only those.  Hints that leave out a kernel a
+    level needs (either merge kernel swapped, long rows left out) stop the
+    rounds and the closure reruns with tested rounds; extra long-row launches
+    are harmless.  Results match the oracle every time."""
+    import torch
+    star = np.eye(30, dtype=np.int64)
+    star[0, 1:22] = 2  # vertex 0 has 22 direct entries: a long row in level 1
+    cases = [synth.csr_pedigree(synth.PedigreeSpec(100_000, 15, 0.7, 1), 3), sp.csr_matrix(star)]
+    for A in cases:
+        ref = orc.dag_closure(A, 64)
+        Ad = sd.canonicalize(sd.from_scipy(A, np.int64))
+        plan = sd.DagPlan(A.shape[0], torch.int64, 64 * (A.nnz + A.shape[0]), A.nnz)
+        C, info = sd.dag_closure(Ad, plan=plan)
+        true = plan.kinds
+        assert true is not None and len(true) == info.nlevels == plan.levels
+        if A.shape[0] == 30:
+            assert true == bytes([1, 3])
+        else:
+            assert any(k & 1 for k in true) and any(not k & 1 for k in true)
+        hints = [true, bytes(k ^ 1 for k in true), bytes(k & 1 for k in true), bytes(k | 2 for k in true)]
+        for hint in hints:
+            plan.kinds = hint
+            C, info = sd.dag_closure(Ad, plan=plan)
+            np.testing.assert_array_equal(C.indptr.cpu().numpy(), ref.indptr)
+            np.testing.assert_array_equal(C.indices.cpu().numpy(), ref.indices)
+            np.testing.assert_array_equal(C.data.cpu().numpy(), ref.data)
+            needed = all((h & 1) == (k & 1) and not (k & ~h & 2) for h, k in zip(hint, true))
+            assert plan.kinds == (hint if needed else true)
+
+
 def test_dag_pedigree_100k_vs_oracle(orc):
     spec = synth.PedigreeSpec(100_000, 15, 0.7, 1)
     A = synth.csr_pedigree(spec, 3)
