@@ -317,6 +317,46 @@ int rbg_batch_destroy(rbg_batch *b);
 int rbg_components(const int64_t *indptr, const int32_t *indices, int32_t n, int32_t *labels, int32_t *flags,
                    int32_t *rounds, rbg_stream_t stream);
 
+/* ---- canonical ordering and permutation -----------------------------------
+ * Replaces sparse/csgraph/_ordering.pyx:17-157 (_get_permutation,
+ * _get_permutation_sparse, _compute_ordering_metrics_sparse) and
+ * _permutation.pyx:38-136 (permute, permute_sparse).
+ * Metrics, one int64 per vertex: anc = sum of MSB(v) over the row's nonzero
+ * entries (MSB(v) = floor(log2 v) for v > 1, else 0), maxrel = the column's
+ * largest entry and at least 0, diag = the vertex's own entry (0 when a CSR
+ * row holds none). */
+int rbg_order_metrics_dense_i32(const int32_t *A, int64_t n, int64_t *anc, int64_t *maxrel, int64_t *diag,
+                                rbg_stream_t stream);
+int rbg_order_metrics_dense_i64(const int64_t *A, int64_t n, int64_t *anc, int64_t *maxrel, int64_t *diag,
+                                rbg_stream_t stream);
+int rbg_order_metrics_csr_i32(const int64_t *indptr, const int32_t *indices, const int32_t *data, int32_t n,
+                              int64_t *anc, int64_t *maxrel, int64_t *diag, rbg_stream_t stream);
+int rbg_order_metrics_csr_i64(const int64_t *indptr, const int32_t *indices, const int64_t *data, int32_t n,
+                              int64_t *anc, int64_t *maxrel, int64_t *diag, rbg_stream_t stream);
+/* perm = the vertices sorted on (-comp_size, comp, -anc, maxrel, -diag, vertex)
+ * (comp: component id per vertex, comp_size: its component's vertex count);
+ * temp: rbg_order_sort_temp_bytes(n) bytes of device memory. */
+int64_t rbg_order_sort_temp_bytes(int32_t n);
+int rbg_order_sort(int32_t n, const int32_t *comp, const int32_t *comp_size, const int64_t *anc,
+                   const int64_t *maxrel, const int64_t *diag, int32_t *perm, void *temp, int64_t temp_bytes,
+                   rbg_stream_t stream);
+/* B[i][j] = A[p[i]][p[j]] (0 for j < i when upper != 0); B must not alias A. */
+int rbg_permute_dense_i32(const int32_t *A, int64_t n, const int32_t *p, int32_t *B, int32_t upper,
+                          rbg_stream_t stream);
+int rbg_permute_dense_i64(const int64_t *A, int64_t n, const int32_t *p, int64_t *B, int32_t upper,
+                          rbg_stream_t stream);
+/* P A P^-1 of a CSR matrix with nnz < 2^32: row i is row p[i] with every
+ * column c renamed to pinv[c], sorted.  Outputs: n + 1 row pointers and nnz
+ * entries; workspace: rbg_permute_csr_workspace_bytes_*(n, nnz) bytes. */
+int64_t rbg_permute_csr_workspace_bytes_i32(int32_t n, int64_t nnz);
+int64_t rbg_permute_csr_workspace_bytes_i64(int32_t n, int64_t nnz);
+int rbg_permute_csr_i32(const int64_t *indptr, const int32_t *indices, const int32_t *data, int32_t n, int64_t nnz,
+                        const int32_t *p, int64_t *out_indptr, int32_t *out_indices, int32_t *out_data,
+                        void *workspace, int64_t ws_bytes, rbg_stream_t stream);
+int rbg_permute_csr_i64(const int64_t *indptr, const int32_t *indices, const int64_t *data, int32_t n, int64_t nnz,
+                        const int32_t *p, int64_t *out_indptr, int32_t *out_indices, int64_t *out_data,
+                        void *workspace, int64_t ws_bytes, rbg_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

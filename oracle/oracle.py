@@ -35,6 +35,10 @@ dag_closure            _transitive_closure_dag.pyx:168-351                   tes
                                                                              tests/reference/test_transitive_closure_dag.py
                                                                              known answers; FW equality on the seeded
                                                                              DAGs; Ahnentafel BFS
+canonical_ordering     reference/ordering.py:13-53, components.py:4-52       tests/reference/test_triangularization.py
+                       (_ordering.pyx:17-47, :160-180)                        golden permutation and matrix; component
+                                                                             grouping of tests/sparse/test_ordering.py;
+                                                                             brute-force key check on random closures
 =====================  ====================================================  =========================================
 
 Parity unpinned (no reference vector or closed form reaches it): the exact
@@ -385,3 +389,39 @@ def py_mat_avos(A, B):
                 acc = py_avos_sum(acc, py_avos_product(A[i][j], B[j][k]))
             out[i][k] = acc
     return out
+
+
+def canonical_ordering(A):
+    """reference/ordering.py:13-53 (avos_canonical_ordering, _get_permutation)
+    with reference/components.py:4-52 (find_components) and
+    reference/permutation.py:3-15 (permute), in numpy.  Returns
+    (B, permutation, {component id: size})."""
+    A = np.asarray(A, dtype=np.int64)
+    n = A.shape[0]
+    adj = (A != 0) | (A != 0).T
+    comp = np.full(n, -1, dtype=np.int64)
+    sizes = {}
+    cid = 0
+    for s0 in range(n):  # components in order of their smallest vertex
+        if comp[s0] >= 0:
+            continue
+        stack = [s0]
+        comp[s0] = cid
+        cnt = 0
+        while stack:
+            v = stack.pop()
+            cnt += 1
+            for w in np.nonzero(adj[v])[0]:
+                if comp[w] < 0:
+                    comp[w] = cid
+                    stack.append(int(w))
+        sizes[cid] = cnt
+        cid += 1
+    msb = np.vectorize(py_msb, otypes=[np.int64])
+    anc = np.where(A != 0, msb(A), 0).sum(axis=1) if n else np.zeros(0, np.int64)
+    maxrel = np.maximum(A.max(axis=0), 0) if n else np.zeros(0, np.int64)
+    diag = np.diagonal(A)
+    basis = sorted(range(n), key=lambda x: (-sizes[comp[x]], comp[x], -anc[x], maxrel[x], -diag[x], x))
+    p = np.array(basis, dtype=np.int64)
+    B = A[np.ix_(p, p)] if n else A.copy()
+    return B, p, sizes

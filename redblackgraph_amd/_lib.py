@@ -93,6 +93,18 @@ SIGNATURES = {
     "rbg_batch_replay": (I32, [P, P]),
     "rbg_batch_destroy": (I32, [P]),
     "rbg_components": (I32, [P, P, I32, P, P, ctypes.POINTER(ctypes.c_int32), P]),
+    "rbg_order_metrics_dense_i32": (I32, [P, I64, P, P, P, P]),
+    "rbg_order_metrics_dense_i64": (I32, [P, I64, P, P, P, P]),
+    "rbg_order_metrics_csr_i32": (I32, [P, P, P, I32, P, P, P, P]),
+    "rbg_order_metrics_csr_i64": (I32, [P, P, P, I32, P, P, P, P]),
+    "rbg_order_sort_temp_bytes": (I64, [I32]),
+    "rbg_order_sort": (I32, [I32, P, P, P, P, P, P, P, I64, P]),
+    "rbg_permute_dense_i32": (I32, [P, I64, P, P, I32, P]),
+    "rbg_permute_dense_i64": (I32, [P, I64, P, P, I32, P]),
+    "rbg_permute_csr_workspace_bytes_i32": (I64, [I32, I64]),
+    "rbg_permute_csr_workspace_bytes_i64": (I64, [I32, I64]),
+    "rbg_permute_csr_i32": (I32, [P, P, P, I32, I64, P, P, P, P, P, I64, P]),
+    "rbg_permute_csr_i64": (I32, [P, P, P, I32, I64, P, P, P, P, P, I64, P]),
 }
 
 

@@ -1,0 +1,343 @@
+// Canonical ordering of a closed graph: the vertices' sort keys, the device
+// sort of the vertices, and the permutation of the matrix (dense or CSR).
+//
+// Reference: sparse/csgraph/_ordering.pyx:17-47 (_get_permutation, dense),
+// :50-157 (_get_permutation_sparse and _compute_ordering_metrics_sparse) and
+// reference/ordering.py:13-53; sparse/csgraph/_permutation.pyx:38-66 (dense
+// permute) and :67-136 (permute_sparse without the triangular filter).
+// Vertices sort on
+//   (-component size, component id, -ancestor count, max relationship,
+//    -diagonal, vertex id)
+// where a vertex's ancestor count is the sum of MSB(v) over the nonzero
+// entries v of its row (MSB(v) = floor(log2 v) for v > 1, else 0: the
+// _rbg_math.pxi loop that shifts while v > 1), its max relationship is the
+// largest entry of its column and at least 0, and its diagonal is its own
+// entry (0 when a CSR row holds none).  Sums and maxima are kept in int64.
+#include <rocprim/device/device_merge_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+#include <rocprim/device/device_segmented_radix_sort.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
+
+#include "rbg_common.hpp"
+
+namespace rbg {
+
+template <typename T>
+__device__ __forceinline__ int64_t ord_msb(T x) {
+  if (x <= 1) return 0;
+  if constexpr (sizeof(T) == 8) return 63 - __clzll((long long)x);
+  else return 31 - __clz((int)x);
+}
+
+constexpr int ORD_NT = 256;
+constexpr int ORD_COL_ROWS = 128;  // rows per workgroup of the column maxima
+
+__global__ void k_ord_fill_i64(int64_t* p, int64_t n, int64_t v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// One workgroup per row: its ancestor count and diagonal.
+template <typename T>
+__global__ __launch_bounds__(ORD_NT) void k_ord_rows_dense(const T* __restrict__ A, int64_t n,
+                                                           int64_t* __restrict__ anc, int64_t* __restrict__ diag) {
+  __shared__ int64_t red[ORD_NT];
+  for (int64_t i = blockIdx.x; i < n; i += gridDim.x) {
+    const T* row = A + i * n;
+    int64_t s = 0;
+    for (int64_t j = threadIdx.x; j < n; j += ORD_NT) {
+      const T v = row[j];
+      if (v != 0) s += ord_msb<T>(v);
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = ORD_NT / 2; o > 0; o >>= 1) {
+      if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      anc[i] = red[0];
+      diag[i] = (int64_t)row[i];
+    }
+    __syncthreads();
+  }
+}
+
+// Workgroup (x, y): columns [256 x, 256 x + 256) over rows [128 y, 128 y + 128);
+// each lane folds its column's maximum into maxrel (filled with 0 first).
+template <typename T>
+__global__ __launch_bounds__(ORD_NT) void k_ord_cols_dense(const T* __restrict__ A, int64_t n,
+                                                           int64_t* __restrict__ maxrel) {
+  const int64_t j = (int64_t)blockIdx.x * ORD_NT + threadIdx.x;
+  if (j >= n) return;
+  const int64_t r0 = (int64_t)blockIdx.y * ORD_COL_ROWS;
+  const int64_t r1 = r0 + ORD_COL_ROWS < n ? r0 + ORD_COL_ROWS : n;
+  int64_t m = 0;
+  for (int64_t i = r0; i < r1; i++) {
+    const int64_t v = (int64_t)A[i * n + j];
+    m = v > m ? v : m;
+  }
+  if (m > 0) atomicMax((long long*)&maxrel[j], (long long)m);
+}
+
+// One lane per row: its ancestor count and diagonal, and its entries folded
+// into their columns' maxima (filled with 0 first).
+template <typename T>
+__global__ __launch_bounds__(ORD_NT) void k_ord_csr(const int64_t* __restrict__ indptr,
+                                                    const int32_t* __restrict__ indices, const T* __restrict__ data,
+                                                    int32_t n, int64_t* __restrict__ anc, int64_t* maxrel,
+                                                    int64_t* __restrict__ diag) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    int64_t s = 0, d = 0;
+    for (int64_t e = indptr[i]; e < indptr[i + 1]; e++) {
+      const int32_t c = indices[e];
+      const T v = data[e];
+      if (v != 0) {
+        s += ord_msb<T>(v);
+        if (v > 0) atomicMax((long long*)&maxrel[c], (long long)v);
+      }
+      if (c == i) d = (int64_t)v;
+    }
+    anc[i] = s;
+    diag[i] = d;
+  }
+}
+
+// The canonical order of two vertices.
+struct OrdLess {
+  const int32_t* comp;
+  const int32_t* csize;
+  const int64_t* anc;
+  const int64_t* maxrel;
+  const int64_t* diag;
+  __device__ bool operator()(const int32_t a, const int32_t b) const {
+    if (csize[a] != csize[b]) return csize[a] > csize[b];
+    if (comp[a] != comp[b]) return comp[a] < comp[b];
+    if (anc[a] != anc[b]) return anc[a] > anc[b];
+    if (maxrel[a] != maxrel[b]) return maxrel[a] < maxrel[b];
+    if (diag[a] != diag[b]) return diag[a] > diag[b];
+    return a < b;
+  }
+};
+
+// B[i][j] = A[p[i]][p[j]] (0 below the diagonal when `upper`)
+template <typename T>
+__global__ __launch_bounds__(ORD_NT) void k_permute_dense(const T* __restrict__ A, int64_t n,
+                                                          const int32_t* __restrict__ p, T* __restrict__ B,
+                                                          int upper) {
+  const int64_t j = (int64_t)blockIdx.x * ORD_NT + threadIdx.x;
+  if (j >= n) return;
+  const int64_t pj = p[j];
+  for (int64_t i = blockIdx.y; i < n; i += gridDim.y)
+    B[i * n + j] = (upper && j < i) ? (T)0 : A[(int64_t)p[i] * n + pj];
+}
+
+__global__ void k_perm_inverse(const int32_t* __restrict__ p, int32_t n, int32_t* __restrict__ pinv,
+                               int64_t* __restrict__ lens, const int64_t* __restrict__ indptr) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t old = p[k];
+    pinv[old] = (int32_t)k;
+    lens[k] = indptr[old + 1] - indptr[old];
+  }
+}
+
+// One wave per new row i: row p[i] of A with its columns renamed through pinv.
+template <typename T>
+__global__ __launch_bounds__(ORD_NT) void k_permute_csr_rows(const int64_t* __restrict__ indptr,
+                                                             const int32_t* __restrict__ indices,
+                                                             const T* __restrict__ data, int32_t n,
+                                                             const int32_t* __restrict__ p,
+                                                             const int32_t* __restrict__ pinv,
+                                                             const int64_t* __restrict__ out_ptr,
+                                                             int32_t* __restrict__ tmp_idx, T* __restrict__ tmp_val) {
+  const int lane = threadIdx.x & 63;
+  const int64_t waves = (int64_t)gridDim.x * (ORD_NT / 64);
+  for (int64_t i = (int64_t)blockIdx.x * (ORD_NT / 64) + (threadIdx.x >> 6); i < n; i += waves) {
+    const int64_t s = indptr[p[i]], len = indptr[p[i] + 1] - s, d = out_ptr[i];
+    for (int64_t k = lane; k < len; k += 64) {
+      tmp_idx[d + k] = pinv[indices[s + k]];
+      tmp_val[d + k] = data[s + k];
+    }
+  }
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+static unsigned key_bits(int32_t n) {
+  unsigned b = 1;
+  while (b < 31 && ((int64_t)1 << b) < (int64_t)n) b++;
+  return b;
+}
+
+template <typename T>
+static size_t permute_csr_sort_bytes(int32_t n, int64_t nnz) {
+  size_t sort_bytes = 0;
+  (void)rocprim::segmented_radix_sort_pairs((void*)nullptr, sort_bytes, (const int32_t*)nullptr, (int32_t*)nullptr,
+                                            (const T*)nullptr, (T*)nullptr, (unsigned)nnz, (unsigned)n,
+                                            (const int64_t*)nullptr, (const int64_t*)nullptr, 0u, key_bits(n),
+                                            (hipStream_t)0);
+  return sort_bytes;
+}
+
+static size_t scan_bytes(int32_t n) {
+  size_t b = 0;
+  (void)rocprim::inclusive_scan((void*)nullptr, b, (const int64_t*)nullptr, (int64_t*)nullptr, (size_t)n,
+                                rocprim::plus<int64_t>(), (hipStream_t)0);
+  return b;
+}
+
+// workspace: pinv (n), row lengths (n), renamed rows (nnz), scan and sort storage
+template <typename T>
+static int64_t permute_csr_ws_bytes(int32_t n, int64_t nnz) {
+  return (int64_t)(align256(4 * (size_t)n) + align256(8 * (size_t)n) + align256(4 * (size_t)nnz) +
+                   align256(sizeof(T) * (size_t)nnz) + align256(scan_bytes(n)) +
+                   align256(permute_csr_sort_bytes<T>(n, nnz)) + 256);
+}
+
+template <typename T>
+static int permute_csr(const int64_t* indptr, const int32_t* indices, const T* data, int32_t n, int64_t nnz,
+                       const int32_t* p, int64_t* out_indptr, int32_t* out_indices, T* out_data, void* ws,
+                       int64_t ws_bytes, hipStream_t s) {
+  if (n <= 0 || nnz < 0 || nnz >= ((int64_t)1 << 32) || !indptr || !p || !out_indptr || !ws) return RBG_E_ARG;
+  if (nnz > 0 && (!indices || !data || !out_indices || !out_data)) return RBG_E_ARG;
+  if (ws_bytes < permute_csr_ws_bytes<T>(n, nnz)) return RBG_E_WORKSPACE;
+  char* w = (char*)(((uintptr_t)ws + 255) & ~(uintptr_t)255);
+  int32_t* pinv = (int32_t*)w;
+  w += align256(4 * (size_t)n);
+  int64_t* lens = (int64_t*)w;
+  w += align256(8 * (size_t)n);
+  int32_t* tmp_idx = (int32_t*)w;
+  w += align256(4 * (size_t)nnz);
+  T* tmp_val = (T*)w;
+  w += align256(sizeof(T) * (size_t)nnz);
+  void* scan_tmp = w;
+  size_t scan_sz = scan_bytes(n);
+  w += align256(scan_sz);
+  void* sort_tmp = w;
+  size_t sort_sz = permute_csr_sort_bytes<T>(n, nnz);
+  const int g = (int)((n + ORD_NT - 1) / ORD_NT < 4096 ? (n + ORD_NT - 1) / ORD_NT : 4096);
+  hipLaunchKernelGGL(k_perm_inverse, dim3(g), dim3(ORD_NT), 0, s, p, n, pinv, lens, indptr);
+  RBG_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_ord_fill_i64, dim3(1), dim3(64), 0, s, out_indptr, (int64_t)1, (int64_t)0);
+  RBG_CHECK_LAUNCH();
+  hipError_t e = rocprim::inclusive_scan(scan_tmp, scan_sz, lens, out_indptr + 1, (size_t)n,
+                                         rocprim::plus<int64_t>(), s);
+  if (e != hipSuccess) return (int)e;
+  if (nnz == 0) return 0;
+  const int gw = (int)((n + 3) / 4 < 4096 ? (n + 3) / 4 : 4096);
+  hipLaunchKernelGGL((k_permute_csr_rows<T>), dim3(gw), dim3(ORD_NT), 0, s, indptr, indices, data, n, p, pinv,
+                     (const int64_t*)out_indptr, tmp_idx, tmp_val);
+  RBG_CHECK_LAUNCH();
+  e = rocprim::segmented_radix_sort_pairs(sort_tmp, sort_sz, (const int32_t*)tmp_idx, out_indices,
+                                          (const T*)tmp_val, out_data, (unsigned)nnz, (unsigned)n,
+                                          (const int64_t*)out_indptr, (const int64_t*)(out_indptr + 1), 0u,
+                                          key_bits(n), s);
+  return e == hipSuccess ? 0 : (int)e;
+}
+
+template <typename T>
+static int order_metrics_dense(const T* A, int64_t n, int64_t* anc, int64_t* maxrel, int64_t* diag, hipStream_t s) {
+  if (!A || n <= 0 || !anc || !maxrel || !diag) return RBG_E_ARG;
+  hipLaunchKernelGGL(k_ord_fill_i64, dim3(64), dim3(ORD_NT), 0, s, maxrel, n, (int64_t)0);
+  RBG_CHECK_LAUNCH();
+  const int rows = (int)(n < 8192 ? n : 8192);
+  hipLaunchKernelGGL((k_ord_rows_dense<T>), dim3(rows), dim3(ORD_NT), 0, s, A, n, anc, diag);
+  RBG_CHECK_LAUNCH();
+  const unsigned cx = (unsigned)((n + ORD_NT - 1) / ORD_NT), cy = (unsigned)((n + ORD_COL_ROWS - 1) / ORD_COL_ROWS);
+  if (cy > 65535) return RBG_E_ARG;
+  hipLaunchKernelGGL((k_ord_cols_dense<T>), dim3(cx, cy), dim3(ORD_NT), 0, s, A, n, maxrel);
+  RBG_CHECK_LAUNCH();
+  return 0;
+}
+
+template <typename T>
+static int order_metrics_csr(const int64_t* indptr, const int32_t* indices, const T* data, int32_t n, int64_t* anc,
+                             int64_t* maxrel, int64_t* diag, hipStream_t s) {
+  if (!indptr || n <= 0 || !anc || !maxrel || !diag) return RBG_E_ARG;
+  hipLaunchKernelGGL(k_ord_fill_i64, dim3(64), dim3(ORD_NT), 0, s, maxrel, (int64_t)n, (int64_t)0);
+  RBG_CHECK_LAUNCH();
+  const int g = (int)((n + ORD_NT - 1) / ORD_NT < 4096 ? (n + ORD_NT - 1) / ORD_NT : 4096);
+  hipLaunchKernelGGL((k_ord_csr<T>), dim3(g), dim3(ORD_NT), 0, s, indptr, indices, data, n, anc, maxrel, diag);
+  RBG_CHECK_LAUNCH();
+  return 0;
+}
+
+template <typename T>
+static int permute_dense(const T* A, int64_t n, const int32_t* p, T* B, int upper, hipStream_t s) {
+  if (!A || !B || !p || n <= 0 || A == B) return RBG_E_ARG;
+  const unsigned gx = (unsigned)((n + ORD_NT - 1) / ORD_NT);
+  const unsigned gy = (unsigned)(n < 16384 ? n : 16384);
+  hipLaunchKernelGGL((k_permute_dense<T>), dim3(gx, gy), dim3(ORD_NT), 0, s, A, n, p, B, upper);
+  RBG_CHECK_LAUNCH();
+  return 0;
+}
+
+static size_t order_sort_bytes(int32_t n) {
+  size_t b = 0;
+  (void)rocprim::merge_sort((void*)nullptr, b, rocprim::counting_iterator<int32_t>(0), (int32_t*)nullptr,
+                            (size_t)n, OrdLess{}, (hipStream_t)0);
+  return b;
+}
+
+}  // namespace rbg
+
+using namespace rbg;
+
+extern "C" {
+
+int rbg_order_metrics_dense_i32(const int32_t* A, int64_t n, int64_t* anc, int64_t* maxrel, int64_t* diag,
+                                rbg_stream_t stream) {
+  return order_metrics_dense<int32_t>(A, n, anc, maxrel, diag, (hipStream_t)stream);
+}
+int rbg_order_metrics_dense_i64(const int64_t* A, int64_t n, int64_t* anc, int64_t* maxrel, int64_t* diag,
+                                rbg_stream_t stream) {
+  return order_metrics_dense<int64_t>(A, n, anc, maxrel, diag, (hipStream_t)stream);
+}
+int rbg_order_metrics_csr_i32(const int64_t* indptr, const int32_t* indices, const int32_t* data, int32_t n,
+                              int64_t* anc, int64_t* maxrel, int64_t* diag, rbg_stream_t stream) {
+  return order_metrics_csr<int32_t>(indptr, indices, data, n, anc, maxrel, diag, (hipStream_t)stream);
+}
+int rbg_order_metrics_csr_i64(const int64_t* indptr, const int32_t* indices, const int64_t* data, int32_t n,
+                              int64_t* anc, int64_t* maxrel, int64_t* diag, rbg_stream_t stream) {
+  return order_metrics_csr<int64_t>(indptr, indices, data, n, anc, maxrel, diag, (hipStream_t)stream);
+}
+
+int64_t rbg_order_sort_temp_bytes(int32_t n) { return n <= 0 ? 0 : (int64_t)order_sort_bytes(n); }
+
+int rbg_order_sort(int32_t n, const int32_t* comp, const int32_t* comp_size, const int64_t* anc,
+                   const int64_t* maxrel, const int64_t* diag, int32_t* perm, void* temp, int64_t temp_bytes,
+                   rbg_stream_t stream) {
+  if (n <= 0 || !comp || !comp_size || !anc || !maxrel || !diag || !perm || !temp) return RBG_E_ARG;
+  size_t b = order_sort_bytes(n);
+  if (temp_bytes < (int64_t)b) return RBG_E_WORKSPACE;
+  hipError_t e = rocprim::merge_sort(temp, b, rocprim::counting_iterator<int32_t>(0), perm, (size_t)n,
+                                     OrdLess{comp, comp_size, anc, maxrel, diag}, (hipStream_t)stream);
+  return e == hipSuccess ? 0 : (int)e;
+}
+
+int rbg_permute_dense_i32(const int32_t* A, int64_t n, const int32_t* p, int32_t* B, int32_t upper,
+                          rbg_stream_t stream) {
+  return permute_dense<int32_t>(A, n, p, B, upper, (hipStream_t)stream);
+}
+int rbg_permute_dense_i64(const int64_t* A, int64_t n, const int32_t* p, int64_t* B, int32_t upper,
+                          rbg_stream_t stream) {
+  return permute_dense<int64_t>(A, n, p, B, upper, (hipStream_t)stream);
+}
+
+int64_t rbg_permute_csr_workspace_bytes_i32(int32_t n, int64_t nnz) { return permute_csr_ws_bytes<int32_t>(n, nnz); }
+int64_t rbg_permute_csr_workspace_bytes_i64(int32_t n, int64_t nnz) { return permute_csr_ws_bytes<int64_t>(n, nnz); }
+
+int rbg_permute_csr_i32(const int64_t* indptr, const int32_t* indices, const int32_t* data, int32_t n, int64_t nnz,
+                        const int32_t* p, int64_t* out_indptr, int32_t* out_indices, int32_t* out_data,
+                        void* workspace, int64_t ws_bytes, rbg_stream_t stream) {
+  return permute_csr<int32_t>(indptr, indices, data, n, nnz, p, out_indptr, out_indices, out_data, workspace,
+                              ws_bytes, (hipStream_t)stream);
+}
+int rbg_permute_csr_i64(const int64_t* indptr, const int32_t* indices, const int64_t* data, int32_t n, int64_t nnz,
+                        const int32_t* p, int64_t* out_indptr, int32_t* out_indices, int64_t* out_data,
+                        void* workspace, int64_t ws_bytes, rbg_stream_t stream) {
+  return permute_csr<int64_t>(indptr, indices, data, n, nnz, p, out_indptr, out_indices, out_data, workspace,
+                              ws_bytes, (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -17,8 +17,10 @@ from ._components import (
     merge_component_matrices,
     get_component_vertices,
 )
+from ._ordering import avos_canonical_ordering, permute, permute_sparse, _get_permutation, _get_permutation_sparse
 
 __all__ = ["CycleError", "DensificationError", "transitive_closure", "transitive_closure_floyd_warshall",
            "transitive_closure_dijkstra", "transitive_closure_squaring", "transitive_closure_adaptive",
            "transitive_closure_dag_sparse", "component_wise_closure", "find_components", "find_components_sparse",
-           "extract_submatrix", "merge_component_matrices", "get_component_vertices"]
+           "extract_submatrix", "merge_component_matrices", "get_component_vertices",
+           "avos_canonical_ordering", "permute", "permute_sparse"]

@@ -1,0 +1,101 @@
+"""Canonical ordering and permutation on the GPU (sparse/csgraph/_ordering.pyx,
+_permutation.pyx) against the oracle's restatement of reference/ordering.py
+and the reference's golden triangularization."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from redblackgraph_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _cg():
+    from redblackgraph_amd.sparse import csgraph
+    return csgraph
+
+
+def test_golden_triangularization(golden):
+    g = golden["canonical_ordering_7"]
+    for A in (np.array(g["A_star"], np.int32), sp.csr_matrix(np.array(g["A_star"], np.int32))):
+        o = _cg().avos_canonical_ordering(A)
+        np.testing.assert_array_equal(o.label_permutation, g["permutation"])
+        B = o.A.toarray() if sp.isspmatrix(o.A) else o.A
+        np.testing.assert_array_equal(B, g["canonical"])
+        assert o.components == {0: 4, 1: 3}
+    g2 = golden["ordering_two_components"]
+    o = _cg().avos_canonical_ordering(sp.csr_matrix(np.array(g2["A"], np.int32)))
+    assert len(o.components) == g2["n_components"] and sum(o.components.values()) == g2["total"]
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.int64])
+@pytest.mark.parametrize("seed", [0, 1])
+def test_ordering_matches_oracle_dense_and_sparse(orc, dtype, seed):
+    """Pedigree closures of several components (the reference's own
+    precondition: A transitively closed): dense and CSR input give the
+    oracle's permutation and permuted matrix."""
+    rng = np.random.default_rng(seed)
+    blocks = []
+    for k in range(4):
+        n = int(rng.integers(20, 120))
+        A = synth.dense_pedigree(synth.PedigreeSpec(n, 5, 0.8, 2), seed * 10 + k)
+        blocks.append(np.asarray(orc.warshall(A).W))
+    C = sp.block_diag(blocks).toarray().astype(dtype)
+    perm_idx = rng.permutation(C.shape[0])  # interleave the components' vertices
+    C = C[np.ix_(perm_idx, perm_idx)]
+    B_ref, p_ref, q_ref = orc.canonical_ordering(C)
+    od = _cg().avos_canonical_ordering(C)
+    np.testing.assert_array_equal(od.label_permutation, p_ref)
+    np.testing.assert_array_equal(od.A, B_ref)
+    assert od.components == q_ref
+    os_ = _cg().avos_canonical_ordering(sp.csr_matrix(C))
+    np.testing.assert_array_equal(os_.label_permutation, p_ref)
+    np.testing.assert_array_equal(os_.A.toarray(), B_ref)
+    assert os_.A.has_sorted_indices
+
+
+def test_ordering_reference_random_closure(orc):
+    """tests/sparse/test_ordering.py:44-75: a random upper triangular matrix
+    closed by Floyd-Warshall; dense and sparse orders agree and match the
+    oracle."""
+    np.random.seed(42)
+    n = 30
+    A = np.triu(np.random.randint(0, 4, size=(n, n)), k=1)
+    np.fill_diagonal(A, np.random.choice([-1, 1], size=n))
+    W = np.asarray(orc.cy_floyd_warshall(A.astype(np.int32)).W).astype(np.int32)
+    _, p_ref, _ = orc.canonical_ordering(W)
+    pd = _cg().avos_canonical_ordering(W).label_permutation
+    ps = _cg().avos_canonical_ordering(sp.csr_matrix(W)).label_permutation
+    np.testing.assert_array_equal(pd, p_ref)
+    np.testing.assert_array_equal(ps, p_ref)
+
+
+@pytest.mark.parametrize("upper", [False, True])
+def test_permute_dense_and_sparse(upper):
+    rng = np.random.default_rng(7)
+    n = 300
+    A = (rng.random((n, n)) < 0.05) * rng.integers(1, 50, (n, n))
+    A = A.astype(np.int64)
+    p = rng.permutation(n).astype(np.int32)
+    ref = A[np.ix_(p, p)]
+    if upper:
+        ref = np.triu(ref)
+    np.testing.assert_array_equal(_cg().permute(A, p, upper), ref)
+    S = _cg().permute_sparse(sp.csr_matrix(A), p, upper)
+    assert S.has_sorted_indices
+    np.testing.assert_array_equal(S.toarray(), ref)
+
+
+def test_ordering_sparse_pedigree_closure(orc):
+    """A 1,500-vertex pedigree closed on the GPU, ordered as CSR: the
+    permutation matches the oracle's (computed on the dense closure) and the
+    result is upper triangular."""
+    from redblackgraph_amd import sparse_dev as sd
+    A = synth.csr_pedigree(synth.PedigreeSpec(1500, 8, 0.7, 1), 4)
+    C, _ = sd.dag_closure(sd.canonicalize(sd.from_scipy(A, np.int64)))
+    Cs = sd.to_scipy(C)
+    _, p_ref, q_ref = orc.canonical_ordering(Cs.toarray())
+    o = _cg().avos_canonical_ordering(Cs)
+    np.testing.assert_array_equal(o.label_permutation, p_ref)
+    assert o.components == q_ref
+    assert sp.triu(o.A).nnz == o.A.nnz

@@ -261,3 +261,48 @@ def test_compositions(orc, golden):
         g = golden[key]
         out, ev = orc.edge_compose(np.array(g["R"], np.int32), g["alpha"], g["beta"], g["rel"])
         np.testing.assert_array_equal(out, g["out"])
+
+
+def test_canonical_ordering_golden(orc, golden):
+    g = golden["canonical_ordering_7"]
+    B, p, q = orc.canonical_ordering(g["A_star"])
+    np.testing.assert_array_equal(p, g["permutation"])
+    np.testing.assert_array_equal(B, g["canonical"])
+    g2 = golden["ordering_two_components"]
+    _, p2, q2 = orc.canonical_ordering(g2["A"])
+    assert len(q2) == g2["n_components"] and sum(q2.values()) == g2["total"]
+    assert sorted(p2.tolist()) == list(range(4))
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_canonical_ordering_keys_brute_force(orc, seed):
+    """Every adjacent pair of the oracle's order obeys the reference's key,
+    recomputed entry by entry from the definition, and components sit in
+    contiguous runs."""
+    A = random_valid_pedigree(40, np.random.default_rng(seed))
+    C = np.asarray(orc.warshall(A).W)
+    B, p, q = orc.canonical_ordering(C)
+    n = C.shape[0]
+    comp = {}
+    for c0 in range(n):  # union-find over row/column nonzeros
+        comp.setdefault(c0, c0)
+    def find(x):
+        while comp[x] != x:
+            x = comp[x]
+        return x
+    for i in range(n):
+        for j in range(n):
+            if C[i][j] != 0:
+                a, b = find(i), find(j)
+                comp[max(a, b)] = min(a, b)
+    root = [find(i) for i in range(n)]
+    ids = {r: k for k, r in enumerate(sorted(set(root)))}
+    size = {r: root.count(r) for r in set(root)}
+    def key(x):
+        anc = sum(orc.py_msb(int(v)) for v in C[x] if v != 0)
+        mr = max([0] + [int(C[j][x]) for j in range(n) if C[j][x] != 0])
+        return (-size[root[x]], ids[root[x]], -anc, mr, -int(C[x][x]), x)
+    keys = [key(int(x)) for x in p]
+    assert keys == sorted(keys)
+    np.testing.assert_array_equal(B, C[np.ix_(p, p)])
+    assert sum(q.values()) == n
