@@ -527,6 +527,83 @@ def bench_serve(args):
     }
 
 
+def cpu_baseline_order(Cs, perm_gpu):
+    """The oracle's vectorised CSR ordering (numpy/scipy restatement of
+    _ordering.pyx:50-157) and a scipy permutation, on the full closure.  Its
+    permutation must equal the GPU's."""
+    import numpy as np
+    from oracle import oracle
+    t0 = time.perf_counter()
+    perm, _ = oracle.canonical_ordering_csr(Cs)
+    B = Cs[perm][:, perm]
+    B.sort_indices()
+    dt = time.perf_counter() - t0
+    assert np.array_equal(perm, perm_gpu), "GPU canonical order differs from the oracle's"
+    return {
+        "value": Cs.nnz / dt, "unit": "entries/s", "cores": 1, "kind": "port",
+        "sample": (f"oracle CSR canonical ordering (numpy/scipy restatement of _ordering.pyx:50-157 and "
+                   f"_permutation.pyx:67-136, 1 thread) on the full closure: {dt:.2f} s for {Cs.nnz} entries; "
+                   f"host CPU: {_cpu_model()}"),
+    }
+
+
+def bench_order(args):
+    """Canonical ordering (components, metrics, sort, CSR permutation) of the
+    1M-vertex closure, on the device."""
+    import numpy as np
+    from redblackgraph_amd import _lib, synth
+    from redblackgraph_amd import sparse_dev as sd
+    from redblackgraph_amd.sparse.csgraph._ordering import canonical_order_device, _permute_csr_device
+    t = _lib.torch()
+    A = synth.csr_pedigree(synth.SPARSE_1M, args.seed)
+    C, info = sd.dag_closure(sd.canonicalize(sd.from_scipy(A, np.int64)))
+    n, nnz = C.n_rows, C.nnz
+    for _ in range(max(1, args.warmup)):
+        perm, B, counts = canonical_order_device(C)
+    t.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        perm, B, counts = canonical_order_device(C)
+    t.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    # the permutation alone, timed with events on the stream it runs on
+    dt = np.dtype(np.int64)
+    e0, e1 = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        _permute_csr_device(C.indptr, C.indices, C.data, n, nnz, dt, perm)
+    e1.record()
+    t.cuda.synchronize()
+    perm_ms = e0.elapsed_time(e1) / args.steps
+    # size-independent checks: a permutation of the vertices, the entry count
+    # kept, and an upper triangular result (a closed pedigree in canonical order)
+    p = perm.cpu().numpy()
+    assert np.array_equal(np.sort(p), np.arange(n))
+    rows = t.repeat_interleave(t.arange(n, device=B.indptr.device), B.indptr[1:] - B.indptr[:-1])
+    below = int((B.indices.to(t.int64) < rows).sum().item())
+    assert B.nnz == nnz and below == 0
+    bytes_alg = 2 * (12 * nnz + 8 * (n + 1)) + 4 * n
+    achieved = bytes_alg / (perm_ms / 1e3)
+    roof = {
+        "bound": "hbm", "kernel": "rbg_permute_csr<long> (rows renamed and copied, then sorted per row)",
+        "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s", "frac": achieved / PEAK_HBM,
+        "traffic": None, "algorithmic_bytes_per_launch": bytes_alg, "avg_launch_ms": perm_ms,
+    }
+    res = {
+        "metric": "canonical ordering entries/sec (components, keys, sort, CSR permutation of a closure)",
+        "value": nnz / (ms / 1e3), "unit": "entries/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int64", "data": "synthetic",
+        "config": {"workload": f"canonical ordering of the 1M-vertex pedigree closure (CSR, int64), seed {args.seed}",
+                   "n": n, "nnz": nnz, "components": int(counts.numel())},
+        "roofline": roof,
+    }
+    if not args.no_cpu_baseline:
+        res["cpu_baseline"] = cpu_baseline_order(sd.to_scipy(C), p)
+    return res
+
+
 def bench_hbmcopy(args):
     """A plain device-to-device copy of 1 GiB (reads 1 GiB, writes 1 GiB):
     the calibration run for the HBM byte counters (FETCH_SIZE, WRITE_SIZE)."""
@@ -557,7 +634,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--workload", default="dense8192",
-                    choices=["dense8192", "dense4096_i64", "dense512", "sparse1m", "batched", "serve", "hbmcopy"])
+                    choices=["dense8192", "dense4096_i64", "dense512", "sparse1m", "batched", "serve", "order1m",
+                             "hbmcopy"])
     ap.add_argument("--serve-count", type=int, default=300)
     ap.add_argument("--serve-rate", type=float, default=150.0)
     ap.add_argument("--serve-slots", type=int, default=4)
@@ -581,6 +659,8 @@ def main():
         res = bench_sparse(args)
     elif args.workload == "hbmcopy":
         res = bench_hbmcopy(args)
+    elif args.workload == "order1m":
+        res = bench_order(args)
     elif args.workload == "serve":
         res = bench_serve(args)
     elif args.workload == "batched":

@@ -425,3 +425,41 @@ def canonical_ordering(A):
     p = np.array(basis, dtype=np.int64)
     B = A[np.ix_(p, p)] if n else A.copy()
     return B, p, sizes
+
+
+def canonical_ordering_csr(A):
+    """sparse/csgraph/_ordering.pyx:50-157 (_get_permutation_sparse and
+    _compute_ordering_metrics_sparse) in vectorised numpy/scipy, for CSR
+    closures too large for canonical_ordering.  Components come from
+    scipy.sparse.csgraph.connected_components (weak), renumbered by each
+    component's smallest vertex as in the reference.  Returns
+    (permutation, {component id: size})."""
+    import scipy.sparse as sp
+    from scipy.sparse.csgraph import connected_components
+    A = sp.csr_matrix(A)
+    n = A.shape[0]
+    _, lab = connected_components(A, directed=True, connection="weak")
+    first = np.full(lab.max() + 1 if n else 0, n, dtype=np.int64)
+    np.minimum.at(first, lab, np.arange(n))
+    order = np.argsort(first, kind="stable")
+    rank = np.empty_like(order)
+    rank[order] = np.arange(order.size)
+    comp = rank[lab]
+    sizes_arr = np.bincount(comp, minlength=order.size)
+    data = A.data.astype(np.int64)
+    nz = data != 0
+    msb = np.zeros(data.size, dtype=np.int64)
+    big = data > 1
+    msb[big] = np.floor(np.log2(data[big].astype(np.float64))).astype(np.int64)
+    fix = big & (np.left_shift(np.int64(1), msb, where=big, out=np.zeros_like(msb)) > data)
+    msb[fix] -= 1  # exact floor(log2) where the float rounding overshoots
+    rows = np.repeat(np.arange(n), np.diff(A.indptr))
+    anc = np.bincount(rows, weights=np.where(nz, msb, 0), minlength=n).astype(np.int64)
+    maxrel = np.zeros(n, dtype=np.int64)
+    np.maximum.at(maxrel, A.indices[nz], data[nz])
+    diag = np.zeros(n, dtype=np.int64)
+    on = A.indices == rows
+    diag[rows[on]] = data[on]
+    # np.lexsort: the last key is the primary one
+    perm = np.lexsort((np.arange(n), -diag, maxrel, -anc, comp, -sizes_arr[comp]))
+    return perm, {int(k): int(v) for k, v in enumerate(sizes_arr)}

@@ -15,6 +15,7 @@ component's smallest vertex as in the reference.
 """
 from __future__ import annotations
 
+import ctypes
 from typing import Dict
 
 import numpy as np
@@ -117,7 +118,7 @@ def _permute_dense_device(dA, n: int, dt, perm, upper: bool):
     return B
 
 
-def _permute_csr_device(indptr, indices, data, n: int, nnz: int, dt, perm):
+def _permute_csr_device(indptr, indices, data, n: int, nnz: int, dt, perm, stream=None):
     t = _lib.torch()
     dev = _lib.device()
     L = _lib.load()
@@ -127,7 +128,7 @@ def _permute_csr_device(indptr, indices, data, n: int, nnz: int, dt, perm):
     oidx = t.empty(max(nnz, 1), dtype=t.int32, device=dev)
     oval = t.empty(max(nnz, 1), dtype=data.dtype, device=dev)
     _lib.call(f"rbg_permute_csr_{sfx}", _lib.ptr(indptr), _lib.ptr(indices), _lib.ptr(data), n, nnz, _lib.ptr(perm),
-              _lib.ptr(optr), _lib.ptr(oidx), _lib.ptr(oval), _lib.ptr(ws), ws.numel(), _lib.stream_handle())
+              _lib.ptr(optr), _lib.ptr(oidx), _lib.ptr(oval), _lib.ptr(ws), ws.numel(), _lib.stream_handle(stream))
     return optr, oidx[:nnz], oval[:nnz]
 
 
@@ -193,3 +194,34 @@ def avos_canonical_ordering(A) -> Ordering:
     perm = _sort(n, q, ids, *_metrics_dense(dA, n, dt))
     B = _permute_dense_device(dA, n, dt, perm, False)
     return Ordering(B.cpu().numpy(), perm.cpu().numpy(), q)
+
+
+def canonical_order_device(C, stream=None):
+    """The canonical ordering of a closed graph held as a device CSR
+    (sparse_dev.DevCSR), without leaving the device: component labels
+    (rbg_components), ids by smallest vertex and sizes, the ordering metrics,
+    the sort and the CSR permutation.  Returns (permutation, permuted DevCSR,
+    component sizes by id), all device tensors."""
+    from ... import sparse_dev as sd
+    t = _lib.torch()
+    dev = C.indices.device
+    n = C.n_rows
+    sh = _lib.stream_handle(stream)
+    dt = np.dtype(np.int64) if C.data.dtype == t.int64 else np.dtype(np.int32)
+    labels = t.empty(max(n, 1), dtype=t.int32, device=dev)
+    flags = t.zeros(2, dtype=t.int32, device=dev)
+    rounds = ctypes.c_int32(0)
+    _lib.call("rbg_components", _lib.ptr(C.indptr), _lib.ptr(C.indices), n, _lib.ptr(labels), _lib.ptr(flags),
+              ctypes.byref(rounds), sh)
+    _, inv, counts = t.unique(labels[:n], sorted=True, return_inverse=True, return_counts=True)
+    comp = inv.to(t.int32)
+    csize = counts.to(t.int32)[inv]
+    anc, maxrel, diag = (t.empty(n, dtype=t.int64, device=dev) for _ in range(3))
+    _lib.call(f"rbg_order_metrics_csr_{_sfx(dt)}", _lib.ptr(C.indptr), _lib.ptr(C.indices), _lib.ptr(C.data), n,
+              _lib.ptr(anc), _lib.ptr(maxrel), _lib.ptr(diag), sh)
+    perm = t.empty(n, dtype=t.int32, device=dev)
+    temp = t.empty(max(1, int(_lib.load().rbg_order_sort_temp_bytes(n))), dtype=t.uint8, device=dev)
+    _lib.call("rbg_order_sort", n, _lib.ptr(comp), _lib.ptr(csize), _lib.ptr(anc), _lib.ptr(maxrel), _lib.ptr(diag),
+              _lib.ptr(perm), _lib.ptr(temp), temp.numel(), sh)
+    optr, oidx, oval = _permute_csr_device(C.indptr, C.indices, C.data, n, int(C.nnz), dt, perm, stream)
+    return perm, sd.DevCSR(optr, oidx, oval, C.shape), counts

@@ -99,3 +99,23 @@ def test_ordering_sparse_pedigree_closure(orc):
     np.testing.assert_array_equal(o.label_permutation, p_ref)
     assert o.components == q_ref
     assert sp.triu(o.A).nnz == o.A.nnz
+
+
+def test_canonical_order_device_matches_host_entry(orc):
+    """The device-only entry (DevCSR in, DevCSR out) gives the same
+    permutation, matrix and component sizes as avos_canonical_ordering."""
+    from redblackgraph_amd import sparse_dev as sd
+    from redblackgraph_amd.sparse.csgraph._ordering import canonical_order_device
+    A = synth.csr_pedigree(synth.PedigreeSpec(4000, 9, 0.7, 1), 8)
+    C, _ = sd.dag_closure(sd.canonicalize(sd.from_scipy(A, np.int64)))
+    perm, B, counts = canonical_order_device(C)
+    Cs = sd.to_scipy(C)
+    ref_perm, ref_q = orc.canonical_ordering_csr(Cs)
+    np.testing.assert_array_equal(perm.cpu().numpy(), ref_perm)
+    assert {i: int(c) for i, c in enumerate(counts.cpu().numpy())} == ref_q
+    Bs = sd.to_scipy(B)
+    R = Cs[ref_perm][:, ref_perm]
+    R.sort_indices()
+    np.testing.assert_array_equal(Bs.indptr, R.indptr)
+    np.testing.assert_array_equal(Bs.indices, R.indices)
+    np.testing.assert_array_equal(Bs.data, R.data)

@@ -306,3 +306,20 @@ def test_canonical_ordering_keys_brute_force(orc, seed):
     assert keys == sorted(keys)
     np.testing.assert_array_equal(B, C[np.ix_(p, p)])
     assert sum(q.values()) == n
+
+
+@pytest.mark.parametrize("seed", [3, 4])
+def test_canonical_ordering_csr_equals_dense(orc, seed):
+    """The vectorised CSR restatement (used as the timed CPU baseline) gives
+    the dense restatement's permutation and component sizes, including values
+    where floor(log2) sits next to a power of two."""
+    rng = np.random.default_rng(seed)
+    blocks = [np.asarray(orc.warshall(random_valid_pedigree(int(rng.integers(10, 40)), rng)).W) for _ in range(3)]
+    C = sp.block_diag(blocks).toarray()
+    C[0, C.shape[0] - 1] = (1 << 40) - 1  # joins two components; MSB 39, a float-rounding edge
+    idx = rng.permutation(C.shape[0])
+    C = C[np.ix_(idx, idx)]
+    _, p_dense, q_dense = orc.canonical_ordering(C)
+    p_csr, q_csr = orc.canonical_ordering_csr(sp.csr_matrix(C))
+    np.testing.assert_array_equal(p_csr, p_dense)
+    assert q_csr == q_dense
