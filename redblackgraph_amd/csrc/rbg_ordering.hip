@@ -80,26 +80,39 @@ __global__ __launch_bounds__(ORD_NT) void k_ord_cols_dense(const T* __restrict__
   if (m > 0) atomicMax((long long*)&maxrel[j], (long long)m);
 }
 
-// One lane per row: its ancestor count and diagonal, and its entries folded
-// into their columns' maxima (filled with 0 first).
+// One wave per row: its ancestor count (a wave sum) and diagonal, and its
+// entries folded into their columns' maxima (filled with 0 first).  The max
+// only grows, so an entry no larger than the stored maximum needs no atomic.
 template <typename T>
 __global__ __launch_bounds__(ORD_NT) void k_ord_csr(const int64_t* __restrict__ indptr,
                                                     const int32_t* __restrict__ indices, const T* __restrict__ data,
                                                     int32_t n, int64_t* __restrict__ anc, int64_t* maxrel,
                                                     int64_t* __restrict__ diag) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    int64_t s = 0, d = 0;
-    for (int64_t e = indptr[i]; e < indptr[i + 1]; e++) {
+  const int lane = threadIdx.x & 63;
+  const int64_t waves = (int64_t)gridDim.x * (ORD_NT / 64);
+  for (int64_t i = (int64_t)blockIdx.x * (ORD_NT / 64) + (threadIdx.x >> 6); i < n; i += waves) {
+    const int64_t e0 = indptr[i], e1 = indptr[i + 1];
+    int64_t s = 0, de = -1;  // de: the row's last diagonal entry, as in the reference's loop
+    for (int64_t e = e0 + lane; e < e1; e += 64) {
       const int32_t c = indices[e];
       const T v = data[e];
       if (v != 0) {
         s += ord_msb<T>(v);
-        if (v > 0) atomicMax((long long*)&maxrel[c], (long long)v);
+        if (v > 0 && (int64_t)v > __atomic_load_n(&maxrel[c], __ATOMIC_RELAXED))
+          atomicMax((long long*)&maxrel[c], (long long)v);
       }
-      if (c == i) d = (int64_t)v;
+      if (c == i) de = e;
     }
-    anc[i] = s;
-    diag[i] = d;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      s += __shfl_xor(s, o);
+      const int64_t od = __shfl_xor(de, o);
+      de = od > de ? od : de;
+    }
+    if (lane == 0) {
+      anc[i] = s;
+      diag[i] = de >= 0 ? (int64_t)data[de] : 0;
+    }
   }
 }
 
@@ -256,7 +269,7 @@ static int order_metrics_csr(const int64_t* indptr, const int32_t* indices, cons
   if (!indptr || n <= 0 || !anc || !maxrel || !diag) return RBG_E_ARG;
   hipLaunchKernelGGL(k_ord_fill_i64, dim3(64), dim3(ORD_NT), 0, s, maxrel, (int64_t)n, (int64_t)0);
   RBG_CHECK_LAUNCH();
-  const int g = (int)((n + ORD_NT - 1) / ORD_NT < 4096 ? (n + ORD_NT - 1) / ORD_NT : 4096);
+  const int g = (int)((n + 3) / 4 < 8192 ? (n + 3) / 4 : 8192);  // four rows (waves) per workgroup
   hipLaunchKernelGGL((k_ord_csr<T>), dim3(g), dim3(ORD_NT), 0, s, indptr, indices, data, n, anc, maxrel, diag);
   RBG_CHECK_LAUNCH();
   return 0;
