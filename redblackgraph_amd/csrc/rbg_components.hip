@@ -30,21 +30,31 @@ static __device__ inline int32_t cc_root(const int32_t* lab, int32_t x) {
   return x;
 }
 
-static __global__ void k_cc_hook(const int64_t* __restrict__ indptr, const int32_t* __restrict__ indices, int32_t n,
-                                 int32_t* lab, int32_t* flag) {
+// One wave per vertex v, a lane per edge (v, w): the roots of both ends, and
+// the larger one hooked under the smaller.  v's root is looked up again for
+// every 64 edges; a root that another hook has just lowered only hooks a
+// vertex of the same component, and the rounds repeat until no edge hooks.
+static __global__ __launch_bounds__(256) void k_cc_hook(const int64_t* __restrict__ indptr,
+                                                        const int32_t* __restrict__ indices, int32_t n, int32_t* lab,
+                                                        int32_t* flag) {
+  const int lane = threadIdx.x & 63;
+  const int64_t waves = (int64_t)gridDim.x * 4;
   bool changed = false;
-  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
-    for (int64_t e = indptr[v]; e < indptr[v + 1]; e++) {
-      const int32_t w = indices[e];
-      const int32_t a = cc_root(lab, (int32_t)v), b = cc_root(lab, w);
-      if (a != b) {
-        const int32_t hi = a > b ? a : b, lo = a > b ? b : a;
-        atomicMin(&lab[hi], lo);
-        changed = true;
+  for (int64_t v = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); v < n; v += waves) {
+    const int64_t e1 = indptr[v + 1];
+    for (int64_t e = indptr[v] + lane; e - lane < e1; e += 64) {
+      const int32_t a = cc_root(lab, (int32_t)v);
+      if (e < e1) {
+        const int32_t b = cc_root(lab, indices[e]);
+        if (a != b) {
+          const int32_t hi = a > b ? a : b, lo = a > b ? b : a;
+          atomicMin(&lab[hi], lo);
+          changed = true;
+        }
       }
     }
   }
-  if (changed) atomicOr(flag, 1);
+  if (__ballot(changed) && lane == 0) atomicOr(flag, 1);
 }
 
 static __global__ void k_cc_compress(int32_t* lab, int32_t n, int32_t* flag_next) {
@@ -66,6 +76,7 @@ int rbg_components(const int64_t* indptr, const int32_t* indices, int32_t n, int
   if (rounds) *rounds = 0;
   if (n == 0) return 0;
   const int grid = (int)((n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096);
+  const int hook_grid = (int)((n + 3) / 4 < 8192 ? (n + 3) / 4 : 8192);  // a wave per vertex
   hipLaunchKernelGGL(k_cc_init, dim3(grid), dim3(256), 0, s, labels, n, flags);
   RBG_CHECK_LAUNCH();
   // flags[0] / flags[1] alternate as "some edge hooked in this round"
@@ -74,7 +85,7 @@ int rbg_components(const int64_t* indptr, const int32_t* indices, int32_t n, int
   for (int32_t r = 0; r <= n; r++) {
     int32_t* cur = flags + (r & 1);
     int32_t* nxt = flags + ((r & 1) ^ 1);
-    hipLaunchKernelGGL(k_cc_hook, dim3(grid), dim3(256), 0, s, indptr, indices, n, labels, cur);
+    hipLaunchKernelGGL(k_cc_hook, dim3(hook_grid), dim3(256), 0, s, indptr, indices, n, labels, cur);
     RBG_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_cc_compress, dim3(grid), dim3(256), 0, s, labels, n, nxt);
     RBG_CHECK_LAUNCH();
