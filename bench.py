@@ -587,7 +587,8 @@ def bench_order(args):
     roof = {
         "bound": "hbm", "kernel": "rbg_permute_csr<long> (rows renamed and copied, then sorted per row)",
         "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s", "frac": achieved / PEAK_HBM,
-        "traffic": None, "algorithmic_bytes_per_launch": bytes_alg, "avg_launch_ms": perm_ms,
+        "traffic": _read_traffic("permute_csr_order1m_pmc.txt"), "algorithmic_bytes_per_launch": bytes_alg,
+        "avg_launch_ms": perm_ms,
     }
     res = {
         "metric": "canonical ordering entries/sec (components, keys, sort, CSR permutation of a closure)",
