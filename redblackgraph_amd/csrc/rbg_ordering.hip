@@ -32,6 +32,11 @@ __device__ __forceinline__ int64_t ord_msb(T x) {
 constexpr int ORD_NT = 256;
 constexpr int ORD_COL_ROWS = 128;  // rows per workgroup of the column maxima
 
+__global__ void k_ord_fill_i32(int32_t* p, int64_t n, int32_t v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
 __global__ void k_ord_fill_i64(int64_t* p, int64_t n, int64_t v) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     p[i] = v;
@@ -154,23 +159,61 @@ __global__ void k_perm_inverse(const int32_t* __restrict__ p, int32_t n, int32_t
   }
 }
 
-// One wave per new row i: row p[i] of A with its columns renamed through pinv.
+// One wave per new row i, one pass: row p[i]'s columns renamed through pinv
+// into LDS, then every entry written straight to its sorted place (its rank:
+// the row's smaller columns, and equal ones earlier in the row, so ranks are
+// a permutation of the row even with repeated columns).  A row longer than
+// PS_MAX is copied renamed to the staging buffer and listed (its output range)
+// for the segmented sort.
+constexpr int PS_MAX = 256;
 template <typename T>
-__global__ __launch_bounds__(ORD_NT) void k_permute_csr_rows(const int64_t* __restrict__ indptr,
-                                                             const int32_t* __restrict__ indices,
-                                                             const T* __restrict__ data, int32_t n,
-                                                             const int32_t* __restrict__ p,
-                                                             const int32_t* __restrict__ pinv,
-                                                             const int64_t* __restrict__ out_ptr,
-                                                             int32_t* __restrict__ tmp_idx, T* __restrict__ tmp_val) {
-  const int lane = threadIdx.x & 63;
+__global__ __launch_bounds__(ORD_NT) void k_permute_csr_fused(const int64_t* __restrict__ indptr,
+                                                              const int32_t* __restrict__ indices,
+                                                              const T* __restrict__ data, int32_t n,
+                                                              const int32_t* __restrict__ p,
+                                                              const int32_t* __restrict__ pinv,
+                                                              const int64_t* __restrict__ out_ptr,
+                                                              int32_t* __restrict__ out_idx, T* __restrict__ out_val,
+                                                              int32_t* __restrict__ tmp_idx, T* __restrict__ tmp_val,
+                                                              int64_t* __restrict__ long_begin,
+                                                              int64_t* __restrict__ long_end, int32_t* long_count) {
+  __shared__ int32_t keys[ORD_NT / 64][PS_MAX];
+  auto wsync = []() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  };
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int32_t* kw = keys[wv];
   const int64_t waves = (int64_t)gridDim.x * (ORD_NT / 64);
-  for (int64_t i = (int64_t)blockIdx.x * (ORD_NT / 64) + (threadIdx.x >> 6); i < n; i += waves) {
+  for (int64_t i = (int64_t)blockIdx.x * (ORD_NT / 64) + wv; i < n; i += waves) {
     const int64_t s = indptr[p[i]], len = indptr[p[i] + 1] - s, d = out_ptr[i];
-    for (int64_t k = lane; k < len; k += 64) {
-      tmp_idx[d + k] = pinv[indices[s + k]];
-      tmp_val[d + k] = data[s + k];
+    if (len > PS_MAX) {
+      for (int64_t k = lane; k < len; k += 64) {
+        tmp_idx[d + k] = pinv[indices[s + k]];
+        tmp_val[d + k] = data[s + k];
+      }
+      if (lane == 0) {
+        const int32_t j = atomicAdd(long_count, 1);
+        long_begin[j] = d;
+        long_end[j] = d + len;
+      }
+      continue;
     }
+    const int L = (int)len;
+    for (int k = lane; k < L; k += 64) kw[k] = pinv[indices[s + k]];
+    wsync();
+    for (int k = lane; k < L; k += 64) {
+      const int32_t key = kw[k];
+      int r = 0;
+      for (int m = 0; m < L; m++) {
+        const int32_t o = kw[m];
+        r += (o < key || (o == key && m < k)) ? 1 : 0;
+      }
+      out_idx[d + r] = key;
+      out_val[d + r] = data[s + k];
+    }
+    wsync();  // the next row restages the keys
   }
 }
 
@@ -199,11 +242,12 @@ static size_t scan_bytes(int32_t n) {
   return b;
 }
 
-// workspace: pinv (n), row lengths (n), renamed rows (nnz), scan and sort storage
+// workspace: pinv (n), row lengths (n), the long rows' output ranges (2 n)
+// and their count, staging for the long rows (nnz), scan and sort storage
 template <typename T>
 static int64_t permute_csr_ws_bytes(int32_t n, int64_t nnz) {
-  return (int64_t)(align256(4 * (size_t)n) + align256(8 * (size_t)n) + align256(4 * (size_t)nnz) +
-                   align256(sizeof(T) * (size_t)nnz) + align256(scan_bytes(n)) +
+  return (int64_t)(align256(4 * (size_t)n) + align256(8 * (size_t)n) + 2 * align256(8 * (size_t)n) + 256 +
+                   align256(4 * (size_t)nnz) + align256(sizeof(T) * (size_t)nnz) + align256(scan_bytes(n)) +
                    align256(permute_csr_sort_bytes<T>(n, nnz)) + 256);
 }
 
@@ -219,6 +263,12 @@ static int permute_csr(const int64_t* indptr, const int32_t* indices, const T* d
   w += align256(4 * (size_t)n);
   int64_t* lens = (int64_t*)w;
   w += align256(8 * (size_t)n);
+  int64_t* long_begin = (int64_t*)w;
+  w += align256(8 * (size_t)n);
+  int64_t* long_end = (int64_t*)w;
+  w += align256(8 * (size_t)n);
+  int32_t* long_count = (int32_t*)w;
+  w += 256;
   int32_t* tmp_idx = (int32_t*)w;
   w += align256(4 * (size_t)nnz);
   T* tmp_val = (T*)w;
@@ -237,14 +287,21 @@ static int permute_csr(const int64_t* indptr, const int32_t* indices, const T* d
                                          rocprim::plus<int64_t>(), s);
   if (e != hipSuccess) return (int)e;
   if (nnz == 0) return 0;
-  const int gw = (int)((n + 3) / 4 < 4096 ? (n + 3) / 4 : 4096);
-  hipLaunchKernelGGL((k_permute_csr_rows<T>), dim3(gw), dim3(ORD_NT), 0, s, indptr, indices, data, n, p, pinv,
-                     (const int64_t*)out_indptr, tmp_idx, tmp_val);
+  const int gw = (int)((n + 3) / 4 < 8192 ? (n + 3) / 4 : 8192);
+  hipLaunchKernelGGL(k_ord_fill_i32, dim3(1), dim3(64), 0, s, long_count, 1, 0);
   RBG_CHECK_LAUNCH();
+  hipLaunchKernelGGL((k_permute_csr_fused<T>), dim3(gw), dim3(ORD_NT), 0, s, indptr, indices, data, n, p, pinv,
+                     (const int64_t*)out_indptr, out_indices, out_data, tmp_idx, tmp_val, long_begin, long_end,
+                     long_count);
+  RBG_CHECK_LAUNCH();
+  int32_t nlong = 0;  // rows longer than PS_MAX: sorted by rocPRIM from the staging buffer
+  e = hipMemcpyAsync(&nlong, long_count, sizeof(nlong), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return (int)e;
+  if (nlong == 0) return 0;
   e = rocprim::segmented_radix_sort_pairs(sort_tmp, sort_sz, (const int32_t*)tmp_idx, out_indices,
-                                          (const T*)tmp_val, out_data, (unsigned)nnz, (unsigned)n,
-                                          (const int64_t*)out_indptr, (const int64_t*)(out_indptr + 1), 0u,
-                                          key_bits(n), s);
+                                          (const T*)tmp_val, out_data, (unsigned)nnz, (unsigned)nlong,
+                                          (const int64_t*)long_begin, (const int64_t*)long_end, 0u, key_bits(n), s);
   return e == hipSuccess ? 0 : (int)e;
 }
 
