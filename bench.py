@@ -17,7 +17,11 @@ Prints one JSON line (the bench contract) with:
   cpu_baseline   the oracle's single-core naive loop (a C restatement of the
                  reference's warshall.c.in) on a bounded sample, scaled to the
                  workload by the N^3 law
-Other workloads (--workload): dense4096_i64, sparse1m, batched.
+The default (--workload all) nests the rest of the metric under
+"sub_results": the sparse 1M-vertex closure, the batched 4096-component
+closure against both per-graph paths, the N=4096 int64 closure and the
+served stream.  Single workloads: --workload dense8192, dense4096_i64,
+dense512, sparse1m, batched, serve, order1m, hbmcopy.
 """
 from __future__ import annotations
 
@@ -68,6 +72,25 @@ def _read_traffic(name: str):
     return None
 
 
+class _one_core:
+    """Pins the calling thread to one core of its affinity set for the CPU
+    baseline (the reference has no CPU parallelism), restoring the set after."""
+
+    def __enter__(self):
+        self.saved = os.sched_getaffinity(0)
+        self.core = min(self.saved)
+        os.sched_setaffinity(0, {self.core})
+        return self
+
+    def __exit__(self, *exc):
+        os.sched_setaffinity(0, self.saved)
+        return False
+
+    def host(self) -> str:
+        return (f"host CPU: {_cpu_model()}; nproc {os.cpu_count()}, affinity set of {len(self.saved)} cores, "
+                f"run pinned to core {self.core}")
+
+
 def cpu_baseline_dense(n_sample: int, n_target: int, dtype):
     """Single-core naive FW (oracle restatement of warshall.c.in) on an
     n_sample pedigree; relaxations/s scaled to entries/s at n_target."""
@@ -77,9 +100,10 @@ def cpu_baseline_dense(n_sample: int, n_target: int, dtype):
     A = synth.dense_pedigree(synth.PedigreeSpec(n_sample, max(4, round(24 * n_sample / 8192)), 0.75, 4), 1,
                              dtype=dtype)
     oracle.lib()
-    t0 = time.perf_counter()
-    oracle.warshall(A)
-    dt = time.perf_counter() - t0
+    with _one_core() as pin:
+        t0 = time.perf_counter()
+        oracle.warshall(A)
+        dt = time.perf_counter() - t0
     rate = n_sample ** 3 / dt  # relaxations per second
     return {
         "value": rate / n_target,  # entries/s = N^2 / (N^3 / rate)
@@ -88,7 +112,8 @@ def cpu_baseline_dense(n_sample: int, n_target: int, dtype):
         "kind": "port",
         "sample": (f"oracle naive k-i-j FW (C restatement of warshall.c.in, gcc -O2, 1 thread) on a "
                    f"seeded N={n_sample} {np.dtype(dtype).name} pedigree: {dt:.2f} s = {rate:.3e} relaxations/s; "
-                   f"value = that rate / {n_target} (N^3 scaling to N={n_target}); host CPU: {_cpu_model()}"),
+                   f"value = that rate / {n_target} (N^3 scaling to N={n_target}: the naive loop runs all N^3 "
+                   f"relaxations whatever the input); {pin.host()}"),
     }
 
 
@@ -96,6 +121,16 @@ def _sampled_steps(args) -> int:
     """Timed steps whose kernels were bracketed by timing events (every
     prof_every-th step, starting with the first)."""
     return (args.steps + args.prof_every - 1) // args.prof_every
+
+
+def _prof_read(kind: int):
+    """(summed ms, launches, recorded work) of one timed launch kind since the
+    last read (rbg_prof_read_work)."""
+    import ctypes
+    from redblackgraph_amd import _lib
+    ms, nl, wk = ctypes.c_double(), ctypes.c_int64(), ctypes.c_double()
+    _lib.call("rbg_prof_read_work", kind, ctypes.byref(ms), ctypes.byref(nl), ctypes.byref(wk))
+    return ms.value, nl.value, wk.value
 
 
 def bench_dense(args, n: int, dtype_name: str):
@@ -121,9 +156,7 @@ def bench_dense(args, n: int, dtype_name: str):
     t.cuda.synchronize()
     t1 = time.perf_counter()
     sampled = _sampled_steps(args)
-    import ctypes
-    pm, pl, tm, tl = ctypes.c_double(), ctypes.c_int64(), ctypes.c_double(), ctypes.c_int64()
-    _lib.call("rbg_fw_prof_read", ctypes.byref(pm), ctypes.byref(pl), ctypes.byref(tm), ctypes.byref(tl))
+    prof = {k: _prof_read(k) for k in (0, 1, 5)}  # panel, 128-tiles, 64-tiles: (ms, launches, relaxations)
     L.rbg_fw_prof_enable(0)
     if out.mode != "fast" or out.status.cycle_count or out.status.events:
         raise RuntimeError(f"unexpected closure outcome: mode={out.mode} status={out.status.flags}")
@@ -135,21 +168,32 @@ def bench_dense(args, n: int, dtype_name: str):
     pcie_ms = (time.perf_counter() - h0) * 1e3
     del Wh
     npad = out.npad
-    relax_per_launch = (npad - FW_BLK) ** 2 * FW_BLK
-    tile_avg_s = tm.value / max(tl.value, 1) / 1e3
-    achieved = relax_per_launch * OPS_PER_RELAX[dtype_name] / tile_avg_s
+    # the dominant kernel: the 128 x 128 (int32) / 64 x 64 (int64) phase-3
+    # tile; achieved = relaxations its launches executed (the triangular
+    # schedule's skipped tiles are not counted) x ops per relaxation / time
+    tm, tl, tw = prof[1]
+    tile_avg_s = tm / max(tl, 1) / 1e3
+    achieved = (tw / max(tl, 1)) * OPS_PER_RELAX[dtype_name] / tile_avg_s
     kname = f"k_fw_tile_fast<{'int' if dtype_name == 'int32' else 'long'}>"
     traffic = _read_traffic(f"fw_tile_fast_dense{n}_{dtype_name}_pmc.txt")
+    sm, sl, sw = prof[5]
+    pm, pl, _ = prof[0]
     roof = {
         "bound": "valu", "kernel": kname,
         "achieved": achieved / 1e12, "peak": PEAK_VALU_OPS / 1e12, "unit": "TOP/s",
         "frac": achieved / PEAK_VALU_OPS,
         "traffic": traffic,
         "ops_per_relaxation": OPS_PER_RELAX[dtype_name],
-        "relaxations_per_launch": relax_per_launch,
-        "avg_launch_ms": tile_avg_s * 1e3, "launches": tl.value,
-        "panel_avg_ms": pm.value / max(pl.value, 1),
-        "kernel_share_of_step": (tm.value + pm.value) / (ms * sampled),
+        "relaxations_per_launch": tw / max(tl, 1),
+        "avg_launch_ms": tile_avg_s * 1e3, "launches": tl,
+        "schedule": out.schedule,
+        "relaxations_executed_per_closure": (tw + sw) / sampled,
+        "relaxations_full_schedule": (npad - FW_BLK) * npad * npad,
+        "small_tile_launches": sl, "small_tile_avg_ms": sm / max(sl, 1),
+        "small_tile_frac": ((sw / sm * 1e3 * OPS_PER_RELAX[dtype_name] / PEAK_VALU_OPS) if sm else None),
+        "panel_avg_ms": pm / max(pl, 1),
+        "panel_ms_per_closure": pm / sampled,
+        "kernel_share_of_step": (tm + sm + pm) / (ms * sampled),
         "timed_steps_sampled": sampled,
     }
     res = {
@@ -173,15 +217,16 @@ def cpu_baseline_sparse(A, nnz_c: int):
     on the full workload."""
     from oracle import oracle
     oracle.lib()
-    t0 = time.perf_counter()
-    r = oracle.dag_closure(A, 64)
-    dt = time.perf_counter() - t0
+    with _one_core() as pin:
+        t0 = time.perf_counter()
+        r = oracle.dag_closure(A, 64)
+        dt = time.perf_counter() - t0
     assert r.indptr[-1] == nnz_c
     return {
         "value": nnz_c / dt, "unit": "entries/s", "cores": 1, "kind": "port",
         "sample": (f"oracle sparse DAG closure (C restatement of _transitive_closure_dag.pyx:168-351, gcc -O2, "
                    f"1 thread) on the full 1M-vertex workload: {dt:.2f} s for {nnz_c} closure entries; "
-                   f"host CPU: {_cpu_model()}"),
+                   f"{pin.host()}"),
     }
 
 
@@ -212,6 +257,17 @@ def bench_sparse(args):
     _lib.call("rbg_prof_read", 2, ctypes.byref(ms_), ctypes.byref(nl))
     L.rbg_prof_enable(0)
     ms = (t1 - t0) * 1e3 / args.steps
+    # first closure of a graph the plan has not seen: no level count, level
+    # kinds or output size known in advance (the device memory pool is warm)
+    first = []
+    for _ in range(3):
+        t.cuda.synchronize()
+        f0 = time.perf_counter()
+        Cf, _ = sd.dag_closure(sd.canonicalize(Ad))
+        t.cuda.synchronize()
+        first.append((time.perf_counter() - f0) * 1e3)
+        del Cf
+    first_ms = sorted(first)[1]
     # host-boundary rate: scipy CSR in, scipy CSR out (PCIe both ways)
     t.cuda.synchronize()
     h0 = time.perf_counter()
@@ -245,7 +301,12 @@ def bench_sparse(args):
         "config": {"workload": f"sparse DAG closure, 1M-vertex pedigree (15 layers of ~66,667, p_parent=0.7, "
                                f"parents from the previous layer), int64 values / int32 indices, seed {args.seed}",
                    "n": n, "nnz_in": int(A.nnz), "nnz_closure": info.nnz, "levels": info.nlevels,
-                   "diameter": info.diameter_candidates, "host_boundary_ms": pcie_ms,
+                   "diameter": info.diameter_candidates,
+                   "primed_plan": "the timed closures reuse the plan of an earlier closure of this graph "
+                                  "(level count, level kinds, output size)",
+                   "first_closure_ms": first_ms,
+                   "first_closure_entries_per_s": info.nnz / (first_ms / 1e3),
+                   "host_boundary_ms": pcie_ms,
                    "host_boundary_entries_per_s": info.nnz / (pcie_ms / 1e3)},
         "roofline": roof,
     }
@@ -263,19 +324,20 @@ def cpu_baseline_batched(sizes, diag, edges, budget_s: float = 15.0):
     oracle.lib()
     order = np.argsort(sizes, kind="stable")
     done_relax, spent, used = 0, 0.0, 0
-    for i in order[:: max(1, len(order) // 256)]:
-        n = int(sizes[i])
-        A = np.zeros((n, n), dtype=np.int32)
-        r, c, v = edges[i]
-        A[r, c] = v
-        np.fill_diagonal(A, diag[i])
-        t0 = time.perf_counter()
-        oracle.warshall(A)
-        spent += time.perf_counter() - t0
-        done_relax += n ** 3
-        used += 1
-        if spent > budget_s:
-            break
+    with _one_core() as pin:
+        for i in order[:: max(1, len(order) // 256)]:
+            n = int(sizes[i])
+            A = np.zeros((n, n), dtype=np.int32)
+            r, c, v = edges[i]
+            A[r, c] = v
+            np.fill_diagonal(A, diag[i])
+            t0 = time.perf_counter()
+            oracle.warshall(A)
+            spent += time.perf_counter() - t0
+            done_relax += n ** 3
+            used += 1
+            if spent > budget_s:
+                break
     rate = done_relax / spent
     entries = float(np.sum(np.asarray(sizes, dtype=np.float64) ** 2))
     relax = float(np.sum(np.asarray(sizes, dtype=np.float64) ** 3))
@@ -283,7 +345,7 @@ def cpu_baseline_batched(sizes, diag, edges, budget_s: float = 15.0):
         "value": entries / (relax / rate), "unit": "entries/s", "cores": 1, "kind": "port",
         "sample": (f"oracle naive FW (C restatement of warshall.c.in, gcc -O2, 1 thread) on {used} of the batch's "
                    f"components (every {max(1, len(order) // 256)}th by size, smallest first): {spent:.2f} s, "
-                   f"{rate:.3e} relaxations/s; value = sum n^2 / (sum n^3 / rate); host CPU: {_cpu_model()}"),
+                   f"{rate:.3e} relaxations/s; value = sum n^2 / (sum n^3 / rate); {pin.host()}"),
     }
 
 
@@ -628,21 +690,54 @@ def bench_hbmcopy(args):
                                             "bytes_written": nbytes}}
 
 
+def _sub(res: dict) -> dict:
+    """A workload's result without the keys the top-level line already holds."""
+    drop = ("metric", "n_gpus", "higher_is_better", "scaling", "vs_baseline", "data")
+    return {k: v for k, v in res.items() if k not in drop}
+
+
+def bench_all(args):
+    """The whole metric in one line: the dense N=8192 closure is the top-level
+    value; the sparse 1M-vertex closure, the batched 4096-component closure
+    (against both per-graph paths), the N=4096 int64 closure and the served
+    stream are nested under sub_results, each with its own steps, roofline and
+    CPU baseline."""
+    import copy
+    res = bench_dense(args, 8192, "int32")
+    subs = {}
+    a = copy.copy(args)
+    print("bench: sparse1m", file=sys.stderr, flush=True)
+    subs["sparse1m"] = _sub(bench_sparse(a))
+    a = copy.copy(args)
+    a.steps, a.warmup = max(2, args.steps // 5), 1
+    print("bench: batched", file=sys.stderr, flush=True)
+    subs["batched"] = _sub(bench_batched(a))
+    a = copy.copy(args)
+    print("bench: dense4096_i64", file=sys.stderr, flush=True)
+    subs["dense4096_i64"] = _sub(bench_dense(a, 4096, "int64"))
+    a = copy.copy(args)
+    print("bench: serve", file=sys.stderr, flush=True)
+    subs["serve"] = _sub(bench_serve(a))
+    res["sub_results"] = subs
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--workload", default="dense8192",
-                    choices=["dense8192", "dense4096_i64", "dense512", "sparse1m", "batched", "serve", "order1m",
-                             "hbmcopy"])
+    ap.add_argument("--workload", default="all",
+                    choices=["all", "dense8192", "dense4096_i64", "dense512", "sparse1m", "batched", "serve",
+                             "order1m", "hbmcopy"])
     ap.add_argument("--serve-count", type=int, default=300)
     ap.add_argument("--serve-rate", type=float, default=150.0)
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-budget-gib", type=int, default=64)
     ap.add_argument("--batch-count", type=int, default=4096)
-    ap.add_argument("--cpu-sample", type=int, default=1024)
+    ap.add_argument("--cpu-sample", type=int, default=1536,
+                    help="order of the dense CPU baseline's sample (about 20 s at 1536 on one core)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--prof-every", type=int, default=4,
                     help="bracket the kernels of every N-th timed step with timing events (1: every step)")
@@ -654,7 +749,9 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU box has one GPU); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    if args.workload == "dense8192":
+    if args.workload == "all":
+        res = bench_all(args)
+    elif args.workload == "dense8192":
         res = bench_dense(args, 8192, "int32")
     elif args.workload == "sparse1m":
         res = bench_sparse(args)

@@ -8,7 +8,8 @@
  *   - Every call is asynchronous on `stream` and performs no host
  *     synchronisation, so it may be captured into a hipGraph -- except the
  *     ones whose comment says they wait (the DAG closure's completion test,
- *     the components' round test, the batched plan's descriptor upload).
+ *     the topological sort's round test, the components' round test, the
+ *     batched plan's descriptor upload).
  *   - Return value: 0 on success, an RBG_E_* code for argument errors, or a
  *     hipError_t (>0) from a failed launch.
  *   - Arithmetic errors (cycles, overflow) are reported through the device
@@ -49,6 +50,8 @@ enum {
   RBG_F_PRECOND_FAIL = 2, /* precheck: input outside the fast-path domain                    */
   RBG_F_CYCLE = 4,        /* DAG paths: a cycle was found                                    */
   RBG_F_CAPACITY = 8,     /* a device-side capacity was exceeded (caller must grow it)        */
+  RBG_F_HAS_LOWER = 16,   /* precheck: some off-diagonal entry lies below the diagonal       */
+  RBG_F_HAS_UPPER = 32,   /* precheck: some off-diagonal entry lies above the diagonal       */
 };
 
 /*
@@ -104,13 +107,17 @@ int64_t rbg_fw_workspace_bytes(int64_t n, int32_t elem_bytes);
  * stream.  rbg_fw_prof_read synchronises those events and returns the summed
  * durations (ms) and launch counts since the last read, then clears them. */
 int rbg_fw_prof_enable(int32_t on);
-/* Same facility for every timed launch kind: 0 FW panel, 1 FW tile, 2 DAG
+/* Same facility for every timed launch kind: 0 FW panel, 1 FW tile, 5 FW small tile, 2 DAG
  * level, 3 SpGEMM rows, 4 batched FW.  on = 1 starts recording afresh, 2
  * resumes and 0 pauses, both keeping what was recorded (so a caller can time
  * a sample of its launches).  rbg_prof_read returns (and clears) the summed
  * duration and count of one kind. */
 int rbg_prof_enable(int32_t on);
 int rbg_prof_read(int32_t kind, double *total_ms, int64_t *launches);
+/* rbg_prof_read, also returning the work the launches recorded: relaxations
+ * executed for the FW tile kinds (1: 128 x 128 tiles, 5: 64 x 64 int32 tiles
+ * of the small-matrix and triangular schedules), 0 for the other kinds. */
+int rbg_prof_read_work(int32_t kind, double *total_ms, int64_t *launches, double *work);
 int rbg_fw_prof_read(double *panel_ms, int64_t *panel_launches, double *tile_ms, int64_t *tile_launches);
 int rbg_fw_precheck_i32(const int32_t *W, int64_t n, int64_t ld, rbg_status *st, rbg_stream_t stream);
 int rbg_fw_precheck_i64(const int64_t *W, int64_t n, int64_t ld, rbg_status *st, rbg_stream_t stream);
@@ -118,6 +125,24 @@ int rbg_fw_dense_i32(int32_t *W, int64_t n, int64_t ld, int32_t width, int32_t i
                      void *workspace, int64_t workspace_bytes, rbg_status *st, rbg_stream_t stream);
 int rbg_fw_dense_i64(int64_t *W, int64_t n, int64_t ld, int32_t width, int32_t is_unsigned, int32_t mode,
                      void *workspace, int64_t workspace_bytes, rbg_status *st, rbg_stream_t stream);
+/* The same closure with the triangular schedule of the Cython FW's
+ * assume_upper_triangular loop (_shortest_path.pyx:365-377), replacing
+ * _floyd_warshall_avos(dist_matrix, assume_upper_triangular) (:340-389).
+ * tri = 1: every off-diagonal entry of W lies below the diagonal (a pedigree
+ * numbered parents first); tri = 2: above it; tri = 0: full schedule.  Per
+ * pivot block only the tiles that block can change are launched (about 1/6
+ * of the full schedule's relaxations), with W, the trajectory diameter and the
+ * status identical to the full schedule's.  rbg_fw_precheck_* reports the
+ * side(s) holding entries (RBG_F_HAS_LOWER / RBG_F_HAS_UPPER).  Fast mode
+ * only: with mode = 1 the full schedule runs whatever tri says.  Unlike the
+ * reference's loop, which reads only the assumed triangle, a wrong tri gives
+ * an unspecified W. */
+int rbg_fw_dense_tri_i32(int32_t *W, int64_t n, int64_t ld, int32_t width, int32_t is_unsigned, int32_t mode,
+                         int32_t tri, void *workspace, int64_t workspace_bytes, rbg_status *st,
+                         rbg_stream_t stream);
+int rbg_fw_dense_tri_i64(int64_t *W, int64_t n, int64_t ld, int32_t width, int32_t is_unsigned, int32_t mode,
+                         int32_t tri, void *workspace, int64_t workspace_bytes, rbg_status *st,
+                         rbg_stream_t stream);
 
 /* ---- dense AVOS product (einsum 'ij,jk', redblack.c.src:554-700) ----------- */
 /* C[M x N] = A[M x K] (x) B[K x N]; row-major with leading dimensions. */
@@ -229,6 +254,24 @@ int rbg_dag_closure_i64(const int64_t *indptr, const int32_t *indices, const int
  * hint left out stops the rounds (st->aux[7] = 0 without RBG_F_CYCLE or
  * RBG_F_CAPACITY: call again with batch > 0); a cycle is still reported. */
 int rbg_dag_level_kinds(void *workspace, int32_t n, uint8_t *host_dst, int32_t count, rbg_stream_t stream);
+/* Topological sort (_topological_sort.pyx:30-163, topological_sort), by the
+ * closure's Kahn levels: level_out[v] = the longest path from v to a vertex
+ * without non-self successors (NULL to skip), order_out = the vertices by
+ * decreasing level, so every edge v -> w (a nonzero A[v][w], w != v) has v
+ * before w, as the reference's order does; permuting A by it gives an upper
+ * triangular matrix.  Within a level the order is unspecified (the reference's
+ * DFS order is one valid topological order; this is another).  workspace:
+ * rbg_dag_workspace_bytes(n, nnz).  A cycle sets RBG_F_CYCLE with st->aux[0]
+ * = a vertex on it (the outputs are then not written); st->aux[6] = levels.
+ * The host waits once every 8 rounds (a graph of L levels takes L rounds). */
+int rbg_topological_sort(const int64_t *indptr, const int32_t *indices, int32_t n, void *workspace,
+                         int64_t ws_bytes, int32_t *level_out, int32_t *order_out, rbg_status *st,
+                         rbg_stream_t stream);
+/* Entry counts by side of the diagonal, for is_upper_triangular
+ * (_topological_sort.pyx:222-265): st->aux[0] += entries with column < row,
+ * st->aux[1] += entries on the diagonal. */
+int rbg_csr_triangles(const int64_t *indptr, const int32_t *indices, int32_t n, rbg_status *st,
+                      rbg_stream_t stream);
 /* Gather arena rows into CSR: out_indptr (n+1, int64) always; indices/data when
  * non-null, into buffers of out_cap entries.  When nnz > out_cap nothing is
  * written to them and st->flags gets RBG_F_CAPACITY (call again with buffers of

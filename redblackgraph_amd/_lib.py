@@ -37,6 +37,7 @@ STATUS_BYTES = ctypes.sizeof(RbgStatus)
 
 EV_OVERFLOW_WIDTH, EV_OVERFLOW_RED, EV_INVALID, EV_SIGN = 1, 2, 4, 8
 F_NEED_GENERAL, F_PRECOND_FAIL, F_CYCLE, F_CAPACITY = 1, 2, 4, 8
+F_HAS_LOWER, F_HAS_UPPER = 16, 32
 NO_KEY = (1 << 64) - 1
 
 # (name, restype, argtypes) of every exported symbol; include/rbg_hip.h is the
@@ -54,12 +55,16 @@ SIGNATURES = {
     "rbg_fw_prof_enable": (I32, [I32]),
     "rbg_prof_enable": (I32, [I32]),
     "rbg_prof_read": (I32, [I32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]),
+    "rbg_prof_read_work": (I32, [I32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64),
+                                 ctypes.POINTER(ctypes.c_double)]),
     "rbg_fw_prof_read": (I32, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64),
                                ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]),
     "rbg_fw_precheck_i32": (I32, [P, I64, I64, P, P]),
     "rbg_fw_precheck_i64": (I32, [P, I64, I64, P, P]),
     "rbg_fw_dense_i32": (I32, [P, I64, I64, I32, I32, I32, P, I64, P, P]),
     "rbg_fw_dense_i64": (I32, [P, I64, I64, I32, I32, I32, P, I64, P, P]),
+    "rbg_fw_dense_tri_i32": (I32, [P, I64, I64, I32, I32, I32, I32, P, I64, P, P]),
+    "rbg_fw_dense_tri_i64": (I32, [P, I64, I64, I32, I32, I32, I32, P, I64, P, P]),
     "rbg_matmul_i32": (I32, [P, I64, P, I64, P, I64, I64, I64, I64, I32, I32, P, P]),
     "rbg_matmul_i64": (I32, [P, I64, P, I64, P, I64, I64, I64, I64, I32, I32, P, P]),
     "rbg_edge_compose_i32": (I32, [P, I64, I64, I64, I32, P, I32, I32, P, P]),
@@ -79,6 +84,8 @@ SIGNATURES = {
     "rbg_dag_level_kinds": (I32, [P, I32, P, I32, P]),
     "rbg_dag_closure_i32": (I32, [P, P, P, I32, P, I64, P, P, I64, I32, P, I32, P, P]),
     "rbg_dag_closure_i64": (I32, [P, P, P, I32, P, I64, P, P, I64, I32, P, I32, P, P]),
+    "rbg_topological_sort": (I32, [P, P, I32, P, I64, P, P, P, P]),
+    "rbg_csr_triangles": (I32, [P, P, I32, P, P]),
     "rbg_dag_gather_i32": (I32, [P, I32, P, P, P, P, P, I64, P, P]),
     "rbg_dag_gather_i64": (I32, [P, I32, P, P, P, P, P, I64, P, P]),
     "rbg_spgemm_rows_i32": (I32, [P, P, P, I32, P, P, P, I32, P, P, P, I64, P, P, I64, I32, I32, P, P]),

@@ -185,10 +185,12 @@ class _Avos(np.ndarray):
         raise ValueError(f"Unexpected dimensionality. self: {self.ndim}, other: {other.ndim}")
 
     def cardinality(self):
-        trace = np.trace(self)
-        m = self.shape[0]
-        c_black = int((m + trace) / 2)
-        return {"red": m - c_black, "black": c_black}
+        """Vertex counts by colour (core/redblack.py:40-47): the diagonal holds
+        -1 for red and 1 for black, so with r + b = n and b - r = trace the
+        black count is (n + trace) / 2."""
+        n = self.shape[0]
+        black = int((n + int(np.trace(self))) / 2)
+        return {"red": n - black, "black": black}
 
     def transitive_closure(self) -> TransitiveClosure:
         W, d = warshall(np.asarray(self))

@@ -119,11 +119,14 @@ static __global__ void k_status_init(rbg_status* st) {
 // Precondition of the fast kernels: every value >= -1, no off-diagonal +-1,
 // every diagonal value in {-1, 0, 1}.  With such diagonals the products that
 // involve a pivot cell (k, k) can neither overflow nor reveal a cycle, which
-// is what lets the fast kernels skip the pivot row and column.
+// is what lets the fast kernels skip the pivot row and column.  Also reports
+// whether any off-diagonal entry lies below (RBG_F_HAS_LOWER) or above
+// (RBG_F_HAS_UPPER) the diagonal: a matrix with entries on one side only
+// can take the triangular schedule.
 template <typename T>
 __global__ void k_fw_precheck(const T* W, int64_t n, int64_t ld, rbg_status* st) {
   int64_t total = n * n;
-  bool bad = false;
+  bool bad = false, lower = false, upper = false;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
        e += (int64_t)gridDim.x * blockDim.x) {
     int64_t i = e / n, j = e % n;
@@ -131,18 +134,34 @@ __global__ void k_fw_precheck(const T* W, int64_t n, int64_t ld, rbg_status* st)
     if (v < (T)-1) bad = true;
     if (i != j && (v == (T)1 || v == (T)-1)) bad = true;
     if (i == j && v > (T)1) bad = true;
+    if (v != 0 && i > j) lower = true;
+    if (v != 0 && i < j) upper = true;
   }
-  if (bad) atomicOr(&st->flags, (uint32_t)RBG_F_PRECOND_FAIL);
+  // one atomic per wave
+  const uint32_t f = (__any(bad) ? (uint32_t)RBG_F_PRECOND_FAIL : 0u) | (__any(lower) ? (uint32_t)RBG_F_HAS_LOWER : 0u) |
+                     (__any(upper) ? (uint32_t)RBG_F_HAS_UPPER : 0u);
+  if ((threadIdx.x & 63) == 0 && f) atomicOr(&st->flags, f);
 }
 
+// E-domain conversion in place.  With st non-null (the triangular schedule)
+// the largest initial value also enters the trajectory maximum: the naive
+// loop writes every cell at k = 0, and on a triangular input with unit
+// diagonals that first write leaves the initial value as it is, while the
+// triangular schedule never visits the cells its pivot blocks cannot change.
 template <typename T, bool ENC>
-__global__ void k_e_convert(T* W, int64_t n, int64_t ld) {
+__global__ void k_e_convert(T* W, int64_t n, int64_t ld, rbg_status* st) {
   int64_t total = n * n;
+  int64_t vmax = 0;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
        e += (int64_t)gridDim.x * blockDim.x) {
     int64_t i = e / n, j = e % n;
     T v = W[i * ld + j];
     W[i * ld + j] = ENC ? e_enc<T>(v) : e_dec<T>(v);
+    if ((int64_t)v > vmax) vmax = (int64_t)v;
+  }
+  if (st) {
+    vmax = wave_max_i64(vmax);
+    if ((threadIdx.x & 63) == 0) record_diameter(st, vmax);
   }
 }
 
@@ -725,7 +744,7 @@ __device__ inline void fw_panel_lines_body(T* W, int64_t n, int64_t ld, int64_t 
 // at 12 more VALU instructions per 128 relaxations to take them apart.
 template <typename T, int RPT, int CPT, bool PANEL_GUARD, bool SHIFT_BYTES>
 __device__ inline void fw_tile_fast_body(T* W, int64_t n, int64_t ld, int64_t K, FwWork ws, int width,
-                                         rbg_status* st, int bid) {
+                                         rbg_status* st, int I, int J) {
   using U = typename unsigned_of<T>::type;
   constexpr U TOP = (U)1 << (sizeof(U) * 8 - 1);
   constexpr int TL = 16 * CPT;  // tile edge
@@ -737,8 +756,6 @@ __device__ inline void fw_tile_fast_body(T* W, int64_t n, int64_t ld, int64_t K,
   __shared__ U Ys[BLK][TL];
   __shared__ int sGa[BLK], sGb[BLK];
   const int tid = threadIdx.x;
-  const int nt = (int)(n / TL);
-  const int I = bid / nt, J = bid % nt;
   const int64_t k0 = K * BLK;
   const int par = (int)(K & 1);
   const U* colT = (const U*)ws.col<T>(par);
@@ -954,21 +971,61 @@ struct FastTile {
   static constexpr int PER_CU = SHIFT_BYTES ? 4 : 3;  // workgroups per CU the registers are sized for
 };
 
+// The tiles one phase-3 launch covers: the rectangle of tile rows
+// [i0, i0 + grid / jn) and tile columns [j0, j0 + jn), workgroup b taking
+// tile (i0 + b / jn, j0 + b % jn).  The full schedule launches the whole
+// matrix (i0 = j0 = 0, jn = n / edge); the triangular schedule only the tiles
+// that pivot block K can change (TriRect).
+struct TileRect {
+  int i0, j0, jn;
+};
+
 // Small int32 matrices (at most SMALL_TILE_MAX_N): 64 x 64 tiles, four times
 // as many workgroups per pivot step, since a 128-tile grid leaves most CUs
-// idle there (N = 512: 16 workgroups for 256 CUs).
+// idle there (N = 512: 16 workgroups for 256 CUs).  The triangular schedule
+// also takes them for the pivot steps whose 128-tile rectangle is small.
 constexpr int64_t SMALL_TILE_MAX_N = 2048;
 template <typename T>
 __global__ __launch_bounds__(NT, 4) void k_fw_tile_fast_small(T* W, int64_t n, int64_t ld, int64_t K, FwWork ws,
-                                                             int width, rbg_status* st) {
-  fw_tile_fast_body<T, 4, 4, true, false>(W, n, ld, K, ws, width, st, blockIdx.x);
+                                                             int width, rbg_status* st, TileRect r) {
+  const int b = blockIdx.x;
+  fw_tile_fast_body<T, 4, 4, true, false>(W, n, ld, K, ws, width, st, r.i0 + b / r.jn, r.j0 + b % r.jn);
 }
 
 template <typename T, int PER_CU = FastTile<T>::PER_CU>
 __global__ __launch_bounds__(NT, PER_CU) void k_fw_tile_fast(T* W, int64_t n, int64_t ld, int64_t K, FwWork ws,
-                                                            int width, rbg_status* st) {
-  fw_tile_fast_body<T, FastTile<T>::RC, FastTile<T>::RC, true, FastTile<T>::SHIFT_BYTES>(W, n, ld, K, ws, width,
-                                                                                        st, blockIdx.x);
+                                                            int width, rbg_status* st, TileRect r) {
+  const int b = blockIdx.x;
+  fw_tile_fast_body<T, FastTile<T>::RC, FastTile<T>::RC, true, FastTile<T>::SHIFT_BYTES>(
+      W, n, ld, K, ws, width, st, r.i0 + b / r.jn, r.j0 + b % r.jn);
+}
+
+// Triangular schedule (the blocked form of the reference's
+// assume_upper_triangular loop, _shortest_path.pyx:365-377).  When every
+// off-diagonal entry lies below the diagonal (tri = 1), a product
+// W[i][k] (x) W[k][j] can be present only for i >= k >= j, and closing keeps
+// the matrix lower triangular; so pivot block [k0, k0 + 32) changes only
+// cells with i > k0 and j < k0 + 31, outside its own rows and columns.  The
+// tiles of edge `edge` that hold such cells form the rectangle of tile rows
+// k0 / edge .. nt - 1 and tile columns 0 .. (k0 + 31) / edge; for tri = 2
+// (upper) it is the mirror image.  Every other tile would only take products
+// with an absent operand, which leave its cells, and the trajectory maximum,
+// as they are.  Returns the rectangle and its tile count.
+inline TileRect tri_rect(int tri, int64_t n, int edge, int64_t K, int* count) {
+  const int nt = (int)(n / edge);
+  const int64_t k0 = K * BLK;
+  const int lo = (int)(k0 / edge), hi = (int)((k0 + BLK - 1) / edge);
+  TileRect r{0, 0, nt};
+  if (tri == 1) {
+    r = {lo, 0, hi + 1};
+    *count = (nt - lo) * (hi + 1);
+  } else if (tri == 2) {
+    r = {0, lo, nt - lo};
+    *count = (hi + 1) * (nt - lo);
+  } else {
+    *count = nt * nt;
+  }
+  return r;
 }
 
 // ---------------------------------------------------------------------------

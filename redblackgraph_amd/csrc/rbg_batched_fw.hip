@@ -106,7 +106,8 @@ static __global__ __launch_bounds__(NT, 3) void k_bfw_tile(const BComp* comps, c
   const Item it = items[blockIdx.x];
   const BComp c = comps[it.comp];
   FwWork ws{c.snap, c.npad, 4};
-  fw_tile_fast_body<int32_t, RC, RC, false, false>(c.W, c.npad, c.npad, K, ws, 32, c.st, it.bid);
+  const int nt = c.npad / (16 * RC);
+  fw_tile_fast_body<int32_t, RC, RC, false, false>(c.W, c.npad, c.npad, K, ws, 32, c.st, it.bid / nt, it.bid % nt);
 }
 
 // Operands of the last block's detections (the panel of the next block would

@@ -21,13 +21,16 @@ namespace rbg {
 
 // ---- optional per-launch timing ---------------------------------------------
 // While enabled, selected launches are bracketed by hipEvents on their stream
-// (bench and profiling only).  Kinds: 0 FW panel, 1 FW tile, 2 DAG level,
-// 3 SpGEMM rows, 4 batched FW.
+// (bench and profiling only).  Kinds: 0 FW panel, 1 FW tile (128 x 128 for
+// int32), 2 DAG level, 3 SpGEMM rows, 4 batched FW, 5 FW tile (64 x 64 int32
+// tiles of the small-matrix and triangular schedules).  Each launch may
+// carry a count of the work it does (relaxations for the tiles).
 constexpr int PROF_KINDS = 8;
 struct Prof {
   bool on = false;
   std::vector<hipEvent_t> pool;
   std::vector<std::pair<int, int>> pairs[PROF_KINDS];
+  std::vector<double> work[PROF_KINDS];
   int used = 0;
   std::mutex mu;
   hipEvent_t take() {
@@ -50,7 +53,7 @@ inline int prof_start(hipStream_t s) {
   (void)hipEventRecord(e, s);
   return idx;
 }
-inline void prof_stop(int kind, int start_idx, hipStream_t s) {
+inline void prof_stop(int kind, int start_idx, hipStream_t s, double work = 0.0) {
   if (start_idx < 0) return;
   std::lock_guard<std::mutex> g(g_prof.mu);
   int idx = g_prof.used;
@@ -58,6 +61,7 @@ inline void prof_stop(int kind, int start_idx, hipStream_t s) {
   if (!e) return;
   (void)hipEventRecord(e, s);
   g_prof.pairs[kind].push_back({start_idx, idx});
+  g_prof.work[kind].push_back(work);
 }
 
 // ---- event recording -------------------------------------------------------

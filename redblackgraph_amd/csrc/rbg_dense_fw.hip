@@ -4,24 +4,33 @@
 
 namespace rbg {
 
-template <typename T>
-static int fw_dense(T* W, int64_t n, int64_t ld, int width, int is_uns, int mode, void* wsp,
+// 128-tile rectangles below this many workgroups (a third of a round of
+// three per CU) run as 64-tiles instead, in the triangular schedule: four
+// times the workgroups for the steps that touch few tiles.
+constexpr int TRI_SMALL_BELOW = 256;
+
+// tri: 0 = full schedule; 1 / 2 = the input's off-diagonal entries all lie
+// below / above the diagonal (fast mode only: general mode always runs the
+// full schedule).
+template <typename T, bool PROF = true>
+static int fw_dense(T* W, int64_t n, int64_t ld, int width, int is_uns, int mode, int tri, void* wsp,
                     int64_t ws_bytes, rbg_status* st, hipStream_t s) {
-  if (!W || !wsp || !st || n <= 0 || n % TILE || ld < n) return RBG_E_ARG;
+  if (!W || !wsp || !st || n <= 0 || n % TILE || ld < n || tri < 0 || tri > 2) return RBG_E_ARG;
   if (ws_bytes < FwWork::bytes(n, (int)sizeof(T))) return RBG_E_WORKSPACE;
   const bool fast = mode == 0;
   if (fast && (width != (int)sizeof(T) * 8 || is_uns)) return RBG_E_ARG;
+  if (!fast) tri = 0;
   FwWork ws{wsp, n, (int)sizeof(T)};
   const int nb = (int)(n / BLK);
   const int nt = (int)(n / TILE);
   const int conv_grid = (int)((n * n + NT * 8 - 1) / (NT * 8)) < 4096 ? (int)((n * n + NT * 8 - 1) / (NT * 8)) : 4096;
   if (fast) {
-    hipLaunchKernelGGL((k_e_convert<T, true>), dim3(conv_grid), dim3(NT), 0, s, W, n, ld);
+    hipLaunchKernelGGL((k_e_convert<T, true>), dim3(conv_grid), dim3(NT), 0, s, W, n, ld, tri ? st : nullptr);
     RBG_CHECK_LAUNCH();
   }
   const int panel_grid = nb > 1 ? 2 * (nb - 1) : 1;
   for (int K = 0; K < nb; K++) {
-    int p0 = prof_start(s);
+    int p0 = PROF ? prof_start(s) : -1;
     if (fast) {
       hipLaunchKernelGGL((k_fw_panel_fast<T>), dim3(nb - 1), dim3(PANEL_NT), 0, s, W, n, ld, (int64_t)K, ws, width,
                          st);
@@ -31,31 +40,38 @@ static int fw_dense(T* W, int64_t n, int64_t ld, int width, int is_uns, int mode
     }
     RBG_CHECK_LAUNCH();
     prof_stop(0, p0, s);
-    int p1 = prof_start(s);
-    bool small = false;
-    if constexpr (sizeof(T) == 4) {
-      if (fast && n <= SMALL_TILE_MAX_N) {
-        small = true;
-        const int ft = (int)(n / 64);
-        hipLaunchKernelGGL((k_fw_tile_fast_small<T>), dim3(ft * ft), dim3(NT), 0, s, W, n, ld, (int64_t)K, ws,
-                           width, st);
+    int p1 = PROF ? prof_start(s) : -1;
+    // relaxations the launch executes: its tiles' cells outside the pivot
+    // block's rows (the pivot tile row is in every rectangle), 32 steps each
+    int edge = TILE, count = nt * nt, kind = 1;
+    TileRect r{0, 0, nt};
+    if (fast) {
+      edge = FastTile<T>::EDGE;
+      r = tri_rect(tri, n, edge, K, &count);
+      if constexpr (sizeof(T) == 4) {
+        if (n <= SMALL_TILE_MAX_N || (tri && count < TRI_SMALL_BELOW)) {
+          edge = 64;
+          r = tri_rect(tri, n, edge, K, &count);
+          kind = 5;
+          hipLaunchKernelGGL((k_fw_tile_fast_small<T>), dim3(count), dim3(NT), 0, s, W, n, ld, (int64_t)K, ws,
+                             width, st, r);
+        }
       }
-    }
-    if (small) {
-    } else if (fast) {
-      const int ft = (int)(n / FastTile<T>::EDGE);
-      hipLaunchKernelGGL((k_fw_tile_fast<T>), dim3(ft * ft), dim3(NT), 0, s, W, n, ld, (int64_t)K, ws, width, st);
+      if (kind == 1)
+        hipLaunchKernelGGL((k_fw_tile_fast<T>), dim3(count), dim3(NT), 0, s, W, n, ld, (int64_t)K, ws, width, st,
+                           r);
     } else {
       hipLaunchKernelGGL((k_fw_tile_general<T>), dim3(nt * nt), dim3(NT), 0, s, W, n, ld, (int64_t)K, ws,
                          width, is_uns, st);
     }
     RBG_CHECK_LAUNCH();
-    prof_stop(1, p1, s);
+    if (PROF)
+      prof_stop(kind, p1, s, ((double)count * edge * edge - (double)BLK * r.jn * edge) * BLK);
   }
   if (fast) {
     hipLaunchKernelGGL((k_fw_capture_last<T, true>), dim3(1), dim3(1), 0, s, st, ws, (int64_t)(nb - 1), n);
     RBG_CHECK_LAUNCH();
-    hipLaunchKernelGGL((k_e_convert<T, false>), dim3(conv_grid), dim3(NT), 0, s, W, n, ld);
+    hipLaunchKernelGGL((k_e_convert<T, false>), dim3(conv_grid), dim3(NT), 0, s, W, n, ld, nullptr);
     RBG_CHECK_LAUNCH();
   } else {
     hipLaunchKernelGGL((k_fw_capture_last<T, false>), dim3(1), dim3(1), 0, s, st, ws, (int64_t)(nb - 1), n);
@@ -93,7 +109,10 @@ int rbg_prof_enable(int32_t on) {
   std::lock_guard<std::mutex> g(g_prof.mu);
   if (on == 1) {  // start afresh
     g_prof.used = 0;
-    for (int k = 0; k < PROF_KINDS; k++) g_prof.pairs[k].clear();
+    for (int k = 0; k < PROF_KINDS; k++) {
+      g_prof.pairs[k].clear();
+      g_prof.work[k].clear();
+    }
   }
   g_prof.on = on != 0;  // 2 resumes and 0 pauses, keeping what was recorded
   bool empty = true;
@@ -102,10 +121,10 @@ int rbg_prof_enable(int32_t on) {
   return 0;
 }
 
-int rbg_prof_read(int32_t kind, double* total_ms, int64_t* launches) {
+int rbg_prof_read_work(int32_t kind, double* total_ms, int64_t* launches, double* work) {
   if (kind < 0 || kind >= PROF_KINDS) return RBG_E_ARG;
   std::lock_guard<std::mutex> g(g_prof.mu);
-  double tot = 0.0;
+  double tot = 0.0, wk = 0.0;
   for (auto& pr : g_prof.pairs[kind]) {
     hipError_t e = hipEventSynchronize(g_prof.pool[pr.second]);
     if (e != hipSuccess) return (int)e;
@@ -114,13 +133,20 @@ int rbg_prof_read(int32_t kind, double* total_ms, int64_t* launches) {
     if (e != hipSuccess) return (int)e;
     tot += ms;
   }
+  for (double w : g_prof.work[kind]) wk += w;
   if (total_ms) *total_ms = tot;
   if (launches) *launches = (int64_t)g_prof.pairs[kind].size();
+  if (work) *work = wk;
   g_prof.pairs[kind].clear();
+  g_prof.work[kind].clear();
   bool empty = true;
   for (int k = 0; k < PROF_KINDS; k++) empty = empty && g_prof.pairs[k].empty();
   if (empty) g_prof.used = 0;
   return 0;
+}
+
+int rbg_prof_read(int32_t kind, double* total_ms, int64_t* launches) {
+  return rbg_prof_read_work(kind, total_ms, launches, nullptr);
 }
 
 int rbg_fw_prof_enable(int32_t on) { return rbg_prof_enable(on); }
@@ -140,13 +166,27 @@ int rbg_fw_precheck_i64(const int64_t* W, int64_t n, int64_t ld, rbg_status* st,
 int rbg_fw_dense_i32(int32_t* W, int64_t n, int64_t ld, int32_t width, int32_t is_unsigned, int32_t mode,
                      void* workspace, int64_t workspace_bytes, rbg_status* st, rbg_stream_t stream) {
   if (width != 8 && width != 16 && width != 32) return RBG_E_ARG;
-  return fw_dense<int32_t>(W, n, ld, width, is_unsigned, mode, workspace, workspace_bytes, st,
+  return fw_dense<int32_t>(W, n, ld, width, is_unsigned, mode, 0, workspace, workspace_bytes, st,
                            (hipStream_t)stream);
 }
 int rbg_fw_dense_i64(int64_t* W, int64_t n, int64_t ld, int32_t width, int32_t is_unsigned, int32_t mode,
                      void* workspace, int64_t workspace_bytes, rbg_status* st, rbg_stream_t stream) {
   if (width != 32 && width != 64) return RBG_E_ARG;
-  return fw_dense<int64_t>(W, n, ld, width, is_unsigned, mode, workspace, workspace_bytes, st,
+  return fw_dense<int64_t>(W, n, ld, width, is_unsigned, mode, 0, workspace, workspace_bytes, st,
+                           (hipStream_t)stream);
+}
+int rbg_fw_dense_tri_i32(int32_t* W, int64_t n, int64_t ld, int32_t width, int32_t is_unsigned, int32_t mode,
+                         int32_t tri, void* workspace, int64_t workspace_bytes, rbg_status* st,
+                         rbg_stream_t stream) {
+  if (width != 8 && width != 16 && width != 32) return RBG_E_ARG;
+  return fw_dense<int32_t>(W, n, ld, width, is_unsigned, mode, tri, workspace, workspace_bytes, st,
+                           (hipStream_t)stream);
+}
+int rbg_fw_dense_tri_i64(int64_t* W, int64_t n, int64_t ld, int32_t width, int32_t is_unsigned, int32_t mode,
+                         int32_t tri, void* workspace, int64_t workspace_bytes, rbg_status* st,
+                         rbg_stream_t stream) {
+  if (width != 32 && width != 64) return RBG_E_ARG;
+  return fw_dense<int64_t>(W, n, ld, width, is_unsigned, mode, tri, workspace, workspace_bytes, st,
                            (hipStream_t)stream);
 }
 

@@ -288,19 +288,28 @@ static int permute_csr(const int64_t* indptr, const int32_t* indices, const T* d
   if (e != hipSuccess) return (int)e;
   if (nnz == 0) return 0;
   const int gw = (int)((n + 3) / 4 < 8192 ? (n + 3) / 4 : 8192);
+  // Rows longer than PS_MAX go to rocPRIM's segmented sort from the staging
+  // buffer.  Their count is known only on the device, so the sort runs over
+  // the most there can be (each holds more than PS_MAX of the nnz entries),
+  // the unused segments cleared to empty ones: no host round trip, and the
+  // call stays capturable.
+  const int64_t nseg = nnz / (PS_MAX + 1) < n ? nnz / (PS_MAX + 1) : n;
   hipLaunchKernelGGL(k_ord_fill_i32, dim3(1), dim3(64), 0, s, long_count, 1, 0);
   RBG_CHECK_LAUNCH();
+  if (nseg > 0) {
+    const int gf = (int)((nseg + ORD_NT - 1) / ORD_NT < 4096 ? (nseg + ORD_NT - 1) / ORD_NT : 4096);
+    hipLaunchKernelGGL(k_ord_fill_i64, dim3(gf), dim3(ORD_NT), 0, s, long_begin, nseg, (int64_t)0);
+    RBG_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_ord_fill_i64, dim3(gf), dim3(ORD_NT), 0, s, long_end, nseg, (int64_t)0);
+    RBG_CHECK_LAUNCH();
+  }
   hipLaunchKernelGGL((k_permute_csr_fused<T>), dim3(gw), dim3(ORD_NT), 0, s, indptr, indices, data, n, p, pinv,
                      (const int64_t*)out_indptr, out_indices, out_data, tmp_idx, tmp_val, long_begin, long_end,
                      long_count);
   RBG_CHECK_LAUNCH();
-  int32_t nlong = 0;  // rows longer than PS_MAX: sorted by rocPRIM from the staging buffer
-  e = hipMemcpyAsync(&nlong, long_count, sizeof(nlong), hipMemcpyDeviceToHost, s);
-  if (e == hipSuccess) e = hipStreamSynchronize(s);
-  if (e != hipSuccess) return (int)e;
-  if (nlong == 0) return 0;
+  if (nseg == 0) return 0;
   e = rocprim::segmented_radix_sort_pairs(sort_tmp, sort_sz, (const int32_t*)tmp_idx, out_indices,
-                                          (const T*)tmp_val, out_data, (unsigned)nnz, (unsigned)nlong,
+                                          (const T*)tmp_val, out_data, (unsigned)nnz, (unsigned)nseg,
                                           (const int64_t*)long_begin, (const int64_t*)long_end, 0u, key_bits(n), s);
   return e == hipSuccess ? 0 : (int)e;
 }

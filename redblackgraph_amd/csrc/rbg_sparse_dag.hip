@@ -404,9 +404,10 @@ __device__ inline void level_slot(const int64_t* __restrict__ indptr, const int3
 // the list lengths in LDS), so every thread issues about (edges / 256)
 // decrements however the lists are spread; the vertices it frees are queued
 // in LDS, placed with one atomic, and given their slot data.
+// SLOTS = false (the topological sort) places the vertices only.
 constexpr int KR_NT = 256;
 constexpr int KR_QCAP = 4096;
-template <typename T>
+template <typename T, bool SLOTS = true>
 __global__ __launch_bounds__(KR_NT) void k_kahn_round(const int64_t* __restrict__ indptr,
                                                       const int32_t* __restrict__ indices, const T* __restrict__ data,
                                                       DagWork ws, int32_t r) {
@@ -416,8 +417,9 @@ __global__ __launch_bounds__(KR_NT) void k_kahn_round(const int64_t* __restrict_
   const int64_t L0 = ctr[C_LSTART];
   if (r == 0) {
     const int64_t Ls = ctr[C_PLACED] - L0;
-    for (int64_t t = (int64_t)blockIdx.x * KR_NT + tid; t < Ls; t += (int64_t)gridDim.x * KR_NT)
-      level_slot<T>(indptr, indices, data, ws, t, ws.order[L0 + t]);
+    if (SLOTS)
+      for (int64_t t = (int64_t)blockIdx.x * KR_NT + tid; t < Ls; t += (int64_t)gridDim.x * KR_NT)
+        level_slot<T>(indptr, indices, data, ws, t, ws.order[L0 + t]);
     return;
   }
   const int64_t f0 = ctr[C_LPREV], f1 = L0;
@@ -474,7 +476,7 @@ __global__ __launch_bounds__(KR_NT) void k_kahn_round(const int64_t* __restrict_
         } else {
           const int64_t g = (int64_t)atomicAdd((unsigned long long*)&ctr[C_PLACED], 1ull);
           ws.order[g] = p;
-          level_slot<T>(indptr, indices, data, ws, g - L0, p);
+          if (SLOTS) level_slot<T>(indptr, indices, data, ws, g - L0, p);
         }
       }
     }
@@ -485,7 +487,7 @@ __global__ __launch_bounds__(KR_NT) void k_kahn_round(const int64_t* __restrict_
     for (int j = tid; j < m; j += KR_NT) {
       const int32_t p = s_q[j];
       ws.order[s_base + j] = p;
-      level_slot<T>(indptr, indices, data, ws, s_base + j - L0, p);
+      if (SLOTS) level_slot<T>(indptr, indices, data, ws, s_base + j - L0, p);
     }
     __syncthreads();  // s_n, s_pre and s_q are reused by the next pass
   }
@@ -1542,6 +1544,48 @@ __global__ void k_dag_cycle_vertex(const int64_t* indptr, const int32_t* indices
   atomicOr(&st->flags, (uint32_t)RBG_F_CYCLE);
 }
 
+// Round r of the topological sort, last step (single lane): cycle and
+// completion tests, and the set-up of round r + 1.
+__global__ void k_topo_round_end(DagWork ws, int32_t r, int32_t n) {
+  int64_t* ctr = ws.counters;
+  if (dag_halted(ctr)) return;
+  const int64_t placed_now = ctr[C_PLACED] - ctr[C_LSTART];
+  if (placed_now > 0) ctr[C_NLEV] = r + 1;
+  if (ctr[C_PLACED] == n) ctr[C_DONE] = 1;
+  else if (placed_now == 0) ctr[C_STUCK] = 1;
+  ctr[C_LPREV] = ctr[C_LSTART];
+  ctr[C_LSTART] = ctr[C_PLACED];
+  ctr[C_ROUND] = r + 1;
+}
+
+// The sort's outputs: level[v], and the vertices by decreasing level (every
+// vertex before its successors; `order` holds them by increasing level).
+__global__ void k_topo_output(DagWork ws, int32_t n, int32_t* level_out, int32_t* order_out) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n || !ws.counters[C_DONE]) return;
+  if (level_out) level_out[i] = ws.level[i];
+  if (order_out) order_out[i] = ws.order[n - 1 - i];
+}
+
+// Entries below (aux[0]) and on (aux[1]) the diagonal of a CSR matrix.
+__global__ void k_csr_triangles(const int64_t* __restrict__ indptr, const int32_t* __restrict__ idx, int32_t n,
+                                rbg_status* st) {
+  int64_t below = 0, diag = 0;
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x)
+    for (int64_t e = indptr[r]; e < indptr[r + 1]; e++) {
+      below += idx[e] < r;
+      diag += idx[e] == r;
+    }
+  for (int o = 32; o > 0; o >>= 1) {
+    below += __shfl_xor(below, o);
+    diag += __shfl_xor(diag, o);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (below) atomicAdd((unsigned long long*)&st->aux[0], (unsigned long long)below);
+    if (diag) atomicAdd((unsigned long long*)&st->aux[1], (unsigned long long)diag);
+  }
+}
+
 __global__ void k_dag_report(DagWork ws, rbg_status* st) {
   const int64_t* c = ws.counters;
   if (c[C_CAP]) atomicOr(&st->flags, (uint32_t)RBG_F_CAPACITY);
@@ -2088,6 +2132,61 @@ int64_t rbg_dag_workspace_bytes(int32_t n, int64_t nnz) { return DagWork::bytes(
 
 RBG_DAG_ENTRY(i32, int32_t)
 RBG_DAG_ENTRY(i64, int64_t)
+
+int rbg_topological_sort(const int64_t* indptr, const int32_t* indices, int32_t n, void* workspace,
+                         int64_t ws_bytes, int32_t* level_out, int32_t* order_out, rbg_status* st,
+                         rbg_stream_t stream) {
+  hipStream_t s = (hipStream_t)stream;
+  if (n <= 0 || !indptr || !indices || !workspace || !st) return RBG_E_ARG;
+  if (ws_bytes < DagWork::bytes(n)) return RBG_E_WORKSPACE;
+  DagWork ws = DagWork::carve(workspace, n);
+  ws.ecap = ((char*)workspace + ws_bytes - (char*)ws.preds) / 4;
+  const int gv = (int)((n + 255) / 256 < 2048 ? (n + 255) / 256 : 2048);
+  hipLaunchKernelGGL(k_dag_init, dim3((n + 255) / 256), dim3(256), 0, s, ws, n);
+  RBG_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_kahn_count, dim3(gv), dim3(256), 0, s, indptr, indices, ws, n);
+  RBG_CHECK_LAUNCH();
+  int rc = scan_exclusive(ws.cnt, ws.pptr, n, ws.sums_a, ws.counters + C_NNZ, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_set_last, dim3(1), dim3(1), 0, s, ws.pptr, (int64_t)n, (const int64_t*)(ws.counters + C_NNZ));
+  RBG_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_kahn_fill, dim3(gv), dim3(256), 0, s, indptr, indices, ws, n);
+  RBG_CHECK_LAUNCH();
+  int64_t host[C_NCTR];
+  for (int32_t r = 0; r <= n; r++) {
+    if (r > 0) {
+      hipLaunchKernelGGL((k_kahn_round<int32_t, false>), dim3(gv), dim3(KR_NT), 0, s, indptr, indices,
+                         (const int32_t*)nullptr, ws, r);
+      RBG_CHECK_LAUNCH();
+    }
+    hipLaunchKernelGGL(k_topo_round_end, dim3(1), dim3(1), 0, s, ws, r, n);
+    RBG_CHECK_LAUNCH();
+    if ((r + 1) % 8 == 0 || r == n) {
+      hipError_t e = hipMemcpyAsync(host, ws.counters, sizeof(host), hipMemcpyDeviceToHost, s);
+      if (e != hipSuccess) return (int)e;
+      e = hipStreamSynchronize(s);
+      if (e != hipSuccess) return (int)e;
+      if (host[C_DONE] || host[C_STUCK] || host[C_CAP]) break;
+    }
+  }
+  hipLaunchKernelGGL(k_dag_report, dim3(1), dim3(1), 0, s, ws, st);
+  RBG_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_dag_cycle_vertex, dim3(1), dim3(1), 0, s, indptr, indices, ws, n, ws.rem[0], st);
+  RBG_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_topo_output, dim3((n + 255) / 256), dim3(256), 0, s, ws, n, level_out, order_out);
+  RBG_CHECK_LAUNCH();
+  return 0;
+}
+
+int rbg_csr_triangles(const int64_t* indptr, const int32_t* indices, int32_t n, rbg_status* st,
+                      rbg_stream_t stream) {
+  if (n < 0 || !st || (n > 0 && (!indptr || !indices))) return RBG_E_ARG;
+  if (n == 0) return 0;
+  const int grid = (int)((n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024);
+  hipLaunchKernelGGL(k_csr_triangles, dim3(grid), dim3(256), 0, (hipStream_t)stream, indptr, indices, n, st);
+  RBG_CHECK_LAUNCH();
+  return 0;
+}
 
 int rbg_dag_level_kinds(void* workspace, int32_t n, uint8_t* host_dst, int32_t count, rbg_stream_t stream) {
   if (n <= 0 || !workspace || !host_dst || count < 0 || count > n) return RBG_E_ARG;

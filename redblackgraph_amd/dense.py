@@ -42,6 +42,7 @@ class FwOutcome:
     status: _lib.RbgStatus
     npad: int
     mode: str  # "fast" | "general"
+    schedule: str = "full"  # "full" | "lower" | "upper" (triangular schedules)
 
     @property
     def diameter(self) -> int:
@@ -52,13 +53,13 @@ class FwOutcome:
         return key // (n * n), (key // n) % n, key % n
 
 
-def _run(work, et: ElemType, mode: int, st: _lib.Status, stream):
+def _run(work, et: ElemType, mode: int, st: _lib.Status, stream, tri: int = 0):
     t = _lib.torch()
     npad = work.shape[0]
     elem = work.element_size()
     ws = _workspace(npad, elem)
-    fn = "rbg_fw_dense_i32" if work.dtype == t.int32 else "rbg_fw_dense_i64"
-    _lib.call(fn, _lib.ptr(work), npad, npad, et.width, int(et.unsigned), mode,
+    fn = "rbg_fw_dense_tri_i32" if work.dtype == t.int32 else "rbg_fw_dense_tri_i64"
+    _lib.call(fn, _lib.ptr(work), npad, npad, et.width, int(et.unsigned), mode, tri,
               _lib.ptr(ws), ws.numel(), st.p, _lib.stream_handle(stream))
 
 
@@ -71,13 +72,21 @@ def status() -> _lib.Status:
     return st
 
 
-def fw_closure(W, et: ElemType, stream=None, force_general: bool = False):
+_TRI = {"full": 0, "lower": 1, "upper": 2}
+
+
+def fw_closure(W, et: ElemType, stream=None, force_general: bool = False, schedule: str = "auto"):
     """Close an n x n device matrix.  Returns (closed n x n tensor, FwOutcome).
 
     The input tensor is not modified.  Runs the fast kernels when the input is
     inside their domain (rbg_fw_precheck) and falls back to the general kernels
-    otherwise -- both on the GPU.
+    otherwise -- both on the GPU.  schedule "auto" takes the triangular
+    schedule when the precheck finds every off-diagonal entry on one side of
+    the diagonal (a pedigree numbered parents first is lower triangular), else
+    the full one; "full" forces the full schedule (the results are the same).
     """
+    if schedule not in ("auto", "full"):
+        raise ValueError(f"unknown schedule {schedule!r}")
     t = _lib.torch()
     n = W.shape[0]
     npad = padded_order(n)
@@ -93,14 +102,22 @@ def fw_closure(W, et: ElemType, stream=None, force_general: bool = False):
     st = status()
     st.reset(stream)
     fast = et.native and not force_general
+    sched = "full"
     if fast:
         fn = "rbg_fw_precheck_i32" if work.dtype == t.int32 else "rbg_fw_precheck_i64"
         _lib.call(fn, _lib.ptr(work), npad, npad, st.p, _lib.stream_handle(stream))
         s = st.read(stream)
         if s.flags & _lib.F_PRECOND_FAIL:
             fast = False
+        elif schedule == "auto":
+            if not s.flags & _lib.F_HAS_UPPER:
+                sched = "lower"
+            elif not s.flags & _lib.F_HAS_LOWER:
+                sched = "upper"
         st.reset(stream)
-    _run(work, et, 0 if fast else 1, st, stream)
+    if not fast:
+        sched = "full"
+    _run(work, et, 0 if fast else 1, st, stream, _TRI[sched])
     s = st.read(stream)
     mode = "fast" if fast else "general"
     # A cycle makes a diagonal value >= 2, after which products through the
@@ -112,7 +129,8 @@ def fw_closure(W, et: ElemType, stream=None, force_general: bool = False):
         _run(work, et, 1, st, stream)
         s = st.read(stream)
         mode = "general"
-    return work[:n, :n], FwOutcome(s, npad, mode)
+        sched = "full"
+    return work[:n, :n], FwOutcome(s, npad, mode, sched)
 
 
 def matmul(A, B, et: ElemType, stream=None):

@@ -139,6 +139,20 @@ def batched_workload(count: int = 4096, lo: int = 16, hi: int = 4096, seed: int 
     return sizes, diag, edges
 
 
+def shuffle_vertices(A: np.ndarray, seed: int) -> np.ndarray:
+    """The same graph with its vertices renumbered by a seeded permutation p
+    (PCG64): B[i][j] = A[p[i]][p[j]].  A pedigree so renumbered is neither
+    lower nor upper triangular."""
+    p = np.random.Generator(np.random.PCG64(seed)).permutation(A.shape[0])
+    return np.ascontiguousarray(A[np.ix_(p, p)])
+
+
+def reverse_vertices(A: np.ndarray) -> np.ndarray:
+    """The same graph numbered backwards (vertex i becomes n - 1 - i): a lower
+    triangular pedigree becomes upper triangular."""
+    return np.ascontiguousarray(A[::-1, ::-1])
+
+
 def random_upper_dag(n: int, seed: int, hi: int = 4) -> np.ndarray:
     """The reference's random DAG construction (tests/sparse/test_dag_sparse_closure.py:77-83):
     legacy RandomState(seed), upper triangle of randint(0, hi), random +-1 diagonal."""

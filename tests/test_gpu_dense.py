@@ -213,6 +213,102 @@ def test_fw_1024_vs_oracle(orc):
     assert out.diameter == ref.diameter
 
 
+@pytest.mark.parametrize("dt,n,reverse", [(np.int32, 200, False), (np.int32, 512, True), (np.int32, 1100, False),
+                                          (np.int32, 1100, True), (np.int64, 300, False), (np.int64, 700, True)])
+def test_fw_triangular_schedule_vs_oracle(orc, dt, n, reverse):
+    """Lower (parents first) and upper (numbered backwards) triangular
+    pedigrees take the triangular schedule; W and the trajectory diameter
+    equal the oracle's naive loop and the full schedule's."""
+    A = synth.dense_pedigree(synth.component_spec(n, 0.8, 4), 17 + n, dtype=dt)
+    if reverse:
+        A = synth.reverse_vertices(A)
+    ref = orc.warshall(A)
+    W, out = _fw(A)
+    assert out.mode == "fast" and out.schedule == ("upper" if reverse else "lower")
+    np.testing.assert_array_equal(W, ref.W)
+    assert out.diameter == ref.diameter
+    W2, out2 = dense.fw_closure(_lib.to_device(A), elem_type(A.dtype), schedule="full")
+    assert out2.schedule == "full"
+    np.testing.assert_array_equal(W2.cpu().numpy(), ref.W)
+    assert out2.diameter == ref.diameter
+
+
+def _digests():
+    import json
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "fw_digests.json")
+    with open(path) as f:
+        return json.load(f)
+
+
+def _workload(name):
+    """The matrix a digest in tests/golden/fw_digests.json was made from."""
+    import importlib.util
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "make_fw_digests.py")
+    spec = importlib.util.spec_from_file_location("make_fw_digests", path)
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod.workload_matrix(name), mod.digest
+
+
+def _check_digest(name, W, out):
+    import hashlib
+    ref = _digests()[name]
+    assert np.dtype(W.dtype).name == ref["dtype"]
+    got = hashlib.sha256(np.ascontiguousarray(W).tobytes()).hexdigest()
+    assert int(np.count_nonzero(W)) == ref["nnz_W"]
+    assert int(W.max()) == ref["max_W"]
+    assert got == ref["sha256_W"]
+    assert out.diameter == ref["diameter"]
+    assert out.status.cycle_count == 0 and out.status.events == 0
+
+
+@pytest.mark.parametrize("name,schedule", [
+    ("dense2304_i32", "lower"),            # parents first: lower triangular
+    ("dense2304_reversed_i32", "upper"),   # numbered backwards: upper triangular
+    ("dense2304_shuffled_i32", "full"),    # renumbered at random: neither
+])
+def test_fw_2304_vs_oracle(name, schedule):
+    """N=2304 int32 (above 2048, so phase 3 runs the 128 x 128 tile kernel
+    k_fw_tile_fast<int> of the N=8192 workload) against the oracle's naive
+    k-i-j loop, run in full when the digest was made: W bit-exact (sha256)
+    and the trajectory diameter (warshall.c.in:89,106).  Each input also runs
+    the full schedule, and the first one general mode."""
+    A, _ = _workload(name)
+    W, out = _fw(A)
+    assert out.mode == "fast" and out.schedule == schedule
+    _check_digest(name, W, out)
+    W2, out2 = dense.fw_closure(_lib.to_device(A), elem_type(A.dtype), schedule="full")
+    assert out2.schedule == "full"
+    _check_digest(name, W2[:A.shape[0], :A.shape[0]].cpu().numpy(), out2)
+    if name == "dense2304_i32":
+        Wg, og = _fw(A, force_general=True)
+        assert og.mode == "general"
+        _check_digest(name, Wg, og)
+
+
+@pytest.mark.parametrize("name", ["dense8192_i32", "dense4096_i64"])
+def test_fw_full_size_vs_oracle(name):
+    """The full-size dense workloads (N=8192 int32, N=4096 int64) against the
+    oracle's naive loop run once in full on the CPU: W digest and trajectory
+    diameter, for the triangular schedule the closure takes, the full
+    schedule, and general mode (the full arithmetic)."""
+    if name not in _digests():
+        pytest.skip(f"oracle digest {name} not generated")
+    A, _ = _workload(name)
+    W, out = _fw(A)
+    assert out.mode == "fast" and out.schedule == "lower"
+    _check_digest(name, W, out)
+    del W
+    W2, out2 = dense.fw_closure(_lib.to_device(A), elem_type(A.dtype), schedule="full")
+    _check_digest(name, W2.cpu().numpy(), out2)
+    del W2
+    Wg, og = _fw(A, force_general=True)
+    assert og.mode == "general"
+    _check_digest(name, Wg, og)
+
+
 def test_fw_8192_properties():
     """Full-size workload: fast vs general arithmetic agree, the closure is a
     fixed point (closing it again changes nothing), lower-triangular input

@@ -119,3 +119,36 @@ def test_canonical_order_device_matches_host_entry(orc):
     np.testing.assert_array_equal(Bs.indptr, R.indptr)
     np.testing.assert_array_equal(Bs.indices, R.indices)
     np.testing.assert_array_equal(Bs.data, R.data)
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.int64])
+def test_permute_sparse_long_rows(dtype):
+    """Rows of more than 256 entries take the staging buffer and the
+    segmented sort; short rows among them take the one-pass ranking.  Both
+    against scipy's A[p][:, p] with sorted indices, entry for entry."""
+    rng = np.random.default_rng(21)
+    n = 6000
+    A = sp.random(n, n, density=0.002, format="lil", random_state=7,
+                  data_rvs=lambda k: rng.integers(2, 1 << 20, k))
+    for r, length in ((11, 257), (900, 1024), (4000, 5000), (5999, 300)):
+        cols = rng.choice(n, size=length, replace=False)
+        A[r, cols] = rng.integers(2, 1 << 20, length)
+    A = sp.csr_matrix(A, dtype=dtype)
+    A.sum_duplicates()
+    lens = np.diff(A.indptr)
+    assert lens.max() == 5000 and (lens > 256).sum() >= 4 and (lens <= 256).sum() > n - 10
+    p = rng.permutation(n).astype(np.int32)
+    B = _cg().permute_sparse(A, p)
+    ref = A[p][:, p].tocsr()
+    ref.sort_indices()
+    assert B.has_sorted_indices
+    np.testing.assert_array_equal(B.indptr, ref.indptr)
+    np.testing.assert_array_equal(B.indices, ref.indices)
+    np.testing.assert_array_equal(B.data, ref.data.astype(dtype))
+    # the triangular filter on the same input
+    Bu = _cg().permute_sparse(A, p, assume_upper_triangular=True)
+    refu = sp.triu(ref).tocsr()
+    refu.sort_indices()
+    np.testing.assert_array_equal(Bu.indptr, refu.indptr)
+    np.testing.assert_array_equal(Bu.indices, refu.indices)
+    np.testing.assert_array_equal(Bu.data, refu.data.astype(dtype))

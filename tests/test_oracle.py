@@ -197,6 +197,62 @@ def test_dag_ahnentafel(orc, seed):
     np.testing.assert_array_equal(r.toarray(40), ahnentafel_closure(A))
 
 
+def _candidate_max_by_ahnentafel(A: np.ndarray) -> int:
+    """max |candidate| of the DAG closure, from the pedigree's meaning: the
+    direct entries, and for each child -> parent edge (v, w) of value a and
+    every ancestor x of w with Ahnentafel number y = C[w][x] (C from the BFS
+    closure), the number of x seen from v through w: ((a - 1) << MSB(y)) + y
+    for y >= 2 (a prefix of the path digits), and a itself for x = w (the
+    parity filter keeps a: fathers are red, mothers black)."""
+    C = ahnentafel_closure(A)
+    best = int(np.abs(A).max()) if A.size else 0
+    n = A.shape[0]
+    for v in range(n):
+        for w in np.nonzero(A[v])[0]:
+            if w == v:
+                continue
+            a = int(A[v, w])
+            for x in np.nonzero(C[w])[0]:
+                y = int(C[w, x])
+                if x == w:
+                    cand = a
+                else:
+                    cand = ((a - 1) << (y.bit_length() - 1)) + y
+                best = max(best, cand)
+    return best
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_dag_candidate_max_pinned(orc, seed):
+    """The Cython DAG closure's diameter rule (_transitive_closure_dag.pyx:
+    261-265, 291-295, 347-349): MSB of the largest |candidate| over direct
+    edges and every nonzero A[v][w] (x) C[w][x].  Checked against the
+    candidates enumerated from the Ahnentafel closure on pedigrees with
+    collapse (several paths to one ancestor), where the largest candidate is
+    a longer path's number and exceeds the largest final value."""
+    rng = np.random.default_rng(900 + seed)
+    A = random_valid_pedigree(48, rng, p=0.9, lookback=3)
+    r = orc.dag_closure(sp.csr_matrix(A), 64)
+    exp = _candidate_max_by_ahnentafel(A)
+    assert r.cand_max == exp
+    assert r.final_max == int(ahnentafel_closure(A).max())
+
+
+def test_dag_candidate_max_differs_from_final():
+    """At least one seeded case separates the two rules in the diameter
+    itself, so a closure that reports MSB(max final) fails the pin above."""
+    diffs = 0
+    for seed in range(6):
+        rng = np.random.default_rng(900 + seed)
+        A = random_valid_pedigree(48, rng, p=0.9, lookback=3)
+        cand = _candidate_max_by_ahnentafel(A)
+        fin = int(ahnentafel_closure(A).max())
+        assert cand >= fin
+        if cand.bit_length() != fin.bit_length():
+            diffs += 1
+    assert diffs > 0
+
+
 def test_topological_sort(orc):
     from redblackgraph_amd.synth import random_upper_dag
     A = sp.csr_matrix(random_upper_dag(30, 5))
