@@ -144,6 +144,24 @@ int rbg_fw_dense_tri_i64(int64_t *W, int64_t n, int64_t ld, int32_t width, int32
                          int32_t tri, void *workspace, int64_t workspace_bytes, rbg_status *st,
                          rbg_stream_t stream);
 
+/* No counterpart in the reference: rbg_fw_dense_* for callers that cannot
+ * allocate device memory (code under hipGraph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The snapshot workspace lives in a
+ * file-scope device array of rbg_fw_noalloc_max_bytes() bytes (8.5 MiB, held
+ * by each GPU the library runs on until the process ends), so these calls
+ * allocate nothing; the caller passes W and st.  Orders whose workspace
+ * (rbg_fw_workspace_bytes) exceeds the array -- int32 n > 17024, int64
+ * n > 8576 -- return RBG_E_WORKSPACE before anything runs.  Every call on a
+ * GPU shares that GPU's array: calls that overlap on one GPU (other streams,
+ * other threads) race and must be serialised by the caller.  Nothing in the
+ * array needs to start cleared.  Results and status are those of
+ * rbg_fw_dense_* with the same arguments (full schedule). */
+int64_t rbg_fw_noalloc_max_bytes(void);
+int rbg_fw_dense_noalloc_i32(int32_t *W, int64_t n, int64_t ld, int32_t width, int32_t is_unsigned, int32_t mode,
+                             rbg_status *st, rbg_stream_t stream);
+int rbg_fw_dense_noalloc_i64(int64_t *W, int64_t n, int64_t ld, int32_t width, int32_t is_unsigned, int32_t mode,
+                             rbg_status *st, rbg_stream_t stream);
+
 /* ---- dense AVOS product (einsum 'ij,jk', redblack.c.src:554-700) ----------- */
 /* C[M x N] = A[M x K] (x) B[K x N]; row-major with leading dimensions. */
 int rbg_matmul_i32(const int32_t *A, int64_t lda, const int32_t *B, int64_t ldb, int32_t *C, int64_t ldc,

@@ -65,6 +65,9 @@ SIGNATURES = {
     "rbg_fw_dense_i64": (I32, [P, I64, I64, I32, I32, I32, P, I64, P, P]),
     "rbg_fw_dense_tri_i32": (I32, [P, I64, I64, I32, I32, I32, I32, P, I64, P, P]),
     "rbg_fw_dense_tri_i64": (I32, [P, I64, I64, I32, I32, I32, I32, P, I64, P, P]),
+    "rbg_fw_noalloc_max_bytes": (I64, []),
+    "rbg_fw_dense_noalloc_i32": (I32, [P, I64, I64, I32, I32, I32, P, P]),
+    "rbg_fw_dense_noalloc_i64": (I32, [P, I64, I64, I32, I32, I32, P, P]),
     "rbg_matmul_i32": (I32, [P, I64, P, I64, P, I64, I64, I64, I64, I32, I32, P, P]),
     "rbg_matmul_i64": (I32, [P, I64, P, I64, P, I64, I64, I64, I64, I32, I32, P, P]),
     "rbg_edge_compose_i32": (I32, [P, I64, I64, I64, I32, P, I32, I32, P, P]),

@@ -90,6 +90,50 @@ static int fw_precheck(const T* W, int64_t n, int64_t ld, rbg_status* st, hipStr
   return 0;
 }
 
+// ---- allocation-free closure ----------------------------------------------
+// The snapshot workspace of rbg_fw_dense_noalloc_* lives in this file-scope
+// device array.  Host code never passes `scratch` itself to a kernel (on the
+// host the name is only a stand-in for the symbol): each device's address of
+// its own copy is looked up once, with that device current, and kept.
+constexpr int64_t NOALLOC_SCRATCH_BYTES = 8912896;  // 8.5 MiB
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static void* g_scratch_ptr[NOALLOC_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+// This device's address of `scratch`, or an error code.
+static int scratch_address(void** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return (int)e;
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return RBG_E_ARG;
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return (int)e;
+    if (!p) return (int)hipErrorInvalidSymbol;
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+template <typename T>
+static int fw_dense_noalloc(T* W, int64_t n, int64_t ld, int width, int is_uns, int mode, rbg_status* st,
+                            hipStream_t s) {
+  if (!W || !st || n <= 0 || n % TILE || ld < n) return RBG_E_ARG;
+  const int64_t need = FwWork::bytes(n, (int)sizeof(T));
+  if (need > NOALLOC_SCRATCH_BYTES) return RBG_E_WORKSPACE;
+  void* ws = nullptr;
+  int rc = scratch_address(&ws);
+  if (rc) return rc;
+  // every snapshot and guard word is written by the panel kernel of a step
+  // before that step (or the next) reads it, so nothing needs clearing
+  return fw_dense<T, false>(W, n, ld, width, is_uns, mode, 0, ws, NOALLOC_SCRATCH_BYTES, st, s);
+}
+
 }  // namespace rbg
 
 using namespace rbg;
@@ -188,6 +232,18 @@ int rbg_fw_dense_tri_i64(int64_t* W, int64_t n, int64_t ld, int32_t width, int32
   if (width != 32 && width != 64) return RBG_E_ARG;
   return fw_dense<int64_t>(W, n, ld, width, is_unsigned, mode, tri, workspace, workspace_bytes, st,
                            (hipStream_t)stream);
+}
+
+int64_t rbg_fw_noalloc_max_bytes(void) { return NOALLOC_SCRATCH_BYTES; }
+int rbg_fw_dense_noalloc_i32(int32_t* W, int64_t n, int64_t ld, int32_t width, int32_t is_unsigned, int32_t mode,
+                             rbg_status* st, rbg_stream_t stream) {
+  if (width != 8 && width != 16 && width != 32) return RBG_E_ARG;
+  return fw_dense_noalloc<int32_t>(W, n, ld, width, is_unsigned, mode, st, (hipStream_t)stream);
+}
+int rbg_fw_dense_noalloc_i64(int64_t* W, int64_t n, int64_t ld, int32_t width, int32_t is_unsigned, int32_t mode,
+                             rbg_status* st, rbg_stream_t stream) {
+  if (width != 32 && width != 64) return RBG_E_ARG;
+  return fw_dense_noalloc<int64_t>(W, n, ld, width, is_unsigned, mode, st, (hipStream_t)stream);
 }
 
 }  // extern "C"
