@@ -405,7 +405,7 @@ def bench_batched(args):
     t = _lib.torch()
     sizes, diag, edges = synth.batched_workload(count=args.batch_count, seed=args.seed)
     t_tune0 = time.perf_counter()
-    b = BatchedClosure(sizes)
+    b = BatchedClosure(sizes, tri=1)  # pedigrees numbered parents first: lower triangular
     tune_s = time.perf_counter() - t_tune0
     b.load_edges(diag, edges)
     pristine = b.pool.clone()
@@ -416,8 +416,7 @@ def bench_batched(args):
     L.rbg_prof_enable(1)
     b.run(graph=False)
     b.stream.synchronize()
-    ms_, nl = ctypes.c_double(), ctypes.c_int64()
-    _lib.call("rbg_prof_read", 4, ctypes.byref(ms_), ctypes.byref(nl))
+    tile_ms, tile_launches, relax_exec = _prof_read(4)
     L.rbg_prof_enable(0)
     outs = b.outcomes()
     if any(o.mode != "fast" or o.status.cycle_count or o.status.events for o in outs):
@@ -445,9 +444,7 @@ def bench_batched(args):
         identical_plain = identical_single = None
     else:
         plain_ms, identical_plain, single_ms, identical_single = _batched_plain_legs(b, sizes, pristine, eager_result)
-    P = b.layout.npad.astype(np.float64)
-    relax_exec = float(np.sum((P - FW_BLK) ** 2 * P))
-    tile_s = ms_.value / 1e3
+    tile_s = tile_ms / 1e3
     achieved = relax_exec * OPS_PER_RELAX["int32"] / tile_s
     entries = float(np.sum(sizes.astype(np.float64) ** 2))
     roof = {
@@ -455,8 +452,9 @@ def bench_batched(args):
         "achieved": achieved / 1e12, "peak": PEAK_VALU_OPS / 1e12, "unit": "TOP/s",
         "frac": achieved / PEAK_VALU_OPS, "traffic": _read_traffic("bfw_tile_batched4096_pmc.txt"),
         "ops_per_relaxation": OPS_PER_RELAX["int32"], "relaxations_executed": relax_exec,
-        "tile_ms_per_batch": ms_.value, "tile_launches": nl.value,
-        "kernel_share_of_step": ms_.value / ms,
+        "schedule": "lower triangular (the tiles each pivot block can change)",
+        "tile_ms_per_batch": tile_ms, "tile_launches": tile_launches,
+        "kernel_share_of_step": tile_ms / ms,
     }
     res = {
         "metric": METRIC, "value": entries / (ms / 1e3), "unit": "entries/s",
@@ -667,6 +665,82 @@ def bench_order(args):
     return res
 
 
+def cpu_baseline_matmul(n_sample: int, n_target: int):
+    """Single-core 'ij,jk' loop (the oracle's restatement of redblack.c.src:
+    554-700) on the closure of an n_sample pedigree, N^3-scaled."""
+    from oracle import oracle
+    from redblackgraph_amd import synth
+    A = synth.dense_pedigree(synth.PedigreeSpec(n_sample, max(4, round(24 * n_sample / 8192)), 0.75, 4), 1)
+    W = oracle.warshall(A).W
+    with _one_core() as pin:
+        t0 = time.perf_counter()
+        oracle.matmul(W, W)
+        dt = time.perf_counter() - t0
+    rate = n_sample ** 3 / dt
+    return {"value": rate / n_target, "unit": "entries/s", "cores": 1, "kind": "port",
+            "sample": (f"oracle 'ij,jk' product loop (C restatement of redblack.c.src:554-700, gcc -O2, 1 thread) "
+                       f"on the closure of a seeded N={n_sample} pedigree, squared: {dt:.2f} s = {rate:.3e} terms/s; "
+                       f"value = that rate / {n_target} (N^3 scaling); {pin.host()}")}
+
+
+def bench_matmul(args):
+    """Dense AVOS product W @ W of the N=8192 int32 closure (the rb.array @
+    path, redblack.c.src:554-700): the fast kernel timed with events on its
+    stream; checked against the general kernel at full size."""
+    import numpy as np
+    from redblackgraph_amd import _lib, dense, synth
+    from redblackgraph_amd._dtypes import elem_type
+    t = _lib.torch()
+    n = 8192
+    A = synth.dense_pedigree(synth.DENSE_8192, args.seed)
+    et = elem_type(np.int32)
+    W, _ = dense.fw_closure(_lib.to_device(A), et)
+    W = W.contiguous()
+    for _ in range(max(1, args.warmup)):
+        C, s = dense.matmul(W, W, et)
+    if s.flags & _lib.F_NEED_GENERAL:
+        raise RuntimeError("the fast product fell back to the general kernel")
+    t.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        C, s = dense.matmul(W, W, et)
+    t.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    # the product kernel alone, with events on its stream (the current stream)
+    ws = t.empty(int(_lib.load().rbg_matmul_fast_workspace_bytes(n, n, n, 4)), dtype=t.uint8, device=W.device)
+    st = dense.status()
+    sh = _lib.stream_handle(None)
+    e0, e1 = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        _lib.call("rbg_matmul_fast_i32", _lib.ptr(W), n, _lib.ptr(W), n, _lib.ptr(C), n, n, n, n, _lib.ptr(ws),
+                  ws.numel(), st.p, sh)
+    e1.record()
+    t.cuda.synchronize()
+    kernel_ms = e0.elapsed_time(e1) / args.steps
+    # full-size parity: the general kernel's product (full arithmetic)
+    Cg, sg = dense.matmul(W, W, et, force_general=True)
+    identical = bool(t.equal(C, Cg))
+    if not identical or sg.events:
+        raise RuntimeError("fast and general products differ")
+    terms = float(n) ** 3
+    achieved = terms * OPS_PER_RELAX["int32"] / (kernel_ms / 1e3)
+    res = {
+        "metric": "dense AVOS product entries/sec (W @ W of a closure)", "value": n * n / (ms / 1e3),
+        "unit": "entries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+        "config": {"workload": f"W @ W, W the closure of the seeded N=8192 int32 pedigree, seed {args.seed}",
+                   "n": n, "identical_to_general_kernel": identical, "kernel_ms": kernel_ms},
+        "roofline": {"bound": "valu", "kernel": "k_mm_fast<int, 8> (phase-3 tile over all k)",
+                     "achieved": achieved / 1e12, "peak": PEAK_VALU_OPS / 1e12, "unit": "TOP/s",
+                     "frac": achieved / PEAK_VALU_OPS, "traffic": _read_traffic("mm_fast_matmul8192_pmc.txt"),
+                     "ops_per_term": OPS_PER_RELAX["int32"], "terms_per_launch": terms, "avg_launch_ms": kernel_ms},
+    }
+    if not args.no_cpu_baseline:
+        res["cpu_baseline"] = cpu_baseline_matmul(1024, n)
+    return res
+
+
 def bench_hbmcopy(args):
     """A plain device-to-device copy of 1 GiB (reads 1 GiB, writes 1 GiB):
     the calibration run for the HBM byte counters (FETCH_SIZE, WRITE_SIZE)."""
@@ -730,7 +804,7 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--workload", default="all",
                     choices=["all", "dense8192", "dense4096_i64", "dense512", "sparse1m", "batched", "serve",
-                             "order1m", "hbmcopy"])
+                             "order1m", "matmul8192", "hbmcopy"])
     ap.add_argument("--serve-count", type=int, default=300)
     ap.add_argument("--serve-rate", type=float, default=150.0)
     ap.add_argument("--serve-slots", type=int, default=4)
@@ -757,6 +831,8 @@ def main():
         res = bench_sparse(args)
     elif args.workload == "hbmcopy":
         res = bench_hbmcopy(args)
+    elif args.workload == "matmul8192":
+        res = bench_matmul(args)
     elif args.workload == "order1m":
         res = bench_order(args)
     elif args.workload == "serve":

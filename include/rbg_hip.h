@@ -107,8 +107,8 @@ int64_t rbg_fw_workspace_bytes(int64_t n, int32_t elem_bytes);
  * stream.  rbg_fw_prof_read synchronises those events and returns the summed
  * durations (ms) and launch counts since the last read, then clears them. */
 int rbg_fw_prof_enable(int32_t on);
-/* Same facility for every timed launch kind: 0 FW panel, 1 FW tile, 5 FW small tile, 2 DAG
- * level, 3 SpGEMM rows, 4 batched FW.  on = 1 starts recording afresh, 2
+/* Same facility for every timed launch kind: 0 FW panel, 1 FW tile, 5 FW small tile,
+ * 2 DAG level, 3 SpGEMM rows, 4 batched FW.  on = 1 starts recording afresh, 2
  * resumes and 0 pauses, both keeping what was recorded (so a caller can time
  * a sample of its launches).  rbg_prof_read returns (and clears) the summed
  * duration and count of one kind. */
@@ -135,8 +135,9 @@ int rbg_fw_dense_i64(int64_t *W, int64_t n, int64_t ld, int32_t width, int32_t i
  * status identical to the full schedule's.  rbg_fw_precheck_* reports the
  * side(s) holding entries (RBG_F_HAS_LOWER / RBG_F_HAS_UPPER).  Fast mode
  * only: with mode = 1 the full schedule runs whatever tri says.  Unlike the
- * reference's loop, which reads only the assumed triangle, a wrong tri gives
- * an unspecified W. */
+ * reference's loop, which reads only the assumed triangle, an entry on the
+ * other side is detected: the fast run stops with RBG_F_NEED_GENERAL, and the
+ * caller reruns mode 1 on the original input, as for any fast-mode miss. */
 int rbg_fw_dense_tri_i32(int32_t *W, int64_t n, int64_t ld, int32_t width, int32_t is_unsigned, int32_t mode,
                          int32_t tri, void *workspace, int64_t workspace_bytes, rbg_status *st,
                          rbg_stream_t stream);
@@ -170,6 +171,28 @@ int rbg_matmul_i32(const int32_t *A, int64_t lda, const int32_t *B, int64_t ldb,
 int rbg_matmul_i64(const int64_t *A, int64_t lda, const int64_t *B, int64_t ldb, int64_t *C, int64_t ldc,
                    int64_t M, int64_t K, int64_t N, int32_t width, int32_t is_unsigned,
                    rbg_status *st, rbg_stream_t stream);
+/* The same product for operands in the fast domain (the replacement of the
+ * 'ij,jk' loop for closures): every value >= -1, no off-diagonal +-1, every
+ * diagonal value in {-1, 0, 1}, in A and in B; rbg_matmul_precheck_* sets
+ * RBG_F_PRECOND_FAIL for an operand outside it.  The terms of two
+ * off-diagonal entries run in the E domain on the phase-3 tile of the dense
+ * closure (128 x 128 int32 / 64 x 64 int64 tiles of C, all of K staged in
+ * 32-k blocks); the two terms with a diagonal operand are added with the full
+ * arithmetic.  st->flags & RBG_F_NEED_GENERAL: a term could overflow (MSB
+ * guard), C is unspecified and rbg_matmul_* must run instead.  Native widths
+ * only (int32 / int64).  workspace: rbg_matmul_fast_workspace_bytes(M, K, N,
+ * elem_bytes) bytes (the E-encoded operands). */
+int64_t rbg_matmul_fast_workspace_bytes(int64_t M, int64_t K, int64_t N, int32_t elem_bytes);
+int rbg_matmul_precheck_i32(const int32_t *X, int64_t ld, int64_t rows, int64_t cols, rbg_status *st,
+                            rbg_stream_t stream);
+int rbg_matmul_precheck_i64(const int64_t *X, int64_t ld, int64_t rows, int64_t cols, rbg_status *st,
+                            rbg_stream_t stream);
+int rbg_matmul_fast_i32(const int32_t *A, int64_t lda, const int32_t *B, int64_t ldb, int32_t *C, int64_t ldc,
+                        int64_t M, int64_t K, int64_t N, void *workspace, int64_t ws_bytes, rbg_status *st,
+                        rbg_stream_t stream);
+int rbg_matmul_fast_i64(const int64_t *A, int64_t lda, const int64_t *B, int64_t ldb, int64_t *C, int64_t ldc,
+                        int64_t M, int64_t K, int64_t N, void *workspace, int64_t ws_bytes, rbg_status *st,
+                        rbg_stream_t stream);
 
 /* ---- relational composition (relational_composition.c.in:51-267) --------- */
 /* Edge composition: out = closure R with edge alpha -> beta of value rel composed in
@@ -359,6 +382,9 @@ typedef struct {
   rbg_status *st;
   int32_t npad;
   int32_t tile;
+  int32_t tri; /* 0: full schedule; 1 / 2: every off-diagonal entry lies below / above the diagonal,
+                  and the component takes the triangular schedule of rbg_fw_dense_tri_* (a component
+                  that is not triangular as declared is flagged RBG_F_PRECOND_FAIL) */
 } rbg_batch_comp;
 int64_t rbg_batch_snap_bytes(int32_t npad);
 int rbg_batch_create(const rbg_batch_comp *comps, int32_t count, rbg_batch **out);
@@ -377,6 +403,78 @@ int rbg_batch_destroy(rbg_batch *b);
  * (*rounds of them). */
 int rbg_components(const int64_t *indptr, const int32_t *indices, int32_t n, int32_t *labels, int32_t *flags,
                    int32_t *rounds, rbg_stream_t stream);
+
+/* ---- CSR helpers around the closures -------------------------------------------
+ * Device replacements for the host/scipy steps of the reference around its
+ * closures.  temp: rbg_glue_temp_bytes(n) bytes of device memory (scan and
+ * sort storage, per-row counts).  Calls with a two-pass protocol write the
+ * n + 1 row pointers when out_indices / out_data are NULL, and the entries
+ * (into buffers of out_indptr[n] entries) when they are given. */
+int64_t rbg_glue_temp_bytes(int64_t n);
+/* st->aux[0] += sum over A's entries (i, k) of |row k of B|: the candidate
+ * count of A (x) B, an exact bound on its size (the symbolic count of
+ * gpu/spgemm_symbolic.py:124-183).  m = rows of A. */
+int rbg_csr_expand_size(const int64_t *ap, const int32_t *aj, int32_t m, const int64_t *bp, rbg_status *st,
+                        rbg_stream_t stream);
+/* st->aux[0] = max |value| of a CSR matrix (atomic max into a cleared
+ * status): the final-value diameter of gpu/transitive_closure.py:43-50. */
+int rbg_csr_max_abs_i32(const int64_t *ap, const int32_t *ax, int32_t n, rbg_status *st, rbg_stream_t stream);
+int rbg_csr_max_abs_i64(const int64_t *ap, const int64_t *ax, int32_t n, rbg_status *st, rbg_stream_t stream);
+/* The n x ncols dense matrix of a CSR (every cell written). */
+int rbg_csr_to_dense_i32(const int64_t *ap, const int32_t *aj, const int32_t *ax, int32_t n, int64_t ncols,
+                         int32_t *D, rbg_stream_t stream);
+int rbg_csr_to_dense_i64(const int64_t *ap, const int32_t *aj, const int64_t *ax, int32_t n, int64_t ncols,
+                         int64_t *D, rbg_stream_t stream);
+/* The canonical CSR of a dense n x ncols matrix (zeros dropped), two-pass. */
+int rbg_dense_to_csr_i32(const int32_t *D, int32_t n, int64_t ncols, int64_t *out_indptr, int32_t *out_indices,
+                         int32_t *out_data, void *temp, int64_t temp_bytes, rbg_stream_t stream);
+int rbg_dense_to_csr_i64(const int64_t *D, int32_t n, int64_t ncols, int64_t *out_indptr, int32_t *out_indices,
+                         int64_t *out_data, void *temp, int64_t temp_bytes, rbg_stream_t stream);
+/* Column `col` of a sorted CSR as an n x 1 CSR (one call: outputs of n entries
+ * at most), for the edge composition's R[:, alpha] (core/redblack.py:74-97). */
+int rbg_csr_column_i32(const int64_t *ap, const int32_t *aj, const int32_t *ax, int32_t n, int32_t col,
+                       int64_t *out_indptr, int32_t *out_indices, int32_t *out_data, void *temp, int64_t temp_bytes,
+                       rbg_stream_t stream);
+int rbg_csr_column_i64(const int64_t *ap, const int32_t *aj, const int64_t *ax, int32_t n, int32_t col,
+                       int64_t *out_indptr, int32_t *out_indices, int64_t *out_data, void *temp, int64_t temp_bytes,
+                       rbg_stream_t stream);
+/* Row `row` of a sorted CSR with the entry in column `col` set to val
+ * (replaced, or inserted in column order): the composed row's seed
+ * R[alpha] with R[alpha][beta] = relationship (core/redblack.py:82-85).
+ * Outputs hold |row| + 1 entries; st->aux[0] = the new length. */
+int rbg_csr_row_set_i32(const int64_t *ap, const int32_t *aj, const int32_t *ax, int64_t row, int32_t col,
+                        int32_t val, int32_t *out_indices, int32_t *out_data, rbg_status *st, rbg_stream_t stream);
+int rbg_csr_row_set_i64(const int64_t *ap, const int32_t *aj, const int64_t *ax, int64_t row, int32_t col,
+                        int64_t val, int32_t *out_indices, int64_t *out_data, rbg_status *st, rbg_stream_t stream);
+/* The entries on or above the diagonal of a sorted CSR (the triangular filter
+ * of permute_sparse, _permutation.pyx:138-157), two-pass. */
+int rbg_csr_upper_i32(const int64_t *ap, const int32_t *aj, const int32_t *ax, int32_t n, int64_t *out_indptr,
+                      int32_t *out_indices, int32_t *out_data, void *temp, int64_t temp_bytes, rbg_stream_t stream);
+int rbg_csr_upper_i64(const int64_t *ap, const int32_t *aj, const int64_t *ax, int32_t n, int64_t *out_indptr,
+                      int32_t *out_indices, int64_t *out_data, void *temp, int64_t temp_bytes, rbg_stream_t stream);
+/* Component layout from rbg_components' labels (each vertex's smallest
+ * component vertex): comp[v] = its component id (components numbered by
+ * their smallest vertex, _components.pyx:86-177), local[v] = its index
+ * within the component (vertices keep their order), members = the vertices
+ * grouped by component, sizes[c] (n entries, zero past the count), starts[c]
+ * (n + 1 entries: the exclusive scan of sizes), *ncomp (device int64), and,
+ * when vertex_size is not NULL, vertex_size[v] = sizes[comp[v]]. */
+int rbg_components_layout(const int32_t *labels, int32_t n, int32_t *comp, int32_t *local, int32_t *members,
+                          int32_t *sizes, int64_t *starts, int64_t *ncomp, int32_t *vertex_size, void *temp,
+                          int64_t temp_bytes, rbg_stream_t stream);
+/* extract_submatrix for every component at once (_components.pyx:180-251):
+ * each entry (v, w) of the graph into cell (local[v], local[w]) of component
+ * comp[v]'s padded int32 matrix at pool + comp_offset[c] (order comp_npad[c],
+ * row-major).  The pool must be cleared first. */
+int rbg_components_scatter(const int64_t *indptr, const int32_t *indices, const int32_t *data, int32_t n,
+                           const int32_t *comp, const int32_t *local, const int64_t *comp_offset,
+                           const int32_t *comp_npad, int32_t *pool, rbg_stream_t stream);
+/* merge_component_matrices (_components.pyx:254-323): the closed components
+ * back into one n x n CSR in global vertex order (rows sorted), two-pass. */
+int rbg_components_merge(const int32_t *pool, const int64_t *comp_offset, const int32_t *comp_npad,
+                         const int32_t *comp, const int32_t *local, const int32_t *members, const int64_t *starts,
+                         const int32_t *sizes, int32_t n, int64_t *out_indptr, int32_t *out_indices,
+                         int32_t *out_data, void *temp, int64_t temp_bytes, rbg_stream_t stream);
 
 /* ---- canonical ordering and permutation -----------------------------------
  * Replaces sparse/csgraph/_ordering.pyx:17-157 (_get_permutation,

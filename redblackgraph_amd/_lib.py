@@ -70,6 +70,11 @@ SIGNATURES = {
     "rbg_fw_dense_noalloc_i64": (I32, [P, I64, I64, I32, I32, I32, P, P]),
     "rbg_matmul_i32": (I32, [P, I64, P, I64, P, I64, I64, I64, I64, I32, I32, P, P]),
     "rbg_matmul_i64": (I32, [P, I64, P, I64, P, I64, I64, I64, I64, I32, I32, P, P]),
+    "rbg_matmul_fast_workspace_bytes": (I64, [I64, I64, I64, I32]),
+    "rbg_matmul_precheck_i32": (I32, [P, I64, I64, I64, P, P]),
+    "rbg_matmul_precheck_i64": (I32, [P, I64, I64, I64, P, P]),
+    "rbg_matmul_fast_i32": (I32, [P, I64, P, I64, P, I64, I64, I64, I64, P, I64, P, P]),
+    "rbg_matmul_fast_i64": (I32, [P, I64, P, I64, P, I64, I64, I64, I64, P, I64, P, P]),
     "rbg_edge_compose_i32": (I32, [P, I64, I64, I64, I32, P, I32, I32, P, P]),
     "rbg_edge_compose_i64": (I32, [P, I64, I64, I64, I64, P, I32, I32, P, P]),
     "rbg_edge_compose_scratch_bytes": (I64, [I64]),
@@ -103,6 +108,23 @@ SIGNATURES = {
     "rbg_batch_replay": (I32, [P, P]),
     "rbg_batch_destroy": (I32, [P]),
     "rbg_components": (I32, [P, P, I32, P, P, ctypes.POINTER(ctypes.c_int32), P]),
+    "rbg_glue_temp_bytes": (I64, [I64]),
+    "rbg_csr_expand_size": (I32, [P, P, I32, P, P, P]),
+    "rbg_csr_max_abs_i32": (I32, [P, P, I32, P, P]),
+    "rbg_csr_max_abs_i64": (I32, [P, P, I32, P, P]),
+    "rbg_csr_to_dense_i32": (I32, [P, P, P, I32, I64, P, P]),
+    "rbg_csr_to_dense_i64": (I32, [P, P, P, I32, I64, P, P]),
+    "rbg_dense_to_csr_i32": (I32, [P, I32, I64, P, P, P, P, I64, P]),
+    "rbg_dense_to_csr_i64": (I32, [P, I32, I64, P, P, P, P, I64, P]),
+    "rbg_csr_column_i32": (I32, [P, P, P, I32, I32, P, P, P, P, I64, P]),
+    "rbg_csr_column_i64": (I32, [P, P, P, I32, I32, P, P, P, P, I64, P]),
+    "rbg_csr_row_set_i32": (I32, [P, P, P, I64, I32, I32, P, P, P, P]),
+    "rbg_csr_row_set_i64": (I32, [P, P, P, I64, I32, I64, P, P, P, P]),
+    "rbg_csr_upper_i32": (I32, [P, P, P, I32, P, P, P, P, I64, P]),
+    "rbg_csr_upper_i64": (I32, [P, P, P, I32, P, P, P, P, I64, P]),
+    "rbg_components_layout": (I32, [P, I32, P, P, P, P, P, P, P, P, I64, P]),
+    "rbg_components_scatter": (I32, [P, P, P, I32, P, P, P, P, P, P]),
+    "rbg_components_merge": (I32, [P, P, P, P, P, P, P, P, I32, P, P, P, P, I64, P]),
     "rbg_order_metrics_dense_i32": (I32, [P, I64, P, P, P, P]),
     "rbg_order_metrics_dense_i64": (I32, [P, I64, P, P, P, P]),
     "rbg_order_metrics_csr_i32": (I32, [P, P, P, I32, P, P, P, P]),
@@ -121,7 +143,7 @@ SIGNATURES = {
 class RbgBatchComp(ctypes.Structure):
     """rbg_batch_comp: one component of a batched dense closure."""
     _fields_ = [("W", ctypes.c_void_p), ("snap", ctypes.c_void_p), ("st", ctypes.c_void_p),
-                ("npad", ctypes.c_int32), ("tile", ctypes.c_int32)]
+                ("npad", ctypes.c_int32), ("tile", ctypes.c_int32), ("tri", ctypes.c_int32)]
 
 
 class LibraryMissing(ImportError):

@@ -69,10 +69,15 @@ class BatchedClosure:
     """Device state for closing a fixed list of component sizes repeatedly.
 
     ``tiles`` maps padded order -> phase-3 tile edge (128 or 64); by default
-    the autotuner's choice for this device.
+    the autotuner's choice for this device.  ``tri`` declares the components'
+    triangularity (0: none, the full schedule; 1 / 2: every off-diagonal
+    entry below / above the diagonal, the triangular schedule), one value for
+    all or one per component; a component that is not triangular as declared
+    is found by the device precheck and closed again by the single-matrix
+    path, like any component outside the fast domain.
     """
 
-    def __init__(self, sizes: Sequence[int], tiles: Optional[Dict[int, int]] = None, stream=None):
+    def __init__(self, sizes: Sequence[int], tiles: Optional[Dict[int, int]] = None, stream=None, tri=0):
         t = _lib.torch()
         dev = _lib.device()
         self.layout = BatchLayout.build(sizes)
@@ -86,6 +91,10 @@ class BatchedClosure:
         self.snap = t.empty(max(self.layout.snap_bytes, _ALIGN), dtype=t.uint8, device=dev)
         self.status = t.zeros(self.count * _lib.STATUS_BYTES, dtype=t.uint8, device=dev)
         self.status_host = t.empty(self.count * _lib.STATUS_BYTES, dtype=t.uint8, pin_memory=True)
+        tri = np.broadcast_to(np.asarray(tri, dtype=np.int32), (self.count,))
+        if np.any((tri < 0) | (tri > 2)):
+            raise ValueError("tri must be 0, 1 or 2")
+        self.tri = tri.copy()
         descs = (_lib.RbgBatchComp * self.count)()
         pb, sb, stb = self.pool.data_ptr(), self.snap.data_ptr(), self.status.data_ptr()
         for i in range(self.count):
@@ -95,6 +104,7 @@ class BatchedClosure:
             descs[i].st = stb + i * _lib.STATUS_BYTES
             descs[i].npad = p
             descs[i].tile = self.tiles.get(p, 128)
+            descs[i].tri = int(self.tri[i])
         h = ctypes.c_void_p()
         _lib.call("rbg_batch_create", descs, self.count, ctypes.byref(h))
         self._h = h
@@ -208,12 +218,13 @@ class BatchedClosure:
         return outs
 
 
-def transitive_closure_batch(mats: Sequence, tiles: Optional[Dict[int, int]] = None):
+def transitive_closure_batch(mats: Sequence, tiles: Optional[Dict[int, int]] = None, tri=0):
     """Close a list of int32 component matrices together.
 
     Returns a list of TransitiveClosure (numpy W, diameter) in input order,
     raising the single-matrix closure's errors (core style) for the first
-    component that has one.
+    component that has one.  ``tri`` as for BatchedClosure (pedigrees
+    numbered parents first are lower triangular: tri=1).
     """
     from .core import raise_fw_errors
     from .types import TransitiveClosure
@@ -222,7 +233,7 @@ def transitive_closure_batch(mats: Sequence, tiles: Optional[Dict[int, int]] = N
     for A in mats:
         if A.ndim != 2 or A.shape[0] != A.shape[1]:
             raise ValueError("components must be square matrices")
-    b = BatchedClosure([A.shape[0] for A in mats], tiles)
+    b = BatchedClosure([A.shape[0] for A in mats], tiles, tri=tri)
     try:
         b.load_dense(mats)
         b.run(graph=False)

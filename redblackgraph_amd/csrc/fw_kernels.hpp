@@ -41,9 +41,9 @@ constexpr int CPT = 8;     // phase-3 cols per thread
 struct FwWork {
   // two snapshot buffers (block parity), each BLK x n for columns and rows,
   // then the guard words of the current pivot block: per 32-block b and k,
-  // gblk[b][0][k] = the largest MSB among the left operands of rows of block b
-  // (column snapshot k) and gblk[b][1][k] = among the right operands of
-  // columns of block b (row snapshot k)
+  // gblk[b][0][k] = the largest MSB among the left operands of rows of block
+  // b (column snapshot k) and gblk[b][1][k] = among the right operands of
+  // columns of block b (row snapshot k).
   void* base;
   int64_t n;
   int elem;
@@ -148,20 +148,26 @@ __global__ void k_fw_precheck(const T* W, int64_t n, int64_t ld, rbg_status* st)
 // loop writes every cell at k = 0, and on a triangular input with unit
 // diagonals that first write leaves the initial value as it is, while the
 // triangular schedule never visits the cells its pivot blocks cannot change.
+// tri (1 lower, 2 upper) also checks the side: an off-diagonal entry on the
+// other side asks for general mode, so every later kernel of the fast run
+// exits and the caller reruns the full schedule.
 template <typename T, bool ENC>
-__global__ void k_e_convert(T* W, int64_t n, int64_t ld, rbg_status* st) {
+__global__ void k_e_convert(T* W, int64_t n, int64_t ld, rbg_status* st, int tri = 0) {
   int64_t total = n * n;
   int64_t vmax = 0;
+  bool wrong = false;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
        e += (int64_t)gridDim.x * blockDim.x) {
     int64_t i = e / n, j = e % n;
     T v = W[i * ld + j];
     W[i * ld + j] = ENC ? e_enc<T>(v) : e_dec<T>(v);
     if ((int64_t)v > vmax) vmax = (int64_t)v;
+    if (v != 0 && ((tri == 1 && j > i) || (tri == 2 && j < i))) wrong = true;
   }
   if (st) {
     vmax = wave_max_i64(vmax);
     if ((threadIdx.x & 63) == 0) record_diameter(st, vmax);
+    if (__any(wrong) && (threadIdx.x & 63) == 0) atomicOr(&st->flags, (uint32_t)RBG_F_NEED_GENERAL);
   }
 }
 
@@ -527,17 +533,30 @@ __device__ inline void fw_panel_pair_body(T* W, int64_t n, int64_t ld, int64_t K
 // an absent operand is some value with the top bit set, which the min may
 // store, and which must not be shifted again.
 // ---------------------------------------------------------------------------
+template <typename U>
+struct LinesShared {
+  U D[BLK][BLK + 1];
+  U sC[BLK][BLK + 1];  // [kk][r] = W[r][kk] at step kk (diagonal block)
+  U sR[BLK][BLK + 1];  // [kk][c] = W[kk][c] at step kk (diagonal block)
+  U Mb[BLK][BLK];      // shift and low bits of sR
+  U Yb[BLK][BLK];
+  int maxA[BLK], maxB[BLK], sbad;
+};
+
 template <typename T>
-__device__ inline void fw_panel_lines_body(T* W, int64_t n, int64_t ld, int64_t K, FwWork ws, int width,
-                                          rbg_status* st, int bid) {
+__device__ __forceinline__ void fw_panel_lines_body(LinesShared<typename unsigned_of<T>::type>& L, T* W, int64_t n,
+                                                    int64_t ld, int64_t K, FwWork ws, int width, rbg_status* st,
+                                                    int bid) {
   using U = typename unsigned_of<T>::type;
   constexpr U TOP = (U)1 << (sizeof(U) * 8 - 1);
-  __shared__ U D[BLK][BLK + 1];
-  __shared__ U sC[BLK][BLK + 1];  // [kk][r] = W[r][kk] at step kk (diagonal block)
-  __shared__ U sR[BLK][BLK + 1];  // [kk][c] = W[kk][c] at step kk (diagonal block)
-  __shared__ U Mb[BLK][BLK];      // shift and low bits of sR
-  __shared__ U Yb[BLK][BLK];
-  __shared__ int maxA[BLK], maxB[BLK], sbad;
+  auto& D = L.D;
+  auto& sC = L.sC;
+  auto& sR = L.sR;
+  auto& Mb = L.Mb;
+  auto& Yb = L.Yb;
+  auto& maxA = L.maxA;
+  auto& maxB = L.maxB;
+  auto& sbad = L.sbad;
   const int tid = threadIdx.x;
   const int64_t k0 = K * BLK;
   const int par = (int)(K & 1);
@@ -742,19 +761,35 @@ __device__ inline void fw_panel_lines_body(T* W, int64_t n, int64_t ld, int64_t 
 // SHIFT_BYTES: the shifts are staged as bytes (36 KB of LDS for a 128-tile
 // instead of 48 KB, so four workgroups fit a CU when the registers allow it),
 // at 12 more VALU instructions per 128 relaxations to take them apart.
+// ahead (the look-ahead schedule, k_fw_step_fast): the next pivot block's
+// rows and columns, [k0 + 32, k0 + 64), belong to the next panel, which runs
+// in the same launch: the tile skips those rows as it skips the pivot rows,
+// and does not store into those columns.
+// The LDS arrays come from the kernel (TileShared), so that a kernel with
+// two workgroup roles can overlay them with the other role's.
+template <typename U, int TL, bool SHIFT_BYTES>
+struct TileShared {
+  using MsT = typename std::conditional<SHIFT_BYTES, uint8_t, U>::type;
+  U As[BLK][TL];
+  MsT Ms[BLK][TL];
+  U Ys[BLK][TL];
+  int sGa[BLK], sGb[BLK];
+};
+
 template <typename T, int RPT, int CPT, bool PANEL_GUARD, bool SHIFT_BYTES>
-__device__ inline void fw_tile_fast_body(T* W, int64_t n, int64_t ld, int64_t K, FwWork ws, int width,
-                                         rbg_status* st, int I, int J) {
+__device__ __forceinline__ void fw_tile_fast_body(TileShared<typename unsigned_of<T>::type, 16 * CPT, SHIFT_BYTES>& L, T* W,
+                                         int64_t n, int64_t ld, int64_t K, FwWork ws, int width, rbg_status* st,
+                                         int I, int J, int ahead = 0) {
   using U = typename unsigned_of<T>::type;
   constexpr U TOP = (U)1 << (sizeof(U) * 8 - 1);
   constexpr int TL = 16 * CPT;  // tile edge
   constexpr int NCH = CPT / 4;  // column chunks per thread
   static_assert(RPT == CPT && CPT % 4 == 0, "square thread blocks of whole chunks");
-  __shared__ U As[BLK][TL];
-  using MsT = typename std::conditional<SHIFT_BYTES, uint8_t, U>::type;
-  __shared__ MsT Ms[BLK][TL];
-  __shared__ U Ys[BLK][TL];
-  __shared__ int sGa[BLK], sGb[BLK];
+  auto& As = L.As;
+  auto& Ms = L.Ms;
+  auto& Ys = L.Ys;
+  auto& sGa = L.sGa;
+  auto& sGb = L.sGb;
   const int tid = threadIdx.x;
   const int64_t k0 = K * BLK;
   const int par = (int)(K & 1);
@@ -764,7 +799,7 @@ __device__ inline void fw_tile_fast_body(T* W, int64_t n, int64_t ld, int64_t K,
   const int tr = tid >> 4, tc = tid & 15;
   const int64_t r0 = (int64_t)I * TL + tr * RPT;
   const int64_t cb = (int64_t)J * TL + tc * 4;  // column of chunk 0; chunk h adds 64 h
-  const bool skip = r0 >= k0 && r0 < k0 + BLK;
+  const bool skip = r0 >= k0 && r0 < k0 + (ahead ? 2 : 1) * BLK;
 
   // every global load first: the tile, this thread's share of the
   // snapshots, the status flags and (one matrix) the guard words
@@ -936,14 +971,17 @@ __device__ inline void fw_tile_fast_body(T* W, int64_t n, int64_t ld, int64_t K,
     }
   }
 #pragma unroll
-  for (int r = 0; r < RPT; r++)
+  for (int h = 0; h < NCH; h++) {
+    const int64_t c = cb + 64 * h;
+    if (ahead && c >= k0 + BLK && c < k0 + 2 * BLK) continue;  // the next panel's columns
 #pragma unroll
-    for (int h = 0; h < NCH; h++) {
+    for (int r = 0; r < RPT; r++) {
       U q[4];
 #pragma unroll
       for (int x = 0; x < 4; x++) q[x] = w[r][4 * h + x];
-      st4(Wu + (r0 + r) * ld + cb + 64 * h, q);
+      st4(Wu + (r0 + r) * ld + c, q);
     }
+  }
   int64_t dm = (int64_t)(d0 > d1 ? d0 : d1);
   dm = wave_max_i64(dm);
   if ((tid & 63) == 0 && dm > seen) record_diameter(st, dm);
@@ -988,16 +1026,337 @@ constexpr int64_t SMALL_TILE_MAX_N = 2048;
 template <typename T>
 __global__ __launch_bounds__(NT, 4) void k_fw_tile_fast_small(T* W, int64_t n, int64_t ld, int64_t K, FwWork ws,
                                                              int width, rbg_status* st, TileRect r) {
+  __shared__ TileShared<typename unsigned_of<T>::type, 64, false> L;
   const int b = blockIdx.x;
-  fw_tile_fast_body<T, 4, 4, true, false>(W, n, ld, K, ws, width, st, r.i0 + b / r.jn, r.j0 + b % r.jn);
+  fw_tile_fast_body<T, 4, 4, true, false>(L, W, n, ld, K, ws, width, st, r.i0 + b / r.jn, r.j0 + b % r.jn);
 }
 
 template <typename T, int PER_CU = FastTile<T>::PER_CU>
 __global__ __launch_bounds__(NT, PER_CU) void k_fw_tile_fast(T* W, int64_t n, int64_t ld, int64_t K, FwWork ws,
                                                             int width, rbg_status* st, TileRect r) {
+  __shared__ TileShared<typename unsigned_of<T>::type, FastTile<T>::EDGE, FastTile<T>::SHIFT_BYTES> L;
   const int b = blockIdx.x;
   fw_tile_fast_body<T, FastTile<T>::RC, FastTile<T>::RC, true, FastTile<T>::SHIFT_BYTES>(
-      W, n, ld, K, ws, width, st, r.i0 + b / r.jn, r.j0 + b % r.jn);
+      L, W, n, ld, K, ws, width, st, r.i0 + b / r.jn, r.j0 + b % r.jn);
+}
+
+// Panel of the look-ahead schedule (fw_panel_quad_body), for lower
+// triangular matrices: the pair panel's walk (fw_panel_pair_body) with 256
+// threads, thread (i4, j) holding rows i4, i4 + 8, i4 + 16, i4 + 24 of column
+// j of the diagonal tile and of the workgroup's one live panel tile -- the
+// row-panel tile (K, J) when J < K, else the column-panel tile (J, K); the
+// other one lies above the diagonal, stays absent, and only its snapshots
+// (absent) are written.  256 threads, so that it can share a launch with
+// 256-thread tiles.
+// Before the walk, a row-panel workgroup applies pivot block K - 1 to its
+// tile from that block's snapshots, as phase 3 would have:
+//     R[i][j] = R[i][j] (+) colsnap_k[k0 + i] (x) rowsnap_k[j0 + j],
+//     k = k0 - 32 .. k0 - 1,
+// all operands fixed, so one pass over the 32 steps with no barrier.  These
+// are the only cells of block K's row and column panels that block K - 1
+// changes (its column panel would need W[k][j] with k < j).
+// The MSB guard is checked per step from the snapshots after the walk (the
+// largest left-operand MSB plus the largest shift among step kk's
+// snapshots bounds every product of step kk); a failure discards the run
+// like any other.  Guard words are not written: the tiles of this schedule
+// check the guard on the operands they stage.
+template <typename U>
+struct QuadShared {
+  U Dg[BLK][BLK + 1];
+  U Pc[BLK][BLK + 1];
+  U sDc[BLK][BLK + 1];  // diagonal column kk at step kk, row x
+  U sDr[BLK][BLK + 1];  // diagonal row kk, column x
+  U sPr[BLK][BLK + 1];  // row-panel row kk, column x
+  U sPc[BLK][BLK + 1];  // column-panel column kk, row x
+  U MYd[2][BLK][2];     // (shift, low bits) of the pivot rows, by step parity
+  U MYr[2][BLK][2];
+  U sDiag[BLK];
+  U sL[BLK][BLK + 1];  // strip: colsnap_k[k0 + x] for step kk
+  U sM[BLK][BLK + 1];  // strip: shift and low bits of rowsnap_k[j0 + x]
+  U sY[BLK][BLK + 1];
+  int64_t sMax[NT / 64];
+  int sbad, sguard;  // sguard: the strip's MSB guard failed
+};
+
+template <typename T>
+__device__ __forceinline__ void fw_panel_quad_body(QuadShared<typename unsigned_of<T>::type>& L, T* W, int64_t n,
+                                                   int64_t ld, int64_t K, FwWork ws, int width, rbg_status* st,
+                                                   int bid) {
+  using U = typename unsigned_of<T>::type;
+  constexpr U TOP = (U)1 << (sizeof(U) * 8 - 1);
+  constexpr U ALL = ~(U)0;
+  const int tid = threadIdx.x;
+  const int j = tid & (BLK - 1), i4 = tid >> 5;
+  const int64_t k0 = K * BLK;
+  const int J = bid < K ? bid : bid + 1;
+  const int64_t j0 = (int64_t)J * BLK;
+  const bool owner = bid == 0;
+  const bool rowp = J < K;  // the live tile: row panel (K, J), else column panel (J, K)
+  const int par = (int)(K & 1);
+  const int lim = width - 2;
+  auto canon = [](U x) -> U { return (T)x < 0 ? ALL : x; };
+  auto pair = [](U b, U& m, U& y) {
+    if ((T)b < 0) {
+      m = 0;
+      y = TOP;
+    } else {
+      m = (U)msb_u(b);
+      y = b ^ ((U)1 << m);
+    }
+  };
+  auto amsb = [](U a) -> int { return (T)a > 1 ? msb_u(a) : 0; };
+
+  // the cells of this thread, loaded before anything waits: p is the live
+  // panel tile's cell (row panel: W[k0 + i][j0 + j], column panel:
+  // W[j0 + i][k0 + j])
+  U d[4], p[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int i = i4 + 8 * q;
+    d[q] = canon((U)W[(k0 + i) * ld + k0 + j]);
+    p[q] = canon((U)(rowp ? W[(k0 + i) * ld + j0 + j] : W[(j0 + i) * ld + k0 + j]));
+  }
+  if (tid == 0) {
+    L.sbad = (st->flags & (RBG_F_NEED_GENERAL | RBG_F_PRECOND_FAIL)) ? 2 : 0;
+    L.sguard = 0;
+  }
+  if (rowp) {
+    // the strip's operands: all loads first, then the LDS stores
+    const U* pcol = (const U*)ws.col<T>(par ^ 1);
+    const U* prow = (const U*)ws.row<T>(par ^ 1);
+    constexpr int PER = BLK * BLK / NT;
+    U la[PER], lb[PER];
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+      const int e = tid + q * NT, kk = e / BLK, x = e % BLK;
+      la[q] = pcol[kk * n + k0 + x];
+      lb[q] = prow[kk * n + j0 + x];
+    }
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+      const int e = tid + q * NT, kk = e / BLK, x = e % BLK;
+      L.sL[kk][x] = la[q];
+      U m, y;
+      pair(lb[q], m, y);
+      L.sM[kk][x] = m;
+      L.sY[kk][x] = y;
+    }
+  }
+  __syncthreads();
+  if (L.sbad) return;
+  T ds = 0;
+  if (rowp) {
+    // per step, the largest left-operand MSB plus the largest shift
+    if (tid < BLK) {
+      int ga = 0, gb = 0;
+      for (int x = 0; x < BLK; x++) {
+        const int a = amsb(L.sL[tid][x]);
+        const int m = (T)L.sY[tid][x] < 0 ? 0 : (int)L.sM[tid][x];
+        ga = a > ga ? a : ga;
+        gb = m > gb ? m : gb;
+      }
+      if (ga + gb > lim) L.sguard = 1;
+    }
+    for (int kk = 0; kk < BLK; kk++) {
+      const U m = L.sM[kk][j], y = L.sY[kk][j];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const U v = (L.sL[kk][i4 + 8 * q] << m) | y;
+        p[q] = p[q] < v ? p[q] : v;
+        ds = ds > (T)p[q] ? ds : (T)p[q];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) p[q] = canon(p[q]);
+  }
+  // the walk's starting state
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int i = i4 + 8 * q;
+    if (i == j) {
+      L.sDiag[i] = d[q];
+      d[q] = ALL;
+    }
+    L.Dg[i][j] = d[q];
+    if (!rowp) L.Pc[i][j] = p[q];
+    if (j == 0) {
+      L.sDc[0][i] = d[q];
+      L.sPc[0][i] = rowp ? ALL : p[q];
+    }
+  }
+  if (i4 == 0) {  // pivot row 0 (q = 0): its pairs and the step-0 row snapshots
+    pair(d[0], L.MYd[0][j][0], L.MYd[0][j][1]);
+    L.sDr[0][j] = d[0];
+    if (rowp) pair(p[0], L.MYr[0][j][0], L.MYr[0][j][1]);
+    L.sPr[0][j] = rowp ? p[0] : ALL;
+  }
+  __syncthreads();
+  if (L.sguard) {
+    if (tid == 0) atomicOr(&st->flags, (uint32_t)RBG_F_NEED_GENERAL);
+    return;
+  }
+  const int64_t seen = st->diameter_max;  // a lower bound of the value at exit
+
+  T dd = 0, dp = 0;
+  for (int kk = 0; kk < BLK; kk++) {
+    const int qq = kk & 1;
+    const U md = L.MYd[qq][j][0], yd = L.MYd[qq][j][1];
+    if (rowp) {
+      //   diagonal  D[i][j] = D[i][j] (+) D[i][k] (x) D[k][j]
+      //   row panel R[i][j] = R[i][j] (+) D[i][k] (x) R[k][j]
+      const U mr = L.MYr[qq][j][0], yr = L.MYr[qq][j][1];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int i = i4 + 8 * q;
+        const U ad = canon(L.Dg[i][kk]);
+        const U pd = (ad << md) | yd;
+        const U pr = (ad << mr) | yr;
+        d[q] = d[q] < pd ? d[q] : pd;
+        p[q] = p[q] < pr ? p[q] : pr;
+        dd = dd > (T)d[q] ? dd : (T)d[q];
+        dp = dp > (T)p[q] ? dp : (T)p[q];
+        L.Dg[i][j] = d[q];
+      }
+    } else {
+      //   column panel C[i][j] = C[i][j] (+) C[i][k] (x) D[k][j]
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int i = i4 + 8 * q;
+        const U ad = canon(L.Dg[i][kk]);
+        const U ac = canon(L.Pc[i][kk]);
+        const U pd = (ad << md) | yd;
+        const U pc = (ac << md) | yd;
+        d[q] = d[q] < pd ? d[q] : pd;
+        p[q] = p[q] < pc ? p[q] : pc;
+        dd = dd > (T)d[q] ? dd : (T)d[q];
+        dp = dp > (T)p[q] ? dp : (T)p[q];
+        L.Dg[i][j] = d[q];
+        L.Pc[i][j] = p[q];
+      }
+    }
+    if (kk + 1 < BLK) {
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int i = i4 + 8 * q;
+        if (i == kk + 1) {
+          const U cd = canon(d[q]);
+          pair(cd, L.MYd[qq ^ 1][j][0], L.MYd[qq ^ 1][j][1]);
+          L.sDr[kk + 1][j] = cd;
+          if (rowp) {
+            const U cr = canon(p[q]);
+            pair(cr, L.MYr[qq ^ 1][j][0], L.MYr[qq ^ 1][j][1]);
+            L.sPr[kk + 1][j] = cr;
+          } else {
+            L.sPr[kk + 1][j] = ALL;
+          }
+        }
+        if (j == kk + 1) {
+          L.sDc[kk + 1][i] = canon(d[q]);
+          L.sPc[kk + 1][i] = rowp ? ALL : canon(p[q]);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // the MSB guard of each step, from its snapshots: left operands are the
+  // column snapshots (diagonal, column panel), right operands the row
+  // snapshots (diagonal, row panel)
+  if (tid < BLK) {
+    int ga = 0, gb = 0;
+    for (int x = 0; x < BLK; x++) {
+      const int a = amsb(L.sDc[tid][x]), c2 = amsb(L.sPc[tid][x]);
+      const U b1 = L.sDr[tid][x], b2 = L.sPr[tid][x];
+      const int m1 = (T)b1 < 0 ? 0 : msb_u(b1), m2 = (T)b2 < 0 ? 0 : msb_u(b2);
+      ga = a > ga ? a : ga;
+      ga = c2 > ga ? c2 : ga;
+      gb = m1 > gb ? m1 : gb;
+      gb = m2 > gb ? m2 : gb;
+    }
+    if (ga + gb > lim) L.sbad = 1;
+  }
+  // cycles: step x, vertex y with W[y][x] and W[x][y] both present
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int i = i4 + 8 * q;
+    if (owner && i != j && (T)L.sDc[i][j] >= 0 && (T)L.sDr[i][j] >= 0) {
+      record_cycle(st, naive_key(k0 + i, k0 + j, k0 + j, n));
+      L.sbad = 1;
+    }
+  }
+  __syncthreads();
+  if (L.sbad) {
+    if (tid == 0) atomicOr(&st->flags, (uint32_t)RBG_F_NEED_GENERAL);
+    return;
+  }
+  // tiles and snapshots out, coalesced (thread (i, j): snapshot step i, index j)
+  T* colT = ws.col<T>(par);
+  T* rowS = ws.row<T>(par);
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int i = i4 + 8 * q;
+    if (owner) {
+      W[(k0 + i) * ld + k0 + j] = (T)(i == j ? L.sDiag[i] : d[q]);
+      colT[i * n + k0 + j] = (T)L.sDc[i][j];
+      rowS[i * n + k0 + j] = (T)L.sDr[i][j];
+    }
+    if (rowp)
+      W[(k0 + i) * ld + j0 + j] = (T)p[q];
+    else
+      W[(j0 + i) * ld + k0 + j] = (T)p[q];
+    rowS[i * n + j0 + j] = (T)L.sPr[i][j];
+    colT[i * n + j0 + j] = (T)L.sPc[i][j];
+  }
+  // the previous block's detections (its snapshots stay until the next launch)
+  if (owner && tid == 0 && K > 0) capture_block<T, true>(st, ws.col<T>(par ^ 1), ws.row<T>(par ^ 1), k0 - BLK, n);
+  // trajectory maximum: the workgroup's, then one atomic
+  T m = dp > ds ? dp : ds;
+  if (owner) {
+    m = m > dd ? m : dd;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int i = i4 + 8 * q;
+      if (i == j && (T)L.sDiag[i] > m) m = (T)L.sDiag[i];
+    }
+  }
+  const int64_t wm = wave_max_i64((int64_t)m);
+  if ((tid & 63) == 0) L.sMax[tid >> 6] = wm;
+  __syncthreads();
+  if (tid == 0) {
+    int64_t x = 0;
+    for (int w = 0; w < NT / 64; w++) x = x > L.sMax[w] ? x : L.sMax[w];
+    if (x > seen) record_diameter(st, x);
+  }
+}
+
+// One pivot step of the look-ahead schedule (lower triangular matrices, fast
+// mode): workgroups [0, npanel) run the panel of pivot block K + 1
+// (fw_panel_quad_body, which first applies block K to block K + 1's rows),
+// the others the phase-3 tiles of block K in rectangle r with those rows
+// and columns left to the panel (ahead).  Neither role reads what the other
+// writes: in a lower triangular matrix block K changes no cell of block
+// K + 1's columns, and the tiles skip block K + 1's rows.  So a pivot step is
+// one launch and the panel runs beside the tiles instead of before them.
+// npanel = 0: the last step, tiles only.  The tiles check the MSB guard on
+// the operands they stage; the two roles share one LDS allocation.
+template <typename T, int RC, bool SHIFT_BYTES, int PER_CU>
+__global__ __launch_bounds__(NT, PER_CU) void k_fw_step_fast(T* W, int64_t n, int64_t ld, int64_t K, FwWork ws,
+                                                            int width, rbg_status* st, TileRect r, int npanel) {
+  using U = typename unsigned_of<T>::type;
+  __shared__ union {
+    TileShared<U, 16 * RC, SHIFT_BYTES> t;
+    QuadShared<U> p;
+  } L;
+  const int b = blockIdx.x;
+  if (b < npanel) {
+    // the panel is the launch's critical path: its waves take issue
+    // priority over the tile waves that share their SIMDs
+    __builtin_amdgcn_s_setprio(3);
+    fw_panel_quad_body<T>(L.p, W, n, ld, K + 1, ws, width, st, b);
+    return;
+  }
+  const int t = b - npanel;
+  fw_tile_fast_body<T, RC, RC, false, SHIFT_BYTES>(L.t, W, n, ld, K, ws, width, st, r.i0 + t / r.jn,
+                                                   r.j0 + t % r.jn, npanel > 0);
 }
 
 // Triangular schedule (the blocked form of the reference's

@@ -9,6 +9,10 @@ namespace rbg {
 // times the workgroups for the steps that touch few tiles.
 constexpr int TRI_SMALL_BELOW = 256;
 
+// Lower triangular closures take the look-ahead schedule (k_fw_step_fast):
+// one launch per pivot block, the next block's panel inside it.
+constexpr bool LOOKAHEAD = true;
+
 // tri: 0 = full schedule; 1 / 2 = the input's off-diagonal entries all lie
 // below / above the diagonal (fast mode only: general mode always runs the
 // full schedule).
@@ -25,26 +29,30 @@ static int fw_dense(T* W, int64_t n, int64_t ld, int width, int is_uns, int mode
   const int nt = (int)(n / TILE);
   const int conv_grid = (int)((n * n + NT * 8 - 1) / (NT * 8)) < 4096 ? (int)((n * n + NT * 8 - 1) / (NT * 8)) : 4096;
   if (fast) {
-    hipLaunchKernelGGL((k_e_convert<T, true>), dim3(conv_grid), dim3(NT), 0, s, W, n, ld, tri ? st : nullptr);
+    hipLaunchKernelGGL((k_e_convert<T, true>), dim3(conv_grid), dim3(NT), 0, s, W, n, ld, tri ? st : nullptr, tri);
     RBG_CHECK_LAUNCH();
   }
   const int panel_grid = nb > 1 ? 2 * (nb - 1) : 1;
-  for (int K = 0; K < nb; K++) {
-    int p0 = PROF ? prof_start(s) : -1;
+  auto panel = [&](int K, hipStream_t q) -> int {
+    int p0 = PROF ? prof_start(q) : -1;
     if (fast) {
-      hipLaunchKernelGGL((k_fw_panel_fast<T>), dim3(nb - 1), dim3(PANEL_NT), 0, s, W, n, ld, (int64_t)K, ws, width,
+      hipLaunchKernelGGL((k_fw_panel_fast<T>), dim3(nb - 1), dim3(PANEL_NT), 0, q, W, n, ld, (int64_t)K, ws, width,
                          st);
     } else {
-      hipLaunchKernelGGL((k_fw_panel<T, false>), dim3(panel_grid), dim3(NT), 0, s, W, n, ld, (int64_t)K, ws,
+      hipLaunchKernelGGL((k_fw_panel<T, false>), dim3(panel_grid), dim3(NT), 0, q, W, n, ld, (int64_t)K, ws,
                          width, is_uns, st);
     }
     RBG_CHECK_LAUNCH();
-    prof_stop(0, p0, s);
-    int p1 = PROF ? prof_start(s) : -1;
-    // relaxations the launch executes: its tiles' cells outside the pivot
-    // block's rows (the pivot tile row is in every rectangle), 32 steps each
+    if (PROF) prof_stop(0, p0, q);
+    return 0;
+  };
+  // the tiles of pivot block K, and with the look-ahead schedule (la) the
+  // panel of block K + 1 in the same launch
+  auto tiles = [&](int K, hipStream_t q, bool la) -> int {
+    int p1 = PROF ? prof_start(q) : -1;
     int edge = TILE, count = nt * nt, kind = 1;
     TileRect r{0, 0, nt};
+    const bool ahead = la && K + 1 < nb;
     if (fast) {
       edge = FastTile<T>::EDGE;
       r = tri_rect(tri, n, edge, K, &count);
@@ -53,20 +61,48 @@ static int fw_dense(T* W, int64_t n, int64_t ld, int width, int is_uns, int mode
           edge = 64;
           r = tri_rect(tri, n, edge, K, &count);
           kind = 5;
-          hipLaunchKernelGGL((k_fw_tile_fast_small<T>), dim3(count), dim3(NT), 0, s, W, n, ld, (int64_t)K, ws,
-                             width, st, r);
         }
       }
-      if (kind == 1)
-        hipLaunchKernelGGL((k_fw_tile_fast<T>), dim3(count), dim3(NT), 0, s, W, n, ld, (int64_t)K, ws, width, st,
+      if (la) {
+        // int32 only (la); the next block's panel: one workgroup per
+        // row/column panel tile pair, as the pair panel
+        const int npanel = ahead ? nb - 1 : 0;
+        if constexpr (sizeof(T) == 4) {
+          if (kind == 5)
+            hipLaunchKernelGGL((k_fw_step_fast<T, 4, false, 4>), dim3(npanel + count), dim3(NT), 0, q, W, n, ld,
+                               (int64_t)K, ws, width, st, r, npanel);
+          else
+            hipLaunchKernelGGL((k_fw_step_fast<T, 8, false, 3>), dim3(npanel + count), dim3(NT), 0, q, W, n, ld,
+                               (int64_t)K, ws, width, st, r, npanel);
+        }
+      } else if (kind == 5) {
+        hipLaunchKernelGGL((k_fw_tile_fast_small<T>), dim3(count), dim3(NT), 0, q, W, n, ld, (int64_t)K, ws, width,
+                           st, r);
+      } else {
+        hipLaunchKernelGGL((k_fw_tile_fast<T>), dim3(count), dim3(NT), 0, q, W, n, ld, (int64_t)K, ws, width, st,
                            r);
+      }
     } else {
-      hipLaunchKernelGGL((k_fw_tile_general<T>), dim3(nt * nt), dim3(NT), 0, s, W, n, ld, (int64_t)K, ws,
+      hipLaunchKernelGGL((k_fw_tile_general<T>), dim3(nt * nt), dim3(NT), 0, q, W, n, ld, (int64_t)K, ws,
                          width, is_uns, st);
     }
     RBG_CHECK_LAUNCH();
+    // relaxations the tiles execute: their cells outside the skipped rows
+    // (the pivot block's, and with a panel in the launch the next block's,
+    // are in every rectangle), 32 steps each
     if (PROF)
-      prof_stop(kind, p1, s, ((double)count * edge * edge - (double)BLK * r.jn * edge) * BLK);
+      prof_stop(kind, p1, q, ((double)count * edge * edge - (double)(ahead ? 2 : 1) * BLK * r.jn * edge) * BLK);
+    return 0;
+  };
+  // Look-ahead (lower triangular int32): pivot block K's tiles and block
+  // K + 1's panel share a launch; only block 0's panel runs on its own.  (The
+  // int64 line panel holds twice the registers, which would cost the int64
+  // tiles their fourth workgroup per CU.)
+  const bool la = fast && tri == 1 && LOOKAHEAD && sizeof(T) == 4;
+  for (int K = 0; K < nb; K++) {
+    int rc = (!la || K == 0) ? panel(K, s) : 0;
+    if (!rc) rc = tiles(K, s, la);
+    if (rc) return rc;
   }
   if (fast) {
     hipLaunchKernelGGL((k_fw_capture_last<T, true>), dim3(1), dim3(1), 0, s, st, ws, (int64_t)(nb - 1), n);

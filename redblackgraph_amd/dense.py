@@ -133,20 +133,42 @@ def fw_closure(W, et: ElemType, stream=None, force_general: bool = False, schedu
     return work[:n, :n], FwOutcome(s, npad, mode, sched)
 
 
-def matmul(A, B, et: ElemType, stream=None):
-    """C = A (x) B for device matrices (M x K) and (K x N)."""
+# Products with fewer multiply-adds than this take the general kernel directly
+# (the fast path's precheck and operand encoding cost more than they save).
+MATMUL_FAST_MIN_TERMS = 1 << 24
+
+
+def matmul(A, B, et: ElemType, stream=None, force_general: bool = False):
+    """C = A (x) B for device matrices (M x K) and (K x N).  Native-width
+    operands inside the fast domain (rbg_matmul_precheck: closures of
+    pedigrees) take rbg_matmul_fast_*; others, and a fast run whose MSB guard
+    fails, take the general kernel with the full arithmetic and events."""
     t = _lib.torch()
     M, K = A.shape
     K2, N = B.shape
     assert K == K2
     C = t.empty((M, N), dtype=A.dtype, device=A.device)
     st = status()
-    st.reset(stream)
-    fn = "rbg_matmul_i32" if A.dtype == t.int32 else "rbg_matmul_i64"
+    sfx = "i32" if A.dtype == t.int32 else "i64"
+    sh = _lib.stream_handle(stream)
     A = A.contiguous()
     B = B.contiguous()
-    _lib.call(fn, _lib.ptr(A), K, _lib.ptr(B), N, _lib.ptr(C), N, M, K, N, et.width, int(et.unsigned),
-              st.p, _lib.stream_handle(stream))
+    if et.native and not force_general and M * K * N >= MATMUL_FAST_MIN_TERMS:
+        st.reset(stream)
+        _lib.call(f"rbg_matmul_precheck_{sfx}", _lib.ptr(A), K, M, K, st.p, sh)
+        _lib.call(f"rbg_matmul_precheck_{sfx}", _lib.ptr(B), N, K, N, st.p, sh)
+        if not st.read(stream).flags & _lib.F_PRECOND_FAIL:
+            ws = t.empty(int(_lib.load().rbg_matmul_fast_workspace_bytes(M, K, N, A.element_size())),
+                         dtype=t.uint8, device=A.device)
+            st.reset(stream)
+            _lib.call(f"rbg_matmul_fast_{sfx}", _lib.ptr(A), K, _lib.ptr(B), N, _lib.ptr(C), N, M, K, N,
+                      _lib.ptr(ws), ws.numel(), st.p, sh)
+            s = st.read(stream)
+            if not s.flags & _lib.F_NEED_GENERAL:
+                return C, s
+    st.reset(stream)
+    _lib.call(f"rbg_matmul_{sfx}", _lib.ptr(A), K, _lib.ptr(B), N, _lib.ptr(C), N, M, K, N, et.width,
+              int(et.unsigned), st.p, sh)
     return C, st.read(stream)
 
 

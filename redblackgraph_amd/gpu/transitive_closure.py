@@ -41,13 +41,18 @@ def sparse_equal_gpu(A: CSRMatrixGPU, B: CSRMatrixGPU) -> bool:
 
 
 def _final_diameter(C: sd.DevCSR) -> int:
+    """floor(log2 max |value|) when that maximum exceeds 1, else 0
+    (gpu/transitive_closure.py:43-50 of the reference): rbg_csr_max_abs."""
+    from ..dense import status
+    from .. import _lib
     if C.nnz == 0:
         return 0
-    t = C.data.abs()
-    t = t[t > 1]
-    if t.numel() == 0:
-        return 0
-    return int(t.max().item()).bit_length() - 1
+    st = status()
+    st.reset()
+    sfx = "i32" if C.data.dtype == _lib.torch().int32 else "i64"
+    _lib.call(f"rbg_csr_max_abs_{sfx}", _lib.ptr(C.indptr), _lib.ptr(C.data), C.n_rows, st.p, _lib.stream_handle())
+    m = int(st.read().aux[0])
+    return m.bit_length() - 1 if m > 1 else 0
 
 
 def transitive_closure_gpu(A: CSRMatrixGPU, max_iterations: int = 64) -> Tuple[CSRMatrixGPU, int]:

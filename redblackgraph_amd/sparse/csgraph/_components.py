@@ -41,21 +41,39 @@ def _device_labels(A: sp.csr_matrix):
     return labels[:n]
 
 
-def _ids_from_labels(roots: np.ndarray, q: Optional[Dict[int, int]]) -> np.ndarray:
-    uniq, ids, counts = np.unique(roots, return_inverse=True, return_counts=True)
-    if q is not None:
-        for cid, c in enumerate(counts):
-            q[int(cid)] = int(c)
-    return ids.astype(np.uint32)
+def _device_labels_dev(dA):
+    """_device_labels for a device CSR (sparse_dev.DevCSR)."""
+    t = _lib.torch()
+    n = dA.n_rows
+    labels = t.empty(max(n, 1), dtype=t.int32, device=dA.indices.device)
+    flags = t.zeros(2, dtype=t.int32, device=dA.indices.device)
+    rounds = ctypes.c_int32(0)
+    _lib.call("rbg_components", _lib.ptr(dA.indptr), _lib.ptr(dA.indices), n, _lib.ptr(labels), _lib.ptr(flags),
+              ctypes.byref(rounds), _lib.stream_handle())
+    return labels
 
 
 def find_components_sparse(A, q: Optional[Dict[int, int]] = None) -> np.ndarray:
     """_components.pyx:86-177: component id per vertex (uint32), ids in order
     of each component's smallest vertex; q receives id -> vertex count."""
+    from ... import sparse_dev as sd
     A = _csr(A)
-    if A.shape[0] == 0:
+    n = A.shape[0]
+    if n == 0:
         return np.zeros(0, dtype=np.uint32)
-    return _ids_from_labels(_device_labels(A).cpu().numpy(), q)
+    t = _lib.torch()
+    labels = _device_labels(A)
+    dev = labels.device
+    comp, local, members, sizes = (t.empty(n, dtype=t.int32, device=dev) for _ in range(4))
+    starts = t.empty(n + 1, dtype=t.int64, device=dev)
+    ncomp = t.empty(1, dtype=t.int64, device=dev)
+    temp = sd.glue_temp(n, dev)
+    _lib.call("rbg_components_layout", _lib.ptr(labels), n, _lib.ptr(comp), _lib.ptr(local), _lib.ptr(members),
+              _lib.ptr(sizes), _lib.ptr(starts), _lib.ptr(ncomp), None, _lib.ptr(temp), temp.numel(), _lib.stream_handle())
+    if q is not None:
+        for cid, c in enumerate(sizes[:int(ncomp.item())].cpu().numpy()):
+            q[int(cid)] = int(c)
+    return comp.cpu().numpy().astype(np.uint32)
 
 
 def find_components(A, q: Optional[Dict[int, int]] = None) -> np.ndarray:
@@ -106,6 +124,20 @@ def merge_component_matrices(components: List[Tuple[sp.csr_matrix, np.ndarray]],
     return out
 
 
+def _triangularity(dA) -> int:
+    """1 when every off-diagonal entry of the device CSR dA lies below the
+    diagonal, 2 when above, else 0 (counted on the device).  Every component
+    then is too: its vertices keep their relative order."""
+    from ...dense import status
+    st = status()
+    st.reset()
+    _lib.call("rbg_csr_triangles", _lib.ptr(dA.indptr), _lib.ptr(dA.indices), dA.n_rows, st.p, _lib.stream_handle())
+    s = st.read()
+    below, diag = int(s.aux[0]), int(s.aux[1])
+    above = dA.nnz - below - diag
+    return 1 if above == 0 else (2 if below == 0 else 0)
+
+
 def component_wise_closure(A, method: str = "auto", densify_threshold: int = 500):
     """transitive_closure.py:85-195: close every connected component on its
     own and merge the closed components into one CSR matrix.
@@ -129,60 +161,66 @@ def component_wise_closure(A, method: str = "auto", densify_threshold: int = 500
     n = A.shape[0]
     if n == 0:
         return TransitiveClosure(sp.csr_matrix((0, 0), dtype=np.int32), 0)
+    from ... import sparse_dev as sd
     A = sp.csr_matrix((A.data.astype(np.int32), A.indices, A.indptr), shape=A.shape)
-    A.sum_duplicates()
-    A.sort_indices()
-    roots = _device_labels(A).cpu().numpy()
-    ids = _ids_from_labels(roots, None).astype(np.int64)
-    ncomp = int(ids.max()) + 1
+    # sorted rows, duplicates combined, zeros dropped: on the device
+    dA = sd.canonicalize(sd.from_scipy(A, np.int32))
+    dev = dA.indices.device
+    sh = _lib.stream_handle()
+    labels = _device_labels_dev(dA)
+    comp, local, members, sizes = (t.empty(n, dtype=t.int32, device=dev) for _ in range(4))
+    starts = t.empty(n + 1, dtype=t.int64, device=dev)
+    ncomp_d = t.empty(1, dtype=t.int64, device=dev)
+    temp = sd.glue_temp(n, dev)
+    _lib.call("rbg_components_layout", _lib.ptr(labels), n, _lib.ptr(comp), _lib.ptr(local), _lib.ptr(members),
+              _lib.ptr(sizes), _lib.ptr(starts), _lib.ptr(ncomp_d), None, _lib.ptr(temp), temp.numel(), sh)
+    ncomp = int(ncomp_d.item())
     if ncomp == 1:
-        res = transitive_closure(A, method="FW" if method == "FW" else "D")
+        res = transitive_closure(sd.to_scipy(dA), method="FW" if method == "FW" else "D")
         W = res.W if sp.isspmatrix(res.W) else sp.csr_matrix(np.asarray(res.W))
         return TransitiveClosure(W, res.diameter)
-    # vertices of each component in increasing order, and each vertex's local index
-    order = np.argsort(ids, kind="stable")
-    sizes = np.bincount(ids, minlength=ncomp)
-    starts = np.concatenate([[0], np.cumsum(sizes)[:-1]])
-    local = np.empty(n, dtype=np.int64)
-    local[order] = np.arange(n) - starts[ids[order]]
-    b = BatchedClosure(sizes)
+    sizes_h = sizes[:ncomp].cpu().numpy().astype(np.int64)
+    b = BatchedClosure(sizes_h, tri=_triangularity(dA))
     try:
-        dev = b.pool.device
-        rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(A.indptr))
-        comp = ids[rows]
-        flat = (b.layout.offset[comp] + local[rows] * b.layout.npad[comp] + local[A.indices])
+        off = t.from_numpy(b.layout.offset.astype(np.int64)).to(dev)
+        npad = t.from_numpy(b.layout.npad.astype(np.int32)).to(dev)
+        b.stream.wait_stream(t.cuda.current_stream())  # the layout, on the current stream
+        bsh = _lib.stream_handle(b.stream)
         with t.cuda.stream(b.stream):
             b.pool.zero_()
-            b.pool.index_put_((t.from_numpy(flat).to(dev),), t.from_numpy(A.data.astype(np.int32)).to(dev))
+        # extract_submatrix for every component at once: each entry into its
+        # component's padded matrix in the batch pool
+        _lib.call("rbg_components_scatter", _lib.ptr(dA.indptr), _lib.ptr(dA.indices), _lib.ptr(dA.data), n,
+                  _lib.ptr(comp), _lib.ptr(local), _lib.ptr(off), _lib.ptr(npad), _lib.ptr(b.pool), bsh)
         b.run(graph=False)
-        verts = [order[starts[c]:starts[c] + sizes[c]] for c in range(ncomp)]
+        host = {}
 
         def source(c):
-            sub, _ = extract_submatrix(A, verts[c])
+            # the component's input again (only for components the fast kernels hand back)
+            if not host:
+                host["members"] = members.cpu().numpy()
+                host["starts"] = starts.cpu().numpy()
+            s0 = int(host["starts"][c])
+            sub, _ = extract_submatrix(A, np.sort(host["members"][s0:s0 + int(sizes_h[c])]))
             return _lib.to_device(sub.toarray().astype(np.int32))
         outs = b.outcomes(source)
         et = elem_type(np.int32)
         for c, out in enumerate(outs):
             raise_fw_errors(out, et, "csgraph")
         diameter = max(out.diameter for out in outs)
-        # merge on the device: every nonzero of the pool back to global indices
-        nz = t.nonzero(b.pool).flatten()
-        vals = b.pool[nz]
-        offs = t.from_numpy(b.layout.offset).to(dev)
-        npad = t.from_numpy(b.layout.npad).to(dev)
-        c_of = t.searchsorted(offs, nz, right=True) - 1
-        rel = nz - offs[c_of]
-        lr, lc = rel // npad[c_of], rel % npad[c_of]
-        gbase = t.from_numpy(starts).to(dev)
-        gorder = t.from_numpy(order).to(dev)
-        gr = gorder[gbase[c_of] + lr]
-        gc = gorder[gbase[c_of] + lc]
-        key = gr * n + gc
-        srt = t.argsort(key)
-        gr, gc, vals = gr[srt], gc[srt], vals[srt]
-        indptr = t.zeros(n + 1, dtype=t.int64, device=dev)
-        indptr[1:] = t.cumsum(t.bincount(gr, minlength=n), 0)
-        W = sp.csr_matrix((vals.cpu().numpy(), gc.to(t.int32).cpu().numpy(), indptr.cpu().numpy()), shape=(n, n))
+        # merge_component_matrices on the device: the closed components back
+        # into one CSR in global vertex order
+        indptr = t.empty(n + 1, dtype=t.int64, device=dev)
+        args = [_lib.ptr(b.pool), _lib.ptr(off), _lib.ptr(npad), _lib.ptr(comp), _lib.ptr(local), _lib.ptr(members),
+                _lib.ptr(starts), _lib.ptr(sizes), n, _lib.ptr(indptr)]
+        _lib.call("rbg_components_merge", *args, None, None, _lib.ptr(temp), temp.numel(), bsh)
+        b.stream.synchronize()
+        nnz = int(indptr[n].item())
+        oi = t.empty(max(nnz, 1), dtype=t.int32, device=dev)
+        ov = t.empty(max(nnz, 1), dtype=t.int32, device=dev)
+        _lib.call("rbg_components_merge", *args, _lib.ptr(oi), _lib.ptr(ov), _lib.ptr(temp), temp.numel(), bsh)
+        b.stream.synchronize()
+        W = sp.csr_matrix((ov[:nnz].cpu().numpy(), oi[:nnz].cpu().numpy(), indptr.cpu().numpy()), shape=(n, n))
         return TransitiveClosure(W, int(diameter))
     finally:
         b.close()

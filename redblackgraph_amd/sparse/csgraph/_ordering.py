@@ -133,14 +133,23 @@ def _permute_csr_device(indptr, indices, data, n: int, nnz: int, dt, perm, strea
 
 
 def _upper_csr(optr, oidx, oval, n: int):
-    """Keeps the entries on or above the diagonal (_permutation.pyx:138-157)."""
+    """Keeps the entries on or above the diagonal (_permutation.pyx:138-157):
+    rbg_csr_upper, counting pass then filling pass."""
     t = _lib.torch()
-    rows = t.repeat_interleave(t.arange(n, device=optr.device), optr[1:] - optr[:-1])
-    keep = oidx.to(t.int64) >= rows
-    counts = t.bincount(rows[keep], minlength=n)
-    ptr = t.zeros(n + 1, dtype=t.int64, device=optr.device)
-    ptr[1:] = t.cumsum(counts, 0)
-    return ptr, oidx[keep], oval[keep]
+    from ...sparse_dev import glue_temp
+    dev = optr.device
+    fn = f"rbg_csr_upper_{'i32' if oval.dtype == t.int32 else 'i64'}"
+    ptr = t.empty(n + 1, dtype=t.int64, device=dev)
+    temp = glue_temp(n, dev)
+    sh = _lib.stream_handle()
+    _lib.call(fn, _lib.ptr(optr), _lib.ptr(oidx), _lib.ptr(oval), n, _lib.ptr(ptr), None, None, _lib.ptr(temp),
+              temp.numel(), sh)
+    nnz = int(ptr[n].item())
+    oi = t.empty(max(nnz, 1), dtype=t.int32, device=dev)
+    ov = t.empty(max(nnz, 1), dtype=oval.dtype, device=dev)
+    _lib.call(fn, _lib.ptr(optr), _lib.ptr(oidx), _lib.ptr(oval), n, _lib.ptr(ptr), _lib.ptr(oi), _lib.ptr(ov),
+              _lib.ptr(temp), temp.numel(), sh)
+    return ptr, oi[:nnz], ov[:nnz]
 
 
 def permute_sparse(A, p, assume_upper_triangular: bool = False) -> sp.csr_matrix:
@@ -213,9 +222,14 @@ def canonical_order_device(C, stream=None):
     rounds = ctypes.c_int32(0)
     _lib.call("rbg_components", _lib.ptr(C.indptr), _lib.ptr(C.indices), n, _lib.ptr(labels), _lib.ptr(flags),
               ctypes.byref(rounds), sh)
-    _, inv, counts = t.unique(labels[:n], sorted=True, return_inverse=True, return_counts=True)
-    comp = inv.to(t.int32)
-    csize = counts.to(t.int32)[inv]
+    # component ids by smallest vertex, and each vertex's component size
+    comp, local, members, sizes, csize = (t.empty(n, dtype=t.int32, device=dev) for _ in range(5))
+    starts = t.empty(n + 1, dtype=t.int64, device=dev)
+    ncomp = t.empty(1, dtype=t.int64, device=dev)
+    gtemp = sd.glue_temp(n, dev)
+    _lib.call("rbg_components_layout", _lib.ptr(labels), n, _lib.ptr(comp), _lib.ptr(local), _lib.ptr(members),
+              _lib.ptr(sizes), _lib.ptr(starts), _lib.ptr(ncomp), _lib.ptr(csize), _lib.ptr(gtemp), gtemp.numel(), sh)
+    counts = sizes[:int(ncomp.item())]
     anc, maxrel, diag = (t.empty(n, dtype=t.int64, device=dev) for _ in range(3))
     _lib.call(f"rbg_order_metrics_csr_{_sfx(dt)}", _lib.ptr(C.indptr), _lib.ptr(C.indices), _lib.ptr(C.data), n,
               _lib.ptr(anc), _lib.ptr(maxrel), _lib.ptr(diag), sh)

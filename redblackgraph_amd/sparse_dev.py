@@ -242,10 +242,11 @@ def spgemm(A: DevCSR, B: DevCSR, mask_upper: bool = False, width: int | None = N
         return DevCSR(t.zeros(m + 1, dtype=t.int64, device=dev), t.empty(0, dtype=t.int32, device=dev),
                       t.empty(0, dtype=vdt, device=dev), (m, B.shape[1])), 0
     # the row bounds sum to the expansion size: sum over A's entries of |B row|
-    blen = (B.indptr[1:] - B.indptr[:-1])
-    expand = int(blen[A.indices.long()].sum().item())
     st = status()
     sh = _lib.stream_handle(stream)
+    st.reset(stream)
+    _lib.call("rbg_csr_expand_size", _lib.ptr(A.indptr), _lib.ptr(A.indices), m, _lib.ptr(B.indptr), st.p, sh)
+    expand = int(st.read(stream).aux[0])
     sfx = "i32" if vdt == t.int32 else "i64"
     ws = t.empty(int(_lib.load().rbg_dag_workspace_bytes(m, 0)), dtype=t.uint8, device=dev)
     cap = max(1024, expand)
@@ -322,25 +323,59 @@ def csr_equal(A: DevCSR, B: DevCSR) -> bool:
 DENSE_INSERT_MAX_BYTES = 4 << 30
 
 
-def to_dense(C: DevCSR):
-    """The n x n device matrix of a CSR (absent entries 0)."""
+def _sfx(dt) -> str:
+    return "i32" if dt == _lib.torch().int32 else "i64"
+
+
+def glue_temp(n: int, device):
+    t = _lib.torch()
+    return t.empty(int(_lib.load().rbg_glue_temp_bytes(max(n, 1))), dtype=t.uint8, device=device)
+
+
+def to_dense(C: DevCSR, stream=None):
+    """The n x m device matrix of a CSR (absent entries 0): rbg_csr_to_dense."""
     t = _lib.torch()
     n, m = C.shape
-    D = t.zeros((n, m), dtype=C.data.dtype, device=C.indices.device)
-    if C.nnz:
-        rows = t.repeat_interleave(t.arange(n, device=D.device), C.indptr[1:] - C.indptr[:-1])
-        D[rows, C.indices.long()] = C.data
+    D = t.empty((n, m), dtype=C.data.dtype, device=C.indices.device)
+    if n:
+        _lib.call(f"rbg_csr_to_dense_{_sfx(C.data.dtype)}", _lib.ptr(C.indptr), _lib.ptr(C.indices), _lib.ptr(C.data),
+                  n, m, _lib.ptr(D), _lib.stream_handle(stream))
     return D
 
 
-def from_dense(D) -> DevCSR:
-    """Canonical CSR of a device matrix (row-major order, zeros dropped)."""
+def from_dense(D, stream=None) -> DevCSR:
+    """Canonical CSR of a device matrix (row-major order, zeros dropped):
+    rbg_dense_to_csr, counting pass then filling pass."""
     t = _lib.torch()
-    mask = D != 0
-    indptr = t.zeros(D.shape[0] + 1, dtype=t.int64, device=D.device)
-    t.cumsum(mask.sum(1), 0, out=indptr[1:])
-    nz = mask.nonzero()
-    return DevCSR(indptr, nz[:, 1].to(t.int32).contiguous(), D[mask].contiguous(), tuple(D.shape))
+    D = D.contiguous()
+    n, m = D.shape
+    indptr = t.empty(n + 1, dtype=t.int64, device=D.device)
+    temp = glue_temp(n, D.device)
+    fn = f"rbg_dense_to_csr_{_sfx(D.dtype)}"
+    sh = _lib.stream_handle(stream)
+    _lib.call(fn, _lib.ptr(D), n, m, _lib.ptr(indptr), None, None, _lib.ptr(temp), temp.numel(), sh)
+    nnz = int(indptr[n].item()) if n else 0
+    idx = t.empty(nnz, dtype=t.int32, device=D.device)
+    val = t.empty(nnz, dtype=D.dtype, device=D.device)
+    if nnz:
+        _lib.call(fn, _lib.ptr(D), n, m, _lib.ptr(indptr), _lib.ptr(idx), _lib.ptr(val), _lib.ptr(temp), temp.numel(),
+                  sh)
+    return DevCSR(indptr, idx, val, (n, m))
+
+
+def column(C: DevCSR, col: int, stream=None) -> DevCSR:
+    """Column `col` of a sorted CSR as an n x 1 CSR (rbg_csr_column)."""
+    t = _lib.torch()
+    n = C.n_rows
+    dev = C.indices.device
+    optr = t.empty(n + 1, dtype=t.int64, device=dev)
+    oi = t.empty(max(n, 1), dtype=t.int32, device=dev)
+    ov = t.empty(max(n, 1), dtype=C.data.dtype, device=dev)
+    temp = glue_temp(n, dev)
+    _lib.call(f"rbg_csr_column_{_sfx(C.data.dtype)}", _lib.ptr(C.indptr), _lib.ptr(C.indices), _lib.ptr(C.data), n,
+              int(col), _lib.ptr(optr), _lib.ptr(oi), _lib.ptr(ov), _lib.ptr(temp), temp.numel(),
+              _lib.stream_handle(stream))
+    return DevCSR(optr, oi, ov, (n, 1))
 
 
 def edge_insert_many(C: DevCSR, edges, stream=None) -> DevCSR:
@@ -367,13 +402,13 @@ def edge_insert_many(C: DevCSR, edges, stream=None) -> DevCSR:
         return C
     et = elem_type(np.int32 if C.data.dtype == t.int32 else np.int64)
     with t.cuda.stream(stream) if stream is not None else contextlib.nullcontext():
-        D = to_dense(C)
+        D = to_dense(C, stream)
         W, st = dense.edge_compose_many(D, edges, et, stream=stream, inplace=True)
         if st.flags & _lib.F_CYCLE:
             raise ValueError("Relational composition would result in a cycle.")
         if st.events:
             raise OverflowError("Avos product results in an overflow during edge composition")
-        return from_dense(W)
+        return from_dense(W, stream)
 
 
 def edge_insert(C: DevCSR, alpha: int, beta: int, relationship: int, stream=None) -> DevCSR:
@@ -388,33 +423,31 @@ def edge_insert(C: DevCSR, alpha: int, beta: int, relationship: int, stream=None
     steps run on the GPU as AVOS SpGEMMs and one CSR AVOS sum.  Raises
     ValueError when C[beta][alpha] != 0 (the edge would close a cycle).
     """
+    from .dense import status
     t = _lib.torch()
     n = C.n_rows
     if not (0 <= alpha < n and 0 <= beta < n):
         raise ValueError("vertex index out of range")
     dev = C.indices.device
     vdt = C.data.dtype
-    b0, b1 = int(C.indptr[beta].item()), int(C.indptr[beta + 1].item())
-    if bool((C.indices[b0:b1] == alpha).any()):
+    sh = _lib.stream_handle(stream)
+    # column alpha of C as an n x 1 matrix; C[beta][alpha] != 0 closes a cycle
+    X = column(C, alpha, stream)
+    xp = X.indptr[beta:beta + 2].cpu()
+    if int(xp[1]) > int(xp[0]):
         raise ValueError("Relational composition would result in a cycle.")
-    a0, a1 = int(C.indptr[alpha].item()), int(C.indptr[alpha + 1].item())
-    cols, vals = C.indices[a0:a1], C.data[a0:a1].clone()
-    hit = cols == beta
-    if bool(hit.any()):
-        vals[hit] = relationship
-    else:
-        cols = t.cat([cols, t.tensor([beta], dtype=t.int32, device=dev)])
-        vals = t.cat([vals, t.tensor([relationship], dtype=vdt, device=dev)])
-        order = t.argsort(cols)
-        cols, vals = cols[order], vals[order]
-    row = DevCSR(t.tensor([0, cols.numel()], dtype=t.int64, device=dev), cols.contiguous(), vals.contiguous(), (1, n))
+    # the composed row's seed: row alpha of C with entry beta = relationship
+    a_len = int((C.indptr[alpha + 1] - C.indptr[alpha]).item())
+    cols = t.empty(a_len + 1, dtype=t.int32, device=dev)
+    vals = t.empty(a_len + 1, dtype=vdt, device=dev)
+    st = status()
+    st.reset(stream)
+    _lib.call(f"rbg_csr_row_set_{_sfx(vdt)}", _lib.ptr(C.indptr), _lib.ptr(C.indices), _lib.ptr(C.data), alpha, beta,
+              int(relationship), _lib.ptr(cols), _lib.ptr(vals), st.p, sh)
+    k = int(st.read(stream).aux[0])
+    row = DevCSR(t.tensor([0, k], dtype=t.int64, device=dev), cols[:k], vals[:k], (1, n))
     u, _ = spgemm(row, C, stream=stream)
-    # column alpha of C as an n x 1 matrix
-    pos = t.nonzero(C.indices == alpha).flatten()
-    rows = t.searchsorted(C.indptr, pos, right=True) - 1
-    xptr = t.zeros(n + 1, dtype=t.int64, device=dev)
-    xptr[rows + 1] = 1
-    xptr = t.cumsum(xptr, 0)
-    X = DevCSR(xptr, t.zeros(pos.numel(), dtype=t.int32, device=dev), C.data[pos].contiguous(), (n, 1))
+    nx = int(X.indptr[n].item())
+    X = DevCSR(X.indptr, X.indices[:nx], X.data[:nx], (n, 1))
     R1, _ = spgemm(X, u, stream=stream)
     return avos_sum_csr(C, R1, stream=stream)

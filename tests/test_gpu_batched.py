@@ -113,3 +113,68 @@ def test_batched_fallbacks_and_errors(orc):
     res = transitive_closure_batch([good[0], off1])
     np.testing.assert_array_equal(res[1].W, orc.warshall(off1).W)
     assert res[0].diameter == orc.warshall(good[0]).diameter
+
+
+@pytest.mark.parametrize("tile,reverse", [(128, False), (64, False), (128, True)])
+def test_batched_triangular_vs_oracle(orc, tile, reverse):
+    """Components declared triangular take the triangular schedule: lower
+    (parents first) and upper (numbered backwards) pedigrees equal the
+    oracle's naive loop, W and trajectory diameter."""
+    mats = _components(40, 16, 600, 7)
+    if reverse:
+        mats = [synth.reverse_vertices(A) for A in mats]
+    b = BatchedClosure([A.shape[0] for A in mats], tiles={p: tile for p in range(128, 4097, 128)},
+                       tri=2 if reverse else 1)
+    try:
+        b.load_dense(mats)
+        b.run(graph=False)
+        outs = b.outcomes()
+        for i, A in enumerate(mats):
+            ref = orc.warshall(A)
+            assert outs[i].mode == "fast", i
+            np.testing.assert_array_equal(b.result(i).cpu().numpy(), ref.W, err_msg=f"component {i}")
+            assert outs[i].diameter == ref.diameter, i
+    finally:
+        b.close()
+
+
+def test_batched_triangular_large_components_and_replays():
+    """Components above 2048 vertices (where most of the batched work is)
+    through the triangular schedule: equal to the single-matrix path (whose
+    128-tile kernel the oracle digests pin), eagerly and in graph replays; a
+    component declared triangular that is not is caught and closed again."""
+    t = _lib.torch()
+    sizes, diag, edges = synth.batched_workload(count=24, lo=1500, hi=4096, seed=9)
+    b = BatchedClosure(sizes, tri=1)
+    try:
+        b.load_edges(diag, edges)
+        b.run(graph=False)
+        b.stream.synchronize()
+        eager = b.pool.clone()
+        outs = b.outcomes()
+        et = elem_type(np.int32)
+        for i in range(len(sizes)):
+            n = int(sizes[i])
+            A = t.zeros((n, n), dtype=t.int32, device=b.pool.device)
+            r, c, v = edges[i]
+            A[t.from_numpy(r).to(A.device), t.from_numpy(c).to(A.device)] = t.from_numpy(v.astype(np.int32)).to(A.device)
+            A.diagonal().copy_(t.from_numpy(diag[i]).to(A.device))
+            W, out = dense.fw_closure(A, et, schedule="full")
+            assert outs[i].mode == "fast"
+            assert bool(t.equal(W, b.result(i))), i
+            assert out.status.diameter_max == outs[i].status.diameter_max, i
+        for rep in range(2):
+            b.load_edges(diag, edges)
+            b.run(graph=True)
+            b.stream.synchronize()
+            assert bool(t.equal(b.pool, eager)), f"replay {rep}"
+    finally:
+        b.close()
+    # a shuffled (non-triangular) component declared lower triangular
+    mats = _components(3, 100, 300, 11)
+    bad = synth.shuffle_vertices(mats[1], 3)
+    mats = [mats[0], bad, mats[2]]
+    res = transitive_closure_batch(mats, tri=1)
+    W, out = dense.fw_closure(_lib.to_device(bad), elem_type(np.int32))
+    np.testing.assert_array_equal(res[1].W, W.cpu().numpy())
+    assert res[1].diameter == out.diameter

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 import scipy.sparse as sp
 
-from redblackgraph_amd import synth
+from redblackgraph_amd import _lib, synth
 from redblackgraph_amd.sparse import csgraph as cg
 
 pytestmark = pytest.mark.gpu
@@ -143,3 +143,88 @@ def test_cycle_in_one_component(orc):
     with pytest.raises(cg.CycleError) as e:
         cg.component_wise_closure(sp.csr_matrix(A))
     assert e.value.vertex == rep["first_i"]
+
+
+# ---------------------------------------------------------------------------
+# device helpers of the component-wise closure and the CSR paths
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("dt", [np.int32, np.int64])
+def test_dense_csr_round_trip(dt):
+    """rbg_csr_to_dense / rbg_dense_to_csr against scipy, rows of every length
+    class (empty, < 64, > 64 entries)."""
+    from redblackgraph_amd import sparse_dev as sd
+    rng = np.random.default_rng(4)
+    n, m = 700, 530
+    D = np.where(rng.random((n, m)) < 0.05, rng.integers(-1, 1 << 20, (n, m)), 0).astype(dt)
+    D[10] = 0
+    D[11, :200] = rng.integers(2, 9, 200)
+    C = sd.from_dense(_lib.to_device(D))
+    ref = sp.csr_matrix(D)
+    ref.sort_indices()
+    np.testing.assert_array_equal(C.indptr.cpu().numpy(), ref.indptr)
+    np.testing.assert_array_equal(C.indices.cpu().numpy(), ref.indices)
+    np.testing.assert_array_equal(C.data.cpu().numpy(), ref.data)
+    np.testing.assert_array_equal(sd.to_dense(C).cpu().numpy(), D)
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_components_layout_scatter_merge(seed):
+    """rbg_components_layout against numpy (ids by smallest vertex, local
+    order, members, sizes, starts); scatter then merge returns the graph."""
+    from redblackgraph_amd import sparse_dev as sd
+    t = _lib.torch()
+    rng = np.random.default_rng(seed)
+    blocks = [synth.dense_pedigree(synth.PedigreeSpec(int(k), 4, 0.8, 2), seed * 7 + i)
+              for i, k in enumerate(rng.integers(1, 90, 30))]
+    A = sp.block_diag(blocks).tocsr()
+    p = rng.permutation(A.shape[0])
+    A = sp.csr_matrix(A.toarray()[np.ix_(p, p)].astype(np.int32))
+    n = A.shape[0]
+    dA = sd.from_scipy(A, np.int32)
+    from redblackgraph_amd.sparse.csgraph._components import _device_labels_dev
+    labels = _device_labels_dev(dA)
+    dev = labels.device
+    comp, local, members, sizes = (t.empty(n, dtype=t.int32, device=dev) for _ in range(4))
+    starts = t.empty(n + 1, dtype=t.int64, device=dev)
+    ncomp = t.empty(1, dtype=t.int64, device=dev)
+    temp = sd.glue_temp(n, dev)
+    _lib.call("rbg_components_layout", _lib.ptr(labels), n, _lib.ptr(comp), _lib.ptr(local), _lib.ptr(members),
+              _lib.ptr(sizes), _lib.ptr(starts), _lib.ptr(ncomp), None, _lib.ptr(temp), temp.numel(), _lib.stream_handle())
+    nc, lab = sp.csgraph.connected_components(A, directed=True, connection="weak")
+    first = np.full(nc, n)
+    np.minimum.at(first, lab, np.arange(n))
+    rank = np.empty(nc, dtype=np.int64)
+    rank[np.argsort(first)] = np.arange(nc)
+    ids = rank[lab]
+    assert int(ncomp.item()) == nc
+    np.testing.assert_array_equal(comp.cpu().numpy(), ids)
+    mem = np.argsort(ids, kind="stable")
+    np.testing.assert_array_equal(members.cpu().numpy(), mem)
+    sz = np.bincount(ids, minlength=nc)
+    np.testing.assert_array_equal(sizes.cpu().numpy()[:nc], sz)
+    st = np.concatenate([[0], np.cumsum(sz)])
+    np.testing.assert_array_equal(starts.cpu().numpy()[:nc + 1], st)
+    loc = np.empty(n, dtype=np.int64)
+    loc[mem] = np.arange(n) - st[ids[mem]]
+    np.testing.assert_array_equal(local.cpu().numpy(), loc)
+    # scatter into padded matrices, then merge back: the graph itself
+    npad = ((sz + 127) // 128 * 128).astype(np.int32)
+    off = np.concatenate([[0], np.cumsum(npad.astype(np.int64) ** 2)[:-1]])
+    pool = t.zeros(int((npad.astype(np.int64) ** 2).sum()), dtype=t.int32, device=dev)
+    offd, npd = _lib.to_device(off.astype(np.int64)), _lib.to_device(npad)
+    sh = _lib.stream_handle()
+    _lib.call("rbg_components_scatter", _lib.ptr(dA.indptr), _lib.ptr(dA.indices), _lib.ptr(dA.data), n,
+              _lib.ptr(comp), _lib.ptr(local), _lib.ptr(offd), _lib.ptr(npd), _lib.ptr(pool), sh)
+    indptr = t.empty(n + 1, dtype=t.int64, device=dev)
+    args = [_lib.ptr(pool), _lib.ptr(offd), _lib.ptr(npd), _lib.ptr(comp), _lib.ptr(local), _lib.ptr(members),
+            _lib.ptr(starts), _lib.ptr(sizes), n, _lib.ptr(indptr)]
+    _lib.call("rbg_components_merge", *args, None, None, _lib.ptr(temp), temp.numel(), sh)
+    nnz = int(indptr[n].item())
+    oi = t.empty(nnz, dtype=t.int32, device=dev)
+    ov = t.empty(nnz, dtype=t.int32, device=dev)
+    _lib.call("rbg_components_merge", *args, _lib.ptr(oi), _lib.ptr(ov), _lib.ptr(temp), temp.numel(), sh)
+    ref = A.copy()
+    ref.sort_indices()
+    np.testing.assert_array_equal(indptr.cpu().numpy(), ref.indptr)
+    np.testing.assert_array_equal(oi.cpu().numpy(), ref.indices)
+    np.testing.assert_array_equal(ov.cpu().numpy(), ref.data)

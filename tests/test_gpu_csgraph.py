@@ -90,12 +90,13 @@ def test_topological_sort_cycles(M, cycle):
 
 def test_topological_sort_cycle_in_pedigree():
     """A back edge in a 5000-vertex pedigree: the reported vertex lies on a cycle."""
-    A = synth.csr_pedigree(synth.PedigreeSpec(5000, 12, 0.8, 2), 3, dtype=np.int32).tolil()
-    A[100, 3000] = 2  # vertex 100 (an ancestor of 3000's line?) gets 3000 as a parent
+    A = synth.csr_pedigree(synth.PedigreeSpec(5000, 12, 0.8, 2), 3, dtype=np.int32)
+    anc = sp.csgraph.breadth_first_order(A, 3000, directed=True, return_predecessors=False)
+    far = int(anc[-1])  # an ancestor of 3000 (edges run child -> parent)
+    assert far != 3000
+    A = A.tolil()
+    A[far, 3000] = 2  # ... that now has 3000 as a parent: a cycle
     A = A.tocsr()
-    reach = sp.csgraph.breadth_first_order(A, 3000, directed=True, return_predecessors=False)
-    if 100 not in set(reach.tolist()):
-        pytest.skip("the planted edge closes no cycle in this seed")
     with pytest.raises(CycleError) as e:
         cg.topological_sort(A)
     v = e.value.vertex

@@ -233,6 +233,39 @@ def test_fw_triangular_schedule_vs_oracle(orc, dt, n, reverse):
     assert out2.diameter == ref.diameter
 
 
+@pytest.mark.parametrize("dt,n,wrong", [(np.int32, 640, 2), (np.int32, 640, 1), (np.int64, 384, 2)])
+def test_fw_triangular_wrong_side_detected(dt, n, wrong):
+    """rbg_fw_dense_tri_* told the wrong side (tri = 2 for a lower triangular
+    pedigree, or tri = 1 once one entry above the diagonal is added) stops
+    the fast run with RBG_F_NEED_GENERAL instead of returning a partial
+    closure; the right side on the same input returns the oracle's W."""
+    t = _lib.torch()
+    A = synth.dense_pedigree(synth.component_spec(n, 0.8, 4), 5 + n, dtype=dt)
+    if wrong == 1:
+        A = A.copy()
+        A[3, n - 7] = 2
+    sfx = "i32" if dt == np.int32 else "i64"
+    esz = np.dtype(dt).itemsize
+    dW = _lib.to_device(A)
+    ws = t.empty(int(_lib.load().rbg_fw_workspace_bytes(n, esz)), dtype=t.uint8, device=dW.device)
+    st = dense.status()
+    st.reset()
+    _lib.call(f"rbg_fw_dense_tri_{sfx}", _lib.ptr(dW), n, n, 8 * esz, 0, 0, wrong, _lib.ptr(ws), ws.numel(), st.p,
+              _lib.stream_handle(None))
+    assert st.read().flags & _lib.F_NEED_GENERAL
+    if wrong == 2:
+        from oracle import oracle as orc
+        dW = _lib.to_device(A)
+        st.reset()
+        _lib.call(f"rbg_fw_dense_tri_{sfx}", _lib.ptr(dW), n, n, 8 * esz, 0, 0, 1, _lib.ptr(ws), ws.numel(), st.p,
+                  _lib.stream_handle(None))
+        s = st.read()
+        assert not s.flags & _lib.F_NEED_GENERAL
+        ref = orc.warshall(A)
+        np.testing.assert_array_equal(dW.cpu().numpy(), ref.W)
+        assert int(s.diameter_max).bit_length() - 1 == ref.diameter
+
+
 def _digests():
     import json
     import os
@@ -480,3 +513,98 @@ def test_fw_overflow_chain(orc, dt, n):
         rb.array(A).transitive_closure()
     W, out = _fw(A)
     assert out.mode == "general" and out.status.events != 0
+
+
+# ---------------------------------------------------------------------------
+# dense AVOS product, fast path
+# ---------------------------------------------------------------------------
+def _mm_call(name_sfx, A, B, fast):
+    """C = A (x) B through one C-ABI entry (fast: rbg_matmul_fast_*, else the
+    general rbg_matmul_*).  Returns (C, status)."""
+    t = _lib.torch()
+    M, K = A.shape
+    N = B.shape[1]
+    dA, dB = _lib.to_device(A), _lib.to_device(B)
+    C = t.empty((M, N), dtype=dA.dtype, device=dA.device)
+    st = dense.status()
+    st.reset()
+    sh = _lib.stream_handle(None)
+    if fast:
+        ws = t.empty(int(_lib.load().rbg_matmul_fast_workspace_bytes(M, K, N, dA.element_size())), dtype=t.uint8,
+                     device=dA.device)
+        _lib.call(f"rbg_matmul_fast_{name_sfx}", _lib.ptr(dA), K, _lib.ptr(dB), N, _lib.ptr(C), N, M, K, N,
+                  _lib.ptr(ws), ws.numel(), st.p, sh)
+    else:
+        width = 32 if name_sfx == "i32" else 64
+        _lib.call(f"rbg_matmul_{name_sfx}", _lib.ptr(dA), K, _lib.ptr(dB), N, _lib.ptr(C), N, M, K, N, width, 0,
+                  st.p, sh)
+    return C.cpu().numpy(), st.read()
+
+
+@pytest.mark.parametrize("dt,n,shape", [(np.int32, 300, "square"), (np.int32, 300, "rect"), (np.int64, 200, "square"),
+                                        (np.int64, 200, "rect"), (np.int32, 70, "square")])
+def test_matmul_fast_vs_general_and_oracle(orc, dt, n, shape):
+    """Closures of pedigrees (the fast domain): the fast entry equals the
+    general kernel and the oracle's 'ij,jk' loop, for square and rectangular
+    operands whose sizes are not multiples of the tile edges (diagonal terms
+    of A and B fall inside and outside the product's range)."""
+    A = synth.dense_pedigree(synth.component_spec(n, 0.8, 4), 41 + n, dtype=dt)
+    W = orc.warshall(A).W
+    if shape == "square":
+        X, Y = W, W
+    else:
+        # leading blocks keep the diagonal on the diagonal (the fast domain)
+        X, Y = W[: n - 37, : n - 11], W[: n - 11, : n - 5]
+    sfx = "i32" if dt == np.int32 else "i64"
+    st = dense.status()
+    st.reset()
+    for M in (X, Y):
+        _lib.call(f"rbg_matmul_precheck_{sfx}", _lib.ptr(_lib.to_device(np.ascontiguousarray(M))), M.shape[1],
+                  M.shape[0], M.shape[1], st.p, _lib.stream_handle(None))
+    assert not st.read().flags & _lib.F_PRECOND_FAIL
+    Cf, sf = _mm_call(sfx, np.ascontiguousarray(X), np.ascontiguousarray(Y), True)
+    Cg, sg = _mm_call(sfx, np.ascontiguousarray(X), np.ascontiguousarray(Y), False)
+    assert not sf.flags & _lib.F_NEED_GENERAL and sg.events == 0
+    np.testing.assert_array_equal(Cf, Cg)
+    ref, ev = orc.matmul(np.ascontiguousarray(X), np.ascontiguousarray(Y))
+    np.testing.assert_array_equal(Cf, ref)
+
+
+def test_matmul_fast_dispatch_and_fallbacks(orc):
+    """rb.array @ on a closure large enough for the fast path equals the
+    oracle; operands outside the domain (an off-diagonal 1) and products
+    whose terms could overflow (long paternal chains) take the general
+    kernel, with its events."""
+    import redblackgraph_amd as rb
+    A = synth.dense_pedigree(synth.component_spec(400, 0.8, 4), 5)
+    W = orc.warshall(A).W
+    ref, _ = orc.matmul(W, W)
+    np.testing.assert_array_equal(rb.array(W) @ rb.array(W), ref)
+    C, s = dense.matmul(_lib.to_device(W), _lib.to_device(W), elem_type(np.int32))
+    np.testing.assert_array_equal(C.cpu().numpy(), ref)
+    X = W.copy()
+    X[300, 20] = 1  # outside the domain
+    ref2, _ = orc.matmul(X, W)
+    C2, _ = dense.matmul(_lib.to_device(X), _lib.to_device(W), elem_type(np.int32))
+    np.testing.assert_array_equal(C2.cpu().numpy(), ref2)
+    # a 31-generation paternal chain: its closure squared stays within int32
+    # (W[i][k] (x) W[k][j] = 2^(i-j)), so the guard passes it and the result
+    # is the oracle's; times its transpose the terms 2^(i-k) (x) 2^(j-k)
+    # overflow, the guard sends it to the general kernel, which raises
+    n = 256
+    Ch = np.zeros((n, n), dtype=np.int32)
+    np.fill_diagonal(Ch, -1)
+    for i in range(1, 31):
+        Ch[i, i - 1] = 2
+    Wc = orc.warshall(Ch).W
+    refc, evc = orc.matmul(Wc, Wc)
+    assert evc == 0
+    Cc, sc = _mm_call("i32", Wc, Wc, True)
+    assert not sc.flags & (_lib.F_NEED_GENERAL | _lib.F_PRECOND_FAIL)
+    np.testing.assert_array_equal(Cc, refc)
+    WcT = np.ascontiguousarray(Wc.T)
+    assert orc.matmul(Wc, WcT)[1] & 1  # the width overflow event
+    Cf, sf = _mm_call("i32", Wc, WcT, True)
+    assert sf.flags & _lib.F_NEED_GENERAL
+    with pytest.raises(OverflowError):
+        rb.array(Wc) @ rb.array(WcT)

@@ -136,7 +136,7 @@ def test_permute_sparse_long_rows(dtype):
     A = sp.csr_matrix(A, dtype=dtype)
     A.sum_duplicates()
     lens = np.diff(A.indptr)
-    assert lens.max() == 5000 and (lens > 256).sum() >= 4 and (lens <= 256).sum() > n - 10
+    assert lens.max() >= 5000 and (lens > 256).sum() >= 4 and (lens <= 256).sum() > n - 10
     p = rng.permutation(n).astype(np.int32)
     B = _cg().permute_sparse(A, p)
     ref = A[p][:, p].tocsr()
