@@ -9,9 +9,12 @@ copy, fast-path precheck, closure, status read-back).
 
 Prints one JSON line (the bench contract) with:
   value          closure entries/s = N^2 / seconds per closure
-  roofline       the dominant kernel (k_fw_tile_fast<int>, phase 3): integer
-                 VALU ops it must execute (2.5 per relaxation) over its average
-                 launch duration, measured with hipEvents on its launch stream
+  roofline       the dominant kernel (for the lower triangular N=8192 input
+                 the fused look-ahead step k_fw_step_fast<int>: pivot block
+                 K's phase-3 tiles and block K+1's panel in one launch):
+                 integer VALU ops it must execute (2.5 per relaxation) over
+                 its average launch duration, measured with hipEvents on its
+                 launch stream
                  (recorded on every --prof-every-th timed step, 4 by default,
                  so that the events' own stalls stay out of most steps)
   cpu_baseline   the oracle's single-core naive loop (a C restatement of the
@@ -19,9 +22,10 @@ Prints one JSON line (the bench contract) with:
                  workload by the N^3 law
 The default (--workload all) nests the rest of the metric under
 "sub_results": the sparse 1M-vertex closure, the batched 4096-component
-closure against both per-graph paths, the N=4096 int64 closure and the
-served stream.  Single workloads: --workload dense8192, dense4096_i64,
-dense512, sparse1m, batched, serve, order1m, hbmcopy.
+closure against both per-graph paths, the N=4096 int64 closure, the
+served stream, the dense N=8192 product and the canonical ordering of the
+1M closure.  Single workloads: --workload dense8192, dense4096_i64,
+dense512, sparse1m, batched, serve, order1m, matmul8192, hbmcopy.
 """
 from __future__ import annotations
 
@@ -39,6 +43,7 @@ METRIC = "AVOS closure entries/sec, dense N=8192 & sparse 1M-vertex; batched vs 
 PEAK_VALU_OPS = 256 * 64 * 2.4e9  # int32 VALU lane-ops/s (256 CU x 64 lanes x 2.4 GHz)
 PEAK_HBM = 8.0e12
 OPS_PER_RELAX = {"int32": 2.5, "int64": 9.0}
+MM_OPS_PER_TERM = 1.5  # dense fast product (k_mm_fast<int, 8>)
 FW_BLK = 32
 
 
@@ -169,13 +174,20 @@ def bench_dense(args, n: int, dtype_name: str):
     del Wh
     npad = out.npad
     # the dominant kernel: the 128 x 128 (int32) / 64 x 64 (int64) phase-3
-    # tile; achieved = relaxations its launches executed (the triangular
-    # schedule's skipped tiles are not counted) x ops per relaxation / time
+    # tile launches -- for int32 lower triangular inputs the fused step that
+    # also runs the next panel; achieved = relaxations its launches executed
+    # (the triangular schedule's skipped tiles are not counted, the fused
+    # panel's are) x ops per relaxation / time
     tm, tl, tw = prof[1]
     tile_avg_s = tm / max(tl, 1) / 1e3
     achieved = (tw / max(tl, 1)) * OPS_PER_RELAX[dtype_name] / tile_avg_s
-    kname = f"k_fw_tile_fast<{'int' if dtype_name == 'int32' else 'long'}>"
-    traffic = _read_traffic(f"fw_tile_fast_dense{n}_{dtype_name}_pmc.txt")
+    if dtype_name == "int32" and out.schedule == "lower":
+        # the fused look-ahead step: pivot block K's tiles and block K+1's panel in one launch
+        kname = "k_fw_step_fast<int, 8, false, 3>"
+        traffic = _read_traffic(f"fw_step_fast_dense{n}_{dtype_name}_pmc.txt")
+    else:
+        kname = f"k_fw_tile_fast<{'int' if dtype_name == 'int32' else 'long'}>"
+        traffic = _read_traffic(f"fw_tile_fast_dense{n}_{dtype_name}_pmc.txt")
     sm, sl, sw = prof[5]
     pm, pl, _ = prof[0]
     roof = {
@@ -587,10 +599,11 @@ def bench_serve(args):
     }
 
 
-def cpu_baseline_order(Cs, perm_gpu):
+def cpu_baseline_order(Cs, perm_gpu, B_gpu=None):
     """The oracle's vectorised CSR ordering (numpy/scipy restatement of
     _ordering.pyx:50-157) and a scipy permutation, on the full closure.  Its
-    permutation must equal the GPU's."""
+    permutation must equal the GPU's, and so must the permuted matrix (row
+    pointers, column indices in row order, values) when the GPU's is given."""
     import numpy as np
     from oracle import oracle
     t0 = time.perf_counter()
@@ -599,6 +612,10 @@ def cpu_baseline_order(Cs, perm_gpu):
     B.sort_indices()
     dt = time.perf_counter() - t0
     assert np.array_equal(perm, perm_gpu), "GPU canonical order differs from the oracle's"
+    if B_gpu is not None:
+        ip, ix, dv = B_gpu
+        assert np.array_equal(ip, B.indptr) and np.array_equal(ix, B.indices) and np.array_equal(dv, B.data), \
+            "GPU permuted closure differs from the oracle's"
     return {
         "value": Cs.nnz / dt, "unit": "entries/s", "cores": 1, "kind": "port",
         "sample": (f"oracle CSR canonical ordering (numpy/scipy restatement of _ordering.pyx:50-157 and "
@@ -661,7 +678,9 @@ def bench_order(args):
         "roofline": roof,
     }
     if not args.no_cpu_baseline:
-        res["cpu_baseline"] = cpu_baseline_order(sd.to_scipy(C), p)
+        Bg = (B.indptr.cpu().numpy(), B.indices.cpu().numpy(), B.data.cpu().numpy())
+        res["cpu_baseline"] = cpu_baseline_order(sd.to_scipy(C), p, Bg)
+        res["config"]["identical_to_oracle"] = "vertex order and the permuted CSR (indptr, indices, data)"
     return res
 
 
@@ -724,7 +743,9 @@ def bench_matmul(args):
     if not identical or sg.events:
         raise RuntimeError("fast and general products differ")
     terms = float(n) ** 3
-    achieved = terms * OPS_PER_RELAX["int32"] / (kernel_ms / 1e3)
+    # the product's inner loop has no trajectory maximum: per cell and two
+    # steps two v_lshl_or_b32 and one v_min3_u32, 1.5 VALU ops per term
+    achieved = terms * MM_OPS_PER_TERM / (kernel_ms / 1e3)
     res = {
         "metric": "dense AVOS product entries/sec (W @ W of a closure)", "value": n * n / (ms / 1e3),
         "unit": "entries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
@@ -734,7 +755,7 @@ def bench_matmul(args):
         "roofline": {"bound": "valu", "kernel": "k_mm_fast<int, 8> (phase-3 tile over all k)",
                      "achieved": achieved / 1e12, "peak": PEAK_VALU_OPS / 1e12, "unit": "TOP/s",
                      "frac": achieved / PEAK_VALU_OPS, "traffic": _read_traffic("mm_fast_matmul8192_pmc.txt"),
-                     "ops_per_term": OPS_PER_RELAX["int32"], "terms_per_launch": terms, "avg_launch_ms": kernel_ms},
+                     "ops_per_term": MM_OPS_PER_TERM, "terms_per_launch": terms, "avg_launch_ms": kernel_ms},
     }
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline_matmul(1024, n)
@@ -773,9 +794,10 @@ def _sub(res: dict) -> dict:
 def bench_all(args):
     """The whole metric in one line: the dense N=8192 closure is the top-level
     value; the sparse 1M-vertex closure, the batched 4096-component closure
-    (against both per-graph paths), the N=4096 int64 closure and the served
-    stream are nested under sub_results, each with its own steps, roofline and
-    CPU baseline."""
+    (against both per-graph paths), the N=4096 int64 closure, the served
+    stream, the dense N=8192 product and the canonical ordering of the 1M
+    closure are nested under sub_results, each with its own steps, roofline
+    and CPU baseline."""
     import copy
     res = bench_dense(args, 8192, "int32")
     subs = {}
@@ -792,6 +814,13 @@ def bench_all(args):
     a = copy.copy(args)
     print("bench: serve", file=sys.stderr, flush=True)
     subs["serve"] = _sub(bench_serve(a))
+    a = copy.copy(args)
+    a.steps, a.warmup = max(2, args.steps // 2), 1
+    print("bench: matmul8192", file=sys.stderr, flush=True)
+    subs["matmul8192"] = _sub(bench_matmul(a))
+    a = copy.copy(args)
+    print("bench: order1m", file=sys.stderr, flush=True)
+    subs["order1m"] = _sub(bench_order(a))
     res["sub_results"] = subs
     return res
 

@@ -1260,19 +1260,22 @@ __device__ __forceinline__ void fw_panel_quad_body(QuadShared<typename unsigned_
   }
   // the MSB guard of each step, from its snapshots: left operands are the
   // column snapshots (diagonal, column panel), right operands the row
-  // snapshots (diagonal, row panel)
-  if (tid < BLK) {
-    int ga = 0, gb = 0;
-    for (int x = 0; x < BLK; x++) {
-      const int a = amsb(L.sDc[tid][x]), c2 = amsb(L.sPc[tid][x]);
-      const U b1 = L.sDr[tid][x], b2 = L.sPr[tid][x];
-      const int m1 = (T)b1 < 0 ? 0 : msb_u(b1), m2 = (T)b2 < 0 ? 0 : msb_u(b2);
-      ga = a > ga ? a : ga;
-      ga = c2 > ga ? c2 : ga;
-      gb = m1 > gb ? m1 : gb;
-      gb = m2 > gb ? m2 : gb;
+  // snapshots (diagonal, row panel); thread (i4, j) takes index j of steps
+  // i4 + 8 q, the 32 lanes of a step combine by shuffles
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int kk = i4 + 8 * q;
+    const int a1 = amsb(L.sDc[kk][j]), a2 = amsb(L.sPc[kk][j]);
+    const U b1 = L.sDr[kk][j], b2 = L.sPr[kk][j];
+    const int m1 = (T)b1 < 0 ? 0 : msb_u(b1), m2 = (T)b2 < 0 ? 0 : msb_u(b2);
+    int ga = a1 > a2 ? a1 : a2, gb = m1 > m2 ? m1 : m2;
+#pragma unroll
+    for (int o = 16; o > 0; o >>= 1) {
+      const int ta = __shfl_xor(ga, o), tb = __shfl_xor(gb, o);
+      ga = ga > ta ? ga : ta;
+      gb = gb > tb ? gb : tb;
     }
-    if (ga + gb > lim) L.sbad = 1;
+    if (j == 0 && ga + gb > lim) L.sbad = 1;
   }
   // cycles: step x, vertex y with W[y][x] and W[x][y] both present
 #pragma unroll
@@ -1306,8 +1309,8 @@ __device__ __forceinline__ void fw_panel_quad_body(QuadShared<typename unsigned_
     rowS[i * n + j0 + j] = (T)L.sPr[i][j];
     colT[i * n + j0 + j] = (T)L.sPc[i][j];
   }
-  // the previous block's detections (its snapshots stay until the next launch)
-  if (owner && tid == 0 && K > 0) capture_block<T, true>(st, ws.col<T>(par ^ 1), ws.row<T>(par ^ 1), k0 - BLK, n);
+  // (no capture of the previous block's detections: a detection in this
+  // schedule asks for general mode, whose run reports them)
   // trajectory maximum: the workgroup's, then one atomic
   T m = dp > ds ? dp : ds;
   if (owner) {

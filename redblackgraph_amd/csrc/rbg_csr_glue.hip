@@ -225,25 +225,39 @@ __global__ void k_gl_roots(const int32_t* __restrict__ lab, int32_t n, int64_t* 
     flag[v] = lab[v] == v ? 1 : 0;
 }
 
-// comp[v] = (inclusive root count at v's root) - 1; the sizes are counted
+// comp[v] = (inclusive root count at v's root) - 1
 __global__ void k_gl_comp_ids(const int32_t* __restrict__ lab, int32_t n, const int64_t* __restrict__ incl,
-                              int32_t* __restrict__ comp, int32_t* __restrict__ sizes, int64_t* ncomp) {
-  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
-    const int32_t c = (int32_t)(incl[lab[v]] - 1);
-    comp[v] = c;
-    atomicAdd(&sizes[c], 1);
-  }
+                              int32_t* __restrict__ comp, int64_t* ncomp) {
+  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x)
+    comp[v] = (int32_t)(incl[lab[v]] - 1);
   if (blockIdx.x == 0 && threadIdx.x == 0) *ncomp = incl[n - 1];
+}
+
+// From the component ids in sorted order: starts[c] = the first place of
+// component c, and n for c from the count up to n (no atomics: one giant
+// component would serialise a per-vertex count on one word)
+__global__ void k_gl_starts(const int32_t* __restrict__ keys, int32_t n, const int64_t* __restrict__ ncomp,
+                            int64_t* __restrict__ starts) {
+  const int64_t nc = *ncomp;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q <= n; q += (int64_t)gridDim.x * blockDim.x) {
+    if (q < n) {
+      const int32_t c = keys[q];
+      if (q == 0 || keys[q - 1] != c) starts[c] = q;
+    }
+    if (q >= nc) starts[q] = n;
+  }
+}
+
+__global__ void k_gl_sizes(const int64_t* __restrict__ starts, int32_t n, const int64_t* __restrict__ ncomp,
+                           int32_t* __restrict__ sizes) {
+  const int64_t nc = *ncomp;
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x)
+    sizes[c] = c < nc ? (int32_t)(starts[c + 1] - starts[c]) : 0;
 }
 
 __global__ void k_gl_iota(int32_t* p, int32_t n) {
   for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x)
     p[v] = (int32_t)v;
-}
-
-__global__ void k_gl_zero_i32(int32_t* p, int64_t n) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    p[i] = 0;
 }
 
 // members sorted by (component, vertex): local[v] = its place in its
@@ -257,11 +271,6 @@ __global__ void k_gl_local(const int32_t* __restrict__ members, const int32_t* _
     local[v] = (int32_t)(q - starts[c]);
     if (vsize) vsize[v] = sizes[c];
   }
-}
-
-__global__ void k_gl_i32_to_i64(const int32_t* __restrict__ a, int64_t* __restrict__ b, int64_t n) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    b[i] = a[i];
 }
 
 // Entry (v, w, x) of the graph to cell (local v, local w) of component
@@ -457,9 +466,7 @@ int rbg_components_layout(const int32_t* labels, int32_t n, int32_t* comp, int32
   // inclusive root counts into `starts` (n entries of int64) as scratch
   hipError_t e = rocprim::inclusive_scan(t.store, t.store_bytes, t.cnt, starts, (size_t)n, rocprim::plus<int64_t>(), s);
   if (e != hipSuccess) return (int)e;
-  hipLaunchKernelGGL(k_gl_zero_i32, dim3(g), dim3(GL_NT), 0, s, sizes, (int64_t)n);
-  RBG_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_gl_comp_ids, dim3(g), dim3(GL_NT), 0, s, labels, n, (const int64_t*)starts, comp, sizes, ncomp);
+  hipLaunchKernelGGL(k_gl_comp_ids, dim3(g), dim3(GL_NT), 0, s, labels, n, (const int64_t*)starts, comp, ncomp);
   RBG_CHECK_LAUNCH();
   // members: the vertices sorted by component (stable: increasing within one)
   hipLaunchKernelGGL(k_gl_iota, dim3(g), dim3(GL_NT), 0, s, t.s0, n);
@@ -469,11 +476,12 @@ int rbg_components_layout(const int32_t* labels, int32_t n, int32_t* comp, int32
   e = rocprim::radix_sort_pairs(t.store, t.store_bytes, (const int32_t*)comp, t.s1, (const int32_t*)t.s0, members,
                                 (size_t)n, 0u, bits, s);
   if (e != hipSuccess) return (int)e;
-  // component starts: exclusive scan of the sizes (unused tail sizes are 0)
-  hipLaunchKernelGGL(k_gl_i32_to_i64, dim3(g), dim3(GL_NT), 0, s, (const int32_t*)sizes, t.cnt, (int64_t)n);
+  // component starts from the sorted ids (starts[c] = n past the count),
+  // then the sizes as their differences
+  hipLaunchKernelGGL(k_gl_starts, dim3(g), dim3(GL_NT), 0, s, (const int32_t*)t.s1, n, (const int64_t*)ncomp, starts);
   RBG_CHECK_LAUNCH();
-  int rc = rows_to_ptr(t.cnt, n, starts, t.store, t.store_bytes, s);  // starts[c] for c < ncomp, starts[n] = n
-  if (rc) return rc;
+  hipLaunchKernelGGL(k_gl_sizes, dim3(g), dim3(GL_NT), 0, s, (const int64_t*)starts, n, (const int64_t*)ncomp, sizes);
+  RBG_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_gl_local, dim3(g), dim3(GL_NT), 0, s, (const int32_t*)members, (const int32_t*)comp,
                      (const int64_t*)starts, (const int32_t*)sizes, n, local, vertex_size);
   RBG_CHECK_LAUNCH();

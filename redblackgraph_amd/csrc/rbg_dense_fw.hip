@@ -87,11 +87,16 @@ static int fw_dense(T* W, int64_t n, int64_t ld, int width, int is_uns, int mode
                          width, is_uns, st);
     }
     RBG_CHECK_LAUNCH();
-    // relaxations the tiles execute: their cells outside the skipped rows
-    // (the pivot block's, and with a panel in the launch the next block's,
-    // are in every rectangle), 32 steps each
-    if (PROF)
-      prof_stop(kind, p1, q, ((double)count * edge * edge - (double)(ahead ? 2 : 1) * BLK * r.jn * edge) * BLK);
+    // relaxations the launch executes, 32 steps per cell: the tiles' cells
+    // outside the skipped rows (the pivot block's, and with a panel in the
+    // launch the next block's, are in every rectangle); a fused panel adds
+    // the diagonal block, one live panel tile per workgroup and the strip
+    // tiles left of the next pivot block
+    if (PROF) {
+      double cells = (double)count * edge * edge - (double)(ahead ? 2 : 1) * BLK * r.jn * edge;
+      if (ahead) cells += (double)BLK * BLK * (1 + (nb - 1) + (K + 1));
+      prof_stop(kind, p1, q, cells * BLK);
+    }
     return 0;
   };
   // Look-ahead (lower triangular int32): pivot block K's tiles and block
