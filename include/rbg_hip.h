@@ -504,9 +504,10 @@ int rbg_permute_dense_i32(const int32_t *A, int64_t n, const int32_t *p, int32_t
                           rbg_stream_t stream);
 int rbg_permute_dense_i64(const int64_t *A, int64_t n, const int32_t *p, int64_t *B, int32_t upper,
                           rbg_stream_t stream);
-/* P A P^-1 of a CSR matrix with nnz < 2^32: row i is row p[i] with every
- * column c renamed to pinv[c], sorted.  Outputs: n + 1 row pointers and nnz
- * entries; workspace: rbg_permute_csr_workspace_bytes_*(n, nnz) bytes. */
+/* P A P^-1 of a CSR matrix (any nnz: 64-bit row pointers and offsets): row i
+ * is row p[i] with every column c renamed to pinv[c], sorted (equal columns
+ * keep their order).  Outputs: n + 1 row pointers and nnz entries;
+ * workspace: rbg_permute_csr_workspace_bytes_*(n, nnz) bytes. */
 int64_t rbg_permute_csr_workspace_bytes_i32(int32_t n, int64_t nnz);
 int64_t rbg_permute_csr_workspace_bytes_i64(int32_t n, int64_t nnz);
 int rbg_permute_csr_i32(const int64_t *indptr, const int32_t *indices, const int32_t *data, int32_t n, int64_t nnz,

@@ -217,6 +217,89 @@ __global__ __launch_bounds__(ORD_NT) void k_permute_csr_fused(const int64_t* __r
   }
 }
 
+// Stable LSD radix sort of the long rows (one workgroup per row, 4-bit
+// digits): thread t owns the contiguous chunk t of its row, so a pass counts
+// each thread's digits in its own LDS column, scans the 16 x 256 counts in
+// (digit, thread) order, and scatters each chunk in order -- stable, so equal
+// columns keep their row order, as the one-pass ranking does.  Passes
+// alternate between the staging buffer and the output; after an even number
+// the row is copied to the output.  64-bit offsets throughout: no size limit.
+constexpr int LR_BITS = 4;
+constexpr int LR_BINS = 1 << LR_BITS;
+template <typename T>
+__global__ __launch_bounds__(ORD_NT) void k_sort_long_rows(int32_t* __restrict__ sk, T* __restrict__ sv,
+                                                           int32_t* __restrict__ ok, T* __restrict__ ov,
+                                                           const int64_t* __restrict__ begin,
+                                                           const int64_t* __restrict__ end,
+                                                           const int32_t* __restrict__ count, int bits) {
+  __shared__ int64_t cnt[LR_BINS][ORD_NT];
+  __shared__ int64_t wsum[ORD_NT / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int32_t nrows = *count;
+  const int passes = (bits + LR_BITS - 1) / LR_BITS;
+  // the scan's share of this thread: digit dd, threads [tt0, tt0 + 16)
+  const int dd = tid / (ORD_NT / LR_BINS), tt0 = (tid % (ORD_NT / LR_BINS)) * LR_BINS;
+  for (int64_t rj = blockIdx.x; rj < nrows; rj += gridDim.x) {
+    const int64_t b0 = begin[rj], len = end[rj] - b0;
+    const int64_t chunk = (len + ORD_NT - 1) / ORD_NT;
+    const int64_t c0 = (int64_t)tid * chunk < len ? (int64_t)tid * chunk : len;
+    const int64_t c1 = c0 + chunk < len ? c0 + chunk : len;
+    int32_t* srck = sk + b0;
+    T* srcv = sv + b0;
+    int32_t* dstk = ok + b0;
+    T* dstv = ov + b0;
+    for (int pass = 0; pass < passes; pass++) {
+      const int sh = pass * LR_BITS;
+#pragma unroll
+      for (int d = 0; d < LR_BINS; d++) cnt[d][tid] = 0;
+      for (int64_t e = c0; e < c1; e++) cnt[(srck[e] >> sh) & (LR_BINS - 1)][tid]++;
+      __syncthreads();
+      // exclusive scan over (digit, thread): this thread's 16 counts ...
+      int64_t run = 0;
+#pragma unroll
+      for (int q = 0; q < LR_BINS; q++) {
+        const int64_t v = cnt[dd][tt0 + q];
+        cnt[dd][tt0 + q] = run;
+        run += v;
+      }
+      // ... offset by the sum of the threads before it (a block scan)
+      int64_t incl = run;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const int64_t u = __shfl_up(incl, o);
+        if (lane >= o) incl += u;
+      }
+      if (lane == 63) wsum[wv] = incl;
+      __syncthreads();
+      int64_t base = incl - run;
+      for (int w = 0; w < wv; w++) base += wsum[w];
+#pragma unroll
+      for (int q = 0; q < LR_BINS; q++) cnt[dd][tt0 + q] += base;
+      __syncthreads();
+      for (int64_t e = c0; e < c1; e++) {
+        const int32_t k = srck[e];
+        const int64_t pos = cnt[(k >> sh) & (LR_BINS - 1)][tid]++;
+        dstk[pos] = k;
+        dstv[pos] = srcv[e];
+      }
+      __syncthreads();  // the pass's writes are done before the next reads them
+      int32_t* tk = srck;
+      srck = dstk;
+      dstk = tk;
+      T* tv = srcv;
+      srcv = dstv;
+      dstv = tv;
+    }
+    if (srck != ok + b0) {
+      for (int64_t e = tid; e < len; e += ORD_NT) {
+        ok[b0 + e] = srck[e];
+        ov[b0 + e] = srcv[e];
+      }
+    }
+    __syncthreads();  // the next row reuses the counts
+  }
+}
+
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 static unsigned key_bits(int32_t n) {
@@ -242,20 +325,24 @@ static size_t scan_bytes(int32_t n) {
   return b;
 }
 
+// Long rows go to rocPRIM's segmented radix sort while its 32-bit sizes hold
+// the matrix, to k_sort_long_rows beyond.
+constexpr int64_t ROCPRIM_SORT_BELOW = (int64_t)1 << 32;
+
 // workspace: pinv (n), row lengths (n), the long rows' output ranges (2 n)
 // and their count, staging for the long rows (nnz), scan and sort storage
 template <typename T>
 static int64_t permute_csr_ws_bytes(int32_t n, int64_t nnz) {
   return (int64_t)(align256(4 * (size_t)n) + align256(8 * (size_t)n) + 2 * align256(8 * (size_t)n) + 256 +
                    align256(4 * (size_t)nnz) + align256(sizeof(T) * (size_t)nnz) + align256(scan_bytes(n)) +
-                   align256(permute_csr_sort_bytes<T>(n, nnz)) + 256);
+                   (nnz < ROCPRIM_SORT_BELOW ? align256(permute_csr_sort_bytes<T>(n, nnz)) : 0) + 256);
 }
 
 template <typename T>
 static int permute_csr(const int64_t* indptr, const int32_t* indices, const T* data, int32_t n, int64_t nnz,
                        const int32_t* p, int64_t* out_indptr, int32_t* out_indices, T* out_data, void* ws,
                        int64_t ws_bytes, hipStream_t s) {
-  if (n <= 0 || nnz < 0 || nnz >= ((int64_t)1 << 32) || !indptr || !p || !out_indptr || !ws) return RBG_E_ARG;
+  if (n <= 0 || nnz < 0 || !indptr || !p || !out_indptr || !ws) return RBG_E_ARG;
   if (nnz > 0 && (!indices || !data || !out_indices || !out_data)) return RBG_E_ARG;
   if (ws_bytes < permute_csr_ws_bytes<T>(n, nnz)) return RBG_E_WORKSPACE;
   char* w = (char*)(((uintptr_t)ws + 255) & ~(uintptr_t)255);
@@ -277,7 +364,8 @@ static int permute_csr(const int64_t* indptr, const int32_t* indices, const T* d
   size_t scan_sz = scan_bytes(n);
   w += align256(scan_sz);
   void* sort_tmp = w;
-  size_t sort_sz = permute_csr_sort_bytes<T>(n, nnz);
+  const bool rp = nnz < ROCPRIM_SORT_BELOW;
+  size_t sort_sz = rp ? permute_csr_sort_bytes<T>(n, nnz) : 0;
   const int g = (int)((n + ORD_NT - 1) / ORD_NT < 4096 ? (n + ORD_NT - 1) / ORD_NT : 4096);
   hipLaunchKernelGGL(k_perm_inverse, dim3(g), dim3(ORD_NT), 0, s, p, n, pinv, lens, indptr);
   RBG_CHECK_LAUNCH();
@@ -308,6 +396,14 @@ static int permute_csr(const int64_t* indptr, const int32_t* indices, const T* d
                      long_count);
   RBG_CHECK_LAUNCH();
   if (nseg == 0) return 0;
+  if (!rp) {
+    const int gl = (int)(nseg < 8192 ? nseg : 8192);
+    hipLaunchKernelGGL((k_sort_long_rows<T>), dim3(gl), dim3(ORD_NT), 0, s, tmp_idx, tmp_val, out_indices, out_data,
+                       (const int64_t*)long_begin, (const int64_t*)long_end, (const int32_t*)long_count,
+                       (int)key_bits(n));
+    RBG_CHECK_LAUNCH();
+    return 0;
+  }
   e = rocprim::segmented_radix_sort_pairs(sort_tmp, sort_sz, (const int32_t*)tmp_idx, out_indices,
                                           (const T*)tmp_val, out_data, (unsigned)nnz, (unsigned)nseg,
                                           (const int64_t*)long_begin, (const int64_t*)long_end, 0u, key_bits(n), s);

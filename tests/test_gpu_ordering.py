@@ -152,3 +152,62 @@ def test_permute_sparse_long_rows(dtype):
     np.testing.assert_array_equal(Bu.indptr, refu.indptr)
     np.testing.assert_array_equal(Bu.indices, refu.indices)
     np.testing.assert_array_equal(Bu.data, refu.data.astype(dtype))
+
+
+@pytest.mark.parametrize("L,need_gib", [((1 << 15) + 1, 100), ((1 << 16) + 1, 130)])
+def test_permute_csr_past_2_31_entries(L, need_gib):
+    """The CSR boundary past 2^31 entries: 65,536 rows of L entries, rows
+    unsorted, through rbg_permute_csr_i32 with a random permutation.
+    L = 32,769: 2,147,549,184 entries, row pointers and output offsets past
+    2^31, long rows sorted by rocPRIM's segmented sort (32-bit sizes).
+    L = 65,537: 4,295,032,832 entries, past 2^32, so the long rows take the
+    library's own 64-bit row sort; each row holds one column twice (k = 0 and
+    k = n), and the two keep their order.  Checked on the device, row block
+    by row block: every row has its L entries, columns non-decreasing, and
+    each entry is the input entry it names (row p[i], column p[j], value a
+    function of its place in the input row).  About 70 and 110 GB of device
+    memory."""
+    from redblackgraph_amd import _lib
+    from redblackgraph_amd.sparse.csgraph._ordering import _permute_csr_device
+    t = _lib.torch()
+    dev = _lib.device()
+    free, _ = t.cuda.mem_get_info(dev)
+    if free < need_gib * 2 ** 30:
+        pytest.skip(f"needs about {need_gib} GiB of free device memory")
+    n, b = 1 << 16, 40503  # b odd: k -> k b mod n is one-to-one on [0, n)
+    binv = pow(b, -1, n)
+    nnz = n * L
+    assert nnz > 2 ** 31
+    kk = t.arange(L, device=dev, dtype=t.int64)
+    indptr = t.arange(n + 1, device=dev, dtype=t.int64) * L
+    indices = t.empty(nnz, dtype=t.int32, device=dev)
+    data = t.empty(nnz, dtype=t.int32, device=dev)
+    R = 1024
+    for r0 in range(0, n, R):  # row r: columns (r + k b) mod n, values 2 + (7 r + 3 k) mod 1000
+        r = t.arange(r0, r0 + R, device=dev, dtype=t.int64)[:, None]
+        indices[r0 * L:(r0 + R) * L] = ((r + kk[None, :] * b) % n).to(t.int32).reshape(-1)
+        data[r0 * L:(r0 + R) * L] = (2 + (7 * r + 3 * kk[None, :]) % 1000).to(t.int32).reshape(-1)
+    del kk
+    g = t.Generator(device=dev)
+    g.manual_seed(5)
+    p = t.randperm(n, device=dev, generator=g).to(t.int32)
+    optr, oidx, oval = _permute_csr_device(indptr, indices, data, n, nnz, np.dtype(np.int32), p)
+    del indices, data
+    t.cuda.synchronize()
+    assert t.equal(optr, indptr)
+    p64 = p.to(t.int64)
+    for i0 in range(0, n, R):
+        e0, e1 = i0 * L, (i0 + R) * L
+        cols = oidx[e0:e1].to(t.int64).reshape(R, L)
+        step = cols[:, 1:] - cols[:, :-1]
+        assert bool((step >= 0).all())
+        pi = p64[i0:i0 + R][:, None]
+        k = ((p64[cols] - pi) % n) * binv % n          # place in input row p[i], modulo n
+        if L > n:                                      # the second of two equal columns is k + n
+            assert int((step == 0).sum()) == R
+            k[:, 1:] += (step == 0).to(t.int64) * n
+        else:
+            assert bool((step > 0).all())
+        assert bool((k < L).all())
+        want = (2 + (7 * pi + 3 * k) % 1000).to(t.int32)
+        assert t.equal(oval[e0:e1].reshape(R, L), want)
