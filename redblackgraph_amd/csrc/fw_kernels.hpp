@@ -543,10 +543,18 @@ struct LinesShared {
   int maxA[BLK], maxB[BLK], sbad;
 };
 
+// The panel lines a step needs: row-panel lines (columns) [r0, r1) and
+// column-panel lines (rows) [c0, c1) -- every line in the full schedule; in
+// the triangular one the lines whose snapshots the step's tile rectangle
+// reads (the others lie on the absent side and are not read).
+struct LineRange {
+  int64_t r0, r1, c0, c1;
+};
+
 template <typename T>
 __device__ __forceinline__ void fw_panel_lines_body(LinesShared<typename unsigned_of<T>::type>& L, T* W, int64_t n,
                                                     int64_t ld, int64_t K, FwWork ws, int width, rbg_status* st,
-                                                    int bid) {
+                                                    int bid, LineRange lr) {
   using U = typename unsigned_of<T>::type;
   constexpr U TOP = (U)1 << (sizeof(U) * 8 - 1);
   auto& D = L.D;
@@ -564,6 +572,15 @@ __device__ __forceinline__ void fw_panel_lines_body(LinesShared<typename unsigne
   T* colT = ws.col<T>(par);
   T* rowS = ws.row<T>(par);
   auto canon = [](U x) -> U { return (T)x < 0 ? ~(U)0 : x; };
+  {
+    // a workgroup with no needed line leaves at once (workgroup 0 owns the
+    // diagonal block and always stays)
+    const int64_t g0 = (int64_t)bid * NT, g1 = g0 + NT;
+    const int64_t ra = g0 > lr.r0 ? g0 : lr.r0, rb = (g1 < n ? g1 : n) < lr.r1 ? (g1 < n ? g1 : n) : lr.r1;
+    const int64_t ca = (g0 > n ? g0 - n : 0) > lr.c0 ? (g0 > n ? g0 - n : 0) : lr.c0;
+    const int64_t cb = (g1 > n ? g1 - n : 0) < lr.c1 ? (g1 > n ? g1 - n : 0) : lr.c1;
+    if (!owner && ra >= rb && ca >= cb) return;
+  }
   if (tid == 0) sbad = (st->flags & (RBG_F_NEED_GENERAL | RBG_F_PRECOND_FAIL)) ? 2 : 0;
   __syncthreads();
   if (sbad) return;
@@ -573,7 +590,8 @@ __device__ __forceinline__ void fw_panel_lines_body(LinesShared<typename unsigne
   // this thread's panel line, loaded while the diagonal block is walked
   const bool rowp = (int64_t)bid * NT + tid < n;
   const int64_t line = rowp ? (int64_t)bid * NT + tid : (int64_t)bid * NT + tid - n;
-  const bool active = (int64_t)bid * NT + tid < 2 * n && !(line >= k0 && line < k0 + BLK);
+  const bool active = (int64_t)bid * NT + tid < 2 * n && !(line >= k0 && line < k0 + BLK) &&
+                      (rowp ? line >= lr.r0 && line < lr.r1 : line >= lr.c0 && line < lr.c1);
   U c[BLK];
   U* Wu = (U*)W;
   if (active) {
@@ -1373,7 +1391,7 @@ __global__ __launch_bounds__(NT, PER_CU) void k_fw_step_fast(T* W, int64_t n, in
 // (upper) it is the mirror image.  Every other tile would only take products
 // with an absent operand, which leave its cells, and the trajectory maximum,
 // as they are.  Returns the rectangle and its tile count.
-inline TileRect tri_rect(int tri, int64_t n, int edge, int64_t K, int* count) {
+__host__ __device__ inline TileRect tri_rect(int tri, int64_t n, int edge, int64_t K, int* count) {
   const int nt = (int)(n / edge);
   const int64_t k0 = K * BLK;
   const int lo = (int)(k0 / edge), hi = (int)((k0 + BLK - 1) / edge);

@@ -113,7 +113,18 @@ static __global__ __launch_bounds__(NT) void k_bfw_panel(const BComp* comps, con
   const BComp c = comps[it.comp];
   FwWork ws{c.snap, c.npad, 4};
   __shared__ LinesShared<uint32_t> L;
-  fw_panel_lines_body<int32_t>(L, c.W, c.npad, c.npad, K, ws, 32, c.st, it.bid);
+  const int64_t n = c.npad;
+  LineRange lr{0, n, 0, n};
+  if (c.tri) {  // the lines the step's tile rectangle reads snapshots at
+    int cnt = 0;
+    const TileRect r = tri_rect(c.tri, n, c.tile, K, &cnt);
+    const int64_t e = c.tile;
+    if (c.tri == 1)
+      lr = LineRange{0, (int64_t)(r.j0 + r.jn) * e, (int64_t)r.i0 * e, n};
+    else
+      lr = LineRange{(int64_t)r.j0 * e, n, 0, (int64_t)(r.i0 + cnt / r.jn) * e};
+  }
+  fw_panel_lines_body<int32_t>(L, c.W, n, n, K, ws, 32, c.st, it.bid, lr);
 }
 
 template <int RC>
