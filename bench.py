@@ -174,16 +174,16 @@ def bench_dense(args, n: int, dtype_name: str):
     del Wh
     npad = out.npad
     # the dominant kernel: the 128 x 128 (int32) / 64 x 64 (int64) phase-3
-    # tile launches -- for int32 lower triangular inputs the fused step that
-    # also runs the next panel; achieved = relaxations its launches executed
+    # tile launches -- for lower triangular inputs the fused step that also
+    # runs the next panel; achieved = relaxations its launches executed
     # (the triangular schedule's skipped tiles are not counted, the fused
     # panel's are) x ops per relaxation / time
     tm, tl, tw = prof[1]
     tile_avg_s = tm / max(tl, 1) / 1e3
     achieved = (tw / max(tl, 1)) * OPS_PER_RELAX[dtype_name] / tile_avg_s
-    if dtype_name == "int32" and out.schedule == "lower":
+    if out.schedule == "lower":
         # the fused look-ahead step: pivot block K's tiles and block K+1's panel in one launch
-        kname = "k_fw_step_fast<int, 8, false, 3>"
+        kname = "k_fw_step_fast<int, 8, false, 3>" if dtype_name == "int32" else "k_fw_step_fast<long, 4, true, 3>"
         traffic = _read_traffic(f"fw_step_fast_dense{n}_{dtype_name}_pmc.txt")
     else:
         kname = f"k_fw_tile_fast<{'int' if dtype_name == 'int32' else 'long'}>"

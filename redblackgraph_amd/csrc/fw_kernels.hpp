@@ -1079,19 +1079,29 @@ __global__ __launch_bounds__(NT, PER_CU) void k_fw_tile_fast(T* W, int64_t n, in
 // like any other.  Guard words are not written: the tiles of this schedule
 // check the guard on the operands they stage.
 template <typename U>
-struct QuadShared {
-  U Dg[BLK][BLK + 1];
-  U Pc[BLK][BLK + 1];
+struct QuadStrip {  // the strip's operands, before the walk
+  U sL[BLK][BLK + 1];        // colsnap_k[k0 + x] for step kk
+  U sY[BLK][BLK + 1];        // low bits of rowsnap_k[j0 + x]
+  uint8_t sM[BLK][BLK + 4];  // and its shift
+};
+template <typename U>
+struct QuadSnaps {     // the walk's snapshots
   U sDc[BLK][BLK + 1];  // diagonal column kk at step kk, row x
   U sDr[BLK][BLK + 1];  // diagonal row kk, column x
   U sPr[BLK][BLK + 1];  // row-panel row kk, column x
   U sPc[BLK][BLK + 1];  // column-panel column kk, row x
-  U MYd[2][BLK][2];     // (shift, low bits) of the pivot rows, by step parity
+};
+template <typename U>
+struct QuadShared {
+  U Dg[BLK][BLK + 1];
+  U Pc[BLK][BLK + 1];
+  union {  // one after the other: the strip's operands, then the snapshots
+    QuadStrip<U> s;
+    QuadSnaps<U> w;
+  } x;
+  U MYd[2][BLK][2];  // (shift, low bits) of the pivot rows, by step parity
   U MYr[2][BLK][2];
   U sDiag[BLK];
-  U sL[BLK][BLK + 1];  // strip: colsnap_k[k0 + x] for step kk
-  U sM[BLK][BLK + 1];  // strip: shift and low bits of rowsnap_k[j0 + x]
-  U sY[BLK][BLK + 1];
   int64_t sMax[NT / 64];
   int sbad, sguard;  // sguard: the strip's MSB guard failed
 };
@@ -1153,11 +1163,11 @@ __device__ __forceinline__ void fw_panel_quad_body(QuadShared<typename unsigned_
 #pragma unroll
     for (int q = 0; q < PER; q++) {
       const int e = tid + q * NT, kk = e / BLK, x = e % BLK;
-      L.sL[kk][x] = la[q];
+      L.x.s.sL[kk][x] = la[q];
       U m, y;
       pair(lb[q], m, y);
-      L.sM[kk][x] = m;
-      L.sY[kk][x] = y;
+      L.x.s.sM[kk][x] = m;
+      L.x.s.sY[kk][x] = y;
     }
   }
   __syncthreads();
@@ -1168,18 +1178,18 @@ __device__ __forceinline__ void fw_panel_quad_body(QuadShared<typename unsigned_
     if (tid < BLK) {
       int ga = 0, gb = 0;
       for (int x = 0; x < BLK; x++) {
-        const int a = amsb(L.sL[tid][x]);
-        const int m = (T)L.sY[tid][x] < 0 ? 0 : (int)L.sM[tid][x];
+        const int a = amsb(L.x.s.sL[tid][x]);
+        const int m = (T)L.x.s.sY[tid][x] < 0 ? 0 : (int)L.x.s.sM[tid][x];
         ga = a > ga ? a : ga;
         gb = m > gb ? m : gb;
       }
       if (ga + gb > lim) L.sguard = 1;
     }
     for (int kk = 0; kk < BLK; kk++) {
-      const U m = L.sM[kk][j], y = L.sY[kk][j];
+      const U m = L.x.s.sM[kk][j], y = L.x.s.sY[kk][j];
 #pragma unroll
       for (int q = 0; q < 4; q++) {
-        const U v = (L.sL[kk][i4 + 8 * q] << m) | y;
+        const U v = (L.x.s.sL[kk][i4 + 8 * q] << m) | y;
         p[q] = p[q] < v ? p[q] : v;
         ds = ds > (T)p[q] ? ds : (T)p[q];
       }
@@ -1187,6 +1197,7 @@ __device__ __forceinline__ void fw_panel_quad_body(QuadShared<typename unsigned_
 #pragma unroll
     for (int q = 0; q < 4; q++) p[q] = canon(p[q]);
   }
+  __syncthreads();  // the strip's operands are read; their LDS takes the snapshots
   // the walk's starting state
 #pragma unroll
   for (int q = 0; q < 4; q++) {
@@ -1198,15 +1209,15 @@ __device__ __forceinline__ void fw_panel_quad_body(QuadShared<typename unsigned_
     L.Dg[i][j] = d[q];
     if (!rowp) L.Pc[i][j] = p[q];
     if (j == 0) {
-      L.sDc[0][i] = d[q];
-      L.sPc[0][i] = rowp ? ALL : p[q];
+      L.x.w.sDc[0][i] = d[q];
+      L.x.w.sPc[0][i] = rowp ? ALL : p[q];
     }
   }
   if (i4 == 0) {  // pivot row 0 (q = 0): its pairs and the step-0 row snapshots
     pair(d[0], L.MYd[0][j][0], L.MYd[0][j][1]);
-    L.sDr[0][j] = d[0];
+    L.x.w.sDr[0][j] = d[0];
     if (rowp) pair(p[0], L.MYr[0][j][0], L.MYr[0][j][1]);
-    L.sPr[0][j] = rowp ? p[0] : ALL;
+    L.x.w.sPr[0][j] = rowp ? p[0] : ALL;
   }
   __syncthreads();
   if (L.sguard) {
@@ -1259,18 +1270,18 @@ __device__ __forceinline__ void fw_panel_quad_body(QuadShared<typename unsigned_
         if (i == kk + 1) {
           const U cd = canon(d[q]);
           pair(cd, L.MYd[qq ^ 1][j][0], L.MYd[qq ^ 1][j][1]);
-          L.sDr[kk + 1][j] = cd;
+          L.x.w.sDr[kk + 1][j] = cd;
           if (rowp) {
             const U cr = canon(p[q]);
             pair(cr, L.MYr[qq ^ 1][j][0], L.MYr[qq ^ 1][j][1]);
-            L.sPr[kk + 1][j] = cr;
+            L.x.w.sPr[kk + 1][j] = cr;
           } else {
-            L.sPr[kk + 1][j] = ALL;
+            L.x.w.sPr[kk + 1][j] = ALL;
           }
         }
         if (j == kk + 1) {
-          L.sDc[kk + 1][i] = canon(d[q]);
-          L.sPc[kk + 1][i] = rowp ? ALL : canon(p[q]);
+          L.x.w.sDc[kk + 1][i] = canon(d[q]);
+          L.x.w.sPc[kk + 1][i] = rowp ? ALL : canon(p[q]);
         }
       }
     }
@@ -1283,8 +1294,8 @@ __device__ __forceinline__ void fw_panel_quad_body(QuadShared<typename unsigned_
 #pragma unroll
   for (int q = 0; q < 4; q++) {
     const int kk = i4 + 8 * q;
-    const int a1 = amsb(L.sDc[kk][j]), a2 = amsb(L.sPc[kk][j]);
-    const U b1 = L.sDr[kk][j], b2 = L.sPr[kk][j];
+    const int a1 = amsb(L.x.w.sDc[kk][j]), a2 = amsb(L.x.w.sPc[kk][j]);
+    const U b1 = L.x.w.sDr[kk][j], b2 = L.x.w.sPr[kk][j];
     const int m1 = (T)b1 < 0 ? 0 : msb_u(b1), m2 = (T)b2 < 0 ? 0 : msb_u(b2);
     int ga = a1 > a2 ? a1 : a2, gb = m1 > m2 ? m1 : m2;
 #pragma unroll
@@ -1299,7 +1310,7 @@ __device__ __forceinline__ void fw_panel_quad_body(QuadShared<typename unsigned_
 #pragma unroll
   for (int q = 0; q < 4; q++) {
     const int i = i4 + 8 * q;
-    if (owner && i != j && (T)L.sDc[i][j] >= 0 && (T)L.sDr[i][j] >= 0) {
+    if (owner && i != j && (T)L.x.w.sDc[i][j] >= 0 && (T)L.x.w.sDr[i][j] >= 0) {
       record_cycle(st, naive_key(k0 + i, k0 + j, k0 + j, n));
       L.sbad = 1;
     }
@@ -1317,15 +1328,15 @@ __device__ __forceinline__ void fw_panel_quad_body(QuadShared<typename unsigned_
     const int i = i4 + 8 * q;
     if (owner) {
       W[(k0 + i) * ld + k0 + j] = (T)(i == j ? L.sDiag[i] : d[q]);
-      colT[i * n + k0 + j] = (T)L.sDc[i][j];
-      rowS[i * n + k0 + j] = (T)L.sDr[i][j];
+      colT[i * n + k0 + j] = (T)L.x.w.sDc[i][j];
+      rowS[i * n + k0 + j] = (T)L.x.w.sDr[i][j];
     }
     if (rowp)
       W[(k0 + i) * ld + j0 + j] = (T)p[q];
     else
       W[(j0 + i) * ld + k0 + j] = (T)p[q];
-    rowS[i * n + j0 + j] = (T)L.sPr[i][j];
-    colT[i * n + j0 + j] = (T)L.sPc[i][j];
+    rowS[i * n + j0 + j] = (T)L.x.w.sPr[i][j];
+    colT[i * n + j0 + j] = (T)L.x.w.sPc[i][j];
   }
   // (no capture of the previous block's detections: a detection in this
   // schedule asks for general mode, whose run reports them)

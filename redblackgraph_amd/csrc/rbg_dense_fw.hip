@@ -64,8 +64,8 @@ static int fw_dense(T* W, int64_t n, int64_t ld, int width, int is_uns, int mode
         }
       }
       if (la) {
-        // int32 only (la); the next block's panel: one workgroup per
-        // row/column panel tile pair, as the pair panel
+        // the next block's panel: one workgroup per row/column panel tile
+        // pair, as the pair panel
         const int npanel = ahead ? nb - 1 : 0;
         if constexpr (sizeof(T) == 4) {
           if (kind == 5)
@@ -74,6 +74,11 @@ static int fw_dense(T* W, int64_t n, int64_t ld, int width, int is_uns, int mode
           else
             hipLaunchKernelGGL((k_fw_step_fast<T, 8, false, 3>), dim3(npanel + count), dim3(NT), 0, q, W, n, ld,
                                (int64_t)K, ws, width, st, r, npanel);
+        } else {
+          // int64: the 64 x 64 tile at three workgroups per CU (the int64
+          // panel's LDS leaves no room for a fourth)
+          hipLaunchKernelGGL((k_fw_step_fast<T, 4, true, 3>), dim3(npanel + count), dim3(NT), 0, q, W, n, ld,
+                             (int64_t)K, ws, width, st, r, npanel);
         }
       } else if (kind == 5) {
         hipLaunchKernelGGL((k_fw_tile_fast_small<T>), dim3(count), dim3(NT), 0, q, W, n, ld, (int64_t)K, ws, width,
@@ -99,11 +104,9 @@ static int fw_dense(T* W, int64_t n, int64_t ld, int width, int is_uns, int mode
     }
     return 0;
   };
-  // Look-ahead (lower triangular int32): pivot block K's tiles and block
-  // K + 1's panel share a launch; only block 0's panel runs on its own.  (The
-  // int64 line panel holds twice the registers, which would cost the int64
-  // tiles their fourth workgroup per CU.)
-  const bool la = fast && tri == 1 && LOOKAHEAD && sizeof(T) == 4;
+  // Look-ahead (lower triangular): pivot block K's tiles and block K + 1's
+  // panel share a launch; only block 0's panel runs on its own.
+  const bool la = fast && tri == 1 && LOOKAHEAD;
   for (int K = 0; K < nb; K++) {
     int rc = (!la || K == 0) ? panel(K, s) : 0;
     if (!rc) rc = tiles(K, s, la);
