@@ -1361,7 +1361,7 @@ __device__ __forceinline__ void fw_panel_quad_body(QuadShared<typename unsigned_
 }
 
 // One pivot step of the look-ahead schedule (lower triangular matrices, fast
-// mode): workgroups [0, npanel) run the panel of pivot block K + 1
+// mode): npanel workgroups run the panel of pivot block K + 1
 // (fw_panel_quad_body, which first applies block K to block K + 1's rows),
 // the others the phase-3 tiles of block K in rectangle r with those rows
 // and columns left to the panel (ahead).  Neither role reads what the other
@@ -1378,15 +1378,25 @@ __global__ __launch_bounds__(NT, PER_CU) void k_fw_step_fast(T* W, int64_t n, in
     TileShared<U, 16 * RC, SHIFT_BYTES> t;
     QuadShared<U> p;
   } L;
+  // int32: the tiles take the first workgroups and the panel the last.  A
+  // launch of the 128-tiles holds one to two rounds; the panel then starts
+  // as the first round drains instead of holding a third of its slots
+  // (12.54 against 12.87 ms per N = 8192 closure).  int64: the panel first.
+  // Its launches hold several rounds of 64-tiles, and a panel dispatched
+  // behind them would end the launch late (7.50 against 6.38 ms per
+  // N = 4096 closure).
+  constexpr bool PANEL_LAST = sizeof(T) == 4;
   const int b = blockIdx.x;
-  if (b < npanel) {
+  const int ntile = (int)gridDim.x - npanel;
+  const int pb = PANEL_LAST ? b - ntile : b;  // this workgroup's panel, if in [0, npanel)
+  if (pb >= 0 && pb < npanel) {
     // the panel is the launch's critical path: its waves take issue
     // priority over the tile waves that share their SIMDs
     __builtin_amdgcn_s_setprio(3);
-    fw_panel_quad_body<T>(L.p, W, n, ld, K + 1, ws, width, st, b);
+    fw_panel_quad_body<T>(L.p, W, n, ld, K + 1, ws, width, st, pb);
     return;
   }
-  const int t = b - npanel;
+  const int t = PANEL_LAST ? b : b - npanel;
   fw_tile_fast_body<T, RC, RC, false, SHIFT_BYTES>(L.t, W, n, ld, K, ws, width, st, r.i0 + t / r.jn,
                                                    r.j0 + t % r.jn, npanel > 0);
 }
