@@ -11,10 +11,11 @@ Prints one JSON line (the bench contract) with:
   value          closure entries/s = N^2 / seconds per closure
   roofline       the dominant kernel (for the lower triangular N=8192 input
                  the fused look-ahead step k_fw_step_fast<int>: pivot block
-                 K's phase-3 tiles and block K+1's panel in one launch):
-                 integer VALU ops it must execute (2.5 per relaxation) over
-                 its average launch duration, measured with hipEvents on its
-                 launch stream
+                 K's phase-3 tiles and block K+1's panel in one launch; for
+                 the N=4096 int64 one the persistent closure k_fw_flow<long>,
+                 every pivot step in one launch): integer VALU ops it must
+                 execute (2.5 per relaxation) over its average launch
+                 duration, measured with hipEvents on its launch stream
                  (recorded on every --prof-every-th timed step, 4 by default,
                  so that the events' own stalls stay out of most steps)
   cpu_baseline   the oracle's single-core naive loop (a C restatement of the
@@ -149,19 +150,21 @@ def bench_dense(args, n: int, dtype_name: str):
     et = elem_type(dtype)
     Ad = _lib.to_device(A)
     L = _lib.load()
+    sched = getattr(args, "dense_schedule", "auto")
     for _ in range(args.warmup):
-        W, out = dense.fw_closure(Ad, et)
+        W, out = dense.fw_closure(Ad, et, schedule=sched)
     t.cuda.synchronize()
     L.rbg_fw_prof_enable(1)
     t.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(args.steps):
         L.rbg_prof_enable(2 if i % args.prof_every == 0 else 0)
-        W, out = dense.fw_closure(Ad, et)
+        W, out = dense.fw_closure(Ad, et, schedule=sched)
     t.cuda.synchronize()
     t1 = time.perf_counter()
     sampled = _sampled_steps(args)
-    prof = {k: _prof_read(k) for k in (0, 1, 5)}  # panel, 128-tiles, 64-tiles: (ms, launches, relaxations)
+    # panel, 128-tiles, 64-tiles, persistent closure: (ms, launches, relaxations)
+    prof = {k: _prof_read(k) for k in (0, 1, 5, 6)}
     L.rbg_fw_prof_enable(0)
     if out.mode != "fast" or out.status.cycle_count or out.status.events:
         raise RuntimeError(f"unexpected closure outcome: mode={out.mode} status={out.status.flags}")
@@ -169,19 +172,22 @@ def bench_dense(args, n: int, dtype_name: str):
     # host-boundary rate: numpy matrix in, numpy closure out (PCIe both ways)
     t.cuda.synchronize()
     h0 = time.perf_counter()
-    Wh = dense.fw_closure(_lib.to_device(A), et)[0].cpu().numpy()
+    Wh = dense.fw_closure(_lib.to_device(A), et, schedule=sched)[0].cpu().numpy()
     pcie_ms = (time.perf_counter() - h0) * 1e3
     del Wh
     npad = out.npad
-    # the dominant kernel: the 128 x 128 (int32) / 64 x 64 (int64) phase-3
-    # tile launches -- for lower triangular inputs the fused step that also
-    # runs the next panel; achieved = relaxations its launches executed
-    # (the triangular schedule's skipped tiles are not counted, the fused
-    # panel's are) x ops per relaxation / time
-    tm, tl, tw = prof[1]
+    # the dominant kernel: for lower triangular inputs the persistent closure
+    # (every pivot step's panels and tiles in one launch), else the 128 x 128
+    # (int32) / 64 x 64 (int64) phase-3 tile launches; achieved = relaxations
+    # its launches executed (the triangular schedule's skipped tiles are not
+    # counted, the panels' are) x ops per relaxation / time
+    tm, tl, tw = prof[6] if prof[6][1] else prof[1]
     tile_avg_s = tm / max(tl, 1) / 1e3
     achieved = (tw / max(tl, 1)) * OPS_PER_RELAX[dtype_name] / tile_avg_s
-    if out.schedule == "lower":
+    if prof[6][1]:
+        kname = "k_fw_flow<int, 8, false, 3>" if dtype_name == "int32" else "k_fw_flow<long, 4, true, 3>"
+        traffic = _read_traffic(f"fw_flow_dense{n}_{dtype_name}_pmc.txt")
+    elif out.schedule in ("lower", "lower-steps"):
         # the fused look-ahead step: pivot block K's tiles and block K+1's panel in one launch
         kname = "k_fw_step_fast<int, 8, false, 3>" if dtype_name == "int32" else "k_fw_step_fast<long, 4, true, 3>"
         traffic = _read_traffic(f"fw_step_fast_dense{n}_{dtype_name}_pmc.txt")
@@ -846,6 +852,9 @@ def main():
                     help="bracket the kernels of every N-th timed step with timing events (1: every step)")
     ap.add_argument("--no-plain", action="store_true",
                     help="batched workload: skip the one-graph-at-a-time comparison legs (profiling runs)")
+    ap.add_argument("--dense-schedule", default="auto", choices=["auto", "steps"],
+                    help="dense workloads: 'steps' keeps one launch per pivot step for lower triangular "
+                         "inputs instead of the persistent closure (a comparison run)")
     args = ap.parse_args()
     args.prof_every = max(1, args.prof_every)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:

@@ -52,6 +52,8 @@ enum {
   RBG_F_CAPACITY = 8,     /* a device-side capacity was exceeded (caller must grow it)        */
   RBG_F_HAS_LOWER = 16,   /* precheck: some off-diagonal entry lies below the diagonal       */
   RBG_F_HAS_UPPER = 32,   /* precheck: some off-diagonal entry lies above the diagonal       */
+  RBG_F_STALLED = 64,     /* a bounded wait of the persistent closure ran out (set with
+                             RBG_F_NEED_GENERAL): rerun, e.g. with tri = 3                  */
 };
 
 /*
@@ -100,15 +102,38 @@ uint64_t rbg_host_avos_product_u64(int64_t a, int64_t b, uint32_t *events);
  * 1 = general (any input).  On return st->diameter_max holds the max trajectory
  * value, st->cycle_* the first/last cycle detections, st->ov_* overflow events,
  * and st->flags & RBG_F_NEED_GENERAL asks for a general-mode rerun on the
- * original input. */
+ * original input.  rbg_fw_workspace_bytes is the workspace rbg_fw_dense_* needs. */
 int64_t rbg_fw_workspace_bytes(int64_t n, int32_t elem_bytes);
+/* The workspace for rbg_fw_dense_tri_* (no reference counterpart; the
+ * triangular schedule's size query): with it a lower triangular fast closure
+ * runs the look-ahead schedule (int32) or the persistent closure (int64, one
+ * launch); with only rbg_fw_workspace_bytes, one plain triangular launch pair
+ * per pivot step. */
+int64_t rbg_fw_tri_workspace_bytes(int64_t n, int32_t elem_bytes);
+/* No counterpart in the reference (schedule introspection, host only, for
+ * tests): the persistent lower triangular closure's work items of order n in
+ * ticket order (elem_bytes = 8: the int64 closure), four int32 per item --
+ * (0, K, idx, 0) for item idx of pivot block K's panel (row panel (K, idx)
+ * for idx < K, else the column panel's rows from 32 (K + 1) + 256 (idx - K)),
+ * (1, K, I, J) for tile (I, J) of pivot step K.  Returns the item count;
+ * items are written when cap >= 4 * count. */
+int64_t rbg_fw_flow_schedule(int64_t n, int32_t elem_bytes, int32_t *items, int64_t cap);
+/* No counterpart in the reference (host only, for tests): each item's waits,
+ * ten (word, target) pairs per item in ticket order -- "counter word `word`
+ * of the closure's counter block >= target", word = -1 for an unused slot.
+ * The block: [4 + K] finished items of pivot block K's panel, [4 + nb + K]
+ * finished tiles of step K, [4 + 2 nb + I nt + J] 1 + the last step tile
+ * (I, J) finished, [4 + 2 nb + nt^2 + K nb + idx] panel item idx of block K
+ * finished (nb = n / 32, nt = n / 64).  Returns the item count; written when
+ * cap >= 20 * count. */
+int64_t rbg_fw_flow_waits(int64_t n, int32_t elem_bytes, int64_t *waits, int64_t cap);
 /* Profiling (no reference counterpart; used by bench.py): while enabled, every
  * panel and tile launch of rbg_fw_dense_* is bracketed by hipEvents on its
  * stream.  rbg_fw_prof_read synchronises those events and returns the summed
  * durations (ms) and launch counts since the last read, then clears them. */
 int rbg_fw_prof_enable(int32_t on);
 /* Same facility for every timed launch kind: 0 FW panel, 1 FW tile, 5 FW small tile,
- * 2 DAG level, 3 SpGEMM rows, 4 batched FW.  on = 1 starts recording afresh, 2
+ * 6 FW persistent closure, 2 DAG level, 3 SpGEMM rows, 4 batched FW.  on = 1 starts recording afresh, 2
  * resumes and 0 pauses, both keeping what was recorded (so a caller can time
  * a sample of its launches).  rbg_prof_read returns (and clears) the summed
  * duration and count of one kind. */
@@ -137,7 +162,12 @@ int rbg_fw_dense_i64(int64_t *W, int64_t n, int64_t ld, int32_t width, int32_t i
  * only: with mode = 1 the full schedule runs whatever tri says.  Unlike the
  * reference's loop, which reads only the assumed triangle, an entry on the
  * other side is detected: the fast run stops with RBG_F_NEED_GENERAL, and the
- * caller reruns mode 1 on the original input, as for any fast-mode miss. */
+ * caller reruns mode 1 on the original input, as for any fast-mode miss.
+ * For int64, tri = 1 with a workspace of rbg_fw_tri_workspace_bytes runs every
+ * pivot step in one persistent launch whose work items wait for each other
+ * under a time limit; should a wait run out (never expected), st->flags gets
+ * RBG_F_STALLED | RBG_F_NEED_GENERAL and the caller reruns on the original
+ * input, with tri = 3 (the same schedule, one launch per pivot step) or mode 1. */
 int rbg_fw_dense_tri_i32(int32_t *W, int64_t n, int64_t ld, int32_t width, int32_t is_unsigned, int32_t mode,
                          int32_t tri, void *workspace, int64_t workspace_bytes, rbg_status *st,
                          rbg_stream_t stream);

@@ -38,6 +38,7 @@ STATUS_BYTES = ctypes.sizeof(RbgStatus)
 EV_OVERFLOW_WIDTH, EV_OVERFLOW_RED, EV_INVALID, EV_SIGN = 1, 2, 4, 8
 F_NEED_GENERAL, F_PRECOND_FAIL, F_CYCLE, F_CAPACITY = 1, 2, 4, 8
 F_HAS_LOWER, F_HAS_UPPER = 16, 32
+F_STALLED = 64
 NO_KEY = (1 << 64) - 1
 
 # (name, restype, argtypes) of every exported symbol; include/rbg_hip.h is the
@@ -52,6 +53,9 @@ SIGNATURES = {
     "rbg_host_avos_sum_i64": (I64, [I64, I64]),
     "rbg_host_avos_product_u64": (U64, [I64, I64, ctypes.POINTER(ctypes.c_uint32)]),
     "rbg_fw_workspace_bytes": (I64, [I64, I32]),
+    "rbg_fw_tri_workspace_bytes": (I64, [I64, I32]),
+    "rbg_fw_flow_schedule": (I64, [I64, I32, P, I64]),
+    "rbg_fw_flow_waits": (I64, [I64, I32, P, I64]),
     "rbg_fw_prof_enable": (I32, [I32]),
     "rbg_prof_enable": (I32, [I32]),
     "rbg_prof_read": (I32, [I32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]),

@@ -39,24 +39,39 @@ constexpr int RPT = 8;     // phase-3 rows per thread
 constexpr int CPT = 8;     // phase-3 cols per thread
 
 struct FwWork {
-  // two snapshot buffers (block parity), each BLK x n for columns and rows,
-  // then the guard words of the current pivot block: per 32-block b and k,
-  // gblk[b][0][k] = the largest MSB among the left operands of rows of block
-  // b (column snapshot k) and gblk[b][1][k] = among the right operands of
-  // columns of block b (row snapshot k).
+  // `ring` snapshot buffers (pivot block K uses buffer K mod ring; two for
+  // the launch-per-step schedules, four for the persistent one), each BLK x n
+  // for columns and rows, then the guard words of the current pivot block:
+  // per 32-block b and k, gblk[b][0][k] = the largest MSB among the left
+  // operands of rows of block b (column snapshot k) and gblk[b][1][k] = among
+  // the right operands of columns of block b (row snapshot k).
   void* base;
   int64_t n;
   int elem;
+  int ring = 2;  // a power of two
+  // lower triangular schedules with the quad panel: every diagonal block as
+  // the closure found it, block K at [K][32][32] (k_fw_diag_copy).  The
+  // panel's workgroups read block K's diagonal block here, not in W, where
+  // the workgroup that owns it stores the closed block when done -- possibly
+  // before another workgroup of the same panel has started.
+  void* dcopy = nullptr;
+  // the persistent closure: each diagonal block's step snapshots, block K's
+  // at [K][2][32][32]: column snapshots [kk][x] = D[x][kk] at step kk, then
+  // row snapshots [kk][x] = D[kk][x] (k_fw_diag_walk)
+  void* dsnap = nullptr;
+  __host__ __device__ int slot(int64_t K) const { return (int)(K & (ring - 1)); }
   template <typename T>
-  __host__ __device__ T* col(int parity) const {
-    return (T*)base + (size_t)parity * 2 * BLK * n;
+  __host__ __device__ T* col(int s) const {
+    return (T*)base + (size_t)s * 2 * BLK * n;
   }
   template <typename T>
-  __host__ __device__ T* row(int parity) const {
-    return (T*)base + (size_t)parity * 2 * BLK * n + (size_t)BLK * n;
+  __host__ __device__ T* row(int s) const {
+    return (T*)base + (size_t)s * 2 * BLK * n + (size_t)BLK * n;
   }
-  __host__ __device__ int* gblk() const { return (int*)((char*)base + (size_t)4 * BLK * n * elem); }
-  static int64_t bytes(int64_t n, int elem) { return (int64_t)4 * BLK * n * elem + (n / BLK) * 2 * BLK * 4 + 256; }
+  __host__ __device__ int* gblk() const { return (int*)((char*)base + (size_t)2 * ring * BLK * n * elem); }
+  static int64_t bytes(int64_t n, int elem, int ring = 2) {
+    return (int64_t)2 * ring * BLK * n * elem + (n / BLK) * 2 * BLK * 4 + 256;
+  }
 };
 
 __device__ inline uint64_t naive_key(int64_t k, int64_t i, int64_t j, int64_t n) {
@@ -185,7 +200,7 @@ __global__ __launch_bounds__(NT) void k_fw_panel(T* W, int64_t n, int64_t ld, in
   const int tid = threadIdx.x;
   const int nb = (int)(n / BLK);
   const int64_t k0 = K * BLK;
-  const int par = (int)(K & 1);
+  const int par = ws.slot(K);
   T* colT = ws.col<T>(par);
   T* rowS = ws.row<T>(par);
 
@@ -199,7 +214,7 @@ __global__ __launch_bounds__(NT) void k_fw_panel(T* W, int64_t n, int64_t ld, in
   }
   const bool owner = blockIdx.x == 0;
 
-  if (owner && tid == 0 && K > 0) capture_block<T, FAST>(st, ws.col<T>(par ^ 1), ws.row<T>(par ^ 1), k0 - BLK, n);
+  if (owner && tid == 0 && K > 0) capture_block<T, FAST>(st, ws.col<T>(ws.slot(K - 1)), ws.row<T>(ws.slot(K - 1)), k0 - BLK, n);
 
   // load tiles (natural domain)
   const int64_t pr0 = role == 2 ? (int64_t)J * BLK : k0;  // panel row origin
@@ -305,6 +320,20 @@ __device__ inline void st4(uint64_t* p, const uint64_t (&v)[4]) {
   *reinterpret_cast<ulonglong2*>(p) = make_ulonglong2(v[0], v[1]);
   *reinterpret_cast<ulonglong2*>(p + 2) = make_ulonglong2(v[2], v[3]);
 }
+// Device-coherent stores (relaxed device-scope atomic stores: written through
+// the XCD's L2), for cells another XCD reads later in the same launch.  Such
+// a store is complete, for every XCD, once the wave's vmcnt has drained.
+template <typename U>
+__device__ __forceinline__ void st_dev(U* p, U v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename U>
+__device__ __forceinline__ void st4_dev(U* p, const U (&v)[4]) {
+#pragma unroll
+  for (int x = 0; x < 4; x++) st_dev(p + x, v[x]);
+}
+// Wait until this wave's memory operations are complete.
+__device__ inline void wait_vm() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
 // ---------------------------------------------------------------------------
 // Panel kernel, fast mode, for one matrix (k_fw_panel_fast): phases 1 and 2
@@ -370,7 +399,7 @@ __device__ inline void fw_panel_pair_body(T* W, int64_t n, int64_t ld, int64_t K
   const int J = bid < K ? bid : bid + 1;
   const int64_t j0 = (int64_t)J * BLK;
   const bool owner = bid == 0;
-  const int par = (int)(K & 1);
+  const int par = ws.slot(K);
   const int lim = width - 2;
   auto canon = [](U x) -> U { return (T)x < 0 ? ALL : x; };
   auto pair = [](U b, U& m, U& y) {
@@ -409,7 +438,7 @@ __device__ inline void fw_panel_pair_body(T* W, int64_t n, int64_t ld, int64_t K
   }
   __syncthreads();
   if (sbad) return;
-  if (owner && tid == 0 && K > 0) capture_block<T, true>(st, ws.col<T>(par ^ 1), ws.row<T>(par ^ 1), k0 - BLK, n);
+  if (owner && tid == 0 && K > 0) capture_block<T, true>(st, ws.col<T>(ws.slot(K - 1)), ws.row<T>(ws.slot(K - 1)), k0 - BLK, n);
   const int64_t seen = st->diameter_max;  // a lower bound of the value at exit
 
   T dd = 0, dr = 0, dc = 0;
@@ -567,7 +596,7 @@ __device__ __forceinline__ void fw_panel_lines_body(LinesShared<typename unsigne
   auto& sbad = L.sbad;
   const int tid = threadIdx.x;
   const int64_t k0 = K * BLK;
-  const int par = (int)(K & 1);
+  const int par = ws.slot(K);
   const bool owner = bid == 0;
   T* colT = ws.col<T>(par);
   T* rowS = ws.row<T>(par);
@@ -584,7 +613,7 @@ __device__ __forceinline__ void fw_panel_lines_body(LinesShared<typename unsigne
   if (tid == 0) sbad = (st->flags & (RBG_F_NEED_GENERAL | RBG_F_PRECOND_FAIL)) ? 2 : 0;
   __syncthreads();
   if (sbad) return;
-  if (owner && tid == 0 && K > 0) capture_block<T, true>(st, ws.col<T>(par ^ 1), ws.row<T>(par ^ 1), k0 - BLK, n);
+  if (owner && tid == 0 && K > 0) capture_block<T, true>(st, ws.col<T>(ws.slot(K - 1)), ws.row<T>(ws.slot(K - 1)), k0 - BLK, n);
   const int64_t seen = st->diameter_max;  // a lower bound of the value at exit
   for (int e = tid; e < BLK * BLK; e += NT) D[e / BLK][e % BLK] = canon((U)W[(k0 + e / BLK) * ld + k0 + e % BLK]);
   // this thread's panel line, loaded while the diagonal block is walked
@@ -810,7 +839,7 @@ __device__ __forceinline__ void fw_tile_fast_body(TileShared<typename unsigned_o
   auto& sGb = L.sGb;
   const int tid = threadIdx.x;
   const int64_t k0 = K * BLK;
-  const int par = (int)(K & 1);
+  const int par = ws.slot(K);
   const U* colT = (const U*)ws.col<T>(par);
   const U* rowS = (const U*)ws.row<T>(par);
 
@@ -840,8 +869,32 @@ __device__ __forceinline__ void fw_tile_fast_body(TileShared<typename unsigned_o
 #pragma unroll
   for (int q = 0; q < SPT; q++) {
     const int e = tid + q * NT, kk = e / GL, x = (e % GL) * 4;
-    ld4(colT + kk * n + (int64_t)I * TL + x, sa[q]);
-    ld4(rowS + kk * n + (int64_t)J * TL + x, sb[q]);
+    if (ahead == 2) {
+      // the persistent closure: left operands of rows above block K and
+      // right operands of columns right of it are absent; block K's own
+      // come from the diagonal block's snapshots
+      const int64_t ri = (int64_t)I * TL + x, cj = (int64_t)J * TL + x;
+      const U* dsn = (const U*)ws.dsnap + K * 2 * BLK * BLK;
+      if (ri < k0) {
+#pragma unroll
+        for (int c = 0; c < 4; c++) sa[q][c] = ~(U)0;
+      } else if (ri < k0 + BLK) {
+        ld4(dsn + kk * BLK + (ri - k0), sa[q]);
+      } else {
+        ld4(colT + kk * n + ri, sa[q]);
+      }
+      if (cj >= k0 + BLK) {
+#pragma unroll
+        for (int c = 0; c < 4; c++) sb[q][c] = ~(U)0;
+      } else if (cj >= k0) {
+        ld4(dsn + BLK * BLK + kk * BLK + (cj - k0), sb[q]);
+      } else {
+        ld4(rowS + kk * n + cj, sb[q]);
+      }
+    } else {
+      ld4(colT + kk * n + (int64_t)I * TL + x, sa[q]);
+      ld4(rowS + kk * n + (int64_t)J * TL + x, sb[q]);
+    }
   }
   const uint32_t flags = st->flags;
   const int64_t seen = st->diameter_max;  // a lower bound of the value at exit
@@ -991,13 +1044,19 @@ __device__ __forceinline__ void fw_tile_fast_body(TileShared<typename unsigned_o
 #pragma unroll
   for (int h = 0; h < NCH; h++) {
     const int64_t c = cb + 64 * h;
-    if (ahead && c >= k0 + BLK && c < k0 + 2 * BLK) continue;  // the next panel's columns
+    // ahead 1: the next panel's columns; ahead 2: every column right of the
+    // pivot block, which a lower triangular step leaves unchanged and later
+    // panels of the persistent closure may be writing
+    if (ahead && c >= k0 + BLK && (ahead == 2 || c < k0 + 2 * BLK)) continue;
 #pragma unroll
     for (int r = 0; r < RPT; r++) {
       U q[4];
 #pragma unroll
       for (int x = 0; x < 4; x++) q[x] = w[r][4 * h + x];
-      st4(Wu + (r0 + r) * ld + c, q);
+      if (ahead == 2)  // the persistent closure: read by the tile's next step, maybe on another XCD
+        st4_dev(Wu + (r0 + r) * ld + c, q);
+      else
+        st4(Wu + (r0 + r) * ld + c, q);
     }
   }
   int64_t dm = (int64_t)(d0 > d1 ? d0 : d1);
@@ -1058,6 +1117,17 @@ __global__ __launch_bounds__(NT, PER_CU) void k_fw_tile_fast(T* W, int64_t n, in
       L, W, n, ld, K, ws, width, st, r.i0 + b / r.jn, r.j0 + b % r.jn);
 }
 
+// The diagonal blocks of W into dst ([K][32][32], FwWork::dcopy): one
+// workgroup per block.  In a lower triangular closure no step changes
+// diagonal block K before pivot block K's panel, so the copy taken before the
+// first step is what that panel must start from.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_fw_diag_copy(const T* W, int64_t ld, T* dst) {
+  const int64_t k0 = (int64_t)blockIdx.x * BLK;
+  for (int e = threadIdx.x; e < BLK * BLK; e += NT)
+    dst[k0 * BLK + e] = W[(k0 + e / BLK) * ld + k0 + e % BLK];
+}
+
 // Panel of the look-ahead schedule (fw_panel_quad_body), for lower
 // triangular matrices: the pair panel's walk (fw_panel_pair_body) with 256
 // threads, thread (i4, j) holding rows i4, i4 + 8, i4 + 16, i4 + 24 of column
@@ -1106,7 +1176,9 @@ struct QuadShared {
   int sbad, sguard;  // sguard: the strip's MSB guard failed
 };
 
-template <typename T>
+// DEV: device-coherent stores (the persistent closure, whose readers of these
+// cells may run on other XCDs in the same launch).
+template <typename T, bool DEV = false>
 __device__ __forceinline__ void fw_panel_quad_body(QuadShared<typename unsigned_of<T>::type>& L, T* W, int64_t n,
                                                    int64_t ld, int64_t K, FwWork ws, int width, rbg_status* st,
                                                    int bid) {
@@ -1120,7 +1192,7 @@ __device__ __forceinline__ void fw_panel_quad_body(QuadShared<typename unsigned_
   const int64_t j0 = (int64_t)J * BLK;
   const bool owner = bid == 0;
   const bool rowp = J < K;  // the live tile: row panel (K, J), else column panel (J, K)
-  const int par = (int)(K & 1);
+  const int par = ws.slot(K);
   const int lim = width - 2;
   auto canon = [](U x) -> U { return (T)x < 0 ? ALL : x; };
   auto pair = [](U b, U& m, U& y) {
@@ -1138,10 +1210,11 @@ __device__ __forceinline__ void fw_panel_quad_body(QuadShared<typename unsigned_
   // panel tile's cell (row panel: W[k0 + i][j0 + j], column panel:
   // W[j0 + i][k0 + j])
   U d[4], p[4];
+  const U* dsrc = (const U*)ws.dcopy + K * BLK * BLK;
 #pragma unroll
   for (int q = 0; q < 4; q++) {
     const int i = i4 + 8 * q;
-    d[q] = canon((U)W[(k0 + i) * ld + k0 + j]);
+    d[q] = canon(dsrc[i * BLK + j]);
     p[q] = canon((U)(rowp ? W[(k0 + i) * ld + j0 + j] : W[(j0 + i) * ld + k0 + j]));
   }
   if (tid == 0) {
@@ -1150,8 +1223,8 @@ __device__ __forceinline__ void fw_panel_quad_body(QuadShared<typename unsigned_
   }
   if (rowp) {
     // the strip's operands: all loads first, then the LDS stores
-    const U* pcol = (const U*)ws.col<T>(par ^ 1);
-    const U* prow = (const U*)ws.row<T>(par ^ 1);
+    const U* pcol = (const U*)ws.col<T>(ws.slot(K - 1));
+    const U* prow = (const U*)ws.row<T>(ws.slot(K - 1));
     constexpr int PER = BLK * BLK / NT;
     U la[PER], lb[PER];
 #pragma unroll
@@ -1323,20 +1396,26 @@ __device__ __forceinline__ void fw_panel_quad_body(QuadShared<typename unsigned_
   // tiles and snapshots out, coalesced (thread (i, j): snapshot step i, index j)
   T* colT = ws.col<T>(par);
   T* rowS = ws.row<T>(par);
+  auto put = [](T* at, T v) {
+    if constexpr (DEV)
+      st_dev(at, v);
+    else
+      *at = v;
+  };
 #pragma unroll
   for (int q = 0; q < 4; q++) {
     const int i = i4 + 8 * q;
     if (owner) {
-      W[(k0 + i) * ld + k0 + j] = (T)(i == j ? L.sDiag[i] : d[q]);
-      colT[i * n + k0 + j] = (T)L.x.w.sDc[i][j];
-      rowS[i * n + k0 + j] = (T)L.x.w.sDr[i][j];
+      put(&W[(k0 + i) * ld + k0 + j], (T)(i == j ? L.sDiag[i] : d[q]));
+      put(&colT[i * n + k0 + j], (T)L.x.w.sDc[i][j]);
+      put(&rowS[i * n + k0 + j], (T)L.x.w.sDr[i][j]);
     }
     if (rowp)
-      W[(k0 + i) * ld + j0 + j] = (T)p[q];
+      put(&W[(k0 + i) * ld + j0 + j], (T)p[q]);
     else
-      W[(j0 + i) * ld + k0 + j] = (T)p[q];
-    rowS[i * n + j0 + j] = (T)L.x.w.sPr[i][j];
-    colT[i * n + j0 + j] = (T)L.x.w.sPc[i][j];
+      put(&W[(j0 + i) * ld + k0 + j], (T)p[q]);
+    put(&rowS[i * n + j0 + j], (T)L.x.w.sPr[i][j]);
+    put(&colT[i * n + j0 + j], (T)L.x.w.sPc[i][j]);
   }
   // (no capture of the previous block's detections: a detection in this
   // schedule asks for general mode, whose run reports them)
@@ -1430,6 +1509,570 @@ __host__ __device__ inline TileRect tri_rect(int tri, int64_t n, int edge, int64
 }
 
 // ---------------------------------------------------------------------------
+// Persistent closure of a lower triangular matrix (fast mode): one launch for
+// every pivot step.  In a lower triangular matrix a product W[i][k] (x)
+// W[k][j] can be present only for i >= k >= j, so:
+//   - no step before K's own changes diagonal block K or block K's columns:
+//     k_fw_diag_walk closes every diagonal block before the launch, leaving
+//     each step's snapshots in dsnap; block K's column panel needs only
+//     those and the input;
+//   - in block K's column panel, row i at step kk reads its own cell
+//     (i, kk), which no earlier step of the block changed: every row is an
+//     independent loop, and the column snapshots are the input's values;
+//   - in block K's row panel, row kk is final at step kk (later steps would
+//     need entries above the diagonal): every column is an independent
+//     forward substitution, and the row snapshots are its final values;
+//   - a tile's snapshot segments above block K's rows and right of its
+//     columns are absent.
+// Work items, one per workgroup, in ticket order:
+//   P(0), then for K = 0 .. nb - 1: P(K + 1) (while K + 1 < nb), T(K)
+// P(K) = the row-panel items R(K, J), J < K (block (K, J), which first takes
+// pivot block K - 1's products, as the tiles of step K - 1 would have), then
+// the column-panel items C(K, g) (256 rows each, from block K + 1 down).
+// T(K) = the phase-3 tiles of block K's triangular rectangle (tri_rect), the
+// tile row holding block K + 2 first.  An item waits only for the items that
+// last wrote what it reads (flow_cond), all of smaller tickets, so the
+// smallest unfinished item can always run: the waits form no cycle, whatever
+// workgroups are resident.  No cell has two writers at once: the tiles of
+// step K skip block K's and block K + 1's rows and store no column right of
+// block K (a lower triangular step leaves those cells unchanged).  A tile may
+// read a cell right of block K that a panel is writing; either value is on
+// the cell's trajectory, the step's products there are absent, and the value
+// is not stored.
+// Cells and snapshots are written with device-coherent stores (through the
+// XCD's L2); a finished item drains its stores, then publishes its counters.
+// A waiting item polls the counters with device-scope loads, under a time
+// limit (a wait that runs out sets RBG_F_STALLED | RBG_F_NEED_GENERAL and
+// every workgroup leaves), then invalidates its XCD's L2 before reading.
+// ---------------------------------------------------------------------------
+constexpr int FLOW_COL_ROWS = NT;  // rows of one column-panel item
+
+struct FlowCtl {
+  // int32 words, zero before the launch: [2] stall, then pdone[nb],
+  // tdone[nb], tstep[nt * nt], pflag[nb][nb]:
+  //   pdone[K]: P(K) items finished; tdone[K]: T(K) tiles finished;
+  //   tstep[I nt + J]: 1 + the last step tile (I, J) finished;
+  //   pflag[K][idx]: 1 once item idx of P(K) finished
+  int* c;
+  int nb, nt, edge;
+  int64_t total;  // items of the closure
+  __host__ __device__ int* stall() const { return c + 2; }
+  __host__ __device__ int64_t pdone(int K) const { return 4 + K; }
+  __host__ __device__ int64_t tdone(int K) const { return 4 + nb + K; }
+  __host__ __device__ int64_t tstep(int I, int J) const { return 4 + 2 * (int64_t)nb + (int64_t)I * nt + J; }
+  __host__ __device__ int64_t pflag(int K, int idx) const {
+    return 4 + 2 * (int64_t)nb + (int64_t)nt * nt + (int64_t)K * nb + idx;
+  }
+  static int64_t words(int nb, int nt) { return 4 + 2 * (int64_t)nb + (int64_t)nt * nt + (int64_t)nb * nb; }
+  __host__ __device__ int sh() const { return edge == 128 ? 2 : 1; }  // log2 of 32-blocks per tile edge
+  // step K's rectangle: tile rows lo(K) .. nt - 1, tile columns 0 .. lo(K)
+  __host__ __device__ int lo(int K) const { return K >> sh(); }
+  __host__ __device__ int64_t tiles(int K) const { return (int64_t)(nt - lo(K)) * (lo(K) + 1); }
+  // the tile row of step K taken first: block K + 2's, whose row panel waits for it
+  __host__ __device__ int prow(int K) const { return K + 2 < nb ? lo(K + 2) : lo(K); }
+  // column-panel items of P(K): the rows below block K, FLOW_COL_ROWS at a time
+  __host__ __device__ int ncol(int K) const { return (nb - 1 - K + 7) >> 3; }
+  __host__ __device__ int np(int K) const { return K + ncol(K); }
+  // sum over m = 1 .. M of ceil(m / 8), M >= -1
+  __host__ __device__ static int64_t s8(int64_t M) {
+    if (M <= 0) return 0;
+    const int64_t Q = M >> 3, R = M & 7;
+    return 4 * Q * (Q + 1) + R * (Q + 1);
+  }
+  // items of P(0) .. P(K)
+  __host__ __device__ int64_t psum(int K) const { return (int64_t)K * (K + 1) / 2 + s8(nb - 1) - s8(nb - 2 - K); }
+  // tiles of T(0) .. T(K - 1): with 2^sh pivot blocks per tile row and
+  // K = Q 2^sh + r, steps q 2^sh .. q 2^sh + 2^sh - 1 share lo = q, so the
+  // sum is 2^sh F(Q) + r (nt - Q)(Q + 1), F(Q) = sum over q < Q of (nt - q)(q + 1)
+  __host__ __device__ int64_t tsum(int K) const {
+    const int64_t Q = K >> sh(), r = K & ((1 << sh()) - 1), N = nt;
+    const int64_t F = N * Q * (Q - 1) / 2 + N * Q - (Q - 1) * Q * (2 * Q - 1) / 6 - Q * (Q - 1) / 2;
+    return (F << sh()) + r * (N - Q) * (Q + 1);
+  }
+  // first ticket of group g = P(g + 1), T(g), 0 <= g < nb
+  __host__ __device__ int64_t before(int g) const { return psum(g) + tsum(g); }
+  __host__ __device__ int64_t count() const { return psum(nb - 1) + tsum(nb); }
+};
+
+// Ticket x (0 <= x < total) -> its item: role 0, item a of P(K) (row panel
+// (K, a) for a < K, else column-panel rows 32 (K + 1) + 256 (a - K) ...);
+// role 1, tile (a, b) of step K.
+struct FlowItem {
+  int role, K, a, b;
+};
+__host__ __device__ __forceinline__ FlowItem flow_item(const FlowCtl& f, int64_t x) {
+  if (x < f.np(0)) return FlowItem{0, 0, (int)x, 0};
+  int lo = 0, hi = f.nb - 1;  // the group of ticket x
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (f.before(mid) <= x)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const int g = lo;
+  uint32_t off = (uint32_t)(x - f.before(g));
+  if (g + 1 < f.nb) {
+    if (off < (uint32_t)f.np(g + 1)) return FlowItem{0, g + 1, (int)off, 0};
+    off -= f.np(g + 1);
+  }
+  const int l = f.lo(g), pr = f.prow(g);
+  const uint32_t jn = (uint32_t)l + 1;
+  if (off < jn) return FlowItem{1, g, pr, (int)off};
+  off -= jn;
+  const int a = l + (int)(off / jn);
+  return FlowItem{1, g, a >= pr ? a + 1 : a, (int)(off % jn)};
+}
+
+// The waits of an item: condition i (0 <= i < FLOW_MAXW) is "counter word
+// `word` >= target"; returns false past the last one.
+//   P(K), K >= 4: all of P(K - 3) and T(K - 4), the last readers of the
+//     snapshot buffer P(K) writes.  A column-panel item waits for nothing
+//     else.  Row panel (K, J) also waits for what its first part reads --
+//     block J's row snapshots of step K - 1 (item (K - 1, J); the diagonal
+//     block's, J = K - 1, come from dsnap) and block K's column snapshots
+//     of step K - 1 (column item 0 of P(K - 1), which also wrote block
+//     (K, K - 1)) -- and for the tile of step K - 2 that holds block (K, J),
+//     the last tile to write it (the tiles of step K - 1 skip block K's rows).
+//   T(K, I, J): the row-panel items of P(K) for tile column J's blocks left
+//     of block K, the column-panel items for tile row I's blocks below it,
+//     and tile (I, J)'s step K - 1 when that step covered it.
+constexpr int FLOW_MAXW = 10;
+__host__ __device__ inline bool flow_cond(const FlowCtl& f, const FlowItem& q, int i, int64_t& word, int& target) {
+  const int bpt = 1 << f.sh(), K = q.K;
+  if (q.role == 0) {
+    if (K >= 4) {
+      if (i == 0) { word = f.pdone(K - 3); target = f.np(K - 3); return true; }
+      if (i == 1) { word = f.tdone(K - 4); target = (int)f.tiles(K - 4); return true; }
+      i -= 2;
+    }
+    const int J = q.a;
+    if (J >= K) return false;  // a column-panel item
+    if (J < K - 1) {
+      if (i == 0) { word = f.pflag(K - 1, J); target = 1; return true; }
+      i--;
+    }
+    if (i == 0) { word = f.pflag(K - 1, K - 1); target = 1; return true; }
+    if (i == 1 && K >= 2 && (J >> f.sh()) <= f.lo(K - 2)) {
+      word = f.tstep(f.lo(K), J >> f.sh());
+      target = K - 1;
+      return true;
+    }
+    return false;
+  }
+  const int I = q.a, J = q.b;
+  // row panels for tile column J's blocks s < K
+  for (int s = J * bpt; s < J * bpt + bpt && s < K; s++) {
+    if (i == 0) { word = f.pflag(K, s); target = 1; return true; }
+    i--;
+  }
+  // column panels for tile row I's blocks s > K: item K + (s - K - 1) / 8
+  int last = -1;
+  for (int s = I * bpt; s < I * bpt + bpt; s++) {
+    if (s <= K) continue;
+    const int g = (s - K - 1) >> 3;
+    if (g == last) continue;
+    last = g;
+    if (i == 0) { word = f.pflag(K, K + g); target = 1; return true; }
+    i--;
+  }
+  if (i == 0 && K >= 1 && J <= f.lo(K - 1)) { word = f.tstep(I, J); target = K; return true; }
+  return false;
+}
+
+__device__ inline int flow_load(const int* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+
+// Wave 0 of a workgroup: lane i waits until condition i of the item holds
+// (counters only grow), all lanes polling together.  False when a wait ran
+// out (then the closure is marked stalled) or another workgroup's did.
+__device__ __forceinline__ bool flow_wait(const FlowCtl& f, const FlowItem& q, int64_t limit, rbg_status* st) {
+  const int lane = threadIdx.x & 63;
+  int64_t word = 0;
+  int target = 0;
+  const bool mine = lane < FLOW_MAXW && flow_cond(f, q, lane, word, target);
+  const int64_t t0 = wall_clock64();
+  for (;;) {
+    const bool ok = !mine || flow_load(f.c + word) >= target;
+    if (__all(ok)) return true;
+    if (flow_load(f.stall())) return false;
+    if (wall_clock64() - t0 > limit) {
+      if (lane == 0) {
+        __hip_atomic_store(f.stall(), 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        atomicOr(&st->flags, (uint32_t)(RBG_F_STALLED | RBG_F_NEED_GENERAL));
+      }
+      return false;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+}
+
+// Every diagonal block of a lower triangular matrix closed before the
+// persistent launch (block K: workgroup K, one cell per thread): the pair
+// panel's walk of the diagonal block alone.  Leaves the closed block in W and
+// each step's snapshots in dsnap[K]; checks the MSB guard of every product,
+// finds cycles from the snapshots, and records the block's trajectory
+// maximum.
+template <typename T>
+__global__ __launch_bounds__(PANEL_NT) void k_fw_diag_walk(T* W, int64_t n, int64_t ld, T* dsnap, int width,
+                                                           rbg_status* st) {
+  using U = typename unsigned_of<T>::type;
+  constexpr U TOP = (U)1 << (sizeof(U) * 8 - 1);
+  constexpr U ALL = ~(U)0;
+  __shared__ U Dg[BLK][BLK + 1];
+  __shared__ U sDc[BLK][BLK + 1];  // column kk at step kk, row x
+  __shared__ U sDr[BLK][BLK + 1];  // row kk at step kk, column x
+  __shared__ U MY[2][BLK][2];      // (shift, low bits) of the pivot row, by step parity
+  __shared__ U sDiag[BLK];
+  __shared__ int64_t sMax[PANEL_NT / 64];
+  __shared__ int sbad;
+  const int tid = threadIdx.x, i = tid >> 5, j = tid & (BLK - 1);
+  const int64_t K = blockIdx.x, k0 = K * BLK;
+  const int lim = width - 2;
+  auto canon = [](U x) -> U { return (T)x < 0 ? ALL : x; };
+  auto pair = [](U b, U& m, U& y) {
+    if ((T)b < 0) {
+      m = 0;
+      y = TOP;
+    } else {
+      m = (U)msb_u(b);
+      y = b ^ ((U)1 << m);
+    }
+  };
+  U d = canon((U)W[(k0 + i) * ld + k0 + j]);
+  if (tid == 0) sbad = (st->flags & (RBG_F_NEED_GENERAL | RBG_F_PRECOND_FAIL)) ? 2 : 0;
+  if (i == j) {
+    sDiag[i] = d;
+    d = ALL;
+  }
+  Dg[i][j] = d;
+  if (i == 0) {
+    pair(d, MY[0][j][0], MY[0][j][1]);
+    sDr[0][j] = d;
+  }
+  if (j == 0) sDc[0][i] = d;
+  __syncthreads();
+  if (sbad) return;
+  const int64_t seen = st->diameter_max;
+  T dd = 0;
+  int g = 0;
+  for (int kk = 0; kk < BLK; kk++) {
+    const int q = kk & 1;
+    const U ad = canon(Dg[i][kk]);
+    const U md = MY[q][j][0], yd = MY[q][j][1];
+    const int gd = ((T)ad > 1 ? msb_u(ad) : 0) + (int)md;
+    g = g > gd ? g : gd;
+    const U pd = (ad << md) | yd;
+    d = d < pd ? d : pd;
+    dd = dd > (T)d ? dd : (T)d;
+    Dg[i][j] = d;
+    if (kk + 1 < BLK) {
+      if (i == kk + 1) {
+        const U cd = canon(d);
+        pair(cd, MY[q ^ 1][j][0], MY[q ^ 1][j][1]);
+        sDr[kk + 1][j] = cd;
+      }
+      if (j == kk + 1) sDc[kk + 1][i] = canon(d);
+    }
+    __syncthreads();
+  }
+  if (g > lim) sbad = 1;
+  // cycles: step x, vertex y with W[y][x] and W[x][y] both present
+  if (i != j && (T)sDc[i][j] >= 0 && (T)sDr[i][j] >= 0) {
+    record_cycle(st, naive_key(k0 + i, k0 + j, k0 + j, n));
+    sbad = 1;
+  }
+  __syncthreads();
+  if (sbad) {
+    if (tid == 0) atomicOr(&st->flags, (uint32_t)RBG_F_NEED_GENERAL);
+    return;
+  }
+  W[(k0 + i) * ld + k0 + j] = (T)(i == j ? sDiag[i] : d);
+  dsnap[K * 2 * BLK * BLK + i * BLK + j] = (T)sDc[i][j];
+  dsnap[K * 2 * BLK * BLK + BLK * BLK + i * BLK + j] = (T)sDr[i][j];
+  T m = dd;
+  if (i == j && (T)sDiag[i] > m) m = (T)sDiag[i];
+  const int64_t wm = wave_max_i64((int64_t)m);
+  if ((tid & 63) == 0) sMax[tid >> 6] = wm;
+  __syncthreads();
+  if (tid == 0) {
+    int64_t x = 0;
+    for (int w = 0; w < PANEL_NT / 64; w++) x = x > sMax[w] ? x : sMax[w];
+    if (x > seen) record_diameter(st, x);
+  }
+}
+
+// Column panel of pivot block K in the persistent closure (lower triangular):
+// rows r = 32 (K + 1) + 256 g + t, one per thread t.  Row r at step kk of the
+// block reads its own cell (r, kk), which no earlier step of the block
+// changed, and block K's diagonal row snapshot kk (dsnap[K]):
+//     C[r][x] = C[r][x] (+) C[r][kk] (x) D_kk[kk][x],   x < kk
+// (products with x >= kk have an absent right operand).  The step's column
+// snapshot of row r is C[r][kk] as the closure found it.  Writes the row's
+// cells and column snapshots with device-coherent stores.
+template <typename U>
+struct FlowColShared {
+  U M[BLK][BLK];  // (shift, low bits) of the diagonal row snapshots [kk][x]
+  U Y[BLK][BLK];
+  int G[BLK];     // per kk, the largest shift among them
+  int64_t sMax[NT / 64];
+  int sbad;
+};
+template <typename T>
+__device__ __forceinline__ void fw_flow_col_body(FlowColShared<typename unsigned_of<T>::type>& L, T* W, int64_t n,
+                                                 int64_t ld, int K, int g, FwWork ws, int width, rbg_status* st) {
+  using U = typename unsigned_of<T>::type;
+  constexpr U TOP = (U)1 << (sizeof(U) * 8 - 1);
+  constexpr U ALL = ~(U)0;
+  auto canon = [](U x) -> U { return (T)x < 0 ? ALL : x; };
+  const int tid = threadIdx.x;
+  const int64_t k0 = (int64_t)K * BLK;
+  const int64_t r = k0 + BLK + (int64_t)g * FLOW_COL_ROWS + tid;
+  const bool live = r < n;
+  U c[BLK];
+  U* Wu = (U*)W;
+  if (live) {
+#pragma unroll
+    for (int h = 0; h < BLK / 4; h++) {
+      U q[4];
+      ld4(Wu + r * ld + k0 + 4 * h, q);
+#pragma unroll
+      for (int x = 0; x < 4; x++) c[4 * h + x] = canon(q[x]);
+    }
+  }
+  const U* drow = (const U*)ws.dsnap + (int64_t)K * 2 * BLK * BLK + BLK * BLK;
+  for (int e = tid; e < BLK * BLK; e += NT) {
+    const U b = drow[e];
+    const bool abs = (T)b < 0;
+    const int m = abs ? 0 : msb_u(b);
+    L.M[e / BLK][e % BLK] = (U)m;
+    L.Y[e / BLK][e % BLK] = abs ? TOP : b ^ ((U)1 << m);
+  }
+  if (tid < BLK) L.G[tid] = 0;
+  if (tid == 0) L.sbad = (st->flags & (RBG_F_NEED_GENERAL | RBG_F_PRECOND_FAIL)) ? 1 : 0;
+  __syncthreads();
+  if (tid < BLK) {
+    int gm = 0;
+    for (int x = 0; x < tid; x++) gm = (int)L.M[tid][x] > gm ? (int)L.M[tid][x] : gm;
+    L.G[tid] = gm;
+  }
+  __syncthreads();
+  if (L.sbad) return;
+  const int64_t seen = st->diameter_max;
+  T dm = 0;
+  int ga = 0;
+  if (live) {
+    T* colT = ws.col<T>(ws.slot(K));
+#pragma unroll
+    for (int kk = 0; kk < BLK; kk++) st_dev(&colT[kk * n + r], (T)c[kk]);
+#pragma unroll
+    for (int kk = 1; kk < BLK; kk++) {
+      const U a = c[kk];  // as the closure found it: no earlier step of the block changed it
+      const int am = (T)a > 1 ? msb_u(a) : 0;
+      ga = am + L.G[kk] > ga ? am + L.G[kk] : ga;
+#pragma unroll
+      for (int x = 0; x < kk; x++) {
+        const U p = (a << L.M[kk][x]) | L.Y[kk][x];
+        c[x] = c[x] < p ? c[x] : p;
+        dm = dm > (T)c[x] ? dm : (T)c[x];
+      }
+    }
+#pragma unroll
+    for (int h = 0; h < BLK / 4; h++) {
+      U q[4];
+#pragma unroll
+      for (int x = 0; x < 4; x++) q[x] = c[4 * h + x];
+      st4_dev(Wu + r * ld + k0 + 4 * h, q);
+    }
+  }
+  if (ga > width - 2) atomicOr(&st->flags, (uint32_t)RBG_F_NEED_GENERAL);
+  const int64_t wm = wave_max_i64((int64_t)dm);
+  if ((tid & 63) == 0 && wm > seen) record_diameter(st, wm);
+}
+
+// Row panel (K, J), J < K, of the persistent closure: block K's rows in block
+// J's columns.  First pivot block K - 1's products (as its tiles would have
+// taken them; they skip block K's rows), from step K - 1's snapshots:
+//     R[i][j] = R[i][j] (+) colsnap_kk[k0 + i] (x) rowsnap_kk[j0 + j]
+// then block K's steps: row kk of the panel is final at step kk (a later step
+// would need an entry above the diagonal), so each column is a forward
+// substitution,
+//     R[i][j] = R[i][j] (+) D_kk[i][kk] (x) R[kk][j],   i > kk,
+// D_kk[i][kk] being block K's diagonal column snapshots (dsnap[K]).  The
+// step's row snapshot is R[kk][j].  Thread t takes column t >> 3, rows
+// (t & 7) + 8 q: the eight threads of a column share a wave, and row kk's
+// value goes to them by a shuffle, with no barrier between steps.
+template <typename U>
+struct FlowRowShared {
+  U R[BLK][BLK + 1];        // the panel, for the transposition
+  U sL[BLK][BLK + 1];       // step K - 1's column snapshots of block K's rows [kk][i]
+  U sY[BLK][BLK + 1];       // step K - 1's row snapshots of block J's columns: low bits [kk][j]
+  uint8_t sM[BLK][BLK + 4]; // and shifts
+  U Dc[BLK][BLK + 1];       // block K's diagonal column snapshots [kk][i]
+  int64_t sMax[NT / 64];
+  int sbad;
+};
+template <typename T>
+__device__ __forceinline__ void fw_flow_row_body(FlowRowShared<typename unsigned_of<T>::type>& L, T* W, int64_t n,
+                                                 int64_t ld, int K, int J, FwWork ws, int width, rbg_status* st) {
+  using U = typename unsigned_of<T>::type;
+  constexpr U TOP = (U)1 << (sizeof(U) * 8 - 1);
+  constexpr U ALL = ~(U)0;
+  auto canon = [](U x) -> U { return (T)x < 0 ? ALL : x; };
+  auto amsb = [](U a) -> int { return (T)a > 1 ? msb_u(a) : 0; };
+  const int tid = threadIdx.x;
+  const int64_t k0 = (int64_t)K * BLK, j0 = (int64_t)J * BLK;
+  constexpr int PER = BLK * BLK / NT;  // 4
+  // every load first: the panel (coalesced), step K - 1's snapshots, block K's diagonal snapshots
+  const U* pcol = (const U*)ws.col<T>(ws.slot(K - 1)) + k0;
+  const U* prow;
+  int64_t prs;  // the row snapshots' stride: the diagonal block's come from dsnap
+  if (J == K - 1) {
+    prow = (const U*)ws.dsnap + (int64_t)(K - 1) * 2 * BLK * BLK + BLK * BLK;
+    prs = BLK;
+  } else {
+    prow = (const U*)ws.row<T>(ws.slot(K - 1)) + j0;
+    prs = n;
+  }
+  const U* dcol = (const U*)ws.dsnap + (int64_t)K * 2 * BLK * BLK;
+  U w[PER], la[PER], lb[PER], ld_[PER];
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    const int e = tid + q * NT, y = e / BLK, x = e % BLK;
+    w[q] = ((const U*)W)[(k0 + y) * ld + j0 + x];
+    la[q] = pcol[y * n + x];
+    lb[q] = prow[y * prs + x];
+    ld_[q] = dcol[y * BLK + x];
+  }
+  if (tid == 0) L.sbad = (st->flags & (RBG_F_NEED_GENERAL | RBG_F_PRECOND_FAIL)) ? 1 : 0;
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    const int e = tid + q * NT, y = e / BLK, x = e % BLK;
+    L.R[y][x] = canon(w[q]);
+    L.sL[y][x] = canon(la[q]);
+    const U b = lb[q];
+    const bool abs = (T)b < 0;
+    const int m = abs ? 0 : msb_u(b);
+    L.sM[y][x] = (uint8_t)m;
+    L.sY[y][x] = abs ? TOP : b ^ ((U)1 << m);
+    L.Dc[y][x] = canon(ld_[q]);
+  }
+  __syncthreads();
+  if (L.sbad) return;
+  const int64_t seen = st->diameter_max;
+  const int lim = width - 2;
+  const int j = tid >> 3, i4 = tid & 7;
+  U r[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) r[q] = L.R[i4 + 8 * q][j];
+  T dm = 0;
+  int ga = 0;
+  // pivot block K - 1
+#pragma unroll 4
+  for (int kk = 0; kk < BLK; kk++) {
+    const U m = L.sM[kk][j], y = L.sY[kk][j];
+    const int mb = (T)y < 0 ? 0 : (int)m;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const U a = L.sL[kk][i4 + 8 * q];
+      const int gq = amsb(a) + mb;
+      ga = gq > ga ? gq : ga;
+      const U v = (a << m) | y;
+      r[q] = r[q] < v ? r[q] : v;
+      dm = dm > (T)r[q] ? dm : (T)r[q];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; q++) r[q] = canon(r[q]);
+  // pivot block K: forward substitution down each column
+  T* rowS = ws.row<T>(ws.slot(K));
+  const int base = (threadIdx.x & 63) & ~7;
+#pragma unroll
+  for (int kk = 0; kk < BLK; kk++) {
+    const U b = canon((U)__shfl((T)r[kk >> 3], base + (kk & 7), 64));
+    if (i4 == (kk & 7)) st_dev(&rowS[kk * n + j0 + j], (T)b);
+    const bool abs = (T)b < 0;
+    const U m = abs ? 0 : (U)msb_u(b);
+    const U y = abs ? TOP : b ^ ((U)1 << m);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      if (i4 + 8 * q > kk) {
+        const U a = L.Dc[kk][i4 + 8 * q];
+        const int gq = amsb(a) + (int)m;
+        ga = gq > ga ? gq : ga;
+        const U v = (a << m) | y;
+        r[q] = r[q] < v ? r[q] : v;
+        dm = dm > (T)r[q] ? dm : (T)r[q];
+      }
+    }
+  }
+  if (ga > lim) atomicOr(&st->flags, (uint32_t)RBG_F_NEED_GENERAL);
+#pragma unroll
+  for (int q = 0; q < 4; q++) st_dev((U*)&W[(k0 + i4 + 8 * q) * ld + j0 + j], r[q]);
+  const int64_t wm = wave_max_i64((int64_t)dm);
+  if ((threadIdx.x & 63) == 0 && wm > seen) record_diameter(st, wm);
+}
+
+static __global__ void k_flow_zero(int* c, int64_t words) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < words; i += (int64_t)gridDim.x * blockDim.x)
+    c[i] = 0;
+}
+
+// One work item per workgroup: the grid holds every item of the closure in
+// ticket order, and workgroup x takes ticket x.  The hardware starts each
+// XCD's workgroups in index order, so a workgroup that has not started
+// leaves every later one of its XCD unstarted too: the smallest unfinished
+// ticket is always running, or starts as soon as an earlier one ends.
+// limit: the longest wait of one item, in wall-clock ticks.
+template <typename T, int RC, bool SHIFT_BYTES, int PER_CU>
+__global__ __launch_bounds__(NT, PER_CU) void k_fw_flow(T* W, int64_t n, int64_t ld, FwWork ws, int width,
+                                                       rbg_status* st, FlowCtl f, int64_t limit) {
+  using U = typename unsigned_of<T>::type;
+  __shared__ union {
+    TileShared<U, 16 * RC, SHIFT_BYTES> t;
+    FlowRowShared<U> r;
+    FlowColShared<U> c;
+  } L;
+  __shared__ int sRole;
+  const FlowItem q = flow_item(f, blockIdx.x);  // the same in every thread
+  if (threadIdx.x < 64) {
+    bool ok = !flow_load(f.stall()) && flow_wait(f, q, limit, st);
+    if (threadIdx.x == 0) {
+      // the items waited for published their cells: drop stale lines
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      sRole = ok ? q.role : -1;
+    }
+  }
+  __syncthreads();
+  if (sRole < 0) return;
+  const int K = q.K, a = q.a, b = q.b;
+  if (q.role == 0) {
+    // the panels are the closure's critical path: their waves take issue
+    // priority over the tile waves that share their SIMDs
+    __builtin_amdgcn_s_setprio(3);
+    if (a < K)
+      fw_flow_row_body<T>(L.r, W, n, ld, K, a, ws, width, st);
+    else
+      fw_flow_col_body<T>(L.c, W, n, ld, K, a - K, ws, width, st);
+  } else {
+    fw_tile_fast_body<T, RC, RC, false, SHIFT_BYTES>(L.t, W, n, ld, K, ws, width, st, a, b, 2);
+  }
+  // publication: the item's cells and snapshots went out with device-coherent
+  // stores; once every wave's stores are complete, the counters may say so
+  wait_vm();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int* c = f.c;
+    if (q.role == 0) {
+      __hip_atomic_store(c + f.pflag(K, a), 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(c + f.pdone(K), 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      __hip_atomic_store(c + f.tstep(a, b), K + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(c + f.tdone(K), 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Phase 3, general mode (natural domain, full arithmetic and event keys).
 // ---------------------------------------------------------------------------
 template <typename T>
@@ -1441,7 +2084,7 @@ __global__ __launch_bounds__(NT) void k_fw_tile_general(T* W, int64_t n, int64_t
   const int nt = (int)(n / TILE);
   const int I = blockIdx.x / nt, J = blockIdx.x % nt;
   const int64_t k0 = K * BLK;
-  const int par = (int)(K & 1);
+  const int par = ws.slot(K);
   const T* colT = ws.col<T>(par);
   const T* rowS = ws.row<T>(par);
   for (int e = tid; e < BLK * TILE; e += NT) {
@@ -1488,7 +2131,7 @@ __global__ __launch_bounds__(NT) void k_fw_tile_general(T* W, int64_t n, int64_t
 
 template <typename T, bool FAST>
 __global__ void k_fw_capture_last(rbg_status* st, FwWork ws, int64_t K, int64_t n) {
-  const int par = (int)(K & 1);
+  const int par = ws.slot(K);
   capture_block<T, FAST>(st, ws.col<T>(par), ws.row<T>(par), K * BLK, n);
 }
 

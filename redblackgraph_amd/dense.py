@@ -26,7 +26,7 @@ def _workspace(npad: int, elem: int):
     key = (dev.index, npad, elem)
     ws = cache.get(key)
     if ws is None:
-        nbytes = int(_lib.load().rbg_fw_workspace_bytes(npad, elem))
+        nbytes = int(_lib.load().rbg_fw_tri_workspace_bytes(npad, elem))
         ws = t.empty(nbytes, dtype=t.uint8, device=dev)
         cache.clear()
         cache[key] = ws
@@ -42,7 +42,9 @@ class FwOutcome:
     status: _lib.RbgStatus
     npad: int
     mode: str  # "fast" | "general"
-    schedule: str = "full"  # "full" | "lower" | "upper" (triangular schedules)
+    # "full" | "lower" | "upper" (triangular schedules; "lower" is the
+    # persistent closure for int64) | "lower-steps" (one launch per pivot step)
+    schedule: str = "full"
 
     @property
     def diameter(self) -> int:
@@ -72,7 +74,7 @@ def status() -> _lib.Status:
     return st
 
 
-_TRI = {"full": 0, "lower": 1, "upper": 2}
+_TRI = {"full": 0, "lower": 1, "upper": 2, "lower-steps": 3}
 
 
 def fw_closure(W, et: ElemType, stream=None, force_general: bool = False, schedule: str = "auto"):
@@ -84,8 +86,10 @@ def fw_closure(W, et: ElemType, stream=None, force_general: bool = False, schedu
     schedule when the precheck finds every off-diagonal entry on one side of
     the diagonal (a pedigree numbered parents first is lower triangular), else
     the full one; "full" forces the full schedule (the results are the same).
+    An int64 lower triangular closure runs as one persistent launch; "steps"
+    keeps one launch per pivot step there (the same results again).
     """
-    if schedule not in ("auto", "full"):
+    if schedule not in ("auto", "full", "steps"):
         raise ValueError(f"unknown schedule {schedule!r}")
     t = _lib.torch()
     n = W.shape[0]
@@ -109,9 +113,9 @@ def fw_closure(W, et: ElemType, stream=None, force_general: bool = False, schedu
         s = st.read(stream)
         if s.flags & _lib.F_PRECOND_FAIL:
             fast = False
-        elif schedule == "auto":
+        elif schedule in ("auto", "steps"):
             if not s.flags & _lib.F_HAS_UPPER:
-                sched = "lower"
+                sched = "lower" if schedule == "auto" else "lower-steps"
             elif not s.flags & _lib.F_HAS_LOWER:
                 sched = "upper"
         st.reset(stream)
@@ -119,6 +123,14 @@ def fw_closure(W, et: ElemType, stream=None, force_general: bool = False, schedu
         sched = "full"
     _run(work, et, 0 if fast else 1, st, stream, _TRI[sched])
     s = st.read(stream)
+    if fast and s.flags & _lib.F_STALLED:
+        # a wait of the persistent closure ran out: the same schedule, one
+        # launch per pivot step, on the original input
+        work = fresh()
+        st.reset(stream)
+        sched = "lower-steps"
+        _run(work, et, 0, st, stream, _TRI[sched])
+        s = st.read(stream)
     mode = "fast" if fast else "general"
     # A cycle makes a diagonal value >= 2, after which products through the
     # pivot cell (skipped by the fast kernels) can raise events of their own:

@@ -82,8 +82,9 @@ class Resident:
 def _dense_bytes(n: int) -> int:
     from .dense import padded_order
     p = padded_order(n)
-    # input copy, padded work matrix, snapshot workspace
-    return 4 * n * n + 4 * p * p + 4 * 4 * 32 * p
+    # input copy, padded work matrix, snapshot workspace (four snapshot
+    # buffers and the persistent closure's counters)
+    return 4 * n * n + 4 * p * p + 8 * 4 * 32 * p + 4 * (p // 64) ** 2
 
 
 def _sparse_close_bytes(nnz: int, n: int) -> int:

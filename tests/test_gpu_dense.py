@@ -233,6 +233,30 @@ def test_fw_triangular_schedule_vs_oracle(orc, dt, n, reverse):
     assert out2.diameter == ref.diameter
 
 
+@pytest.mark.parametrize("dt,n", [(np.int64, 128), (np.int64, 300), (np.int64, 900), (np.int64, 2000),
+                                  (np.int64, 3000), (np.int32, 700)])
+def test_fw_persistent_closure_vs_steps_and_oracle(orc, dt, n):
+    """The persistent lower triangular closure (int64: one launch, work items
+    waiting on each other) against the same schedule launched step by step and
+    the oracle: W and the trajectory diameter, three runs each (the items'
+    order on the GPU differs from run to run), no wait running out.  (int32
+    lower triangular closures launch step by step either way.)"""
+    A = synth.dense_pedigree(synth.component_spec(n, 0.8, 4), 29 + n, dtype=dt)
+    ref = orc.warshall(A) if n <= 1100 else None
+    Ws, outs = dense.fw_closure(_lib.to_device(A), elem_type(A.dtype), schedule="steps")
+    assert outs.schedule == "lower-steps"
+    Ws = Ws.cpu().numpy()
+    if ref is not None:
+        np.testing.assert_array_equal(Ws, ref.W)
+        assert outs.diameter == ref.diameter
+    for _ in range(3):
+        W, out = _fw(A)
+        assert out.mode == "fast" and out.schedule == "lower"
+        assert not out.status.flags & (_lib.F_STALLED | _lib.F_NEED_GENERAL)
+        np.testing.assert_array_equal(W, Ws)
+        assert out.diameter == outs.diameter
+
+
 @pytest.mark.parametrize("dt,n,wrong", [(np.int32, 640, 2), (np.int32, 640, 1), (np.int64, 384, 2)])
 def test_fw_triangular_wrong_side_detected(dt, n, wrong):
     """rbg_fw_dense_tri_* told the wrong side (tri = 2 for a lower triangular
@@ -247,7 +271,7 @@ def test_fw_triangular_wrong_side_detected(dt, n, wrong):
     sfx = "i32" if dt == np.int32 else "i64"
     esz = np.dtype(dt).itemsize
     dW = _lib.to_device(A)
-    ws = t.empty(int(_lib.load().rbg_fw_workspace_bytes(n, esz)), dtype=t.uint8, device=dW.device)
+    ws = t.empty(int(_lib.load().rbg_fw_tri_workspace_bytes(n, esz)), dtype=t.uint8, device=dW.device)
     st = dense.status()
     st.reset()
     _lib.call(f"rbg_fw_dense_tri_{sfx}", _lib.ptr(dW), n, n, 8 * esz, 0, 0, wrong, _lib.ptr(ws), ws.numel(), st.p,
@@ -295,6 +319,7 @@ def _check_digest(name, W, out):
     assert got == ref["sha256_W"]
     assert out.diameter == ref["diameter"]
     assert out.status.cycle_count == 0 and out.status.events == 0
+    assert not out.status.flags & _lib.F_STALLED
 
 
 @pytest.mark.parametrize("name,schedule", [
@@ -315,6 +340,10 @@ def test_fw_2304_vs_oracle(name, schedule):
     W2, out2 = dense.fw_closure(_lib.to_device(A), elem_type(A.dtype), schedule="full")
     assert out2.schedule == "full"
     _check_digest(name, W2[:A.shape[0], :A.shape[0]].cpu().numpy(), out2)
+    if schedule == "lower":  # the persistent closure above; one launch per pivot step here
+        W3, out3 = dense.fw_closure(_lib.to_device(A), elem_type(A.dtype), schedule="steps")
+        assert out3.schedule == "lower-steps"
+        _check_digest(name, W3[:A.shape[0], :A.shape[0]].cpu().numpy(), out3)
     if name == "dense2304_i32":
         Wg, og = _fw(A, force_general=True)
         assert og.mode == "general"
@@ -334,6 +363,10 @@ def test_fw_full_size_vs_oracle(name):
     assert out.mode == "fast" and out.schedule == "lower"
     _check_digest(name, W, out)
     del W
+    W3, out3 = dense.fw_closure(_lib.to_device(A), elem_type(A.dtype), schedule="steps")
+    assert out3.schedule == "lower-steps"
+    _check_digest(name, W3.cpu().numpy(), out3)
+    del W3
     W2, out2 = dense.fw_closure(_lib.to_device(A), elem_type(A.dtype), schedule="full")
     _check_digest(name, W2.cpu().numpy(), out2)
     del W2

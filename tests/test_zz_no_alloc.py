@@ -95,10 +95,11 @@ def test_noalloc_refuses_oversize():
     n = 128
     while int(L.rbg_fw_workspace_bytes(n, 4)) <= cap:
         n += 128
-    dummy = t.zeros(16, dtype=t.int32, device=_lib.device())
+    # a matrix of the full order, so that even a wrongly accepted call stays in bounds
+    W = t.zeros((n, n), dtype=t.int32, device=_lib.device())
     st = _lib.Status()
     st.reset()
-    rc = L.rbg_fw_dense_noalloc_i32(_lib.ptr(dummy), n, n, 32, 0, 0, st.p, _lib.stream_handle(None))
+    rc = L.rbg_fw_dense_noalloc_i32(_lib.ptr(W), n, n, 32, 0, 0, st.p, _lib.stream_handle(None))
     assert rc == -2  # RBG_E_WORKSPACE, before anything runs
     assert int(L.rbg_fw_workspace_bytes(n - 128, 4)) <= cap  # n - 128 still fits
     assert n - 128 >= 16384
