@@ -113,9 +113,11 @@ int64_t rbg_fw_tri_workspace_bytes(int64_t n, int32_t elem_bytes);
 /* No counterpart in the reference (schedule introspection, host only, for
  * tests): the persistent lower triangular closure's work items of order n in
  * ticket order (elem_bytes = 8: the int64 closure), four int32 per item --
- * (0, K, idx, 0) for item idx of pivot block K's panel (row panel (K, idx)
- * for idx < K, else the column panel's rows from 32 (K + 1) + 256 (idx - K)),
- * (1, K, I, J) for tile (I, J) of pivot step K.  Returns the item count;
+ * (0, K, idx, 0) for item idx of pivot block K's panel -- with c = 1 when
+ * block K has column-panel rows (K < n / 32 - 1), else 0: idx = 0 < c the
+ * column panel's first 64 rows (from 32 (K + 1)), c <= idx < K + c row panel
+ * (K, idx - c), idx >= K + c the column panel's rows from 32 (K + 1) +
+ * 64 (idx - K) -- and (1, K, I, J) for tile (I, J) of pivot step K.  Returns the item count;
  * items are written when cap >= 4 * count. */
 int64_t rbg_fw_flow_schedule(int64_t n, int32_t elem_bytes, int32_t *items, int64_t cap);
 /* No counterpart in the reference (host only, for tests): each item's waits,

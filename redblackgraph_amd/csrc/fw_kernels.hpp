@@ -1526,9 +1526,11 @@ __host__ __device__ inline TileRect tri_rect(int tri, int64_t n, int edge, int64
 //     columns are absent.
 // Work items, one per workgroup, in ticket order:
 //   P(0), then for K = 0 .. nb - 1: P(K + 1) (while K + 1 < nb), T(K)
-// P(K) = the row-panel items R(K, J), J < K (block (K, J), which first takes
-// pivot block K - 1's products, as the tiles of step K - 1 would have), then
-// the column-panel items C(K, g) (256 rows each, from block K + 1 down).
+// P(K) = the column-panel items C(K, g) (64 rows each, from block K + 1
+// down) and the row-panel items R(K, J), J < K (block (K, J), which first
+// takes pivot block K - 1's products, as the tiles of step K - 1 would have),
+// in the order C(K, 0) (block K + 1's rows, which R(K + 1, .) read), R(K, 0 ..
+// K - 1), C(K, 1 ..).
 // T(K) = the phase-3 tiles of block K's triangular rectangle (tri_rect), the
 // tile row holding block K + 2 first.  An item waits only for the items that
 // last wrote what it reads (flow_cond), all of smaller tickets, so the
@@ -1545,7 +1547,7 @@ __host__ __device__ inline TileRect tri_rect(int tri, int64_t n, int edge, int64
 // limit (a wait that runs out sets RBG_F_STALLED | RBG_F_NEED_GENERAL and
 // every workgroup leaves), then invalidates its XCD's L2 before reading.
 // ---------------------------------------------------------------------------
-constexpr int FLOW_COL_ROWS = NT;  // rows of one column-panel item
+constexpr int FLOW_COL_ROWS = 64;  // rows of one column-panel item: 4 threads per row
 
 struct FlowCtl {
   // int32 words, zero before the launch: [2] stall, then pdone[nb],
@@ -1570,17 +1572,28 @@ struct FlowCtl {
   __host__ __device__ int64_t tiles(int K) const { return (int64_t)(nt - lo(K)) * (lo(K) + 1); }
   // the tile row of step K taken first: block K + 2's, whose row panel waits for it
   __host__ __device__ int prow(int K) const { return K + 2 < nb ? lo(K + 2) : lo(K); }
-  // column-panel items of P(K): the rows below block K, FLOW_COL_ROWS at a time
-  __host__ __device__ int ncol(int K) const { return (nb - 1 - K + 7) >> 3; }
+  // column-panel items of P(K): the rows below block K, two blocks at a time
+  __host__ __device__ int ncol(int K) const { return (nb - K) >> 1; }
   __host__ __device__ int np(int K) const { return K + ncol(K); }
-  // sum over m = 1 .. M of ceil(m / 8), M >= -1
-  __host__ __device__ static int64_t s8(int64_t M) {
-    if (M <= 0) return 0;
-    const int64_t Q = M >> 3, R = M & 7;
-    return 4 * Q * (Q + 1) + R * (Q + 1);
+  // P(K)'s item numbers: C(K, 0), R(K, 0 .. K - 1), C(K, 1 ..)
+  __host__ __device__ int ridx(int K, int J) const { return J + (ncol(K) > 0 ? 1 : 0); }
+  __host__ __device__ int cidx(int K, int g) const { return g == 0 ? 0 : K + g; }
+  // item idx of P(K): a row panel (sets J) or a column panel (sets g)
+  __host__ __device__ bool is_row(int K, int idx, int& J, int& g) const {
+    const int c = ncol(K) > 0 ? 1 : 0;
+    if (c && idx == 0) { g = 0; return false; }
+    if (idx < K + c) { J = idx - c; return true; }
+    g = idx - K;
+    return false;
   }
-  // items of P(0) .. P(K)
-  __host__ __device__ int64_t psum(int K) const { return (int64_t)K * (K + 1) / 2 + s8(nb - 1) - s8(nb - 2 - K); }
+  // sum over m = 1 .. M of ceil(m / 2), M >= -1
+  __host__ __device__ static int64_t s2(int64_t M) {
+    if (M <= 0) return 0;
+    const int64_t Q = M >> 1, R = M & 1;
+    return Q * (Q + 1) + R * (Q + 1);
+  }
+  // items of P(0) .. P(K): ncol(K') = ceil((nb - 1 - K') / 2)
+  __host__ __device__ int64_t psum(int K) const { return (int64_t)K * (K + 1) / 2 + s2(nb - 1) - s2(nb - 2 - K); }
   // tiles of T(0) .. T(K - 1): with 2^sh pivot blocks per tile row and
   // K = Q 2^sh + r, steps q 2^sh .. q 2^sh + 2^sh - 1 share lo = q, so the
   // sum is 2^sh F(Q) + r (nt - Q)(Q + 1), F(Q) = sum over q < Q of (nt - q)(q + 1)
@@ -1594,9 +1607,8 @@ struct FlowCtl {
   __host__ __device__ int64_t count() const { return psum(nb - 1) + tsum(nb); }
 };
 
-// Ticket x (0 <= x < total) -> its item: role 0, item a of P(K) (row panel
-// (K, a) for a < K, else column-panel rows 32 (K + 1) + 256 (a - K) ...);
-// role 1, tile (a, b) of step K.
+// Ticket x (0 <= x < total) -> its item: role 0, item a of P(K) (FlowCtl::
+// is_row tells which panel); role 1, tile (a, b) of step K.
 struct FlowItem {
   int role, K, a, b;
 };
@@ -1646,13 +1658,13 @@ __host__ __device__ inline bool flow_cond(const FlowCtl& f, const FlowItem& q, i
       if (i == 1) { word = f.tdone(K - 4); target = (int)f.tiles(K - 4); return true; }
       i -= 2;
     }
-    const int J = q.a;
-    if (J >= K) return false;  // a column-panel item
+    int J = 0, g = 0;
+    if (!f.is_row(K, q.a, J, g)) return false;  // a column-panel item
     if (J < K - 1) {
-      if (i == 0) { word = f.pflag(K - 1, J); target = 1; return true; }
+      if (i == 0) { word = f.pflag(K - 1, f.ridx(K - 1, J)); target = 1; return true; }
       i--;
     }
-    if (i == 0) { word = f.pflag(K - 1, K - 1); target = 1; return true; }
+    if (i == 0) { word = f.pflag(K - 1, f.cidx(K - 1, 0)); target = 1; return true; }
     if (i == 1 && K >= 2 && (J >> f.sh()) <= f.lo(K - 2)) {
       word = f.tstep(f.lo(K), J >> f.sh());
       target = K - 1;
@@ -1663,17 +1675,17 @@ __host__ __device__ inline bool flow_cond(const FlowCtl& f, const FlowItem& q, i
   const int I = q.a, J = q.b;
   // row panels for tile column J's blocks s < K
   for (int s = J * bpt; s < J * bpt + bpt && s < K; s++) {
-    if (i == 0) { word = f.pflag(K, s); target = 1; return true; }
+    if (i == 0) { word = f.pflag(K, f.ridx(K, s)); target = 1; return true; }
     i--;
   }
-  // column panels for tile row I's blocks s > K: item K + (s - K - 1) / 8
+  // column panels for tile row I's blocks s > K: C(K, (s - K - 1) / 2)
   int last = -1;
   for (int s = I * bpt; s < I * bpt + bpt; s++) {
     if (s <= K) continue;
-    const int g = (s - K - 1) >> 3;
+    const int g = (s - K - 1) >> 1;
     if (g == last) continue;
     last = g;
-    if (i == 0) { word = f.pflag(K, K + g); target = 1; return true; }
+    if (i == 0) { word = f.pflag(K, f.cidx(K, g)); target = 1; return true; }
     i--;
   }
   if (i == 0 && K >= 1 && J <= f.lo(K - 1)) { word = f.tstep(I, J); target = K; return true; }
@@ -1802,19 +1814,21 @@ __global__ __launch_bounds__(PANEL_NT) void k_fw_diag_walk(T* W, int64_t n, int6
 }
 
 // Column panel of pivot block K in the persistent closure (lower triangular):
-// rows r = 32 (K + 1) + 256 g + t, one per thread t.  Row r at step kk of the
-// block reads its own cell (r, kk), which no earlier step of the block
-// changed, and block K's diagonal row snapshot kk (dsnap[K]):
+// rows r = 32 (K + 1) + 64 g + t / 4.  Row r at step kk of the block reads
+// its own cell (r, kk), which no earlier step of the block changed, and
+// block K's diagonal row snapshot kk (dsnap[K]):
 //     C[r][x] = C[r][x] (+) C[r][kk] (x) D_kk[kk][x],   x < kk
-// (products with x >= kk have an absent right operand).  The step's column
-// snapshot of row r is C[r][kk] as the closure found it.  Writes the row's
-// cells and column snapshots with device-coherent stores.
+// (products with x >= kk have an absent right operand).  Every cell's
+// sequence needs only the row as the closure found it, so the four threads
+// of a row take every fourth cell (x = t % 4 + 4 m).  The step's column
+// snapshot of row r is C[r][kk] as the closure found it.  Writes the cells
+// and the column snapshots with device-coherent stores.
 template <typename U>
 struct FlowColShared {
-  U M[BLK][BLK];  // (shift, low bits) of the diagonal row snapshots [kk][x]
+  U C0[FLOW_COL_ROWS][BLK + 1];  // the rows as the closure found them
+  U M[BLK][BLK];                 // (shift, low bits) of the diagonal row snapshots [kk][x]
   U Y[BLK][BLK];
-  int G[BLK];     // per kk, the largest shift among them
-  int64_t sMax[NT / 64];
+  int G[BLK];                    // per kk, the largest shift among them
   int sbad;
 };
 template <typename T>
@@ -1824,20 +1838,22 @@ __device__ __forceinline__ void fw_flow_col_body(FlowColShared<typename unsigned
   constexpr U TOP = (U)1 << (sizeof(U) * 8 - 1);
   constexpr U ALL = ~(U)0;
   auto canon = [](U x) -> U { return (T)x < 0 ? ALL : x; };
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, h = tid & 3, row = tid >> 2;
   const int64_t k0 = (int64_t)K * BLK;
-  const int64_t r = k0 + BLK + (int64_t)g * FLOW_COL_ROWS + tid;
+  const int64_t r = k0 + BLK + (int64_t)g * FLOW_COL_ROWS + row;
   const bool live = r < n;
-  U c[BLK];
   U* Wu = (U*)W;
-  if (live) {
+  {
+    // thread (row, h) loads cells 8 h .. 8 h + 7 of its row
+    U v[8];
 #pragma unroll
-    for (int h = 0; h < BLK / 4; h++) {
-      U q[4];
-      ld4(Wu + r * ld + k0 + 4 * h, q);
-#pragma unroll
-      for (int x = 0; x < 4; x++) c[4 * h + x] = canon(q[x]);
+    for (int q = 0; q < 8; q++) v[q] = ALL;
+    if (live) {
+      ld4(Wu + r * ld + k0 + 8 * h, *reinterpret_cast<U(*)[4]>(&v[0]));
+      ld4(Wu + r * ld + k0 + 8 * h + 4, *reinterpret_cast<U(*)[4]>(&v[4]));
     }
+#pragma unroll
+    for (int q = 0; q < 8; q++) L.C0[row][8 * h + q] = canon(v[q]);
   }
   const U* drow = (const U*)ws.dsnap + (int64_t)K * 2 * BLK * BLK + BLK * BLK;
   for (int e = tid; e < BLK * BLK; e += NT) {
@@ -1847,7 +1863,6 @@ __device__ __forceinline__ void fw_flow_col_body(FlowColShared<typename unsigned
     L.M[e / BLK][e % BLK] = (U)m;
     L.Y[e / BLK][e % BLK] = abs ? TOP : b ^ ((U)1 << m);
   }
-  if (tid < BLK) L.G[tid] = 0;
   if (tid == 0) L.sbad = (st->flags & (RBG_F_NEED_GENERAL | RBG_F_PRECOND_FAIL)) ? 1 : 0;
   __syncthreads();
   if (tid < BLK) {
@@ -1862,27 +1877,28 @@ __device__ __forceinline__ void fw_flow_col_body(FlowColShared<typename unsigned
   int ga = 0;
   if (live) {
     T* colT = ws.col<T>(ws.slot(K));
+    U c[BLK / 4];  // cells h + 4 m
 #pragma unroll
-    for (int kk = 0; kk < BLK; kk++) st_dev(&colT[kk * n + r], (T)c[kk]);
-#pragma unroll
+    for (int m = 0; m < BLK / 4; m++) {
+      c[m] = L.C0[row][h + 4 * m];
+      st_dev(&colT[(h + 4 * m) * n + r], (T)c[m]);
+    }
     for (int kk = 1; kk < BLK; kk++) {
-      const U a = c[kk];  // as the closure found it: no earlier step of the block changed it
+      const U a = L.C0[row][kk];  // as the closure found it: no earlier step of the block changed it
       const int am = (T)a > 1 ? msb_u(a) : 0;
       ga = am + L.G[kk] > ga ? am + L.G[kk] : ga;
 #pragma unroll
-      for (int x = 0; x < kk; x++) {
-        const U p = (a << L.M[kk][x]) | L.Y[kk][x];
-        c[x] = c[x] < p ? c[x] : p;
-        dm = dm > (T)c[x] ? dm : (T)c[x];
+      for (int m = 0; m < BLK / 4; m++) {
+        const int x = h + 4 * m;
+        if (x < kk) {
+          const U p = (a << L.M[kk][x]) | L.Y[kk][x];
+          c[m] = c[m] < p ? c[m] : p;
+          dm = dm > (T)c[m] ? dm : (T)c[m];
+        }
       }
     }
 #pragma unroll
-    for (int h = 0; h < BLK / 4; h++) {
-      U q[4];
-#pragma unroll
-      for (int x = 0; x < 4; x++) q[x] = c[4 * h + x];
-      st4_dev(Wu + r * ld + k0 + 4 * h, q);
-    }
+    for (int m = 0; m < BLK / 4; m++) st_dev(&Wu[r * ld + k0 + h + 4 * m], c[m]);
   }
   if (ga > width - 2) atomicOr(&st->flags, (uint32_t)RBG_F_NEED_GENERAL);
   const int64_t wm = wave_max_i64((int64_t)dm);
@@ -2049,10 +2065,11 @@ __global__ __launch_bounds__(NT, PER_CU) void k_fw_flow(T* W, int64_t n, int64_t
     // the panels are the closure's critical path: their waves take issue
     // priority over the tile waves that share their SIMDs
     __builtin_amdgcn_s_setprio(3);
-    if (a < K)
-      fw_flow_row_body<T>(L.r, W, n, ld, K, a, ws, width, st);
+    int J = 0, g = 0;
+    if (f.is_row(K, a, J, g))
+      fw_flow_row_body<T>(L.r, W, n, ld, K, J, ws, width, st);
     else
-      fw_flow_col_body<T>(L.c, W, n, ld, K, a - K, ws, width, st);
+      fw_flow_col_body<T>(L.c, W, n, ld, K, g, ws, width, st);
   } else {
     fw_tile_fast_body<T, RC, RC, false, SHIFT_BYTES>(L.t, W, n, ld, K, ws, width, st, a, b, 2);
   }

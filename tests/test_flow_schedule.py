@@ -46,7 +46,19 @@ def _layout(n, elem):
 
 
 def _ncol(nb, K):
-    return (nb - 1 - K + 7) // 8
+    return (nb - K) // 2
+
+
+def _panel_item(nb, K, idx):
+    """Item idx of P(K): ("c", g) for the column panel's rows from block
+    K + 1 + 2 g (two blocks each), ("r", J) for row panel (K, J).  Order:
+    C(K, 0), R(K, 0 .. K - 1), C(K, 1 ..)."""
+    c = 1 if _ncol(nb, K) > 0 else 0
+    if c and idx == 0:
+        return "c", 0
+    if idx < K + c:
+        return "r", idx - c
+    return "c", idx - K
 
 
 def _all_items(n, elem):
@@ -108,16 +120,15 @@ def test_flow_tickets_cover_items_and_wait_backwards(n, elem):
 def _blocks(item, nb, edge):
     """(writes, reads, loose reads) of an item as sets of 32 x 32 blocks
     (row block, column block); loose reads are read but neither relaxed nor
-    stored.  Item idx < K of P(K) is row panel (K, idx); the others are the
-    column panel's rows from block K + 1 + 8 (idx - K), eight blocks each.
-    (The diagonal blocks are closed before the launch.)"""
+    stored.  (The diagonal blocks are closed before the launch.)"""
     role, K, a, b = item
     if role == 0:
-        if a < K:
-            blk = {(K, a)}
+        kind, v = _panel_item(nb, K, a)
+        if kind == "r":
+            blk = {(K, v)}
         else:
-            s0 = K + 1 + 8 * (a - K)
-            blk = {(s, K) for s in range(s0, min(s0 + 8, nb))}
+            s0 = K + 1 + 2 * v
+            blk = {(s, K) for s in range(s0, min(s0 + 2, nb))}
         return blk, set(blk), set()
     per = edge // BLK
     rows = [r for r in range(a * per, a * per + per) if r not in (K, K + 1)]
@@ -135,11 +146,12 @@ def _snap_blocks(item, nb, edge):
     panels of P(K + 1)."""
     role, K, a, b = item
     if role == 0:
-        if a < K:  # row panel (K, a): its strip reads step K - 1's
-            r = {(K - 1, "c", K)} | ({(K - 1, "r", a)} if a < K - 1 else set())
-            return {(K, "r", a)}, r
-        s0 = K + 1 + 8 * (a - K)
-        return {(K, "c", s) for s in range(s0, min(s0 + 8, nb))}, set()
+        kind, v = _panel_item(nb, K, a)
+        if kind == "r":  # row panel (K, v): its strip reads step K - 1's
+            r = {(K - 1, "c", K)} | ({(K - 1, "r", v)} if v < K - 1 else set())
+            return {(K, "r", v)}, r
+        s0 = K + 1 + 2 * v
+        return {(K, "c", s) for s in range(s0, min(s0 + 2, nb))}, set()
     per = edge // BLK
     r = {(K, "c", s) for s in range(a * per, a * per + per) if s > K}
     r |= {(K, "r", s) for s in range(b * per, b * per + per) if s < K}
