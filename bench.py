@@ -10,12 +10,11 @@ copy, fast-path precheck, closure, status read-back).
 Prints one JSON line (the bench contract) with:
   value          closure entries/s = N^2 / seconds per closure
   roofline       the dominant kernel (for the lower triangular N=8192 input
-                 the fused look-ahead step k_fw_step_fast<int>: pivot block
-                 K's phase-3 tiles and block K+1's panel in one launch; for
-                 the N=4096 int64 one the persistent closure k_fw_flow<long>,
-                 every pivot step in one launch): integer VALU ops it must
-                 execute (2.5 per relaxation) over its average launch
-                 duration, measured with hipEvents on its launch stream
+                 the persistent closure k_fw_flow<int>: every pivot step's
+                 panels and phase-3 tiles in one launch, work items waiting
+                 on each other): integer VALU ops it must execute (2.5 per
+                 relaxation) over its average launch duration, measured with
+                 hipEvents on its launch stream
                  (recorded on every --prof-every-th timed step, 4 by default,
                  so that the events' own stalls stay out of most steps)
   cpu_baseline   the oracle's single-core naive loop (a C restatement of the

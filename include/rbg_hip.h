@@ -106,13 +106,13 @@ uint64_t rbg_host_avos_product_u64(int64_t a, int64_t b, uint32_t *events);
 int64_t rbg_fw_workspace_bytes(int64_t n, int32_t elem_bytes);
 /* The workspace for rbg_fw_dense_tri_* (no reference counterpart; the
  * triangular schedule's size query): with it a lower triangular fast closure
- * runs the look-ahead schedule (int32) or the persistent closure (int64, one
- * launch); with only rbg_fw_workspace_bytes, one plain triangular launch pair
- * per pivot step. */
+ * runs as one persistent launch (tri = 3: the look-ahead schedule, one launch
+ * per pivot step); with only rbg_fw_workspace_bytes, one plain triangular
+ * launch pair per pivot step. */
 int64_t rbg_fw_tri_workspace_bytes(int64_t n, int32_t elem_bytes);
 /* No counterpart in the reference (schedule introspection, host only, for
  * tests): the persistent lower triangular closure's work items of order n in
- * ticket order (elem_bytes = 8: the int64 closure), four int32 per item --
+ * ticket order (elem_bytes 4 or 8: the int32 or int64 closure), four int32 per item --
  * (0, K, idx, 0) for item idx of pivot block K's panel -- with c = 1 when
  * block K has column-panel rows (K < n / 32 - 1), else 0: idx = 0 < c the
  * column panel's first 64 rows (from 32 (K + 1)), c <= idx < K + c row panel
@@ -126,8 +126,8 @@ int64_t rbg_fw_flow_schedule(int64_t n, int32_t elem_bytes, int32_t *items, int6
  * The block: [4 + K] finished items of pivot block K's panel, [4 + nb + K]
  * finished tiles of step K, [4 + 2 nb + I nt + J] 1 + the last step tile
  * (I, J) finished, [4 + 2 nb + nt^2 + K nb + idx] panel item idx of block K
- * finished (nb = n / 32, nt = n / 64).  Returns the item count; written when
- * cap >= 20 * count. */
+ * finished (nb = n / 32; nt = n / tile edge: 128 for int32 above n = 2048,
+ * else 64).  Returns the item count; written when cap >= 20 * count. */
 int64_t rbg_fw_flow_waits(int64_t n, int32_t elem_bytes, int64_t *waits, int64_t cap);
 /* Profiling (no reference counterpart; used by bench.py): while enabled, every
  * panel and tile launch of rbg_fw_dense_* is bracketed by hipEvents on its
@@ -165,7 +165,7 @@ int rbg_fw_dense_i64(int64_t *W, int64_t n, int64_t ld, int32_t width, int32_t i
  * reference's loop, which reads only the assumed triangle, an entry on the
  * other side is detected: the fast run stops with RBG_F_NEED_GENERAL, and the
  * caller reruns mode 1 on the original input, as for any fast-mode miss.
- * For int64, tri = 1 with a workspace of rbg_fw_tri_workspace_bytes runs every
+ * tri = 1 with a workspace of rbg_fw_tri_workspace_bytes runs every
  * pivot step in one persistent launch whose work items wait for each other
  * under a time limit; should a wait run out (never expected), st->flags gets
  * RBG_F_STALLED | RBG_F_NEED_GENERAL and the caller reruns on the original

@@ -327,10 +327,37 @@ template <typename U>
 __device__ __forceinline__ void st_dev(U* p, U v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-template <typename U>
-__device__ __forceinline__ void st4_dev(U* p, const U (&v)[4]) {
+// Four cells with 8-byte device-scope stores (a 4-byte store per cell costs
+// the persistent int32 closure a third of its time); the caller drains them
+// with wait_vm() before publishing.
+__device__ __forceinline__ void st4_dev(uint32_t* p, const uint32_t (&v)[4]) {
+  st_dev((uint64_t*)p, (uint64_t)v[0] | ((uint64_t)v[1] << 32));
+  st_dev((uint64_t*)p + 1, (uint64_t)v[2] | ((uint64_t)v[3] << 32));
+}
+__device__ __forceinline__ void st4_dev(uint64_t* p, const uint64_t (&v)[4]) {
 #pragma unroll
   for (int x = 0; x < 4; x++) st_dev(p + x, v[x]);
+}
+// A raw buffer resource over [base, base + 2 GiB) for the stores below.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, 0x7fffffff, 0x00020000);
+}
+// Four cells at byte offset `off` of a buffer resource with one or two
+// 16-byte device-scope stores (cache policy sc1, the bit device-scope atomic
+// stores carry): written through the XCD's L2; the caller drains them with
+// wait_vm() before publishing.  (The compiler sees these stores, and keeps
+// their data registers until the hardware has read them.)
+__device__ __forceinline__ void st4_buf(__amdgpu_buffer_rsrc_t rs, int off, const uint32_t (&v)[4]) {
+  typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+  const v4u d = {v[0], v[1], v[2], v[3]};
+  __builtin_amdgcn_raw_buffer_store_b128(d, rs, off, 0, 16);
+}
+__device__ __forceinline__ void st4_buf(__amdgpu_buffer_rsrc_t rs, int off, const uint64_t (&v)[4]) {
+  typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+  const v4u d0 = {(uint32_t)v[0], (uint32_t)(v[0] >> 32), (uint32_t)v[1], (uint32_t)(v[1] >> 32)};
+  const v4u d1 = {(uint32_t)v[2], (uint32_t)(v[2] >> 32), (uint32_t)v[3], (uint32_t)(v[3] >> 32)};
+  __builtin_amdgcn_raw_buffer_store_b128(d0, rs, off, 0, 16);
+  __builtin_amdgcn_raw_buffer_store_b128(d1, rs, off + 16, 0, 16);
 }
 // Wait until this wave's memory operations are complete.
 __device__ inline void wait_vm() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
@@ -823,7 +850,9 @@ struct TileShared {
   int sGa[BLK], sGb[BLK];
 };
 
-template <typename T, int RPT, int CPT, bool PANEL_GUARD, bool SHIFT_BYTES>
+// DEV: device-coherent stores (the persistent closure, whose next reader of
+// these cells may run on another XCD in the same launch).
+template <typename T, int RPT, int CPT, bool PANEL_GUARD, bool SHIFT_BYTES, bool DEV = false>
 __device__ __forceinline__ void fw_tile_fast_body(TileShared<typename unsigned_of<T>::type, 16 * CPT, SHIFT_BYTES>& L, T* W,
                                          int64_t n, int64_t ld, int64_t K, FwWork ws, int width, rbg_status* st,
                                          int I, int J, int ahead = 0) {
@@ -1041,6 +1070,9 @@ __device__ __forceinline__ void fw_tile_fast_body(TileShared<typename unsigned_o
       }
     }
   }
+  // device-coherent stores: 16-byte buffer stores through a resource based
+  // at the tile's first cell (offsets within a tile fit 32 bits at any order)
+  const __amdgpu_buffer_rsrc_t rs = buf_rsrc(Wu + (int64_t)I * TL * ld + (int64_t)J * TL);
 #pragma unroll
   for (int h = 0; h < NCH; h++) {
     const int64_t c = cb + 64 * h;
@@ -1053,8 +1085,8 @@ __device__ __forceinline__ void fw_tile_fast_body(TileShared<typename unsigned_o
       U q[4];
 #pragma unroll
       for (int x = 0; x < 4; x++) q[x] = w[r][4 * h + x];
-      if (ahead == 2)  // the persistent closure: read by the tile's next step, maybe on another XCD
-        st4_dev(Wu + (r0 + r) * ld + c, q);
+      if constexpr (DEV)
+        st4_buf(rs, (int)(((r0 + r - (int64_t)I * TL) * ld + c - (int64_t)J * TL) * sizeof(U)), q);
       else
         st4(Wu + (r0 + r) * ld + c, q);
     }
@@ -1831,13 +1863,19 @@ struct FlowColShared {
   int G[BLK];                    // per kk, the largest shift among them
   int sbad;
 };
-template <typename T>
+template <typename T, bool DEV = true>
 __device__ __forceinline__ void fw_flow_col_body(FlowColShared<typename unsigned_of<T>::type>& L, T* W, int64_t n,
                                                  int64_t ld, int K, int g, FwWork ws, int width, rbg_status* st) {
   using U = typename unsigned_of<T>::type;
   constexpr U TOP = (U)1 << (sizeof(U) * 8 - 1);
   constexpr U ALL = ~(U)0;
   auto canon = [](U x) -> U { return (T)x < 0 ? ALL : x; };
+  auto put = [](T* at, T v) {
+    if constexpr (DEV)
+      st_dev(at, v);
+    else
+      *at = v;
+  };
   const int tid = threadIdx.x, h = tid & 3, row = tid >> 2;
   const int64_t k0 = (int64_t)K * BLK;
   const int64_t r = k0 + BLK + (int64_t)g * FLOW_COL_ROWS + row;
@@ -1881,7 +1919,7 @@ __device__ __forceinline__ void fw_flow_col_body(FlowColShared<typename unsigned
 #pragma unroll
     for (int m = 0; m < BLK / 4; m++) {
       c[m] = L.C0[row][h + 4 * m];
-      st_dev(&colT[(h + 4 * m) * n + r], (T)c[m]);
+      put(&colT[(h + 4 * m) * n + r], (T)c[m]);
     }
     for (int kk = 1; kk < BLK; kk++) {
       const U a = L.C0[row][kk];  // as the closure found it: no earlier step of the block changed it
@@ -1898,7 +1936,7 @@ __device__ __forceinline__ void fw_flow_col_body(FlowColShared<typename unsigned
       }
     }
 #pragma unroll
-    for (int m = 0; m < BLK / 4; m++) st_dev(&Wu[r * ld + k0 + h + 4 * m], c[m]);
+    for (int m = 0; m < BLK / 4; m++) put((T*)&Wu[r * ld + k0 + h + 4 * m], (T)c[m]);
   }
   if (ga > width - 2) atomicOr(&st->flags, (uint32_t)RBG_F_NEED_GENERAL);
   const int64_t wm = wave_max_i64((int64_t)dm);
@@ -1927,7 +1965,7 @@ struct FlowRowShared {
   int64_t sMax[NT / 64];
   int sbad;
 };
-template <typename T>
+template <typename T, bool DEV = true>
 __device__ __forceinline__ void fw_flow_row_body(FlowRowShared<typename unsigned_of<T>::type>& L, T* W, int64_t n,
                                                  int64_t ld, int K, int J, FwWork ws, int width, rbg_status* st) {
   using U = typename unsigned_of<T>::type;
@@ -1935,6 +1973,12 @@ __device__ __forceinline__ void fw_flow_row_body(FlowRowShared<typename unsigned
   constexpr U ALL = ~(U)0;
   auto canon = [](U x) -> U { return (T)x < 0 ? ALL : x; };
   auto amsb = [](U a) -> int { return (T)a > 1 ? msb_u(a) : 0; };
+  auto put = [](T* at, T v) {
+    if constexpr (DEV)
+      st_dev(at, v);
+    else
+      *at = v;
+  };
   const int tid = threadIdx.x;
   const int64_t k0 = (int64_t)K * BLK, j0 = (int64_t)J * BLK;
   constexpr int PER = BLK * BLK / NT;  // 4
@@ -2005,7 +2049,7 @@ __device__ __forceinline__ void fw_flow_row_body(FlowRowShared<typename unsigned
 #pragma unroll
   for (int kk = 0; kk < BLK; kk++) {
     const U b = canon((U)__shfl((T)r[kk >> 3], base + (kk & 7), 64));
-    if (i4 == (kk & 7)) st_dev(&rowS[kk * n + j0 + j], (T)b);
+    if (i4 == (kk & 7)) put(&rowS[kk * n + j0 + j], (T)b);
     const bool abs = (T)b < 0;
     const U m = abs ? 0 : (U)msb_u(b);
     const U y = abs ? TOP : b ^ ((U)1 << m);
@@ -2023,7 +2067,7 @@ __device__ __forceinline__ void fw_flow_row_body(FlowRowShared<typename unsigned
   }
   if (ga > lim) atomicOr(&st->flags, (uint32_t)RBG_F_NEED_GENERAL);
 #pragma unroll
-  for (int q = 0; q < 4; q++) st_dev((U*)&W[(k0 + i4 + 8 * q) * ld + j0 + j], r[q]);
+  for (int q = 0; q < 4; q++) put(&W[(k0 + i4 + 8 * q) * ld + j0 + j], (T)r[q]);
   const int64_t wm = wave_max_i64((int64_t)dm);
   if ((threadIdx.x & 63) == 0 && wm > seen) record_diameter(st, wm);
 }
@@ -2071,7 +2115,7 @@ __global__ __launch_bounds__(NT, PER_CU) void k_fw_flow(T* W, int64_t n, int64_t
     else
       fw_flow_col_body<T>(L.c, W, n, ld, K, g, ws, width, st);
   } else {
-    fw_tile_fast_body<T, RC, RC, false, SHIFT_BYTES>(L.t, W, n, ld, K, ws, width, st, a, b, 2);
+    fw_tile_fast_body<T, RC, RC, false, SHIFT_BYTES, true>(L.t, W, n, ld, K, ws, width, st, a, b, 2);
   }
   // publication: the item's cells and snapshots went out with device-coherent
   // stores; once every wave's stores are complete, the counters may say so

@@ -9,14 +9,12 @@ namespace rbg {
 // times the workgroups for the steps that touch few tiles.
 constexpr int TRI_SMALL_BELOW = 256;
 
-// Lower triangular closures take the look-ahead schedule (k_fw_step_fast):
-// one launch per pivot block, the next block's panel inside it; int64 ones,
-// with a workspace of rbg_fw_tri_workspace_bytes, the persistent closure
-// (k_fw_flow): one launch for all pivot blocks (N = 4096: 5.84 against
-// 6.40 ms per closure on one box).  The int32 closure keeps one launch per
-// step: its 128-tiles take about 27 us, and the persistent launch's waits
-// and in-order dispatch cost more than the per-launch tails they remove
-// (N = 8192: 15.0 against 12.6 ms; with 64-tiles 17.9 ms).
+// Lower triangular closures with a workspace of rbg_fw_tri_workspace_bytes
+// take the persistent closure (k_fw_flow): one launch for all pivot blocks
+// (one box: N = 8192 int32 10.87 ms, N = 4096 int64 4.34 ms per closure);
+// with a smaller workspace, or tri = 3, the look-ahead schedule
+// (k_fw_step_fast): one launch per pivot block, the next block's panel
+// inside it (12.56 and 6.36 ms).
 constexpr bool LOOKAHEAD = true;
 constexpr bool FLOW = true;
 
@@ -35,10 +33,10 @@ static int64_t flow_ws_bytes(int64_t n, int elem) {
   return flow_ctl_offset(n, elem) + FlowCtl::words((int)(n / BLK), (int)(n / 64)) * 4;
 }
 
-// The persistent closure's schedule for order n (int64, 64-tiles); counters
-// in `wsp`.
+// The persistent closure's schedule for order n: int32 takes 128-tiles
+// (64-tiles up to SMALL_TILE_MAX_N), int64 64-tiles; counters in `wsp`.
 static FlowCtl flow_ctl(int64_t n, int elem, void* wsp) {
-  const int edge = 64;
+  const int edge = (elem == 4 && n > SMALL_TILE_MAX_N) ? 128 : 64;
   FlowCtl f{wsp ? (int*)((char*)wsp + flow_ctl_offset(n, elem)) : nullptr, (int)(n / BLK), (int)(n / edge), edge, 0};
   f.total = f.count();
   return f;
@@ -73,7 +71,7 @@ static int fw_dense(T* W, int64_t n, int64_t ld, int width, int is_uns, int mode
   // panel share a launch; only block 0's panel runs on its own.  With room
   // for it, the persistent closure instead.
   const bool la = fast && tri == 1 && LOOKAHEAD && ws_bytes >= la_ws_bytes(n, elem);
-  const bool flow = la && FLOW && sizeof(T) == 8 && !per_step && ws_bytes >= flow_ws_bytes(n, elem);
+  const bool flow = la && FLOW && !per_step && ws_bytes >= flow_ws_bytes(n, elem);
   FwWork ws{wsp, n, elem, flow ? 4 : 2};
   const int nb = (int)(n / BLK);
   const int nt = (int)(n / TILE);
@@ -187,6 +185,10 @@ static int fw_dense(T* W, int64_t n, int64_t ld, int width, int is_uns, int mode
     int p = PROF ? prof_start(s) : -1;
     if constexpr (sizeof(T) == 8)
       hipLaunchKernelGGL((k_fw_flow<T, 4, true, 3>), grid, dim3(NT), 0, s, W, n, ld, ws, width, st, f, limit);
+    else if (edge == 128)
+      hipLaunchKernelGGL((k_fw_flow<T, 8, false, 3>), grid, dim3(NT), 0, s, W, n, ld, ws, width, st, f, limit);
+    else
+      hipLaunchKernelGGL((k_fw_flow<T, 4, false, 4>), grid, dim3(NT), 0, s, W, n, ld, ws, width, st, f, limit);
     RBG_CHECK_LAUNCH();
     if (PROF) prof_stop(6, p, s, cells * BLK);
   } else {
@@ -280,7 +282,7 @@ int64_t rbg_fw_workspace_bytes(int64_t n, int32_t elem_bytes) { return FwWork::b
 int64_t rbg_fw_tri_workspace_bytes(int64_t n, int32_t elem_bytes) { return flow_ws_bytes(n, elem_bytes); }
 
 int64_t rbg_fw_flow_schedule(int64_t n, int32_t elem_bytes, int32_t* items, int64_t cap) {
-  if (n <= 0 || n % TILE || elem_bytes != 8) return RBG_E_ARG;
+  if (n <= 0 || n % TILE || (elem_bytes != 4 && elem_bytes != 8)) return RBG_E_ARG;
   const FlowCtl f = flow_ctl(n, elem_bytes, nullptr);
   if (items && cap >= 4 * f.total) {
     for (int64_t x = 0; x < f.total; x++) {
@@ -295,7 +297,7 @@ int64_t rbg_fw_flow_schedule(int64_t n, int32_t elem_bytes, int32_t* items, int6
 }
 
 int64_t rbg_fw_flow_waits(int64_t n, int32_t elem_bytes, int64_t* waits, int64_t cap) {
-  if (n <= 0 || n % TILE || elem_bytes != 8) return RBG_E_ARG;
+  if (n <= 0 || n % TILE || (elem_bytes != 4 && elem_bytes != 8)) return RBG_E_ARG;
   const FlowCtl f = flow_ctl(n, elem_bytes, nullptr);
   if (waits && cap >= 2 * FLOW_MAXW * f.total) {
     for (int64_t x = 0; x < f.total; x++) {

@@ -43,7 +43,7 @@ class FwOutcome:
     npad: int
     mode: str  # "fast" | "general"
     # "full" | "lower" | "upper" (triangular schedules; "lower" is the
-    # persistent closure for int64) | "lower-steps" (one launch per pivot step)
+    # persistent closure) | "lower-steps" (one launch per pivot step)
     schedule: str = "full"
 
     @property
@@ -86,8 +86,8 @@ def fw_closure(W, et: ElemType, stream=None, force_general: bool = False, schedu
     schedule when the precheck finds every off-diagonal entry on one side of
     the diagonal (a pedigree numbered parents first is lower triangular), else
     the full one; "full" forces the full schedule (the results are the same).
-    An int64 lower triangular closure runs as one persistent launch; "steps"
-    keeps one launch per pivot step there (the same results again).
+    A lower triangular closure runs as one persistent launch; "steps" keeps
+    one launch per pivot step there (the same results again).
     """
     if schedule not in ("auto", "full", "steps"):
         raise ValueError(f"unknown schedule {schedule!r}")

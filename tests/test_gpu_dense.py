@@ -233,14 +233,14 @@ def test_fw_triangular_schedule_vs_oracle(orc, dt, n, reverse):
     assert out2.diameter == ref.diameter
 
 
-@pytest.mark.parametrize("dt,n", [(np.int64, 128), (np.int64, 300), (np.int64, 900), (np.int64, 2000),
-                                  (np.int64, 3000), (np.int32, 700)])
+@pytest.mark.parametrize("dt,n", [(np.int32, 128), (np.int32, 700), (np.int32, 2100), (np.int32, 3000),
+                                  (np.int64, 128), (np.int64, 300), (np.int64, 900), (np.int64, 2000),
+                                  (np.int64, 3000)])
 def test_fw_persistent_closure_vs_steps_and_oracle(orc, dt, n):
-    """The persistent lower triangular closure (int64: one launch, work items
-    waiting on each other) against the same schedule launched step by step and
-    the oracle: W and the trajectory diameter, three runs each (the items'
-    order on the GPU differs from run to run), no wait running out.  (int32
-    lower triangular closures launch step by step either way.)"""
+    """The persistent lower triangular closure (one launch, work items waiting
+    on each other) against the same schedule launched step by step and the
+    oracle: W and the trajectory diameter, three runs each (the items' order
+    on the GPU differs from run to run), no wait running out."""
     A = synth.dense_pedigree(synth.component_spec(n, 0.8, 4), 29 + n, dtype=dt)
     ref = orc.warshall(A) if n <= 1100 else None
     Ws, outs = dense.fw_closure(_lib.to_device(A), elem_type(A.dtype), schedule="steps")
