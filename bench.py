@@ -464,12 +464,17 @@ def bench_batched(args):
     tile_s = tile_ms / 1e3
     achieved = relax_exec * OPS_PER_RELAX["int32"] / tile_s
     entries = float(np.sum(sizes.astype(np.float64) ** 2))
+    persistent = info["persistent_components"] == len(sizes)
     roof = {
-        "bound": "valu", "kernel": "k_bfw_tile<8>/<4> (batched phase 3)",
+        "bound": "valu",
+        "kernel": ("k_bfw_flow (every component's panel items and tiles in one persistent launch)" if persistent
+                   else "k_bfw_tile<8>/<4> (batched phase 3)"),
         "achieved": achieved / 1e12, "peak": PEAK_VALU_OPS / 1e12, "unit": "TOP/s",
-        "frac": achieved / PEAK_VALU_OPS, "traffic": _read_traffic("bfw_tile_batched4096_pmc.txt"),
+        "frac": achieved / PEAK_VALU_OPS,
+        "traffic": _read_traffic("bfw_flow_batched4096_pmc.txt" if persistent else "bfw_tile_batched4096_pmc.txt"),
         "ops_per_relaxation": OPS_PER_RELAX["int32"], "relaxations_executed": relax_exec,
-        "schedule": "lower triangular (the tiles each pivot block can change)",
+        "schedule": ("lower triangular, persistent: every component's work items in one grid" if persistent
+                     else "lower triangular (the tiles each pivot block can change)"),
         "tile_ms_per_batch": tile_ms, "tile_launches": tile_launches,
         "kernel_share_of_step": tile_ms / ms,
     }
@@ -492,13 +497,57 @@ def bench_batched(args):
                    "batched_vs_single_matrix_path": single_ms / ms if single_ms else None,
                    "identical_to_single_matrix_path": identical_single,
                    "launches_per_batch": info["launches"], "pivot_steps": info["steps"],
+                   "persistent_items": info["persistent_items"],
                    "tiles": {str(k): v for k, v in sorted(b.tiles.items())}, "autotune_s": tune_s},
         "roofline": roof,
     }
+    res["config"]["persistent"] = _batched_persistent_leg(args, sizes, pristine, eager_result, b.stream)
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline_batched(sizes, diag, edges)
     b.close()
     return res
+
+
+def _batched_persistent_leg(args, sizes, pristine, eager_result, stream):
+    """The same batch closed by the persistent launch (every component's work
+    items in one grid): ms per graph replay, the launch's share and VALU
+    fraction, and whether the pool equals the per-step launches' result."""
+    from redblackgraph_amd import _lib
+    from redblackgraph_amd.batched import BatchedClosure
+    t = _lib.torch()
+    L = _lib.load()
+    p = BatchedClosure(sizes, tri=1, persistent=True, stream=stream)
+    try:
+        info = p.info()
+        with t.cuda.stream(p.stream):
+            p.pool.copy_(pristine)
+        L.rbg_prof_enable(1)
+        p.run(graph=False)
+        p.stream.synchronize()
+        launch_ms, _, relax = _prof_read(4)
+        L.rbg_prof_enable(0)
+        flags = sorted(set(int(s.flags) for s in p.statuses()))
+        with t.cuda.stream(p.stream):
+            for _ in range(args.warmup):
+                p.pool.copy_(pristine)
+                p.run(graph=True)
+        p.stream.synchronize()
+        t0 = time.perf_counter()
+        with t.cuda.stream(p.stream):
+            for _ in range(args.steps):
+                p.pool.copy_(pristine)
+                p.run(graph=True)
+        p.stream.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        identical = bool(t.equal(p.pool, eager_result))
+        achieved = relax * OPS_PER_RELAX["int32"] / (launch_ms / 1e3)
+        return {"ms_per_step": ms, "launch_ms": launch_ms, "frac": achieved / PEAK_VALU_OPS,
+                "relaxations_executed": relax, "work_items": info["persistent_items"],
+                "launches_per_batch": info["launches"], "status_flags": flags,
+                "identical_to_per_step_launches": identical,
+                "kernel": "k_bfw_flow (every component's panel items and tiles in one persistent launch)"}
+    finally:
+        p.close()
 
 
 def bench_serve(args):

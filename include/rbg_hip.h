@@ -110,25 +110,34 @@ int64_t rbg_fw_workspace_bytes(int64_t n, int32_t elem_bytes);
  * per pivot step); with only rbg_fw_workspace_bytes, one plain triangular
  * launch pair per pivot step. */
 int64_t rbg_fw_tri_workspace_bytes(int64_t n, int32_t elem_bytes);
+/* No counterpart in the reference (test hook): the longest wait of one
+ * persistent work item, in device wall-clock ticks, for the closures
+ * launched after the call (single-matrix and batched); 0 restores the
+ * default of two seconds.  A wait that runs out ends every workgroup of that
+ * closure and flags it RBG_F_STALLED | RBG_F_NEED_GENERAL.  Returns the
+ * previous override. */
+int64_t rbg_fw_flow_wait_limit(int64_t ticks);
 /* No counterpart in the reference (schedule introspection, host only, for
  * tests): the persistent lower triangular closure's work items of order n in
- * ticket order (elem_bytes 4 or 8: the int32 or int64 closure), four int32 per item --
+ * ticket order (elem_bytes 4 or 8: the int32 or int64 closure; edge the tile
+ * edge, 64 or 128, or 0 for the single-matrix closure's choice: 128 for int32
+ * above n = 2048, else 64), four int32 per item --
  * (0, K, idx, 0) for item idx of pivot block K's panel -- with c = 1 when
  * block K has column-panel rows (K < n / 32 - 1), else 0: idx = 0 < c the
  * column panel's first 64 rows (from 32 (K + 1)), c <= idx < K + c row panel
  * (K, idx - c), idx >= K + c the column panel's rows from 32 (K + 1) +
  * 64 (idx - K) -- and (1, K, I, J) for tile (I, J) of pivot step K.  Returns the item count;
  * items are written when cap >= 4 * count. */
-int64_t rbg_fw_flow_schedule(int64_t n, int32_t elem_bytes, int32_t *items, int64_t cap);
+int64_t rbg_fw_flow_schedule(int64_t n, int32_t elem_bytes, int32_t edge, int32_t *items, int64_t cap);
 /* No counterpart in the reference (host only, for tests): each item's waits,
  * ten (word, target) pairs per item in ticket order -- "counter word `word`
  * of the closure's counter block >= target", word = -1 for an unused slot.
  * The block: [4 + K] finished items of pivot block K's panel, [4 + nb + K]
  * finished tiles of step K, [4 + 2 nb + I nt + J] 1 + the last step tile
  * (I, J) finished, [4 + 2 nb + nt^2 + K nb + idx] panel item idx of block K
- * finished (nb = n / 32; nt = n / tile edge: 128 for int32 above n = 2048,
- * else 64).  Returns the item count; written when cap >= 20 * count. */
-int64_t rbg_fw_flow_waits(int64_t n, int32_t elem_bytes, int64_t *waits, int64_t cap);
+ * finished (nb = n / 32; nt = n / tile edge, edge as for the schedule).
+ * Returns the item count; written when cap >= 20 * count. */
+int64_t rbg_fw_flow_waits(int64_t n, int32_t elem_bytes, int32_t edge, int64_t *waits, int64_t cap);
 /* Profiling (no reference counterpart; used by bench.py): while enabled, every
  * panel and tile launch of rbg_fw_dense_* is bracketed by hipEvents on its
  * stream.  rbg_fw_prof_read synchronises those events and returns the summed
@@ -395,9 +404,16 @@ int rbg_csr_add_i64(const int64_t *ap, const int32_t *aj, const int64_t *ax, con
  * transitive_closure.py:85-195 component_wise_closure, each through
  * warshall.c.in:50-107).  Every component is an int32 matrix of padded order
  * npad (multiple of 128, rows contiguous, zero padding), closed in place with
- * the fast kernels' arithmetic; `snap` is its snapshot workspace
- * (rbg_batch_snap_bytes(npad)), `st` its status, `tile` the phase-3 tile edge
- * (128 or 64) for its size bucket.  After a run each status holds what the
+ * the fast kernels' arithmetic; `snap` is its snapshot workspace of
+ * `snap_bytes` bytes (at least rbg_batch_snap_bytes(npad)), `st` its status,
+ * `tile` the phase-3 tile edge (128 or 64) for its size bucket.  Lower
+ * triangular components (tri = 1) whose workspace holds
+ * rbg_batch_tri_snap_bytes(npad) are closed together by one persistent launch
+ * (the work items of rbg_fw_dense_tri_*'s persistent closure, every such
+ * component's in one grid); the others by one panel and tile launches per
+ * pivot step.  A wait of the persistent launch that runs out (never seen)
+ * flags its component RBG_F_STALLED | RBG_F_NEED_GENERAL.  After a run each
+ * status holds what the
  * single-matrix fast path would report; RBG_F_PRECOND_FAIL, RBG_F_NEED_GENERAL
  * or a nonzero cycle_count mean the component must be closed again from its
  * input by rbg_fw_dense_i32 in general mode (its matrix is then unspecified).
@@ -405,8 +421,10 @@ int rbg_csr_add_i64(const int64_t *ap, const int32_t *aj, const int64_t *ax, con
  * (device memory owned by the handle); run launches the whole schedule on
  * `stream`; capture records it into a hipGraph owned by the handle (stream
  * must not be the legacy null stream) and replay launches that graph.
- * info: [0] count [1] pivot-block steps [2] launches [3] panel items
- *       [4] 128-tiles [5] 64-tiles [6] 128x128 chunks [7] graph captured. */
+ * info (10 entries): [0] count [1] pivot-block steps [2] launches [3] panel
+ *       items [4] 128-tiles [5] 64-tiles (the per-step launches) [6] 128x128
+ *       chunks [7] graph captured [8] components of the persistent launch
+ *       [9] its work items. */
 typedef struct rbg_batch rbg_batch;
 typedef struct {
   int32_t *W;
@@ -417,8 +435,17 @@ typedef struct {
   int32_t tri; /* 0: full schedule; 1 / 2: every off-diagonal entry lies below / above the diagonal,
                   and the component takes the triangular schedule of rbg_fw_dense_tri_* (a component
                   that is not triangular as declared is flagged RBG_F_PRECOND_FAIL) */
+  int64_t snap_bytes;
 } rbg_batch_comp;
 int64_t rbg_batch_snap_bytes(int32_t npad);
+int64_t rbg_batch_tri_snap_bytes(int32_t npad);
+/* No counterpart in the reference (host only, for tests): the persistent
+ * launch's tickets for components of padded orders npad[] (largest first)
+ * with tile edges edge[] (64 or 128): out[2 x] the component and out[2 x + 1]
+ * the component's own ticket (rbg_fw_flow_schedule's order) of ticket x.
+ * Returns the ticket count; written when cap >= 2 * count. */
+int64_t rbg_batch_flow_tickets(const int32_t *npad, const int32_t *edge, int32_t count, int64_t *out,
+                               int64_t cap);
 int rbg_batch_create(const rbg_batch_comp *comps, int32_t count, rbg_batch **out);
 int rbg_batch_info(const rbg_batch *b, int64_t *info);
 int rbg_batch_run(rbg_batch *b, rbg_stream_t stream);

@@ -54,8 +54,9 @@ SIGNATURES = {
     "rbg_host_avos_product_u64": (U64, [I64, I64, ctypes.POINTER(ctypes.c_uint32)]),
     "rbg_fw_workspace_bytes": (I64, [I64, I32]),
     "rbg_fw_tri_workspace_bytes": (I64, [I64, I32]),
-    "rbg_fw_flow_schedule": (I64, [I64, I32, P, I64]),
-    "rbg_fw_flow_waits": (I64, [I64, I32, P, I64]),
+    "rbg_fw_flow_schedule": (I64, [I64, I32, I32, P, I64]),
+    "rbg_fw_flow_wait_limit": (I64, [I64]),
+    "rbg_fw_flow_waits": (I64, [I64, I32, I32, P, I64]),
     "rbg_fw_prof_enable": (I32, [I32]),
     "rbg_prof_enable": (I32, [I32]),
     "rbg_prof_read": (I32, [I32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]),
@@ -105,6 +106,8 @@ SIGNATURES = {
     "rbg_csr_add_i64": (I32, [P, P, P, P, P, P, I32, I32, P, P, P, P, P, P, P, P]),
     "rbg_spgemm_rows_i64": (I32, [P, P, P, I32, P, P, P, I32, P, P, P, I64, P, P, I64, I32, I32, P, P]),
     "rbg_batch_snap_bytes": (I64, [I32]),
+    "rbg_batch_tri_snap_bytes": (I64, [I32]),
+    "rbg_batch_flow_tickets": (I64, [P, P, I32, P, I64]),
     "rbg_batch_create": (I32, [P, I32, P]),
     "rbg_batch_info": (I32, [P, P]),
     "rbg_batch_run": (I32, [P, P]),
@@ -147,7 +150,8 @@ SIGNATURES = {
 class RbgBatchComp(ctypes.Structure):
     """rbg_batch_comp: one component of a batched dense closure."""
     _fields_ = [("W", ctypes.c_void_p), ("snap", ctypes.c_void_p), ("st", ctypes.c_void_p),
-                ("npad", ctypes.c_int32), ("tile", ctypes.c_int32), ("tri", ctypes.c_int32)]
+                ("npad", ctypes.c_int32), ("tile", ctypes.c_int32), ("tri", ctypes.c_int32),
+                ("snap_bytes", ctypes.c_int64)]
 
 
 class LibraryMissing(ImportError):

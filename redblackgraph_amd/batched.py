@@ -8,7 +8,10 @@ warshall.c.in:50-107).  Here every component is padded to a multiple of 128
 and laid out contiguously in one int32 pool in HBM; the C ABI
 (rbg_batch_create / run / capture / replay) walks all components' pivot
 blocks together, largest components first, with the phase-3 tile shape picked
-per size bucket (see autotune.py).
+per size bucket (see autotune.py).  Lower triangular components (pedigrees
+numbered parents first) are instead closed together by one persistent launch:
+every component's work items in one grid, each waiting only on its own
+component's counters.
 
 Each component's result equals the single-matrix closure
 (dense.fw_closure): W, diameter, and the error behaviour.  Components whose
@@ -37,32 +40,40 @@ def _round(x: int, a: int = _ALIGN) -> int:
 @dataclass
 class BatchLayout:
     """Where each component lives: pool offsets (elements), padded orders,
-    snapshot offsets (bytes)."""
+    snapshot offsets and sizes (bytes)."""
     sizes: np.ndarray
     npad: np.ndarray
     offset: np.ndarray
     snap_offset: np.ndarray
+    snap_size: np.ndarray
     pool_elems: int
     snap_bytes: int
 
     @classmethod
-    def build(cls, sizes: Sequence[int]) -> "BatchLayout":
+    def build(cls, sizes: Sequence[int], persistent=None) -> "BatchLayout":
+        """``persistent``: per component, whether its workspace should hold
+        the persistent closure's (rbg_batch_tri_snap_bytes)."""
         L = _lib.load()
         sizes = np.asarray(sizes, dtype=np.int64)
         npad = np.array([padded_order(int(n)) for n in sizes], dtype=np.int64)
+        pers = np.zeros(len(sizes), dtype=bool) if persistent is None else np.asarray(persistent, dtype=bool)
         off = np.zeros(len(sizes), dtype=np.int64)
         soff = np.zeros(len(sizes), dtype=np.int64)
+        ssz = np.zeros(len(sizes), dtype=np.int64)
         pos = spos = 0
-        snap_cache: Dict[int, int] = {}
+        snap_cache: Dict[tuple, int] = {}
         for i, p in enumerate(npad):
             off[i] = pos
             pos += int(p) * int(p)  # multiples of 128 x 128: every component stays 64 KiB aligned
             soff[i] = spos
-            b = snap_cache.get(int(p))
+            key = (int(p), bool(pers[i]))
+            b = snap_cache.get(key)
             if b is None:
-                b = snap_cache[int(p)] = int(L.rbg_batch_snap_bytes(int(p)))
+                q = L.rbg_batch_tri_snap_bytes if key[1] else L.rbg_batch_snap_bytes
+                b = snap_cache[key] = int(q(int(p)))
+            ssz[i] = b
             spos += _round(b)
-        return cls(sizes, npad, off, soff, pos, spos)
+        return cls(sizes, npad, off, soff, ssz, pos, spos)
 
 
 class BatchedClosure:
@@ -74,13 +85,24 @@ class BatchedClosure:
     entry below / above the diagonal, the triangular schedule), one value for
     all or one per component; a component that is not triangular as declared
     is found by the device precheck and closed again by the single-matrix
-    path, like any component outside the fast domain.
+    path, like any component outside the fast domain.  ``persistent``: lower
+    triangular components (tri 1) are closed by one persistent launch
+    instead of one panel and tile launches per pivot step (the same results;
+    on MI355X the 4096-component workload takes 411 ms that way against
+    380 ms per step: every work item pays a few us of decode, polling and
+    publication, and the batch's panels are many small items).
     """
 
-    def __init__(self, sizes: Sequence[int], tiles: Optional[Dict[int, int]] = None, stream=None, tri=0):
+    def __init__(self, sizes: Sequence[int], tiles: Optional[Dict[int, int]] = None, stream=None, tri=0,
+                 persistent: bool = False):
         t = _lib.torch()
         dev = _lib.device()
-        self.layout = BatchLayout.build(sizes)
+        count = len(sizes)
+        tri = np.broadcast_to(np.asarray(tri, dtype=np.int32), (count,))
+        if np.any((tri < 0) | (tri > 2)):
+            raise ValueError("tri must be 0, 1 or 2")
+        self.tri = tri.copy()
+        self.layout = BatchLayout.build(sizes, (self.tri == 1) & bool(persistent))
         self.count = len(self.layout.sizes)
         if tiles is None:
             from .autotune import tile_plan
@@ -91,10 +113,6 @@ class BatchedClosure:
         self.snap = t.empty(max(self.layout.snap_bytes, _ALIGN), dtype=t.uint8, device=dev)
         self.status = t.zeros(self.count * _lib.STATUS_BYTES, dtype=t.uint8, device=dev)
         self.status_host = t.empty(self.count * _lib.STATUS_BYTES, dtype=t.uint8, pin_memory=True)
-        tri = np.broadcast_to(np.asarray(tri, dtype=np.int32), (self.count,))
-        if np.any((tri < 0) | (tri > 2)):
-            raise ValueError("tri must be 0, 1 or 2")
-        self.tri = tri.copy()
         descs = (_lib.RbgBatchComp * self.count)()
         pb, sb, stb = self.pool.data_ptr(), self.snap.data_ptr(), self.status.data_ptr()
         for i in range(self.count):
@@ -105,6 +123,7 @@ class BatchedClosure:
             descs[i].npad = p
             descs[i].tile = self.tiles.get(p, 128)
             descs[i].tri = int(self.tri[i])
+            descs[i].snap_bytes = int(self.layout.snap_size[i])
         h = ctypes.c_void_p()
         _lib.call("rbg_batch_create", descs, self.count, ctypes.byref(h))
         self._h = h
@@ -125,9 +144,10 @@ class BatchedClosure:
             pass
 
     def info(self) -> dict:
-        buf = (ctypes.c_int64 * 8)()
+        buf = (ctypes.c_int64 * 10)()
         _lib.call("rbg_batch_info", self._h, buf)
-        keys = ["count", "steps", "launches", "panel_items", "tiles128", "tiles64", "chunks", "captured"]
+        keys = ["count", "steps", "launches", "panel_items", "tiles128", "tiles64", "chunks", "captured",
+                "persistent_components", "persistent_items"]
         return dict(zip(keys, [int(x) for x in buf]))
 
     # -- inputs -------------------------------------------------------------

@@ -28,6 +28,8 @@
 //            otherwise the kernels raise RBG_F_NEED_GENERAL.
 #include <type_traits>
 
+#include <atomic>
+
 #include "rbg_common.hpp"
 
 namespace rbg {
@@ -1590,6 +1592,9 @@ struct FlowCtl {
   int* c;
   int nb, nt, edge;
   int64_t total;  // items of the closure
+  // the items in ticket order, packed (flow_pack), or null: then each
+  // workgroup finds its item by flow_item's search
+  const uint32_t* items = nullptr;
   __host__ __device__ int* stall() const { return c + 2; }
   __host__ __device__ int64_t pdone(int K) const { return 4 + K; }
   __host__ __device__ int64_t tdone(int K) const { return 4 + nb + K; }
@@ -1597,7 +1602,7 @@ struct FlowCtl {
   __host__ __device__ int64_t pflag(int K, int idx) const {
     return 4 + 2 * (int64_t)nb + (int64_t)nt * nt + (int64_t)K * nb + idx;
   }
-  static int64_t words(int nb, int nt) { return 4 + 2 * (int64_t)nb + (int64_t)nt * nt + (int64_t)nb * nb; }
+  __host__ __device__ static int64_t words(int nb, int nt) { return 4 + 2 * (int64_t)nb + (int64_t)nt * nt + (int64_t)nb * nb; }
   __host__ __device__ int sh() const { return edge == 128 ? 2 : 1; }  // log2 of 32-blocks per tile edge
   // step K's rectangle: tile rows lo(K) .. nt - 1, tile columns 0 .. lo(K)
   __host__ __device__ int lo(int K) const { return K >> sh(); }
@@ -1681,6 +1686,17 @@ __host__ __device__ __forceinline__ FlowItem flow_item(const FlowCtl& f, int64_t
 //   T(K, I, J): the row-panel items of P(K) for tile column J's blocks left
 //     of block K, the column-panel items for tile row I's blocks below it,
 //     and tile (I, J)'s step K - 1 when that step covered it.
+// An item packed into 32 bits (role 1, K 10, a 11, b 10): a table of them
+// spares every workgroup flow_item's search (several us of dependent scalar
+// arithmetic per item).  Orders up to 32 x FLOW_PACK_MAX_NB.
+constexpr int FLOW_PACK_MAX_NB = 1024;
+__host__ __device__ inline uint32_t flow_pack(const FlowItem& q) {
+  return (uint32_t)q.role << 31 | (uint32_t)q.K << 21 | (uint32_t)q.a << 10 | (uint32_t)q.b;
+}
+__host__ __device__ inline FlowItem flow_unpack(uint32_t v) {
+  return FlowItem{(int)(v >> 31), (int)((v >> 21) & 1023), (int)((v >> 10) & 2047), (int)(v & 1023)};
+}
+
 constexpr int FLOW_MAXW = 10;
 __host__ __device__ inline bool flow_cond(const FlowCtl& f, const FlowItem& q, int i, int64_t& word, int& target) {
   const int bpt = 1 << f.sh(), K = q.K;
@@ -1734,12 +1750,20 @@ __device__ __forceinline__ bool flow_wait(const FlowCtl& f, const FlowItem& q, i
   int64_t word = 0;
   int target = 0;
   const bool mine = lane < FLOW_MAXW && flow_cond(f, q, lane, word, target);
-  const int64_t t0 = wall_clock64();
+  // lane FLOW_MAXW reads the stall word in the same round trip
+  const bool watch = lane == FLOW_MAXW;
+  if (watch) word = f.stall() - f.c;
+  bool timing = false;
+  int64_t t0 = 0;
   for (;;) {
-    const bool ok = !mine || flow_load(f.c + word) >= target;
-    if (__all(ok)) return true;
-    if (flow_load(f.stall())) return false;
-    if (wall_clock64() - t0 > limit) {
+    const int v = (mine || watch) ? flow_load(f.c + word) : 0;
+    if (__any(watch && v != 0)) return false;
+    if (__all(!mine || v >= target)) return true;
+    const int64_t now = wall_clock64();
+    if (!timing) {
+      timing = true;
+      t0 = now;
+    } else if (now - t0 > limit) {
       if (lane == 0) {
         __hip_atomic_store(f.stall(), 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         atomicOr(&st->flags, (uint32_t)(RBG_F_STALLED | RBG_F_NEED_GENERAL));
@@ -1757,8 +1781,8 @@ __device__ __forceinline__ bool flow_wait(const FlowCtl& f, const FlowItem& q, i
 // finds cycles from the snapshots, and records the block's trajectory
 // maximum.
 template <typename T>
-__global__ __launch_bounds__(PANEL_NT) void k_fw_diag_walk(T* W, int64_t n, int64_t ld, T* dsnap, int width,
-                                                           rbg_status* st) {
+__device__ __forceinline__ void fw_diag_walk_body(T* W, int64_t n, int64_t ld, T* dsnap, int width, rbg_status* st,
+                                                  int64_t K) {
   using U = typename unsigned_of<T>::type;
   constexpr U TOP = (U)1 << (sizeof(U) * 8 - 1);
   constexpr U ALL = ~(U)0;
@@ -1770,7 +1794,7 @@ __global__ __launch_bounds__(PANEL_NT) void k_fw_diag_walk(T* W, int64_t n, int6
   __shared__ int64_t sMax[PANEL_NT / 64];
   __shared__ int sbad;
   const int tid = threadIdx.x, i = tid >> 5, j = tid & (BLK - 1);
-  const int64_t K = blockIdx.x, k0 = K * BLK;
+  const int64_t k0 = K * BLK;
   const int lim = width - 2;
   auto canon = [](U x) -> U { return (T)x < 0 ? ALL : x; };
   auto pair = [](U b, U& m, U& y) {
@@ -2072,38 +2096,52 @@ __device__ __forceinline__ void fw_flow_row_body(FlowRowShared<typename unsigned
   if ((threadIdx.x & 63) == 0 && wm > seen) record_diameter(st, wm);
 }
 
+template <typename T>
+__global__ __launch_bounds__(PANEL_NT) void k_fw_diag_walk(T* W, int64_t n, int64_t ld, T* dsnap, int width,
+                                                           rbg_status* st) {
+  fw_diag_walk_body<T>(W, n, ld, dsnap, width, st, blockIdx.x);
+}
+
 static __global__ void k_flow_zero(int* c, int64_t words) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < words; i += (int64_t)gridDim.x * blockDim.x)
     c[i] = 0;
 }
 
-// One work item per workgroup: the grid holds every item of the closure in
-// ticket order, and workgroup x takes ticket x.  The hardware starts each
-// XCD's workgroups in index order, so a workgroup that has not started
-// leaves every later one of its XCD unstarted too: the smallest unfinished
-// ticket is always running, or starts as soon as an earlier one ends.
-// limit: the longest wait of one item, in wall-clock ticks.
-template <typename T, int RC, bool SHIFT_BYTES, int PER_CU>
-__global__ __launch_bounds__(NT, PER_CU) void k_fw_flow(T* W, int64_t n, int64_t ld, FwWork ws, int width,
-                                                       rbg_status* st, FlowCtl f, int64_t limit) {
-  using U = typename unsigned_of<T>::type;
-  __shared__ union {
+// The packed item table of a closure (FlowCtl::items), one thread per ticket.
+static __global__ void k_flow_table(FlowCtl f, uint32_t* items) {
+  for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < f.total; x += (int64_t)gridDim.x * blockDim.x)
+    items[x] = flow_pack(flow_item(f, x));
+}
+
+// The shared memory of one persistent work item: a tile's, a row panel's or a
+// column panel's.
+template <typename U, int RC, bool SHIFT_BYTES>
+struct FlowShared {
+  union {
     TileShared<U, 16 * RC, SHIFT_BYTES> t;
     FlowRowShared<U> r;
     FlowColShared<U> c;
-  } L;
-  __shared__ int sRole;
-  const FlowItem q = flow_item(f, blockIdx.x);  // the same in every thread
+  };
+  int role;
+};
+
+// Work item q of the persistent closure of one matrix: waits for the items
+// that last wrote what it reads, runs, then publishes its counters.
+// limit: the longest wait, in wall-clock ticks.
+template <typename T, int RC, bool SHIFT_BYTES>
+__device__ __forceinline__ void fw_flow_item(FlowShared<typename unsigned_of<T>::type, RC, SHIFT_BYTES>& L,
+                                             const FlowItem& q, T* W, int64_t n, int64_t ld, const FwWork& ws,
+                                             int width, rbg_status* st, const FlowCtl& f, int64_t limit) {
   if (threadIdx.x < 64) {
-    bool ok = !flow_load(f.stall()) && flow_wait(f, q, limit, st);
+    bool ok = flow_wait(f, q, limit, st);
     if (threadIdx.x == 0) {
       // the items waited for published their cells: drop stale lines
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-      sRole = ok ? q.role : -1;
+      L.role = ok ? q.role : -1;
     }
   }
   __syncthreads();
-  if (sRole < 0) return;
+  if (L.role < 0) return;
   const int K = q.K, a = q.a, b = q.b;
   if (q.role == 0) {
     // the panels are the closure's critical path: their waves take issue
@@ -2131,6 +2169,19 @@ __global__ __launch_bounds__(NT, PER_CU) void k_fw_flow(T* W, int64_t n, int64_t
       __hip_atomic_fetch_add(c + f.tdone(K), 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
+}
+
+// One work item per workgroup: the grid holds every item of the closure in
+// ticket order, and workgroup x takes ticket x.  The hardware starts each
+// XCD's workgroups in index order, so a workgroup that has not started
+// leaves every later one of its XCD unstarted too: the smallest unfinished
+// ticket is always running, or starts as soon as an earlier one ends.
+template <typename T, int RC, bool SHIFT_BYTES, int PER_CU>
+__global__ __launch_bounds__(NT, PER_CU) void k_fw_flow(T* W, int64_t n, int64_t ld, FwWork ws, int width,
+                                                       rbg_status* st, FlowCtl f, int64_t limit) {
+  __shared__ FlowShared<typename unsigned_of<T>::type, RC, SHIFT_BYTES> L;
+  const FlowItem q = f.items ? flow_unpack(f.items[blockIdx.x]) : flow_item(f, blockIdx.x);
+  fw_flow_item<T, RC, SHIFT_BYTES>(L, q, W, n, ld, ws, width, st, f, limit);
 }
 
 // ---------------------------------------------------------------------------
@@ -2194,6 +2245,70 @@ template <typename T, bool FAST>
 __global__ void k_fw_capture_last(rbg_status* st, FwWork ws, int64_t K, int64_t n) {
   const int par = ws.slot(K);
   capture_block<T, FAST>(st, ws.col<T>(par), ws.row<T>(par), K * BLK, n);
+}
+
+// Workspace layouts: FwWork's snapshot buffers (two, or four for the
+// persistent closure) and guard words; for the lower triangular schedules
+// the diagonal-block copy (FwWork::dcopy, n x 32); for the persistent
+// closure the diagonal blocks' step snapshots, its counters (FlowCtl; tile
+// rows counted for the smallest tile edge, 64) and its packed item table.
+// rbg_fw_workspace_bytes is the first, rbg_fw_tri_workspace_bytes the
+// largest.
+static inline int64_t align256(int64_t x) { return (x + 255) / 256 * 256; }
+static inline int64_t dcopy_offset(int64_t n, int elem, int ring) { return align256(FwWork::bytes(n, elem, ring)); }
+static inline int64_t la_ws_bytes(int64_t n, int elem) { return dcopy_offset(n, elem, 2) + n * BLK * elem; }
+static inline int64_t dsnap_offset(int64_t n, int elem) { return align256(dcopy_offset(n, elem, 4) + n * BLK * elem); }
+static inline int64_t flow_ctl_offset(int64_t n, int elem) { return align256(dsnap_offset(n, elem) + 2 * n * BLK * elem); }
+static inline int64_t flow_items_offset(int64_t n, int elem) {
+  return align256(flow_ctl_offset(n, elem) + FlowCtl::words((int)(n / BLK), (int)(n / 64)) * 4);
+}
+
+// The persistent closure's schedule for order n with tiles of `edge` (128 or
+// 64), counters in `wsp` (no item table).
+static inline FlowCtl flow_ctl(int64_t n, int elem, void* wsp, int edge) {
+  FlowCtl f{wsp ? (int*)((char*)wsp + flow_ctl_offset(n, elem)) : nullptr, (int)(n / BLK), (int)(n / edge), edge, 0};
+  f.total = f.count();
+  return f;
+}
+
+// The single-matrix choice: int32 takes 128-tiles (64-tiles up to
+// SMALL_TILE_MAX_N), int64 64-tiles; the item table follows the counters.
+static inline int flow_edge(int64_t n, int elem) { return (elem == 4 && n > SMALL_TILE_MAX_N) ? 128 : 64; }
+static inline FlowCtl flow_ctl(int64_t n, int elem, void* wsp) {
+  FlowCtl f = flow_ctl(n, elem, wsp, flow_edge(n, elem));
+  if (wsp && f.nb <= FLOW_PACK_MAX_NB) f.items = (const uint32_t*)((char*)wsp + flow_items_offset(n, elem));
+  return f;
+}
+static inline int64_t flow_ws_bytes(int64_t n, int elem) {
+  return flow_items_offset(n, elem) + 4 * flow_ctl(n, elem, nullptr).total;
+}
+
+// Relaxations a persistent closure of order n executes, counted in cells x 32
+// steps: the tiles' cells outside the skipped rows; a row panel's strip (one
+// block) and forward substitution (496 of a block's cells per 32 steps), a
+// column panel's 496 per 32 rows (the diagonal blocks run before the launch).
+static inline double flow_relaxations(const FlowCtl& f, int64_t n) {
+  double cells = 0;
+  for (int K = 0; K < f.nb; K++) {
+    const int jn = f.lo(K) + 1;
+    cells += (double)f.tiles(K) * f.edge * f.edge - (double)(K + 1 < f.nb ? 2 : 1) * BLK * jn * f.edge;
+    cells += (double)K * (BLK * BLK + 496) + (double)(n - (int64_t)(K + 1) * BLK) * 496 / BLK;
+  }
+  return cells * BLK;
+}
+
+// The longest wait of one persistent work item: two seconds of the device's
+// wall clock (a step's tiles take well under a millisecond at N = 8192), or
+// the test override (rbg_fw_flow_wait_limit).
+extern std::atomic<int64_t> g_flow_limit_override;
+static inline int64_t flow_limit_ticks() {
+  const int64_t o = g_flow_limit_override.load();
+  if (o > 0) return o;
+  int dev = 0, khz = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz < 1)
+    khz = 100000;
+  return (int64_t)khz * 1000 * 2;
 }
 
 }  // namespace rbg

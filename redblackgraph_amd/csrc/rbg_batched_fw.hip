@@ -11,6 +11,10 @@
 // panel lines) and one tile launch per tile shape, each over the work of all
 // running components, so a launch carries the panels (or tiles) of thousands
 // of components at once.
+// Lower triangular components whose workspace holds the persistent closure's
+// (rbg_batch_tri_snap_bytes) instead take one persistent launch for all of
+// them: the single-matrix persistent closure's work items (k_fw_flow), every
+// component's in one grid, each waiting only on its own component's counters.
 // Arithmetic, snapshots, event keys and the MSB guard are the single-matrix
 // fast kernels' (fw_kernels.hpp), per component.  A component whose input is
 // outside the fast domain, or that needs general arithmetic or saw a cycle,
@@ -29,8 +33,90 @@ struct BComp {
   rbg_status* st;
   int32_t npad;
   int32_t tile;
-  int32_t tri;  // 0 full schedule; 1 / 2: entries declared below / above the diagonal only
+  int32_t tri;   // 0 full schedule; 1 / 2: entries declared below / above the diagonal only
+  int32_t flow;  // 1: closed by the persistent launch (snapshot ring of four)
 };
+
+// A component of the persistent launch: its matrix, snapshot workspace (with
+// the diagonal blocks' step snapshots at dsnap), status and padded order; its
+// counters and schedule follow from its class.
+struct BFlow {
+  int32_t* W;
+  void* snap;
+  void* dsnap;
+  rbg_status* st;
+  int64_t n;
+};
+
+// The persistent launch's components of one padded order and tile edge, in
+// launch order (largest order first).  Tickets run class by class and, within
+// a class, by the components' own tickets with the components in lockstep:
+// ticket base + t cnt + m is own ticket t of component first + m.  Every wait
+// of an item names an item of its own component with a smaller own ticket,
+// so a smaller ticket of the launch, as in the single-matrix launch.  The
+// class's counters: `words` per component from word `ctl` of the launch's
+// counter array; its packed items (flow_pack, in own-ticket order) from entry
+// `items` of the launch's item table.
+struct BClass {
+  int64_t base;
+  int64_t ctl;
+  int64_t items;
+  int32_t first;
+  int32_t cnt;
+  int32_t n;
+  int32_t edge;
+  int32_t words;
+  int32_t pad;
+};
+constexpr int BFLOW_MAX_CLASSES = 64;
+// Passed by value: the decode reads kernel arguments only.
+struct BClasses {
+  int32_t count = 0;
+  int32_t dual = 0;  // some class has 64-tiles
+  int64_t total = 0;
+  int64_t ctl_words = 0;
+  int64_t item_words = 0;
+  BClass c[BFLOW_MAX_CLASSES];
+};
+
+// Classes of components with these (padded order, tile edge), in launch
+// order; false when they need more than BFLOW_MAX_CLASSES classes, more than
+// 2^31 - 1 tickets (one workgroup per ticket) or an order past the packed
+// items' range.
+static bool bclass_plan(const std::vector<std::pair<int, int>>& ne, BClasses& k) {
+  k = BClasses{};
+  for (size_t j = 0; j < ne.size();) {
+    size_t e = j;
+    while (e < ne.size() && ne[e] == ne[j]) e++;
+    if (k.count == BFLOW_MAX_CLASSES || ne[j].first / BLK > FLOW_PACK_MAX_NB) return false;
+    const FlowCtl f = flow_ctl(ne[j].first, 4, nullptr, ne[j].second);
+    BClass& c = k.c[k.count++];
+    c = BClass{k.total, k.ctl_words, k.item_words, (int32_t)j, (int32_t)(e - j), ne[j].first, ne[j].second,
+               (int32_t)FlowCtl::words(f.nb, f.nt), 0};
+    k.total += f.total * (int64_t)(e - j);
+    k.ctl_words += (int64_t)c.words * (e - j);
+    k.item_words += f.total;
+    k.dual |= c.edge == 64;
+    j = e;
+  }
+  return k.total <= 0x7fffffffLL;
+}
+
+// Ticket x -> class j, component, own ticket.
+__host__ __device__ inline void bclass_locate(const BClasses& k, int64_t x, int& j, int& comp, int64_t& local) {
+  int lo = 0, hi = k.count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (k.c[mid].base <= x)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  j = lo;
+  const uint32_t d = (uint32_t)(x - k.c[lo].base), cnt = (uint32_t)k.c[lo].cnt;
+  comp = k.c[lo].first + (int)(d % cnt);
+  local = (int64_t)(d / cnt);
+}
 
 struct Item {
   int32_t comp;
@@ -144,9 +230,50 @@ static __global__ void k_bfw_capture(const BComp* comps, int32_t count) {
   if (c >= count) return;
   const BComp b = comps[c];
   const int64_t nb = b.npad / BLK;
-  const int par = (int)((nb - 1) & 1);
-  FwWork ws{b.snap, b.npad, 4};
+  FwWork ws{b.snap, b.npad, 4, b.flow ? 4 : 2};
+  const int par = ws.slot(nb - 1);
   capture_block<int32_t, true>(b.st, ws.col<int32_t>(par), ws.row<int32_t>(par), (nb - 1) * BLK, b.npad);
+}
+
+// The persistent launch's pre-passes: item = (component, diagonal block),
+// every diagonal block closed with its step snapshots (k_fw_diag_walk); and
+// the counters cleared.
+static __global__ __launch_bounds__(PANEL_NT) void k_bfw_diag_walk(const BFlow* comps, const Item* items) {
+  const Item it = items[blockIdx.x];
+  const BFlow c = comps[it.comp];
+  fw_diag_walk_body<int32_t>(c.W, c.n, c.n, (int32_t*)c.dsnap, 32, c.st, it.bid);
+}
+
+static __global__ void k_bfw_flow_zero(int* ctl, int64_t words) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < words; i += (int64_t)gridDim.x * blockDim.x)
+    ctl[i] = 0;
+}
+
+// Every work item of every persistent component, one per workgroup, in the
+// ticket order of BClass; workgroup x takes ticket x.  The class, component
+// and counters follow from the kernel arguments alone and the item from one
+// table load; the component's pointers load meanwhile.
+// DUAL: some class has 64-tiles (else every class has 128-tiles).
+template <bool DUAL>
+static __global__ __launch_bounds__(NT, 3) void k_bfw_flow(const BFlow* comps, int* ctl, const uint32_t* items,
+                                                          BClasses k, int64_t limit) {
+  __shared__ union {
+    FlowShared<uint32_t, 8, false> e128;
+    FlowShared<uint32_t, 4, false> e64;
+  } L;
+  int j = 0, comp = 0;
+  int64_t local = 0;
+  bclass_locate(k, blockIdx.x, j, comp, local);
+  const int n = k.c[j].n, edge = k.c[j].edge;
+  const FlowCtl f{ctl + k.c[j].ctl + (int64_t)(comp - k.c[j].first) * k.c[j].words, n / BLK, n / edge, edge, 0};
+  const FlowItem q = flow_unpack(items[k.c[j].items + local]);
+  const BFlow c = comps[comp];
+  FwWork ws{c.snap, n, 4, 4};
+  ws.dsnap = c.dsnap;
+  if (!DUAL || edge == 128)
+    fw_flow_item<int32_t, 8, false>(L.e128, q, c.W, n, n, ws, 32, c.st, f, limit);
+  else
+    fw_flow_item<int32_t, 4, false>(L.e64, q, c.W, n, n, ws, 32, c.st, f, limit);
 }
 
 }  // namespace rbg
@@ -156,7 +283,17 @@ using namespace rbg;
 struct rbg_batch {
   int32_t count = 0;
   int32_t maxnb = 0;
+  int32_t stepnb = 0;  // pivot steps of the per-step launches (components outside the persistent launch)
   BComp* d_comps = nullptr;
+  // the persistent launch: its components, classes, counters, diagonal-walk items
+  int32_t n_flow = 0;
+  int64_t n_walk = 0;
+  double flow_work = 0;  // relaxations it executes
+  BClasses classes;
+  BFlow* d_flow = nullptr;
+  Item* d_walk = nullptr;
+  int* d_ctl = nullptr;
+  uint32_t* d_fitems = nullptr;
   // chunks | panels | tiles 128 | tiles 64 (full-schedule components) |
   // triangular components' tiles 128, step by step | the same for tiles 64
   Item* d_items = nullptr;
@@ -182,9 +319,30 @@ static int batch_launch(rbg_batch* b, hipStream_t s) {
   RBG_CHECK_LAUNCH();
   hipLaunchKernelGGL((k_bfw_convert<true>), dim3(b->n_chunk), dim3(NT), 0, s, b->d_comps, chunks);
   RBG_CHECK_LAUNCH();
-  for (int K = 0; K < b->maxnb; K++) {
-    hipLaunchKernelGGL(k_bfw_panel, dim3(b->panel_cnt[K]), dim3(NT), 0, s, b->d_comps, panels, (int64_t)K);
+  if (b->n_flow) {
+    hipLaunchKernelGGL(k_bfw_diag_walk, dim3(b->n_walk), dim3(PANEL_NT), 0, s, b->d_flow, b->d_walk);
     RBG_CHECK_LAUNCH();
+    const int64_t words = b->classes.ctl_words;
+    const int zg = (int)((words + NT - 1) / NT < 2048 ? (words + NT - 1) / NT : 2048);
+    hipLaunchKernelGGL(k_bfw_flow_zero, dim3(zg), dim3(NT), 0, s, b->d_ctl, words);
+    RBG_CHECK_LAUNCH();
+    const int64_t limit = flow_limit_ticks();
+    const dim3 grid((unsigned)b->classes.total);
+    int pf = prof_start(s);
+    if (b->classes.dual)
+      hipLaunchKernelGGL((k_bfw_flow<true>), grid, dim3(NT), 0, s, b->d_flow, b->d_ctl, b->d_fitems, b->classes,
+                         limit);
+    else
+      hipLaunchKernelGGL((k_bfw_flow<false>), grid, dim3(NT), 0, s, b->d_flow, b->d_ctl, b->d_fitems, b->classes,
+                         limit);
+    RBG_CHECK_LAUNCH();
+    prof_stop(4, pf, s, b->flow_work);
+  }
+  for (int K = 0; K < b->stepnb; K++) {
+    if (b->panel_cnt[K]) {
+      hipLaunchKernelGGL(k_bfw_panel, dim3(b->panel_cnt[K]), dim3(NT), 0, s, b->d_comps, panels, (int64_t)K);
+      RBG_CHECK_LAUNCH();
+    }
     int pi = prof_start(s);
     if (b->t128_cnt[K]) {
       hipLaunchKernelGGL((k_bfw_tile<8>), dim3(b->t128_cnt[K]), dim3(NT), 0, s, b->d_comps, t128, (int64_t)K);
@@ -217,6 +375,32 @@ extern "C" {
 
 int64_t rbg_batch_snap_bytes(int32_t npad) { return FwWork::bytes(npad, 4); }
 
+// the persistent launch's workspace without the counters (the launch keeps
+// every component's in one array of its own)
+int64_t rbg_batch_tri_snap_bytes(int32_t npad) { return flow_ctl_offset(npad, 4); }
+
+int64_t rbg_batch_flow_tickets(const int32_t* npad, const int32_t* edge, int32_t count, int64_t* out, int64_t cap) {
+  if (!npad || !edge || count <= 0) return RBG_E_ARG;
+  std::vector<std::pair<int, int>> ne;
+  for (int i = 0; i < count; i++) {
+    if (npad[i] < CHUNK || npad[i] % CHUNK || (edge[i] != 64 && edge[i] != 128)) return RBG_E_ARG;
+    if (i && npad[i] > npad[i - 1]) return RBG_E_ARG;  // launch order: largest first
+    ne.push_back({npad[i], edge[i]});
+  }
+  BClasses k;
+  if (!bclass_plan(ne, k)) return RBG_E_ARG;
+  if (out && cap >= 2 * k.total) {
+    for (int64_t x = 0; x < k.total; x++) {
+      int j = 0, comp = 0;
+      int64_t local = 0;
+      bclass_locate(k, x, j, comp, local);
+      out[2 * x] = comp;
+      out[2 * x + 1] = local;
+    }
+  }
+  return k.total;
+}
+
 int rbg_batch_create(const rbg_batch_comp* comps, int32_t count, rbg_batch** out) {
   if (!comps || count <= 0 || !out) return RBG_E_ARG;
   *out = nullptr;
@@ -226,17 +410,51 @@ int rbg_batch_create(const rbg_batch_comp* comps, int32_t count, rbg_batch** out
     if (!q.W || !q.snap || !q.st || q.npad < CHUNK || q.npad % CHUNK) return RBG_E_ARG;
     if (q.tile != 128 && q.tile != 64) return RBG_E_ARG;
     if (q.tri < 0 || q.tri > 2) return RBG_E_ARG;
-    c[i] = BComp{q.W, q.snap, q.st, q.npad, q.tile, q.tri};
+    if (q.snap_bytes < FwWork::bytes(q.npad, 4)) return RBG_E_WORKSPACE;
+    c[i] = BComp{q.W, q.snap, q.st, q.npad, q.tile, q.tri, q.tri == 1 && q.snap_bytes >= flow_ctl_offset(q.npad, 4)};
   }
   // largest first: the components running at step K are a prefix
   std::stable_sort(c.begin(), c.end(), [](const BComp& a, const BComp& b) { return a.npad > b.npad; });
+  // the persistent launch: lower triangular components with its workspace,
+  // in classes of equal order and tile edge (else they take the per-step
+  // launches)
+  BClasses classes;
+  {
+    std::vector<std::pair<int, int>> ne;
+    for (int i = 0; i < count; i++)
+      if (c[i].flow) ne.push_back({c[i].npad, c[i].tile});
+    if (!bclass_plan(ne, classes)) {
+      for (auto& x : c) x.flow = 0;
+      classes = BClasses{};
+    }
+  }
   auto* b = new rbg_batch();
   b->count = count;
   b->maxnb = c[0].npad / BLK;
+  b->classes = classes;
+  std::vector<BFlow> fl;
+  std::vector<Item> walk;
+  for (int i = 0; i < count; i++) {
+    if (!c[i].flow) {
+      b->stepnb = std::max(b->stepnb, c[i].npad / BLK);
+      continue;
+    }
+    for (int K = 0; K < c[i].npad / BLK; K++) walk.push_back({(int)fl.size(), K});
+    fl.push_back(BFlow{c[i].W, c[i].snap, (char*)c[i].snap + dsnap_offset(c[i].npad, 4), c[i].st, c[i].npad});
+    b->flow_work += flow_relaxations(flow_ctl(c[i].npad, 4, nullptr, c[i].tile), c[i].npad);
+  }
+  b->n_flow = (int32_t)fl.size();
+  b->n_walk = (int64_t)walk.size();
+  std::vector<uint32_t> fitems;
+  for (int j = 0; j < classes.count; j++) {
+    const FlowCtl f = flow_ctl(classes.c[j].n, 4, nullptr, classes.c[j].edge);
+    for (int64_t x = 0; x < f.total; x++) fitems.push_back(flow_pack(flow_item(f, x)));
+  }
   std::vector<Item> chunks, panels, t128, t64, tri128, tri64;
   for (int i = 0; i < count; i++) {
     const int nc = c[i].npad / CHUNK, nt = c[i].npad / c[i].tile;
     for (int q = 0; q < nc * nc; q++) chunks.push_back({i, q});
+    if (c[i].flow) continue;
     for (int q = 0; q < lines_groups(c[i].npad); q++) panels.push_back({i, q});
     if (c[i].tri) continue;
     auto& tl = c[i].tile == 128 ? t128 : t64;
@@ -253,6 +471,7 @@ int rbg_batch_create(const rbg_batch_comp* comps, int32_t count, rbg_batch** out
   b->tri64_off.assign(b->maxnb + 1, 0);
   b->work.assign(b->maxnb, 0.0);
   for (int i = 0; i < count; i++) {
+    if (c[i].flow) continue;
     const int nb = c[i].npad / BLK, nt = c[i].npad / c[i].tile;
     for (int K = 0; K < nb; K++) {
       b->panel_cnt[K] += lines_groups(c[i].npad);
@@ -266,7 +485,7 @@ int rbg_batch_create(const rbg_batch_comp* comps, int32_t count, rbg_batch** out
   // (tri_rect), of every component still running, in component order
   for (int K = 0; K < b->maxnb; K++) {
     for (int i = 0; i < count && c[i].npad / BLK > K; i++) {
-      if (!c[i].tri) continue;
+      if (!c[i].tri || c[i].flow) continue;
       const int nt = c[i].npad / c[i].tile;
       int cnt = 0;
       const TileRect r = tri_rect(c[i].tri, c[i].npad, c[i].tile, K, &cnt);
@@ -287,9 +506,23 @@ int rbg_batch_create(const rbg_batch_comp* comps, int32_t count, rbg_batch** out
   if (e == hipSuccess) e = hipMalloc((void**)&b->d_items, sizeof(Item) * all.size());
   if (e == hipSuccess) e = hipMemcpy(b->d_comps, c.data(), sizeof(BComp) * count, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(b->d_items, all.data(), sizeof(Item) * all.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess && b->n_flow) {
+    e = hipMalloc((void**)&b->d_flow, sizeof(BFlow) * fl.size());
+    if (e == hipSuccess) e = hipMalloc((void**)&b->d_walk, sizeof(Item) * walk.size());
+    if (e == hipSuccess) e = hipMalloc((void**)&b->d_ctl, sizeof(int) * classes.ctl_words);
+    if (e == hipSuccess) e = hipMalloc((void**)&b->d_fitems, sizeof(uint32_t) * fitems.size());
+    if (e == hipSuccess)
+      e = hipMemcpy(b->d_fitems, fitems.data(), sizeof(uint32_t) * fitems.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(b->d_flow, fl.data(), sizeof(BFlow) * fl.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(b->d_walk, walk.data(), sizeof(Item) * walk.size(), hipMemcpyHostToDevice);
+  }
   if (e != hipSuccess) {
     (void)hipFree(b->d_comps);
     (void)hipFree(b->d_items);
+    (void)hipFree(b->d_flow);
+    (void)hipFree(b->d_walk);
+    (void)hipFree(b->d_ctl);
+    (void)hipFree(b->d_fitems);
     delete b;
     return (int)e;
   }
@@ -299,10 +532,10 @@ int rbg_batch_create(const rbg_batch_comp* comps, int32_t count, rbg_batch** out
 
 int rbg_batch_info(const rbg_batch* b, int64_t* info) {
   if (!b || !info) return RBG_E_ARG;
-  int64_t launches = 5;
-  for (int K = 0; K < b->maxnb; K++)
-    launches += 1 + (b->t128_cnt[K] > 0) + (b->t64_cnt[K] > 0) + (b->tri128_off[K + 1] > b->tri128_off[K]) +
-                (b->tri64_off[K + 1] > b->tri64_off[K]);
+  int64_t launches = 5 + (b->n_flow ? 3 : 0);
+  for (int K = 0; K < b->stepnb; K++)
+    launches += (b->panel_cnt[K] > 0) + (b->t128_cnt[K] > 0) + (b->t64_cnt[K] > 0) +
+                (b->tri128_off[K + 1] > b->tri128_off[K]) + (b->tri64_off[K + 1] > b->tri64_off[K]);
   info[0] = b->count;
   info[1] = b->maxnb;
   info[2] = launches;
@@ -311,6 +544,8 @@ int rbg_batch_info(const rbg_batch* b, int64_t* info) {
   info[5] = b->n_t64 + b->n_tri64;
   info[6] = b->n_chunk;
   info[7] = b->exec != nullptr;
+  info[8] = b->n_flow;
+  info[9] = b->classes.total;
   return 0;
 }
 
@@ -359,6 +594,10 @@ int rbg_batch_destroy(rbg_batch* b) {
   if (b->graph) (void)hipGraphDestroy(b->graph);
   (void)hipFree(b->d_comps);
   (void)hipFree(b->d_items);
+  (void)hipFree(b->d_flow);
+  (void)hipFree(b->d_walk);
+  (void)hipFree(b->d_ctl);
+  (void)hipFree(b->d_fitems);
   delete b;
   return 0;
 }

@@ -4,6 +4,8 @@
 
 namespace rbg {
 
+std::atomic<int64_t> g_flow_limit_override{0};
+
 // 128-tile rectangles below this many workgroups (a third of a round of
 // three per CU) run as 64-tiles instead, in the triangular schedule: four
 // times the workgroups for the steps that touch few tiles.
@@ -17,40 +19,6 @@ constexpr int TRI_SMALL_BELOW = 256;
 // inside it (12.56 and 6.36 ms).
 constexpr bool LOOKAHEAD = true;
 constexpr bool FLOW = true;
-
-// Workspace layouts: FwWork's snapshot buffers (two, or four for the
-// persistent closure) and guard words; for the lower triangular schedules
-// the diagonal-block copy (FwWork::dcopy, n x 32); for the persistent
-// closure its counters (FlowCtl; tile rows counted for the smallest tile
-// edge, 64).  rbg_fw_workspace_bytes is the first, rbg_fw_tri_workspace_bytes
-// the largest.
-static int64_t align256(int64_t x) { return (x + 255) / 256 * 256; }
-static int64_t dcopy_offset(int64_t n, int elem, int ring) { return align256(FwWork::bytes(n, elem, ring)); }
-static int64_t la_ws_bytes(int64_t n, int elem) { return dcopy_offset(n, elem, 2) + n * BLK * elem; }
-static int64_t dsnap_offset(int64_t n, int elem) { return align256(dcopy_offset(n, elem, 4) + n * BLK * elem); }
-static int64_t flow_ctl_offset(int64_t n, int elem) { return align256(dsnap_offset(n, elem) + 2 * n * BLK * elem); }
-static int64_t flow_ws_bytes(int64_t n, int elem) {
-  return flow_ctl_offset(n, elem) + FlowCtl::words((int)(n / BLK), (int)(n / 64)) * 4;
-}
-
-// The persistent closure's schedule for order n: int32 takes 128-tiles
-// (64-tiles up to SMALL_TILE_MAX_N), int64 64-tiles; counters in `wsp`.
-static FlowCtl flow_ctl(int64_t n, int elem, void* wsp) {
-  const int edge = (elem == 4 && n > SMALL_TILE_MAX_N) ? 128 : 64;
-  FlowCtl f{wsp ? (int*)((char*)wsp + flow_ctl_offset(n, elem)) : nullptr, (int)(n / BLK), (int)(n / edge), edge, 0};
-  f.total = f.count();
-  return f;
-}
-
-// The longest wait of one persistent work item: two seconds of the device's
-// wall clock (a step's tiles take well under a millisecond at N = 8192).
-static int64_t flow_limit_ticks() {
-  int dev = 0, khz = 0;
-  if (hipGetDevice(&dev) != hipSuccess ||
-      hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz < 1)
-    khz = 100000;
-  return (int64_t)khz * 1000 * 2;
-}
 
 // tri: 0 = full schedule; 1 / 2 = the input's off-diagonal entries all lie
 // below / above the diagonal (fast mode only: general mode always runs the
@@ -166,20 +134,15 @@ static int fw_dense(T* W, int64_t n, int64_t ld, int width, int is_uns, int mode
     const FlowCtl f = flow_ctl(n, elem, wsp);
     const int edge = f.edge;
     if (f.total > 0x7fffffffLL) return RBG_E_ARG;  // one workgroup per item
-    // relaxations, counted in cells x 32 steps: the tiles' cells outside
-    // the skipped rows; a row panel's strip (one block) and forward
-    // substitution (496 of a block's cells per 32 steps), a column panel's
-    // 496 per 32 rows (the diagonal blocks run before the launch)
-    double cells = 0;
-    for (int K = 0; K < nb; K++) {
-      const int jn = f.lo(K) + 1;
-      cells += (double)f.tiles(K) * edge * edge - (double)(K + 1 < nb ? 2 : 1) * BLK * jn * edge;
-      cells += (double)K * (BLK * BLK + 496) + (double)(n - (int64_t)(K + 1) * BLK) * 496 / BLK;
-    }
     const int64_t words = FlowCtl::words(nb, f.nt);
     hipLaunchKernelGGL(k_flow_zero, dim3((int)((words + NT - 1) / NT < 1024 ? (words + NT - 1) / NT : 1024)),
                        dim3(NT), 0, s, f.c, words);
     RBG_CHECK_LAUNCH();
+    if (f.items) {
+      const int tg = (int)((f.total + NT - 1) / NT < 1024 ? (f.total + NT - 1) / NT : 1024);
+      hipLaunchKernelGGL(k_flow_table, dim3(tg), dim3(NT), 0, s, f, (uint32_t*)f.items);
+      RBG_CHECK_LAUNCH();
+    }
     const int64_t limit = flow_limit_ticks();
     const dim3 grid((unsigned)f.total);
     int p = PROF ? prof_start(s) : -1;
@@ -190,7 +153,7 @@ static int fw_dense(T* W, int64_t n, int64_t ld, int width, int is_uns, int mode
     else
       hipLaunchKernelGGL((k_fw_flow<T, 4, false, 4>), grid, dim3(NT), 0, s, W, n, ld, ws, width, st, f, limit);
     RBG_CHECK_LAUNCH();
-    if (PROF) prof_stop(6, p, s, cells * BLK);
+    if (PROF) prof_stop(6, p, s, flow_relaxations(f, n));
   } else {
     for (int K = 0; K < nb; K++) {
       int rc = (!la || K == 0) ? panel(K, s) : 0;
@@ -279,11 +242,14 @@ int rbg_status_init(rbg_status* st, rbg_stream_t stream) {
 
 int64_t rbg_fw_workspace_bytes(int64_t n, int32_t elem_bytes) { return FwWork::bytes(n, elem_bytes); }
 
+int64_t rbg_fw_flow_wait_limit(int64_t ticks) { return g_flow_limit_override.exchange(ticks > 0 ? ticks : 0); }
+
 int64_t rbg_fw_tri_workspace_bytes(int64_t n, int32_t elem_bytes) { return flow_ws_bytes(n, elem_bytes); }
 
-int64_t rbg_fw_flow_schedule(int64_t n, int32_t elem_bytes, int32_t* items, int64_t cap) {
-  if (n <= 0 || n % TILE || (elem_bytes != 4 && elem_bytes != 8)) return RBG_E_ARG;
-  const FlowCtl f = flow_ctl(n, elem_bytes, nullptr);
+int64_t rbg_fw_flow_schedule(int64_t n, int32_t elem_bytes, int32_t edge, int32_t* items, int64_t cap) {
+  if (n <= 0 || n % TILE || (elem_bytes != 4 && elem_bytes != 8) || (edge && edge != 64 && edge != 128))
+    return RBG_E_ARG;
+  const FlowCtl f = flow_ctl(n, elem_bytes, nullptr, edge ? edge : flow_edge(n, elem_bytes));
   if (items && cap >= 4 * f.total) {
     for (int64_t x = 0; x < f.total; x++) {
       const FlowItem q = flow_item(f, x);
@@ -296,9 +262,10 @@ int64_t rbg_fw_flow_schedule(int64_t n, int32_t elem_bytes, int32_t* items, int6
   return f.total;
 }
 
-int64_t rbg_fw_flow_waits(int64_t n, int32_t elem_bytes, int64_t* waits, int64_t cap) {
-  if (n <= 0 || n % TILE || (elem_bytes != 4 && elem_bytes != 8)) return RBG_E_ARG;
-  const FlowCtl f = flow_ctl(n, elem_bytes, nullptr);
+int64_t rbg_fw_flow_waits(int64_t n, int32_t elem_bytes, int32_t edge, int64_t* waits, int64_t cap) {
+  if (n <= 0 || n % TILE || (elem_bytes != 4 && elem_bytes != 8) || (edge && edge != 64 && edge != 128))
+    return RBG_E_ARG;
+  const FlowCtl f = flow_ctl(n, elem_bytes, nullptr, edge ? edge : flow_edge(n, elem_bytes));
   if (waits && cap >= 2 * FLOW_MAXW * f.total) {
     for (int64_t x = 0; x < f.total; x++) {
       const FlowItem q = flow_item(f, x);

@@ -1,10 +1,12 @@
-"""Host-side logic of the batched closure: pool layout and the autotuner's
-cache (no GPU calls)."""
+"""Host-side logic of the batched closure: pool layout, the persistent
+launch's tickets and the autotuner's cache (no GPU calls)."""
+import ctypes
 import json
 
 import numpy as np
+import pytest
 
-from redblackgraph_amd import autotune
+from redblackgraph_amd import _lib, autotune
 from redblackgraph_amd.batched import BatchLayout
 
 
@@ -17,6 +19,69 @@ def test_layout_offsets_and_padding():
     assert np.all(L.offset % (128 * 128) == 0)
     assert np.all(L.snap_offset % 256 == 0)
     assert np.all(np.diff(L.snap_offset) >= 4 * 32 * L.npad[:-1] * 4)
+    assert np.all(np.diff(L.snap_offset) >= L.snap_size[:-1])
+    P = BatchLayout.build(sizes, [True, False, True, True, False])
+    assert np.all(np.diff(P.snap_offset) >= P.snap_size[:-1]) and P.snap_offset[-1] + P.snap_size[-1] <= P.snap_bytes
+    lib = _lib.load()
+    for i, p in enumerate(P.npad):
+        q = lib.rbg_batch_tri_snap_bytes(int(p)) if i in (0, 2, 3) else lib.rbg_batch_snap_bytes(int(p))
+        assert P.snap_size[i] == q
+        assert lib.rbg_batch_tri_snap_bytes(int(p)) > lib.rbg_batch_snap_bytes(int(p))
+
+
+def _tickets(npad, edge):
+    lib = _lib.load()
+    a = np.asarray(npad, dtype=np.int32)
+    e = np.asarray(edge, dtype=np.int32)
+    vp = lambda x: x.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+    total = lib.rbg_batch_flow_tickets(vp(a), vp(e), len(a), None, 0)
+    out = np.zeros(2 * total, dtype=np.int64)
+    assert lib.rbg_batch_flow_tickets(vp(a), vp(e), len(a), vp(out), out.size) == total
+    return out.reshape(total, 2)
+
+
+@pytest.mark.parametrize("npad,edge", [
+    ([4096, 2048, 1024, 1024, 512, 384, 128, 128, 128], [128] * 9),
+    ([1280, 1280, 1152, 640, 256, 256, 128], [128, 64, 64, 128, 64, 64, 128]),
+    ([128], [128]),
+])
+def test_persistent_batch_tickets(npad, edge):
+    """Every ticket of the persistent batched launch names one work item of
+    one component; each component's items all appear, once, in its own
+    ticket order (so every wait, which names an item of the same component
+    with a smaller own ticket, names a smaller ticket of the launch); and the
+    components of one order and tile edge run in lockstep, largest first."""
+    lib = _lib.load()
+    tk = _tickets(npad, edge)
+    comp, local = tk[:, 0], tk[:, 1]
+    for c, (p, e) in enumerate(zip(npad, edge)):
+        own = local[comp == c]
+        total = lib.rbg_fw_flow_schedule(p, 4, e, None, 0)
+        np.testing.assert_array_equal(own, np.arange(total))
+    first = [np.flatnonzero(comp == c)[0] for c in range(len(npad))]
+    assert first == sorted(first) and first[0] == 0
+    n0 = [lib.rbg_fw_flow_schedule(p, 4, e, None, 0) for p, e in zip(npad, edge)]
+    assert len(tk) == sum(n0)
+    # lockstep: within a run of equal (order, edge) the components take turns
+    x = 0
+    while x < len(tk):
+        c0 = comp[x]
+        run = [c for c in range(c0, len(npad)) if (npad[c], edge[c]) == (npad[c0], edge[c0])]
+        m = len(run)
+        blk = comp[x:x + m * n0[c0]].reshape(n0[c0], m)
+        np.testing.assert_array_equal(blk, np.tile(np.asarray(run), (n0[c0], 1)))
+        x += m * n0[c0]
+
+
+def test_persistent_batch_tickets_reject_bad_order():
+    lib = _lib.load()
+    a = np.asarray([128, 256], dtype=np.int32)
+    e = np.asarray([128, 128], dtype=np.int32)
+    vp = lambda x: x.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+    assert lib.rbg_batch_flow_tickets(vp(a), vp(e), 2, None, 0) < 0
+    e = np.asarray([128, 96], dtype=np.int32)
+    a = np.asarray([256, 128], dtype=np.int32)
+    assert lib.rbg_batch_flow_tickets(vp(a), vp(e), 2, None, 0) < 0
 
 
 def test_tuning_batch_sizes():

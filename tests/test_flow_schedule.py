@@ -22,23 +22,24 @@ BLK = 32
 MAXW = 10
 
 
-def _edge(n, elem):
-    return 128 if elem == 4 and n > 2048 else 64
+def _edge(n, elem, edge=0):
+    """The tile edge: `edge`, or with 0 the single-matrix closure's choice."""
+    return edge or (128 if elem == 4 and n > 2048 else 64)
 
 
-def _schedule(n, elem):
+def _schedule(n, elem, edge=0):
     L = _lib.load()
-    total = L.rbg_fw_flow_schedule(n, elem, None, 0)
+    total = L.rbg_fw_flow_schedule(n, elem, edge, None, 0)
     assert total > 0
     buf = np.zeros(4 * total, dtype=np.int32)
-    assert L.rbg_fw_flow_schedule(n, elem, buf.ctypes.data_as(ctypes.c_void_p), buf.size) == total
+    assert L.rbg_fw_flow_schedule(n, elem, edge, buf.ctypes.data_as(ctypes.c_void_p), buf.size) == total
     w = np.zeros(2 * MAXW * total, dtype=np.int64)
-    assert L.rbg_fw_flow_waits(n, elem, w.ctypes.data_as(ctypes.c_void_p), w.size) == total
+    assert L.rbg_fw_flow_waits(n, elem, edge, w.ctypes.data_as(ctypes.c_void_p), w.size) == total
     return [tuple(int(v) for v in row) for row in buf.reshape(total, 4)], w.reshape(total, MAXW, 2)
 
 
-def _layout(n, elem):
-    nb, edge = n // BLK, _edge(n, elem)
+def _layout(n, elem, edge=0):
+    nb, edge = n // BLK, _edge(n, elem, edge)
     nt = n // edge
     lo = [K * BLK // edge for K in range(nb)]
     return nb, nt, edge, lo
@@ -60,8 +61,8 @@ def _panel_item(nb, K, idx):
     return "c", idx - K
 
 
-def _all_items(n, elem):
-    nb, nt, edge, lo = _layout(n, elem)
+def _all_items(n, elem, edge=0):
+    nb, nt, edge, lo = _layout(n, elem, edge)
     items = set()
     for K in range(nb):
         items |= {(0, K, idx, 0) for idx in range(K + _ncol(nb, K))}
@@ -69,10 +70,10 @@ def _all_items(n, elem):
     return items
 
 
-def _deps(n, elem, sched, waits):
+def _deps(n, elem, sched, waits, edge=0):
     """Per ticket: the tickets its conditions wait for (a condition on a
     counter holds once every item it counts has finished)."""
-    nb, nt, edge, lo = _layout(n, elem)
+    nb, nt, edge, lo = _layout(n, elem, edge)
     ticket = {it: x for x, it in enumerate(sched)}
     panels_of, tiles_of = {}, {}
     for x, it in enumerate(sched):
@@ -103,14 +104,19 @@ def _deps(n, elem, sched, waits):
     return out
 
 
-@pytest.mark.parametrize("n,elem", [(128, 8), (512, 4), (1152, 8), (2176, 4), (4096, 8), (8192, 4)])
-def test_flow_tickets_cover_items_and_wait_backwards(n, elem):
-    sched, waits = _schedule(n, elem)
-    assert len(sched) == len(_all_items(n, elem)) and set(sched) == _all_items(n, elem)
-    for x, dx in enumerate(_deps(n, elem, sched, waits)):
+# (n, elem, edge): the single-matrix closure's shapes (edge 0) and the
+# batched closure's 128-tiles at small orders
+@pytest.mark.parametrize("n,elem,edge", [(128, 8, 0), (512, 4, 0), (1152, 8, 0), (2176, 4, 0), (4096, 8, 0),
+                                         (8192, 4, 0), (128, 4, 128), (384, 4, 128), (1024, 4, 128),
+                                         (640, 4, 64)])
+def test_flow_tickets_cover_items_and_wait_backwards(n, elem, edge):
+    sched, waits = _schedule(n, elem, edge)
+    items = _all_items(n, elem, edge)
+    assert len(sched) == len(items) and set(sched) == items
+    for x, dx in enumerate(_deps(n, elem, sched, waits, edge)):
         assert all(d < x for d in dx), (sched[x], [sched[d] for d in dx if d >= x])
     # within each step's tiles, block K + 2's tile row comes first
-    nb, nt, edge, lo = _layout(n, elem)
+    nb, nt, edge, lo = _layout(n, elem, edge)
     for K in range(nb - 2):
         tiles = [it for it in sched if it[0] == 1 and it[1] == K]
         assert all(t[2] == lo[K + 2] for t in tiles[: lo[K] + 1])
@@ -157,11 +163,12 @@ def _snap_blocks(item, nb, edge):
     return set(), r
 
 
-@pytest.mark.parametrize("n,elem", [(256, 8), (1024, 4), (2048, 8), (2560, 4)])
-def test_flow_unordered_items_touch_disjoint_blocks(n, elem):
-    sched, waits = _schedule(n, elem)
-    nb, nt, edge, lo = _layout(n, elem)
-    deps = _deps(n, elem, sched, waits)
+@pytest.mark.parametrize("n,elem,edge", [(256, 8, 0), (1024, 4, 0), (2048, 8, 0), (2560, 4, 0), (128, 4, 128),
+                                         (256, 4, 128), (768, 4, 128), (1152, 4, 128)])
+def test_flow_unordered_items_touch_disjoint_blocks(n, elem, edge):
+    sched, waits = _schedule(n, elem, edge)
+    nb, nt, edge, lo = _layout(n, elem, edge)
+    deps = _deps(n, elem, sched, waits, edge)
     anc = [0] * len(sched)  # ancestors as bit sets over tickets
     for x, dx in enumerate(deps):
         m = 0

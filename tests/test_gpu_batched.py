@@ -1,12 +1,14 @@
 """GPU parity of the batched dense closure: every component equals the CPU
 oracle's naive loop (W and diameter) and the single-matrix GPU path, for both
-tile shapes, eagerly and through hipGraph replays; components outside the fast
-domain fall back to the general kernels with the single-matrix errors."""
+tile shapes, eagerly and through hipGraph replays; the persistent launch of
+lower triangular components equals the per-step launches; components outside
+the fast domain fall back to the general kernels with the single-matrix
+errors."""
 import numpy as np
 import pytest
 
 from redblackgraph_amd import _lib, dense, synth
-from redblackgraph_amd._dtypes import elem_type
+from redblackgraph_amd._dtypes import elem_type, msb
 from redblackgraph_amd.batched import BatchedClosure, transitive_closure_batch
 from helpers import overflow_chain
 
@@ -115,16 +117,19 @@ def test_batched_fallbacks_and_errors(orc):
     assert res[0].diameter == orc.warshall(good[0]).diameter
 
 
-@pytest.mark.parametrize("tile,reverse", [(128, False), (64, False), (128, True)])
-def test_batched_triangular_vs_oracle(orc, tile, reverse):
+@pytest.mark.parametrize("tile,reverse,persistent", [(128, False, False), (64, False, False), (128, True, False),
+                                                     (128, False, True), (64, False, True)])
+def test_batched_triangular_vs_oracle(orc, tile, reverse, persistent):
     """Components declared triangular take the triangular schedule: lower
     (parents first) and upper (numbered backwards) pedigrees equal the
-    oracle's naive loop, W and trajectory diameter."""
+    oracle's naive loop, W and trajectory diameter, by per-step launches and
+    by the persistent launch."""
     mats = _components(40, 16, 600, 7)
     if reverse:
         mats = [synth.reverse_vertices(A) for A in mats]
     b = BatchedClosure([A.shape[0] for A in mats], tiles={p: tile for p in range(128, 4097, 128)},
-                       tri=2 if reverse else 1)
+                       tri=2 if reverse else 1, persistent=persistent)
+    assert b.info()["persistent_components"] == (len(mats) if persistent else 0)
     try:
         b.load_dense(mats)
         b.run(graph=False)
@@ -138,14 +143,16 @@ def test_batched_triangular_vs_oracle(orc, tile, reverse):
         b.close()
 
 
-def test_batched_triangular_large_components_and_replays():
+@pytest.mark.parametrize("persistent", [False, True])
+def test_batched_triangular_large_components_and_replays(persistent):
     """Components above 2048 vertices (where most of the batched work is)
-    through the triangular schedule: equal to the single-matrix path (whose
-    128-tile kernel the oracle digests pin), eagerly and in graph replays; a
-    component declared triangular that is not is caught and closed again."""
+    through the triangular schedule, per step and persistent: equal to the
+    single-matrix path (whose 128-tile kernel the oracle digests pin),
+    eagerly and in graph replays; a component declared triangular that is
+    not is caught and closed again."""
     t = _lib.torch()
     sizes, diag, edges = synth.batched_workload(count=24, lo=1500, hi=4096, seed=9)
-    b = BatchedClosure(sizes, tri=1)
+    b = BatchedClosure(sizes, tri=1, persistent=persistent)
     try:
         b.load_edges(diag, edges)
         b.run(graph=False)
@@ -178,3 +185,75 @@ def test_batched_triangular_large_components_and_replays():
     W, out = dense.fw_closure(_lib.to_device(bad), elem_type(np.int32))
     np.testing.assert_array_equal(res[1].W, W.cpu().numpy())
     assert res[1].diameter == out.diameter
+
+
+def test_batched_persistent_equals_per_step_launches(orc):
+    """Lower triangular components in the persistent launch (128- and
+    64-tiles in one grid), beside full-schedule components in per-step
+    launches of the same batch: every matrix and status equals the per-step
+    schedule's, eagerly and in graph replays, and the oracle's on a sample."""
+    t = _lib.torch()
+    mats = _components(48, 16, 1300, 21)
+    tri = [0 if i % 3 == 0 else 1 for i in range(len(mats))]
+    tiles = {p: (64 if (p // 128) % 2 else 128) for p in range(128, 4097, 128)}
+    runs = {}
+    for persistent in (True, False):
+        b = BatchedClosure([A.shape[0] for A in mats], tiles=tiles, tri=tri, persistent=persistent)
+        try:
+            info = b.info()
+            assert info["persistent_components"] == (sum(tri) if persistent else 0)
+            b.load_dense(mats)
+            b.run(graph=False)
+            b.stream.synchronize()
+            eager = b.pool.clone()
+            sts = [(s.flags, s.diameter_max, s.cycle_count, s.events) for s in b.statuses()]
+            if persistent:
+                for rep in range(3):
+                    b.load_dense(mats)
+                    b.run(graph=True)
+                    b.stream.synchronize()
+                    assert bool(t.equal(b.pool, eager)), f"replay {rep}"
+                    assert [(s.flags, s.diameter_max, s.cycle_count, s.events) for s in b.statuses()] == sts
+            runs[persistent] = (eager, sts, b.layout)
+        finally:
+            b.close()
+    (pool_p, st_p, lay), (pool_s, st_s, _) = runs[True], runs[False]
+    assert st_p == st_s
+    assert all(f == 0 for f, _, _, _ in st_p)
+    assert bool(t.equal(pool_p, pool_s))
+    for i in (1, 2, 4, 5, 7):
+        n, o, p = int(lay.sizes[i]), int(lay.offset[i]), int(lay.npad[i])
+        ref = orc.warshall(mats[i])
+        np.testing.assert_array_equal(pool_p[o:o + p * p].view(p, p)[:n, :n].cpu().numpy(), ref.W)
+        assert msb(int(st_p[i][1])) == ref.diameter, i
+
+
+def test_batched_persistent_wait_limit_reruns():
+    """Waits of the persistent batched launch that run out (a one-tick limit
+    set from the host) flag their components stalled; outcomes() closes those
+    again through the single-matrix path, and every component equals an
+    unforced run."""
+    t = _lib.torch()
+    L = _lib.load()
+    mats = _components(24, 200, 1500, 33)
+    b = BatchedClosure([A.shape[0] for A in mats], tri=1, persistent=True)
+    try:
+        assert b.info()["persistent_components"] == len(mats)
+        b.load_dense(mats)
+        b.run(graph=False)
+        outs0 = b.outcomes()
+        ref = [b.result(i).clone() for i in range(len(mats))]
+        prev = L.rbg_fw_flow_wait_limit(1)
+        try:
+            b.load_dense(mats)
+            b.run(graph=False)
+            stalled = [i for i, s in enumerate(b.statuses()) if s.flags & _lib.F_STALLED]
+            outs1 = b.outcomes(lambda i: _lib.to_device(mats[i]))
+        finally:
+            L.rbg_fw_flow_wait_limit(prev)
+        assert stalled
+        for i in range(len(mats)):
+            assert bool(t.equal(b.result(i), ref[i])), i
+            assert outs1[i].diameter == outs0[i].diameter, i
+    finally:
+        b.close()

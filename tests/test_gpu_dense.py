@@ -257,6 +257,26 @@ def test_fw_persistent_closure_vs_steps_and_oracle(orc, dt, n):
         assert out.diameter == outs.diameter
 
 
+@pytest.mark.parametrize("dt,n", [(np.int32, 2100), (np.int64, 900)])
+def test_fw_persistent_wait_limit_reruns_steps(dt, n):
+    """A wait of the persistent closure that runs out, forced from the host
+    with a one-tick limit: every workgroup ends, the closure is flagged
+    stalled, and fw_closure reruns the per-step schedule from the input --
+    the same W, diameter and status as an unforced closure."""
+    L = _lib.load()
+    A = synth.dense_pedigree(synth.component_spec(n, 0.8, 4), 29 + n, dtype=dt)
+    W0, out0 = _fw(A)
+    assert out0.schedule == "lower"
+    prev = L.rbg_fw_flow_wait_limit(1)
+    try:
+        W1, out1 = _fw(A)
+    finally:
+        L.rbg_fw_flow_wait_limit(prev)
+    assert out1.mode == "fast" and out1.schedule == "lower-steps"
+    np.testing.assert_array_equal(W1, W0)
+    assert out1.diameter == out0.diameter and out1.status.flags == out0.status.flags == 0
+
+
 @pytest.mark.parametrize("dt,n,wrong", [(np.int32, 640, 2), (np.int32, 640, 1), (np.int64, 384, 2)])
 def test_fw_triangular_wrong_side_detected(dt, n, wrong):
     """rbg_fw_dense_tri_* told the wrong side (tri = 2 for a lower triangular
