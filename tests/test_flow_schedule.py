@@ -113,8 +113,12 @@ def test_flow_tickets_cover_items_and_wait_backwards(n, elem, edge):
     sched, waits = _schedule(n, elem, edge)
     items = _all_items(n, elem, edge)
     assert len(sched) == len(items) and set(sched) == items
+    # round of an item: K for P(K), K + 1 for T(K); every wait names an
+    # earlier round, so no two items of a round ever wait for each other
+    rnd = [it[1] + (it[0] == 1) for it in sched]
     for x, dx in enumerate(_deps(n, elem, sched, waits, edge)):
         assert all(d < x for d in dx), (sched[x], [sched[d] for d in dx if d >= x])
+        assert all(rnd[d] < rnd[x] for d in dx), (sched[x], [sched[d] for d in dx if rnd[d] >= rnd[x]])
     # within each step's tiles, block K + 2's tile row comes first
     nb, nt, edge, lo = _layout(n, elem, edge)
     for K in range(nb - 2):
