@@ -87,14 +87,16 @@ class BatchedClosure:
     is found by the device precheck and closed again by the single-matrix
     path, like any component outside the fast domain.  ``persistent``: lower
     triangular components (tri 1) are closed by one persistent launch
-    instead of one panel and tile launches per pivot step (the same results;
-    on MI355X the 4096-component workload takes 411 ms that way against
-    380 ms per step: every work item pays a few us of decode, polling and
-    publication, and the batch's panels are many small items).
+    instead of one panel and tile launches per pivot step (the same results);
+    True for all of them, or a padded order from which on (the others keep
+    the per-step launches).  On MI355X the 4096-component workload takes
+    408 ms all persistent against 376 ms per step: every work item pays a few
+    us of decode, polling and publication, and the per-step launches of a
+    mixed batch fill each other's tails.
     """
 
     def __init__(self, sizes: Sequence[int], tiles: Optional[Dict[int, int]] = None, stream=None, tri=0,
-                 persistent: bool = False):
+                 persistent=False):
         t = _lib.torch()
         dev = _lib.device()
         count = len(sizes)
@@ -102,7 +104,12 @@ class BatchedClosure:
         if np.any((tri < 0) | (tri > 2)):
             raise ValueError("tri must be 0, 1 or 2")
         self.tri = tri.copy()
-        self.layout = BatchLayout.build(sizes, (self.tri == 1) & bool(persistent))
+        npad = np.array([padded_order(int(n)) for n in sizes], dtype=np.int64)
+        if isinstance(persistent, (bool, np.bool_)):
+            pers = np.full(count, bool(persistent))
+        else:
+            pers = npad >= int(persistent)
+        self.layout = BatchLayout.build(sizes, (self.tri == 1) & pers)
         self.count = len(self.layout.sizes)
         if tiles is None:
             from .autotune import tile_plan
