@@ -99,7 +99,9 @@ uint64_t rbg_host_avos_product_u64(int64_t a, int64_t b, uint32_t *events);
 /* Dense closure in place on an n x n row-major matrix with leading dimension ld.
  * n must be a multiple of 128 (the facade pads with zero rows/columns, which the
  * semiring ignores).  mode: 0 = fast (values must pass rbg_fw_precheck_*),
- * 1 = general (any input).  On return st->diameter_max holds the max trajectory
+ * 1 = general (any input), 2 = fast on a matrix rbg_fw_load_* wrote (already
+ * in the fast kernels' encoding; st still holding the load's flags, which the
+ * closure checks against tri and clears).  On return st->diameter_max holds the max trajectory
  * value, st->cycle_* the first/last cycle detections, st->ov_* overflow events,
  * and st->flags & RBG_F_NEED_GENERAL asks for a general-mode rerun on the
  * original input.  rbg_fw_workspace_bytes is the workspace rbg_fw_dense_* needs. */
@@ -155,6 +157,18 @@ int rbg_prof_read(int32_t kind, double *total_ms, int64_t *launches);
  * of the small-matrix and triangular schedules), 0 for the other kinds. */
 int rbg_prof_read_work(int32_t kind, double *total_ms, int64_t *launches, double *work);
 int rbg_fw_prof_read(double *panel_ms, int64_t *panel_launches, double *tile_ms, int64_t *tile_launches);
+/* No counterpart in the reference (the fast path's copy-in, one pass over the
+ * input where a copy, rbg_fw_precheck_* and the fast closure's encoding took
+ * three): W (npad x npad, row stride npad, 16-byte aligned, npad a multiple
+ * of 4) = A (n x n, row stride lda) zero-padded, in the fast kernels'
+ * encoding, with rbg_fw_precheck_*'s flags in st and the largest input value
+ * in st->aux[7].  Without RBG_F_PRECOND_FAIL, close W with mode 2 and the
+ * schedule the flags allow; with it, W is unspecified (copy A again and run
+ * mode 1). */
+int rbg_fw_load_i32(const int32_t *A, int64_t n, int64_t lda, int32_t *W, int64_t npad, rbg_status *st,
+                    rbg_stream_t stream);
+int rbg_fw_load_i64(const int64_t *A, int64_t n, int64_t lda, int64_t *W, int64_t npad, rbg_status *st,
+                    rbg_stream_t stream);
 int rbg_fw_precheck_i32(const int32_t *W, int64_t n, int64_t ld, rbg_status *st, rbg_stream_t stream);
 int rbg_fw_precheck_i64(const int64_t *W, int64_t n, int64_t ld, rbg_status *st, rbg_stream_t stream);
 int rbg_fw_dense_i32(int32_t *W, int64_t n, int64_t ld, int32_t width, int32_t is_unsigned, int32_t mode,

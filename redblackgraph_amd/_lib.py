@@ -66,6 +66,8 @@ SIGNATURES = {
                                ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]),
     "rbg_fw_precheck_i32": (I32, [P, I64, I64, P, P]),
     "rbg_fw_precheck_i64": (I32, [P, I64, I64, P, P]),
+    "rbg_fw_load_i32": (I32, [P, I64, I64, P, I64, P, P]),
+    "rbg_fw_load_i64": (I32, [P, I64, I64, P, I64, P, P]),
     "rbg_fw_dense_i32": (I32, [P, I64, I64, I32, I32, I32, P, I64, P, P]),
     "rbg_fw_dense_i64": (I32, [P, I64, I64, I32, I32, I32, P, I64, P, P]),
     "rbg_fw_dense_tri_i32": (I32, [P, I64, I64, I32, I32, I32, I32, P, I64, P, P]),

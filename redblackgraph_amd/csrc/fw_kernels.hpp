@@ -160,6 +160,81 @@ __global__ void k_fw_precheck(const T* W, int64_t n, int64_t ld, rbg_status* st)
   if ((threadIdx.x & 63) == 0 && f) atomicOr(&st->flags, f);
 }
 
+// The fast path's copy-in, one pass: W (npad x npad, zero padding) = the
+// E-domain encoding of A (n x n, row stride lda), with k_fw_precheck's flags
+// and the largest input value in aux[FW_AUX_INIT] (the triangular schedules
+// count it in the trajectory maximum: k_fw_take_initial).  Workgroups take
+// rows; each thread four consecutive columns per pass.
+constexpr int FW_AUX_INIT = 7;
+template <typename T>
+__global__ __launch_bounds__(NT) void k_fw_load(const T* A, int64_t n, int64_t lda, T* W, int64_t npad,
+                                                 rbg_status* st) {
+  bool bad = false, lower = false, upper = false;
+  int64_t vmax = 0;
+  const bool vec = (lda & 3) == 0 && (n & 3) == 0 && ((uintptr_t)A & (4 * sizeof(T) - 1)) == 0;
+  for (int64_t i = blockIdx.x; i < npad; i += gridDim.x) {
+    for (int64_t j0 = 4 * (int64_t)threadIdx.x; j0 < npad; j0 += 4 * NT) {
+      T v[4] = {0, 0, 0, 0};
+      if (i < n && j0 < n) {
+        if (vec) {
+          if constexpr (sizeof(T) == 4) {
+            const uint4 q = *(const uint4*)(A + i * lda + j0);
+            v[0] = (T)q.x, v[1] = (T)q.y, v[2] = (T)q.z, v[3] = (T)q.w;
+          } else {
+            const ulonglong2 q0 = *(const ulonglong2*)(A + i * lda + j0);
+            const ulonglong2 q1 = *(const ulonglong2*)(A + i * lda + j0 + 2);
+            v[0] = (T)q0.x, v[1] = (T)q0.y, v[2] = (T)q1.x, v[3] = (T)q1.y;
+          }
+        } else {
+#pragma unroll
+          for (int c = 0; c < 4; c++)
+            if (j0 + c < n) v[c] = A[i * lda + j0 + c];
+        }
+      }
+      T o[4];
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        const int64_t j = j0 + c;
+        const T x = v[c];
+        if (x < (T)-1) bad = true;
+        if (i != j && (x == (T)1 || x == (T)-1)) bad = true;
+        if (i == j && x > (T)1) bad = true;
+        if (x != 0 && i > j) lower = true;
+        if (x != 0 && i < j) upper = true;
+        vmax = (int64_t)x > vmax ? (int64_t)x : vmax;
+        o[c] = e_enc<T>(x);
+      }
+      if constexpr (sizeof(T) == 4) {
+        *(uint4*)(W + i * npad + j0) = make_uint4((uint32_t)o[0], (uint32_t)o[1], (uint32_t)o[2], (uint32_t)o[3]);
+      } else {
+        *(ulonglong2*)(W + i * npad + j0) = make_ulonglong2((uint64_t)o[0], (uint64_t)o[1]);
+        *(ulonglong2*)(W + i * npad + j0 + 2) = make_ulonglong2((uint64_t)o[2], (uint64_t)o[3]);
+      }
+    }
+  }
+  const uint32_t f = (__any(bad) ? (uint32_t)RBG_F_PRECOND_FAIL : 0u) | (__any(lower) ? (uint32_t)RBG_F_HAS_LOWER : 0u) |
+                     (__any(upper) ? (uint32_t)RBG_F_HAS_UPPER : 0u);
+  vmax = wave_max_i64(vmax);
+  if ((threadIdx.x & 63) == 0) {
+    if (f) atomicOr(&st->flags, f);
+    if (vmax > 0) atomicMax((long long*)&st->aux[FW_AUX_INIT], (long long)vmax);
+  }
+}
+
+// A closure of a matrix k_fw_load wrote (mode 2): the side the caller chose
+// must match the load's flags (else general mode is asked for, as a wrong
+// side in k_e_convert); a triangular schedule takes the initial maximum.
+// The load's side flags and the maximum are then cleared, leaving the status
+// as the closure of a plain copy would.
+static __global__ void k_fw_take_initial(rbg_status* st, int tri) {
+  const uint32_t f = st->flags;
+  if ((tri == 1 && (f & RBG_F_HAS_UPPER)) || (tri == 2 && (f & RBG_F_HAS_LOWER)))
+    atomicOr(&st->flags, (uint32_t)RBG_F_NEED_GENERAL);
+  if (tri) record_diameter(st, st->aux[FW_AUX_INIT]);
+  atomicAnd(&st->flags, ~(uint32_t)(RBG_F_HAS_LOWER | RBG_F_HAS_UPPER));
+  st->aux[FW_AUX_INIT] = 0;
+}
+
 // E-domain conversion in place.  With st non-null (the triangular schedule)
 // the largest initial value also enters the trajectory maximum: the naive
 // loop writes every cell at k = 0, and on a triangular input with unit

@@ -27,9 +27,12 @@ constexpr bool FLOW = true;
 template <typename T, bool PROF = true>
 static int fw_dense(T* W, int64_t n, int64_t ld, int width, int is_uns, int mode, int tri, void* wsp,
                     int64_t ws_bytes, rbg_status* st, hipStream_t s) {
-  if (!W || !wsp || !st || n <= 0 || n % TILE || ld < n || tri < 0 || tri > 3) return RBG_E_ARG;
+  if (!W || !wsp || !st || n <= 0 || n % TILE || ld < n || tri < 0 || tri > 3 || mode < 0 || mode > 2)
+    return RBG_E_ARG;
   if (ws_bytes < FwWork::bytes(n, (int)sizeof(T))) return RBG_E_WORKSPACE;
-  const bool fast = mode == 0;
+  // mode 2: fast, W written by rbg_fw_load_* (already in the E domain)
+  const bool loaded = mode == 2;
+  const bool fast = mode == 0 || loaded;
   if (fast && (width != (int)sizeof(T) * 8 || is_uns)) return RBG_E_ARG;
   const bool per_step = tri == 3;
   if (per_step) tri = 1;
@@ -44,7 +47,10 @@ static int fw_dense(T* W, int64_t n, int64_t ld, int width, int is_uns, int mode
   const int nb = (int)(n / BLK);
   const int nt = (int)(n / TILE);
   const int conv_grid = (int)((n * n + NT * 8 - 1) / (NT * 8)) < 4096 ? (int)((n * n + NT * 8 - 1) / (NT * 8)) : 4096;
-  if (fast) {
+  if (loaded) {
+    hipLaunchKernelGGL(k_fw_take_initial, dim3(1), dim3(1), 0, s, st, tri);
+    RBG_CHECK_LAUNCH();
+  } else if (fast) {
     hipLaunchKernelGGL((k_e_convert<T, true>), dim3(conv_grid), dim3(NT), 0, s, W, n, ld, tri ? st : nullptr, tri);
     RBG_CHECK_LAUNCH();
   }
@@ -170,6 +176,15 @@ static int fw_dense(T* W, int64_t n, int64_t ld, int width, int is_uns, int mode
     hipLaunchKernelGGL((k_fw_capture_last<T, false>), dim3(1), dim3(1), 0, s, st, ws, (int64_t)(nb - 1), n);
     RBG_CHECK_LAUNCH();
   }
+  return 0;
+}
+
+template <typename T>
+static int fw_load(const T* A, int64_t n, int64_t lda, T* W, int64_t npad, rbg_status* st, hipStream_t s) {
+  if (!A || !W || !st || n <= 0 || lda < n || npad < n || npad % 4 || ((uintptr_t)W & 15)) return RBG_E_ARG;
+  const int grid = (int)(npad < 4096 ? npad : 4096);
+  hipLaunchKernelGGL((k_fw_load<T>), dim3(grid), dim3(NT), 0, s, A, n, lda, W, npad, st);
+  RBG_CHECK_LAUNCH();
   return 0;
 }
 
@@ -333,6 +348,14 @@ int rbg_fw_prof_read(double* panel_ms, int64_t* panel_launches, double* tile_ms,
   return rbg_prof_read(1, tile_ms, tile_launches);
 }
 
+int rbg_fw_load_i32(const int32_t* A, int64_t n, int64_t lda, int32_t* W, int64_t npad, rbg_status* st,
+                    rbg_stream_t stream) {
+  return fw_load<int32_t>(A, n, lda, W, npad, st, (hipStream_t)stream);
+}
+int rbg_fw_load_i64(const int64_t* A, int64_t n, int64_t lda, int64_t* W, int64_t npad, rbg_status* st,
+                    rbg_stream_t stream) {
+  return fw_load<int64_t>(A, n, lda, W, npad, st, (hipStream_t)stream);
+}
 int rbg_fw_precheck_i32(const int32_t* W, int64_t n, int64_t ld, rbg_status* st, rbg_stream_t stream) {
   return fw_precheck<int32_t>(W, n, ld, st, (hipStream_t)stream);
 }

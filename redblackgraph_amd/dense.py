@@ -102,26 +102,29 @@ def fw_closure(W, et: ElemType, stream=None, force_general: bool = False, schedu
         w[:n, :n] = W
         return w
 
-    work = fresh()
     st = status()
     st.reset(stream)
     fast = et.native and not force_general
     sched = "full"
     if fast:
-        fn = "rbg_fw_precheck_i32" if work.dtype == t.int32 else "rbg_fw_precheck_i64"
-        _lib.call(fn, _lib.ptr(work), npad, npad, st.p, _lib.stream_handle(stream))
+        # one pass: copy, fast-domain check and encoding (rbg_fw_load_*)
+        src = W if W.stride(1) == 1 else W.contiguous()
+        work = t.empty((npad, npad), dtype=W.dtype, device=W.device)
+        fn = "rbg_fw_load_i32" if work.dtype == t.int32 else "rbg_fw_load_i64"
+        _lib.call(fn, _lib.ptr(src), n, src.stride(0), _lib.ptr(work), npad, st.p, _lib.stream_handle(stream))
         s = st.read(stream)
         if s.flags & _lib.F_PRECOND_FAIL:
             fast = False
+            st.reset(stream)
         elif schedule in ("auto", "steps"):
             if not s.flags & _lib.F_HAS_UPPER:
                 sched = "lower" if schedule == "auto" else "lower-steps"
             elif not s.flags & _lib.F_HAS_LOWER:
                 sched = "upper"
-        st.reset(stream)
     if not fast:
+        work = fresh()
         sched = "full"
-    _run(work, et, 0 if fast else 1, st, stream, _TRI[sched])
+    _run(work, et, 2 if fast else 1, st, stream, _TRI[sched])
     s = st.read(stream)
     if fast and s.flags & _lib.F_STALLED:
         # a wait of the persistent closure ran out: the same schedule, one

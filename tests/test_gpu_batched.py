@@ -197,11 +197,13 @@ def test_batched_persistent_equals_per_step_launches(orc):
     tri = [0 if i % 3 == 0 else 1 for i in range(len(mats))]
     tiles = {p: (64 if (p // 128) % 2 else 128) for p in range(128, 4097, 128)}
     runs = {}
-    for persistent in (True, False):
+    for persistent in (True, False, 512):
         b = BatchedClosure([A.shape[0] for A in mats], tiles=tiles, tri=tri, persistent=persistent)
         try:
             info = b.info()
-            assert info["persistent_components"] == (sum(tri) if persistent else 0)
+            npad = [dense.padded_order(A.shape[0]) for A in mats]
+            want = {True: sum(tri), False: 0, 512: sum(t_ and p >= 512 for t_, p in zip(tri, npad))}[persistent]
+            assert info["persistent_components"] == want
             b.load_dense(mats)
             b.run(graph=False)
             b.stream.synchronize()
@@ -218,7 +220,8 @@ def test_batched_persistent_equals_per_step_launches(orc):
         finally:
             b.close()
     (pool_p, st_p, lay), (pool_s, st_s, _) = runs[True], runs[False]
-    assert st_p == st_s
+    assert st_p == st_s == runs[512][1]
+    assert bool(t.equal(runs[512][0], pool_s))
     assert all(f == 0 for f, _, _, _ in st_p)
     assert bool(t.equal(pool_p, pool_s))
     for i in (1, 2, 4, 5, 7):

@@ -257,6 +257,52 @@ def test_fw_persistent_closure_vs_steps_and_oracle(orc, dt, n):
         assert out.diameter == outs.diameter
 
 
+@pytest.mark.parametrize("dt,n,lda,bad", [(np.int32, 300, 300, False), (np.int32, 261, 300, False),
+                                          (np.int64, 130, 133, False), (np.int32, 200, 200, True)])
+def test_fw_load_equals_copy_precheck_and_encoding(dt, n, lda, bad):
+    """rbg_fw_load_* (the fast path's one-pass copy-in) against what it
+    replaces: a zero-padded copy, rbg_fw_precheck_*'s flags, the encoding
+    (absent -> all ones, -1 -> 0, v >= 1 -> v) and the largest input value in
+    aux[7]; then a closure through it against one through a plain copy.  Row
+    strides wider than n and orders that are not multiples of four take the
+    scalar loads."""
+    t = _lib.torch()
+    A = synth.dense_pedigree(synth.component_spec(n, 0.8, 4), 7 + n, dtype=dt)
+    if bad:
+        A = A.copy()
+        A[5, 3] = 1  # an off-diagonal 1: outside the fast domain
+    big = np.zeros((n, lda), dtype=dt)
+    big[:, :n] = A
+    Bd = _lib.to_device(big)[:, :n]  # a view with row stride lda
+    npad = dense.padded_order(n)
+    sfx = "i32" if dt == np.int32 else "i64"
+    st = dense.status()
+    st.reset()
+    Wl = t.empty((npad, npad), dtype=Bd.dtype, device=Bd.device)
+    _lib.call(f"rbg_fw_load_{sfx}", _lib.ptr(Bd), n, lda, _lib.ptr(Wl), npad, st.p, _lib.stream_handle(None))
+    s1 = st.read()
+    P = t.zeros((npad, npad), dtype=Bd.dtype, device=Bd.device)
+    P[:n, :n] = _lib.to_device(A)
+    st.reset()
+    _lib.call(f"rbg_fw_precheck_{sfx}", _lib.ptr(P), npad, npad, st.p, _lib.stream_handle(None))
+    s2 = st.read()
+    assert s1.flags == s2.flags
+    assert bool(s1.flags & _lib.F_PRECOND_FAIL) == bad
+    assert s1.aux[7] == int(A.max())
+    if bad:
+        return
+    # the encoding: absent (0) -> all ones, -1 -> 0, v >= 1 -> v
+    enc = P.clone()
+    enc[P == 0] = -1
+    enc[P == -1] = 0
+    assert bool(t.equal(Wl, enc))
+    # a closure through the fused load equals one through a plain copy
+    W1, o1 = dense.fw_closure(Bd, elem_type(A.dtype))
+    W2, o2 = dense.fw_closure(_lib.to_device(A), elem_type(A.dtype))
+    assert bool(t.equal(W1, W2)) and o1.status.diameter_max == o2.status.diameter_max
+    assert o1.status.flags == 0 and o1.status.aux[7] == 0
+
+
 @pytest.mark.parametrize("dt,n", [(np.int32, 2100), (np.int64, 900)])
 def test_fw_persistent_wait_limit_reruns_steps(dt, n):
     """A wait of the persistent closure that runs out, forced from the host
