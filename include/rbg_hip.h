@@ -417,7 +417,7 @@ int rbg_csr_add_i64(const int64_t *ap, const int32_t *aj, const int64_t *ax, con
  * Replaces closing components one at a time (sparse/csgraph/
  * transitive_closure.py:85-195 component_wise_closure, each through
  * warshall.c.in:50-107).  Every component is an int32 matrix of padded order
- * npad (multiple of 128, rows contiguous, zero padding), closed in place with
+ * npad (multiple of 128, rows contiguous, zero padding, 16-byte aligned), closed in place with
  * the fast kernels' arithmetic; `snap` is its snapshot workspace of
  * `snap_bytes` bytes (at least rbg_batch_snap_bytes(npad)), `st` its status,
  * `tile` the phase-3 tile edge (128 or 64) for its size bucket.  Lower

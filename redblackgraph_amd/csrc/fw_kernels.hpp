@@ -221,6 +221,29 @@ __global__ __launch_bounds__(NT) void k_fw_load(const T* A, int64_t n, int64_t l
   }
 }
 
+// The fast path's decode in place (every cell of the n x n matrix): rows per
+// workgroup, four consecutive cells per thread and pass (ld and W 16-byte
+// aligned; k_e_convert otherwise).
+template <typename T>
+__global__ __launch_bounds__(NT) void k_e_decode4(T* W, int64_t n, int64_t ld) {
+  for (int64_t i = blockIdx.x; i < n; i += gridDim.x)
+    for (int64_t j0 = 4 * (int64_t)threadIdx.x; j0 < n; j0 += 4 * NT) {
+      T* p = W + i * ld + j0;
+      if constexpr (sizeof(T) == 4) {
+        uint4 q = *(const uint4*)p;
+        q.x = (uint32_t)e_dec<T>((T)q.x), q.y = (uint32_t)e_dec<T>((T)q.y);
+        q.z = (uint32_t)e_dec<T>((T)q.z), q.w = (uint32_t)e_dec<T>((T)q.w);
+        *(uint4*)p = q;
+      } else {
+        ulonglong2 a = *(const ulonglong2*)p, b = *(const ulonglong2*)(p + 2);
+        a.x = (uint64_t)e_dec<T>((T)a.x), a.y = (uint64_t)e_dec<T>((T)a.y);
+        b.x = (uint64_t)e_dec<T>((T)b.x), b.y = (uint64_t)e_dec<T>((T)b.y);
+        *(ulonglong2*)p = a;
+        *(ulonglong2*)(p + 2) = b;
+      }
+    }
+}
+
 // A closure of a matrix k_fw_load wrote (mode 2): the side the caller chose
 // must match the load's flags (else general mode is asked for, as a wrong
 // side in k_e_convert); a triangular schedule takes the initial maximum.

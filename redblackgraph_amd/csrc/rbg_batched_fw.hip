@@ -146,34 +146,12 @@ static __global__ void k_bfw_init(const BComp* comps, int32_t count) {
   for (int i = 0; i < 8; i++) st->aux[i] = 0;
 }
 
-// Fast-domain precheck of one 128 x 128 chunk (rules of k_fw_precheck).
-static __global__ __launch_bounds__(NT) void k_bfw_precheck(const BComp* comps, const Item* items) {
-  __shared__ int sbad;
-  const Item it = items[blockIdx.x];
-  const BComp c = comps[it.comp];
-  const int nc = c.npad / CHUNK;
-  const int64_t i0 = (int64_t)(it.bid / nc) * CHUNK, j0 = (int64_t)(it.bid % nc) * CHUNK;
-  if (threadIdx.x == 0) sbad = 0;
-  __syncthreads();
-  bool bad = false;
-  for (int e = threadIdx.x; e < CHUNK * CHUNK; e += NT) {
-    const int64_t i = i0 + e / CHUNK, j = j0 + e % CHUNK;
-    const int32_t v = c.W[i * c.npad + j];
-    if (v < -1) bad = true;
-    if (i != j && (v == 1 || v == -1)) bad = true;
-    if (i == j && v > 1) bad = true;
-    // a component declared triangular must be: its schedule skips the rest
-    if (v != 0 && ((c.tri == 1 && i < j) || (c.tri == 2 && i > j))) bad = true;
-  }
-  if (bad) sbad = 1;
-  __syncthreads();
-  if (threadIdx.x == 0 && sbad) atomicOr(&c.st->flags, (uint32_t)RBG_F_PRECOND_FAIL);
-}
-
-// Domain conversion of one 128 x 128 chunk.  Entering the E domain, a
-// triangular component's initial values also enter its trajectory maximum
-// (the triangular schedule never visits the tiles its pivot blocks cannot
-// change; see k_e_convert).
+// Domain conversion of one 128 x 128 chunk, four cells per thread and pass.
+// Entering the E domain it also checks the fast domain (rules of
+// k_fw_precheck, and a component declared triangular must be: its schedule
+// skips the rest), and a triangular component's initial values enter its
+// trajectory maximum (the triangular schedule never visits the tiles its
+// pivot blocks cannot change; see k_e_convert).
 template <bool ENC>
 static __global__ __launch_bounds__(NT) void k_bfw_convert(const BComp* comps, const Item* items) {
   const Item it = items[blockIdx.x];
@@ -181,15 +159,32 @@ static __global__ __launch_bounds__(NT) void k_bfw_convert(const BComp* comps, c
   const int nc = c.npad / CHUNK;
   const int64_t i0 = (int64_t)(it.bid / nc) * CHUNK, j0 = (int64_t)(it.bid % nc) * CHUNK;
   int64_t vmax = 0;
-  for (int e = threadIdx.x; e < CHUNK * CHUNK; e += NT) {
-    int32_t* p = c.W + (i0 + e / CHUNK) * c.npad + j0 + e % CHUNK;
-    const int32_t v = *p;
-    *p = ENC ? e_enc<int32_t>(v) : e_dec<int32_t>(v);
-    vmax = v > vmax ? v : vmax;
+  bool bad = false;
+  for (int e = 4 * threadIdx.x; e < CHUNK * CHUNK; e += 4 * NT) {
+    const int64_t i = i0 + e / CHUNK, jb = j0 + e % CHUNK;
+    uint4* p = (uint4*)(c.W + i * c.npad + jb);
+    const uint4 q = *p;
+    int32_t v[4] = {(int32_t)q.x, (int32_t)q.y, (int32_t)q.z, (int32_t)q.w};
+#pragma unroll
+    for (int x = 0; x < 4; x++) {
+      const int64_t j = jb + x;
+      if (ENC) {
+        if (v[x] < -1) bad = true;
+        if (i != j && (v[x] == 1 || v[x] == -1)) bad = true;
+        if (i == j && v[x] > 1) bad = true;
+        if (v[x] != 0 && ((c.tri == 1 && i < j) || (c.tri == 2 && i > j))) bad = true;
+        vmax = v[x] > vmax ? v[x] : vmax;
+      }
+      v[x] = ENC ? e_enc<int32_t>(v[x]) : e_dec<int32_t>(v[x]);
+    }
+    *p = make_uint4((uint32_t)v[0], (uint32_t)v[1], (uint32_t)v[2], (uint32_t)v[3]);
   }
-  if (ENC && c.tri) {
-    vmax = wave_max_i64(vmax);
-    if ((threadIdx.x & 63) == 0) record_diameter(c.st, vmax);
+  if (ENC) {
+    if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(&c.st->flags, (uint32_t)RBG_F_PRECOND_FAIL);
+    if (c.tri) {
+      vmax = wave_max_i64(vmax);
+      if ((threadIdx.x & 63) == 0) record_diameter(c.st, vmax);
+    }
   }
 }
 
@@ -315,8 +310,6 @@ static int batch_launch(rbg_batch* b, hipStream_t s) {
   const Item* t64 = t128 + b->n_t128;
   const Item* tri128 = t64 + b->n_t64;
   const Item* tri64 = tri128 + b->n_tri128;
-  hipLaunchKernelGGL(k_bfw_precheck, dim3(b->n_chunk), dim3(NT), 0, s, b->d_comps, chunks);
-  RBG_CHECK_LAUNCH();
   hipLaunchKernelGGL((k_bfw_convert<true>), dim3(b->n_chunk), dim3(NT), 0, s, b->d_comps, chunks);
   RBG_CHECK_LAUNCH();
   if (b->n_flow) {
@@ -407,7 +400,7 @@ int rbg_batch_create(const rbg_batch_comp* comps, int32_t count, rbg_batch** out
   std::vector<BComp> c(count);
   for (int i = 0; i < count; i++) {
     const rbg_batch_comp& q = comps[i];
-    if (!q.W || !q.snap || !q.st || q.npad < CHUNK || q.npad % CHUNK) return RBG_E_ARG;
+    if (!q.W || !q.snap || !q.st || q.npad < CHUNK || q.npad % CHUNK || ((uintptr_t)q.W & 15)) return RBG_E_ARG;
     if (q.tile != 128 && q.tile != 64) return RBG_E_ARG;
     if (q.tri < 0 || q.tri > 2) return RBG_E_ARG;
     if (q.snap_bytes < FwWork::bytes(q.npad, 4)) return RBG_E_WORKSPACE;
@@ -532,7 +525,7 @@ int rbg_batch_create(const rbg_batch_comp* comps, int32_t count, rbg_batch** out
 
 int rbg_batch_info(const rbg_batch* b, int64_t* info) {
   if (!b || !info) return RBG_E_ARG;
-  int64_t launches = 5 + (b->n_flow ? 3 : 0);
+  int64_t launches = 4 + (b->n_flow ? 3 : 0);
   for (int K = 0; K < b->stepnb; K++)
     launches += (b->panel_cnt[K] > 0) + (b->t128_cnt[K] > 0) + (b->t64_cnt[K] > 0) +
                 (b->tri128_off[K + 1] > b->tri128_off[K]) + (b->tri64_off[K + 1] > b->tri64_off[K]);

@@ -170,7 +170,10 @@ static int fw_dense(T* W, int64_t n, int64_t ld, int width, int is_uns, int mode
   if (fast) {
     hipLaunchKernelGGL((k_fw_capture_last<T, true>), dim3(1), dim3(1), 0, s, st, ws, (int64_t)(nb - 1), n);
     RBG_CHECK_LAUNCH();
-    hipLaunchKernelGGL((k_e_convert<T, false>), dim3(conv_grid), dim3(NT), 0, s, W, n, ld, nullptr);
+    if (ld % 4 == 0 && ((uintptr_t)W & 15) == 0)
+      hipLaunchKernelGGL((k_e_decode4<T>), dim3((unsigned)(n < 4096 ? n : 4096)), dim3(NT), 0, s, W, n, ld);
+    else
+      hipLaunchKernelGGL((k_e_convert<T, false>), dim3(conv_grid), dim3(NT), 0, s, W, n, ld, nullptr);
     RBG_CHECK_LAUNCH();
   } else {
     hipLaunchKernelGGL((k_fw_capture_last<T, false>), dim3(1), dim3(1), 0, s, st, ws, (int64_t)(nb - 1), n);
