@@ -422,7 +422,9 @@ def bench_batched(args):
     t = _lib.torch()
     sizes, diag, edges = synth.batched_workload(count=args.batch_count, seed=args.seed)
     t_tune0 = time.perf_counter()
-    b = BatchedClosure(sizes, tri=1)  # pedigrees numbered parents first: lower triangular
+    # pedigrees numbered parents first: lower triangular; the schedule is the
+    # autotuner's choice for this mix of orders
+    b = BatchedClosure(sizes, tri=1, persistent="auto")
     tune_s = time.perf_counter() - t_tune0
     b.load_edges(diag, edges)
     pristine = b.pool.clone()
@@ -498,7 +500,8 @@ def bench_batched(args):
                    "identical_to_single_matrix_path": identical_single,
                    "launches_per_batch": info["launches"], "pivot_steps": info["steps"],
                    "persistent_items": info["persistent_items"],
-                   "tiles": {str(k): v for k, v in sorted(b.tiles.items())}, "autotune_s": tune_s},
+                   "tiles": {str(k): v for k, v in sorted(b.tiles.items())}, "schedule": b.schedule,
+                   "autotune_s": tune_s},
         "roofline": roof,
     }
     res["config"]["persistent"] = _batched_persistent_leg(args, sizes, pristine, eager_result, b.stream)

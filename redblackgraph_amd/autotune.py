@@ -1,4 +1,4 @@
-"""Tile-shape autotuner for the batched dense closure.
+"""Tile-shape and schedule autotuner for the batched dense closure.
 
 For each size bucket (padded order, a multiple of 128) the batched launcher
 can run phase 3 with 128 x 128 tiles (8 x 8 cells per thread) or 64 x 64
@@ -8,12 +8,21 @@ workgroups per pivot block.  The tuner closes a batch of seeded synthetic
 pedigrees of the bucket's order with each shape, times it with HIP events,
 and keeps the faster one.
 
+For a batch of lower triangular components it also picks the schedule:
+the per-step launches or the persistent launch (BatchedClosure's
+``persistent``).  Which one is faster depends on the batch's mix of orders
+-- the per-step launches of many orders fill each other's tails, the
+persistent launch wins on batches of large components -- so the tuner times
+both on synthetic pedigrees of the batch's padded orders, keyed by their
+histogram.
+
 Results are cached as JSON, keyed by device name and bucket, in
 $RBG_AUTOTUNE_CACHE (default ~/.cache/redblackgraph_amd/autotune.json); a
 table shipped with the package (autotune_defaults.json) seeds the cache.
 """
 from __future__ import annotations
 
+import hashlib
 import json
 import os
 import threading
@@ -127,3 +136,62 @@ def tile_plan(npads: Iterable[int], measure: bool = True, retune: bool = False, 
         e = table.get(_key(dev, p))
         plan[p] = int(e["tile"]) if e else 128
     return plan
+
+
+SCHEDULES = ("steps", "persistent")
+
+
+def _schedule_key(dev: str, npads: Iterable[int]) -> str:
+    u, c = np.unique(np.asarray(list(npads), dtype=np.int64), return_counts=True)
+    digest = hashlib.sha1(",".join(f"{a}:{b}" for a, b in zip(u, c)).encode()).hexdigest()[:16]
+    return f"{dev}|batched_fw_schedule|{digest}"
+
+
+def measure_schedule(npads: Iterable[int], tiles: Dict[int, int], reps: int = 2, seed: int = 7) -> Dict[str, float]:
+    """Milliseconds per eager run of a batch of lower triangular synthetic
+    pedigrees with these padded orders, per schedule."""
+    from . import synth
+    from .batched import BatchedClosure
+    t = _lib.torch()
+    npads = [int(p) for p in npads]
+    pool = {p: [synth.component_edges(synth.component_spec(p), seed * 1009 + i) for i in range(4)]
+            for p in sorted(set(npads))}
+    diag = [pool[p][i % 4][0] for i, p in enumerate(npads)]
+    edges = [pool[p][i % 4][1] for i, p in enumerate(npads)]
+    out = {}
+    for name in SCHEDULES:
+        b = BatchedClosure(npads, tiles=tiles, tri=1, persistent=name == "persistent")
+        try:
+            b.load_edges(diag, edges)
+            b.run(graph=False)  # warm
+            ev0, ev1 = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
+            ev0.record(b.stream)
+            for _ in range(reps):
+                b.run(graph=False)
+            ev1.record(b.stream)
+            ev1.synchronize()
+            out[name] = ev0.elapsed_time(ev1) / reps
+        finally:
+            b.close()
+            del b
+    return out
+
+
+def schedule_plan(npads: Iterable[int], tiles: Dict[int, int], measure: bool = True, retune: bool = False,
+                  dev: str | None = None) -> str:
+    """"steps" or "persistent" for a batch of lower triangular components of
+    these padded orders on this device (or the device named ``dev``),
+    measuring an unknown mix when ``measure`` is set; "steps" when unknown."""
+    npads = [int(p) for p in npads]
+    dev = dev or device_key()
+    key = _schedule_key(dev, npads)
+    with _lock:
+        table = _load(DEFAULTS_PATH)
+        table.update(_load(cache_path()))
+        if (retune or key not in table) and measure:
+            ms = measure_schedule(npads, tiles)
+            table[key] = {"schedule": min(ms, key=ms.get), "ms": {k: round(v, 3) for k, v in ms.items()},
+                          "components": len(npads)}
+            _save(cache_path(), table)
+        e = table.get(key)
+    return e["schedule"] if e else "steps"

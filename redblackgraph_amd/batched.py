@@ -88,11 +88,13 @@ class BatchedClosure:
     path, like any component outside the fast domain.  ``persistent``: lower
     triangular components (tri 1) are closed by one persistent launch
     instead of one panel and tile launches per pivot step (the same results);
-    True for all of them, or a padded order from which on (the others keep
-    the per-step launches).  On MI355X the 4096-component workload takes
-    408 ms all persistent against 376 ms per step: every work item pays a few
-    us of decode, polling and publication, and the per-step launches of a
-    mixed batch fill each other's tails.
+    True for all of them, a padded order from which on (the others keep
+    the per-step launches), or "auto": the autotuner's choice for this mix
+    of orders (autotune.schedule_plan).  On MI355X the 4096-component
+    workload takes 396 ms all persistent against 364 ms per step (every work
+    item pays a few us of decode, polling and publication, and the per-step
+    launches of a mixed batch fill each other's tails); batches of large
+    components run 20-28% faster persistent.
     """
 
     def __init__(self, sizes: Sequence[int], tiles: Optional[Dict[int, int]] = None, stream=None, tri=0,
@@ -105,16 +107,24 @@ class BatchedClosure:
             raise ValueError("tri must be 0, 1 or 2")
         self.tri = tri.copy()
         npad = np.array([padded_order(int(n)) for n in sizes], dtype=np.int64)
-        if isinstance(persistent, (bool, np.bool_)):
-            pers = np.full(count, bool(persistent))
-        else:
-            pers = npad >= int(persistent)
-        self.layout = BatchLayout.build(sizes, (self.tri == 1) & pers)
-        self.count = len(self.layout.sizes)
         if tiles is None:
             from .autotune import tile_plan
-            tiles = tile_plan(sorted(set(int(p) for p in self.layout.npad)))
+            tiles = tile_plan(sorted(set(int(p) for p in npad)))
         self.tiles = {int(k): int(v) for k, v in tiles.items()}
+        if isinstance(persistent, str):
+            if persistent != "auto":
+                raise ValueError(f"unknown schedule {persistent!r}")
+            from .autotune import schedule_plan
+            lower = npad[self.tri == 1]
+            persistent = bool(lower.size) and schedule_plan(lower, self.tiles) == "persistent"
+        if isinstance(persistent, (bool, np.bool_)):
+            pers = np.full(count, bool(persistent))
+            self.schedule = "persistent" if persistent else "steps"
+        else:
+            pers = npad >= int(persistent)
+            self.schedule = f"persistent from order {int(persistent)}"
+        self.layout = BatchLayout.build(sizes, (self.tri == 1) & pers)
+        self.count = len(self.layout.sizes)
         self.stream = stream if stream is not None else t.cuda.Stream()
         self.pool = t.zeros(self.layout.pool_elems, dtype=t.int32, device=dev)
         self.snap = t.empty(max(self.layout.snap_bytes, _ALIGN), dtype=t.uint8, device=dev)

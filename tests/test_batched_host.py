@@ -99,6 +99,21 @@ def test_tile_plan_reads_cache(tmp_path, monkeypatch):
     assert plan == {256: 64, 512: 128}
 
 
+def test_schedule_plan_reads_cache(tmp_path, monkeypatch):
+    """The schedule choice is keyed by the histogram of padded orders (the
+    order of the components does not matter) and defaults to the per-step
+    launches when the mix is unknown and not measured."""
+    dev = "TestGPU|gfx950|256cu"
+    npads = [128, 4096, 128, 2048]
+    key = autotune._schedule_key(dev, npads)
+    assert key == autotune._schedule_key(dev, sorted(npads)) != autotune._schedule_key(dev, npads[:3])
+    cache = tmp_path / "tune.json"
+    cache.write_text(json.dumps({key: {"schedule": "persistent", "ms": {}}}))
+    monkeypatch.setenv("RBG_AUTOTUNE_CACHE", str(cache))
+    assert autotune.schedule_plan(npads, {}, measure=False, dev=dev) == "persistent"
+    assert autotune.schedule_plan([256, 256], {}, measure=False, dev=dev) == "steps"
+
+
 def test_shipped_tile_table(tmp_path, monkeypatch):
     """The table shipped for MI355X (gfx950, 256 CUs) covers every padded
     order up to 4096 and seeds the plan without measuring."""
@@ -111,6 +126,12 @@ def test_shipped_tile_table(tmp_path, monkeypatch):
     monkeypatch.setenv("RBG_AUTOTUNE_CACHE", str(tmp_path / "empty.json"))
     plan = autotune.tile_plan(range(128, 4097, 128), measure=False, dev=dev)
     assert set(plan.values()) <= set(autotune.TILES) and len(plan) == 32
+    # the bench workload's schedule ships too, so a fresh install measures nothing
+    from redblackgraph_amd import synth
+    from redblackgraph_amd.dense import padded_order
+    npads = [padded_order(int(n)) for n in synth.batched_workload(count=4096, seed=0)[0]]
+    e = table[autotune._schedule_key(dev, npads)]
+    assert e["schedule"] in autotune.SCHEDULES and e["schedule"] == min(e["ms"], key=e["ms"].get)
 
 
 def test_serving_warmup_trace_shapes():

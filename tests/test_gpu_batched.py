@@ -260,3 +260,30 @@ def test_batched_persistent_wait_limit_reruns():
             assert outs1[i].diameter == outs0[i].diameter, i
     finally:
         b.close()
+
+
+def test_batched_schedule_autotuned(tmp_path, monkeypatch):
+    """persistent="auto": the autotuner times both schedules on synthetic
+    pedigrees of the batch's padded orders, caches the faster one, and the
+    batch closed by it equals the per-step launches' closure."""
+    from redblackgraph_amd import autotune
+    t = _lib.torch()
+    monkeypatch.setenv("RBG_AUTOTUNE_CACHE", str(tmp_path / "tune.json"))
+    mats = _components(20, 100, 1300, 41)
+    b = BatchedClosure([A.shape[0] for A in mats], tri=1, persistent="auto")
+    s = BatchedClosure([A.shape[0] for A in mats], tri=1)
+    try:
+        assert b.schedule in autotune.SCHEDULES
+        assert b.info()["persistent_components"] == (len(mats) if b.schedule == "persistent" else 0)
+        key = autotune._schedule_key(autotune.device_key(), b.layout.npad)
+        import json
+        e = json.load(open(tmp_path / "tune.json"))[key]
+        assert e["schedule"] == b.schedule and set(e["ms"]) == set(autotune.SCHEDULES)
+        for x in (b, s):
+            x.load_dense(mats)
+            x.run(graph=False)
+            x.stream.synchronize()
+        assert bool(t.equal(b.pool, s.pool))
+    finally:
+        b.close()
+        s.close()
