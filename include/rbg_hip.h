@@ -422,7 +422,7 @@ int rbg_csr_add_i64(const int64_t *ap, const int32_t *aj, const int64_t *ax, con
  * `snap_bytes` bytes (at least rbg_batch_snap_bytes(npad)), `st` its status,
  * `tile` the phase-3 tile edge (128 or 64) for its size bucket.  Lower
  * triangular components (tri = 1) whose workspace holds
- * rbg_batch_tri_snap_bytes(npad) are closed together by one persistent launch
+ * rbg_batch_tri_snap_bytes(npad) and that ask for it (`persistent`) are closed together by one persistent launch
  * (the work items of rbg_fw_dense_tri_*'s persistent closure, every such
  * component's in one grid); the others by one panel and tile launches per
  * pivot step.  A wait of the persistent launch that runs out (never seen)
@@ -449,6 +449,8 @@ typedef struct {
   int32_t tri; /* 0: full schedule; 1 / 2: every off-diagonal entry lies below / above the diagonal,
                   and the component takes the triangular schedule of rbg_fw_dense_tri_* (a component
                   that is not triangular as declared is flagged RBG_F_PRECOND_FAIL) */
+  int32_t persistent; /* tri = 1 with rbg_batch_tri_snap_bytes: 1 = the persistent launch, 0 = the per-step
+                         launches with every diagonal block closed before the steps */
   int64_t snap_bytes;
 } rbg_batch_comp;
 int64_t rbg_batch_snap_bytes(int32_t npad);

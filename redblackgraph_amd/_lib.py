@@ -153,7 +153,7 @@ class RbgBatchComp(ctypes.Structure):
     """rbg_batch_comp: one component of a batched dense closure."""
     _fields_ = [("W", ctypes.c_void_p), ("snap", ctypes.c_void_p), ("st", ctypes.c_void_p),
                 ("npad", ctypes.c_int32), ("tile", ctypes.c_int32), ("tri", ctypes.c_int32),
-                ("snap_bytes", ctypes.c_int64)]
+                ("persistent", ctypes.c_int32), ("snap_bytes", ctypes.c_int64)]
 
 
 class LibraryMissing(ImportError):

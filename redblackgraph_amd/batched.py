@@ -123,7 +123,10 @@ class BatchedClosure:
         else:
             pers = npad >= int(persistent)
             self.schedule = f"persistent from order {int(persistent)}"
-        self.layout = BatchLayout.build(sizes, (self.tri == 1) & pers)
+        # lower triangular components get the persistent closure's workspace
+        # either way: the per-step launches use its diagonal snapshots
+        self.layout = BatchLayout.build(sizes, self.tri == 1)
+        self.persistent_mask = (self.tri == 1) & pers
         self.count = len(self.layout.sizes)
         self.stream = stream if stream is not None else t.cuda.Stream()
         self.pool = t.zeros(self.layout.pool_elems, dtype=t.int32, device=dev)
@@ -141,6 +144,7 @@ class BatchedClosure:
             descs[i].tile = self.tiles.get(p, 128)
             descs[i].tri = int(self.tri[i])
             descs[i].snap_bytes = int(self.layout.snap_size[i])
+            descs[i].persistent = int(self.persistent_mask[i])
         h = ctypes.c_void_p()
         _lib.call("rbg_batch_create", descs, self.count, ctypes.byref(h))
         self._h = h

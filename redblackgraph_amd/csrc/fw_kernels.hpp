@@ -707,10 +707,13 @@ struct LineRange {
   int64_t r0, r1, c0, c1;
 };
 
+// dsn (lower triangular components): block K's step snapshots, the diagonal
+// block already closed in W with its events and guard (k_fw_diag_walk); the
+// workgroup takes the snapshots instead of walking the block.
 template <typename T>
 __device__ __forceinline__ void fw_panel_lines_body(LinesShared<typename unsigned_of<T>::type>& L, T* W, int64_t n,
                                                     int64_t ld, int64_t K, FwWork ws, int width, rbg_status* st,
-                                                    int bid, LineRange lr) {
+                                                    int bid, LineRange lr, const T* dsn = nullptr) {
   using U = typename unsigned_of<T>::type;
   constexpr U TOP = (U)1 << (sizeof(U) * 8 - 1);
   auto& D = L.D;
@@ -742,7 +745,14 @@ __device__ __forceinline__ void fw_panel_lines_body(LinesShared<typename unsigne
   if (sbad) return;
   if (owner && tid == 0 && K > 0) capture_block<T, true>(st, ws.col<T>(ws.slot(K - 1)), ws.row<T>(ws.slot(K - 1)), k0 - BLK, n);
   const int64_t seen = st->diameter_max;  // a lower bound of the value at exit
-  for (int e = tid; e < BLK * BLK; e += NT) D[e / BLK][e % BLK] = canon((U)W[(k0 + e / BLK) * ld + k0 + e % BLK]);
+  if (dsn) {
+    for (int e = tid; e < BLK * BLK; e += NT) {
+      sC[e / BLK][e % BLK] = (U)dsn[e];
+      sR[e / BLK][e % BLK] = (U)dsn[BLK * BLK + e];
+    }
+  } else {
+    for (int e = tid; e < BLK * BLK; e += NT) D[e / BLK][e % BLK] = canon((U)W[(k0 + e / BLK) * ld + k0 + e % BLK]);
+  }
   // this thread's panel line, loaded while the diagonal block is walked
   const bool rowp = (int64_t)bid * NT + tid < n;
   const int64_t line = rowp ? (int64_t)bid * NT + tid : (int64_t)bid * NT + tid - n;
@@ -775,7 +785,7 @@ __device__ __forceinline__ void fw_panel_lines_body(LinesShared<typename unsigne
   for (int x = 0; x < 4; x++) dv[x] = D[i][j0 + x];
   T dmax = 0;
   bool bad = false;
-  for (int kk = 0; kk < BLK; kk++) {
+  for (int kk = 0; kk < (dsn ? 0 : BLK); kk++) {
     const U aD = canon(D[i][kk]);
     U bD[4];
 #pragma unroll
@@ -847,7 +857,7 @@ __device__ __forceinline__ void fw_panel_lines_body(LinesShared<typename unsigne
   if (owner) {
     for (int e = tid; e < BLK * BLK; e += NT) {
       const int r = e / BLK, cc = e % BLK;
-      W[(k0 + r) * ld + k0 + cc] = (T)D[r][cc];
+      if (!dsn) W[(k0 + r) * ld + k0 + cc] = (T)D[r][cc];
       colT[r * n + k0 + cc] = (T)sC[r][cc];
       rowS[r * n + k0 + cc] = (T)sR[r][cc];
     }

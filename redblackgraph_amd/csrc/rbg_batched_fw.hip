@@ -35,6 +35,9 @@ struct BComp {
   int32_t tile;
   int32_t tri;   // 0 full schedule; 1 / 2: entries declared below / above the diagonal only
   int32_t flow;  // 1: closed by the persistent launch (snapshot ring of four)
+  // lower triangular, per-step launches: every diagonal block closed before
+  // the steps (k_bfw_prewalk), its step snapshots here for the panels
+  void* dsnap;
 };
 
 // A component of the persistent launch: its matrix, snapshot workspace (with
@@ -205,7 +208,17 @@ static __global__ __launch_bounds__(NT) void k_bfw_panel(const BComp* comps, con
     else
       lr = LineRange{(int64_t)r.j0 * e, n, 0, (int64_t)(r.i0 + cnt / r.jn) * e};
   }
-  fw_panel_lines_body<int32_t>(L, c.W, n, n, K, ws, 32, c.st, it.bid, lr);
+  fw_panel_lines_body<int32_t>(L, c.W, n, n, K, ws, 32, c.st, it.bid, lr,
+                               c.dsnap ? (const int32_t*)c.dsnap + K * 2 * BLK * BLK : nullptr);
+}
+
+// The per-step lower triangular components' diagonal blocks, closed before
+// the steps (no step changes block K before step K in such a matrix): item =
+// (component, block).
+static __global__ __launch_bounds__(PANEL_NT) void k_bfw_prewalk(const BComp* comps, const Item* items) {
+  const Item it = items[blockIdx.x];
+  const BComp c = comps[it.comp];
+  fw_diag_walk_body<int32_t>(c.W, c.npad, c.npad, (int32_t*)c.dsnap, 32, c.st, it.bid);
 }
 
 template <int RC>
@@ -293,6 +306,8 @@ struct rbg_batch {
   // triangular components' tiles 128, step by step | the same for tiles 64
   Item* d_items = nullptr;
   int64_t n_chunk = 0, n_panel = 0, n_t128 = 0, n_t64 = 0, n_tri128 = 0, n_tri64 = 0;
+  Item* d_prewalk = nullptr;  // (component, block) of the per-step components with dsnap
+  int64_t n_prewalk = 0;
   std::vector<int64_t> panel_cnt, t128_cnt, t64_cnt;  // per step: items of running components
   std::vector<int64_t> tri128_off, tri64_off;         // per step: [off[K], off[K + 1]) of the step lists
   std::vector<double> work;                           // per step: relaxations its tile launches execute
@@ -330,6 +345,10 @@ static int batch_launch(rbg_batch* b, hipStream_t s) {
                          limit);
     RBG_CHECK_LAUNCH();
     prof_stop(4, pf, s, b->flow_work);
+  }
+  if (b->n_prewalk) {
+    hipLaunchKernelGGL(k_bfw_prewalk, dim3((unsigned)b->n_prewalk), dim3(PANEL_NT), 0, s, b->d_comps, b->d_prewalk);
+    RBG_CHECK_LAUNCH();
   }
   for (int K = 0; K < b->stepnb; K++) {
     if (b->panel_cnt[K]) {
@@ -404,7 +423,10 @@ int rbg_batch_create(const rbg_batch_comp* comps, int32_t count, rbg_batch** out
     if (q.tile != 128 && q.tile != 64) return RBG_E_ARG;
     if (q.tri < 0 || q.tri > 2) return RBG_E_ARG;
     if (q.snap_bytes < FwWork::bytes(q.npad, 4)) return RBG_E_WORKSPACE;
-    c[i] = BComp{q.W, q.snap, q.st, q.npad, q.tile, q.tri, q.tri == 1 && q.snap_bytes >= flow_ctl_offset(q.npad, 4)};
+    const bool room = q.tri == 1 && q.snap_bytes >= flow_ctl_offset(q.npad, 4);
+    const int flow = room && q.persistent;
+    c[i] = BComp{q.W, q.snap, q.st, q.npad, q.tile, q.tri, flow,
+                 room && !flow ? (char*)q.snap + dsnap_offset(q.npad, 4) : nullptr};
   }
   // largest first: the components running at step K are a prefix
   std::stable_sort(c.begin(), c.end(), [](const BComp& a, const BComp& b) { return a.npad > b.npad; });
@@ -495,10 +517,20 @@ int rbg_batch_create(const rbg_batch_comp* comps, int32_t count, rbg_batch** out
   std::vector<Item> all;
   all.reserve(chunks.size() + panels.size() + t128.size() + t64.size() + tri128.size() + tri64.size());
   for (auto* v : {&chunks, &panels, &t128, &t64, &tri128, &tri64}) all.insert(all.end(), v->begin(), v->end());
+  std::vector<Item> prewalk;
+  for (int i = 0; i < count; i++)
+    if (c[i].dsnap)
+      for (int q = 0; q < c[i].npad / BLK; q++) prewalk.push_back({i, q});
+  b->n_prewalk = (int64_t)prewalk.size();
   hipError_t e = hipMalloc((void**)&b->d_comps, sizeof(BComp) * count);
   if (e == hipSuccess) e = hipMalloc((void**)&b->d_items, sizeof(Item) * all.size());
   if (e == hipSuccess) e = hipMemcpy(b->d_comps, c.data(), sizeof(BComp) * count, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(b->d_items, all.data(), sizeof(Item) * all.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess && b->n_prewalk) {
+    e = hipMalloc((void**)&b->d_prewalk, sizeof(Item) * prewalk.size());
+    if (e == hipSuccess)
+      e = hipMemcpy(b->d_prewalk, prewalk.data(), sizeof(Item) * prewalk.size(), hipMemcpyHostToDevice);
+  }
   if (e == hipSuccess && b->n_flow) {
     e = hipMalloc((void**)&b->d_flow, sizeof(BFlow) * fl.size());
     if (e == hipSuccess) e = hipMalloc((void**)&b->d_walk, sizeof(Item) * walk.size());
@@ -512,6 +544,7 @@ int rbg_batch_create(const rbg_batch_comp* comps, int32_t count, rbg_batch** out
   if (e != hipSuccess) {
     (void)hipFree(b->d_comps);
     (void)hipFree(b->d_items);
+    (void)hipFree(b->d_prewalk);
     (void)hipFree(b->d_flow);
     (void)hipFree(b->d_walk);
     (void)hipFree(b->d_ctl);
@@ -525,7 +558,7 @@ int rbg_batch_create(const rbg_batch_comp* comps, int32_t count, rbg_batch** out
 
 int rbg_batch_info(const rbg_batch* b, int64_t* info) {
   if (!b || !info) return RBG_E_ARG;
-  int64_t launches = 4 + (b->n_flow ? 3 : 0);
+  int64_t launches = 4 + (b->n_flow ? 3 : 0) + (b->n_prewalk ? 1 : 0);
   for (int K = 0; K < b->stepnb; K++)
     launches += (b->panel_cnt[K] > 0) + (b->t128_cnt[K] > 0) + (b->t64_cnt[K] > 0) +
                 (b->tri128_off[K + 1] > b->tri128_off[K]) + (b->tri64_off[K + 1] > b->tri64_off[K]);
@@ -587,6 +620,7 @@ int rbg_batch_destroy(rbg_batch* b) {
   if (b->graph) (void)hipGraphDestroy(b->graph);
   (void)hipFree(b->d_comps);
   (void)hipFree(b->d_items);
+  (void)hipFree(b->d_prewalk);
   (void)hipFree(b->d_flow);
   (void)hipFree(b->d_walk);
   (void)hipFree(b->d_ctl);
