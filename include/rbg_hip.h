@@ -450,7 +450,8 @@ typedef struct {
                   and the component takes the triangular schedule of rbg_fw_dense_tri_* (a component
                   that is not triangular as declared is flagged RBG_F_PRECOND_FAIL) */
   int32_t persistent; /* tri = 1 with rbg_batch_tri_snap_bytes: 1 = the persistent launch, 0 = the per-step
-                         launches with every diagonal block closed before the steps */
+                         launches; per step, a triangular component (tri = 1 or 2) with that workspace has
+                         every diagonal block closed before the steps */
   int64_t snap_bytes;
 } rbg_batch_comp;
 int64_t rbg_batch_snap_bytes(int32_t npad);

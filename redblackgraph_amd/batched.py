@@ -123,9 +123,9 @@ class BatchedClosure:
         else:
             pers = npad >= int(persistent)
             self.schedule = f"persistent from order {int(persistent)}"
-        # lower triangular components get the persistent closure's workspace
-        # either way: the per-step launches use its diagonal snapshots
-        self.layout = BatchLayout.build(sizes, self.tri == 1)
+        # triangular components get the persistent closure's workspace either
+        # way: the per-step launches use its diagonal snapshots
+        self.layout = BatchLayout.build(sizes, self.tri != 0)
         self.persistent_mask = (self.tri == 1) & pers
         self.count = len(self.layout.sizes)
         self.stream = stream if stream is not None else t.cuda.Stream()

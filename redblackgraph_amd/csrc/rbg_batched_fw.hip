@@ -35,8 +35,8 @@ struct BComp {
   int32_t tile;
   int32_t tri;   // 0 full schedule; 1 / 2: entries declared below / above the diagonal only
   int32_t flow;  // 1: closed by the persistent launch (snapshot ring of four)
-  // lower triangular, per-step launches: every diagonal block closed before
-  // the steps (k_bfw_prewalk), its step snapshots here for the panels
+  // triangular, per-step launches: every diagonal block closed before the
+  // steps (k_bfw_prewalk), its step snapshots here for the panels
   void* dsnap;
 };
 
@@ -212,8 +212,8 @@ static __global__ __launch_bounds__(NT) void k_bfw_panel(const BComp* comps, con
                                c.dsnap ? (const int32_t*)c.dsnap + K * 2 * BLK * BLK : nullptr);
 }
 
-// The per-step lower triangular components' diagonal blocks, closed before
-// the steps (no step changes block K before step K in such a matrix): item =
+// The per-step triangular components' diagonal blocks, closed before the
+// steps (no step changes block K before step K in such a matrix): item =
 // (component, block).
 static __global__ __launch_bounds__(PANEL_NT) void k_bfw_prewalk(const BComp* comps, const Item* items) {
   const Item it = items[blockIdx.x];
@@ -423,8 +423,11 @@ int rbg_batch_create(const rbg_batch_comp* comps, int32_t count, rbg_batch** out
     if (q.tile != 128 && q.tile != 64) return RBG_E_ARG;
     if (q.tri < 0 || q.tri > 2) return RBG_E_ARG;
     if (q.snap_bytes < FwWork::bytes(q.npad, 4)) return RBG_E_WORKSPACE;
-    const bool room = q.tri == 1 && q.snap_bytes >= flow_ctl_offset(q.npad, 4);
-    const int flow = room && q.persistent;
+    // a triangular component's diagonal block K is final before step K (a
+    // product into it needs an operand on the absent side), so its blocks
+    // can be closed before the steps; the persistent launch takes lower ones
+    const bool room = q.tri != 0 && q.snap_bytes >= flow_ctl_offset(q.npad, 4);
+    const int flow = room && q.tri == 1 && q.persistent;
     c[i] = BComp{q.W, q.snap, q.st, q.npad, q.tile, q.tri, flow,
                  room && !flow ? (char*)q.snap + dsnap_offset(q.npad, 4) : nullptr};
   }
