@@ -896,8 +896,9 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-budget-gib", type=int, default=64)
     ap.add_argument("--batch-count", type=int, default=4096)
-    ap.add_argument("--cpu-sample", type=int, default=1536,
-                    help="order of the dense CPU baseline's sample (about 20 s at 1536 on one core)")
+    ap.add_argument("--cpu-sample", type=int, default=2048,
+                    help="order of the dense CPU baseline's sample, run in full and N^3-scaled to the workload "
+                         "(about 26 s at 2048 on one core)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--prof-every", type=int, default=4,
                     help="bracket the kernels of every N-th timed step with timing events (1: every step)")
