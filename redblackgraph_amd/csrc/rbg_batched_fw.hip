@@ -159,8 +159,12 @@ template <bool ENC>
 static __global__ __launch_bounds__(NT) void k_bfw_convert(const BComp* comps, const Item* items) {
   const Item it = items[blockIdx.x];
   const BComp c = comps[it.comp];
-  const int nc = c.npad / CHUNK;
-  const int64_t i0 = (int64_t)(it.bid / nc) * CHUNK, j0 = (int64_t)(it.bid % nc) * CHUNK;
+  const int nc = c.npad / CHUNK, ci = it.bid / nc, cj = it.bid % nc;
+  const int64_t i0 = (int64_t)ci * CHUNK, j0 = (int64_t)cj * CHUNK;
+  // a chunk wholly on a triangular component's absent side is never read or
+  // written by its closure: checked (it must be all zero), never converted
+  const bool off = (c.tri == 1 && cj > ci) || (c.tri == 2 && cj < ci);
+  if (!ENC && off) return;
   int64_t vmax = 0;
   bool bad = false;
   for (int e = 4 * threadIdx.x; e < CHUNK * CHUNK; e += 4 * NT) {
@@ -180,7 +184,7 @@ static __global__ __launch_bounds__(NT) void k_bfw_convert(const BComp* comps, c
       }
       v[x] = ENC ? e_enc<int32_t>(v[x]) : e_dec<int32_t>(v[x]);
     }
-    *p = make_uint4((uint32_t)v[0], (uint32_t)v[1], (uint32_t)v[2], (uint32_t)v[3]);
+    if (!off) *p = make_uint4((uint32_t)v[0], (uint32_t)v[1], (uint32_t)v[2], (uint32_t)v[3]);
   }
   if (ENC) {
     if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(&c.st->flags, (uint32_t)RBG_F_PRECOND_FAIL);
