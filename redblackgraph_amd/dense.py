@@ -108,7 +108,7 @@ def fw_closure(W, et: ElemType, stream=None, force_general: bool = False, schedu
     sched = "full"
     if fast:
         # one pass: copy, fast-domain check and encoding (rbg_fw_load_*)
-        src = W if W.stride(1) == 1 else W.contiguous()
+        src = W if (W.stride(1) == 1 and W.stride(0) >= n) else W.contiguous()
         work = t.empty((npad, npad), dtype=W.dtype, device=W.device)
         fn = "rbg_fw_load_i32" if work.dtype == t.int32 else "rbg_fw_load_i64"
         _lib.call(fn, _lib.ptr(src), n, src.stride(0), _lib.ptr(work), npad, st.p, _lib.stream_handle(stream))

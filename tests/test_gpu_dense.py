@@ -303,6 +303,22 @@ def test_fw_load_equals_copy_precheck_and_encoding(dt, n, lda, bad):
     assert o1.status.flags == 0 and o1.status.aux[7] == 0
 
 
+def test_fw_closure_of_views():
+    """Input views the one-pass load cannot read with a row stride (a
+    transposed view, a broadcast row) close like their contiguous copies."""
+    t = _lib.torch()
+    et = elem_type(np.int32)
+    A = synth.dense_pedigree(synth.component_spec(300, 0.8, 4), 11)
+    Ad = _lib.to_device(A)
+    Wt, ot = dense.fw_closure(Ad.t(), et)
+    W2, o2 = dense.fw_closure(Ad.t().contiguous(), et)
+    assert bool(t.equal(Wt, W2)) and ot.diameter == o2.diameter and ot.schedule == "upper"
+    row = t.zeros(5, dtype=t.int32, device=Ad.device)
+    B = row.unsqueeze(0).expand(5, 5)  # row stride 0: all absent
+    Wb, ob = dense.fw_closure(B, et)
+    assert bool(t.equal(Wb, t.zeros((5, 5), dtype=t.int32, device=Ad.device)))
+
+
 @pytest.mark.parametrize("dt,n", [(np.int32, 2100), (np.int64, 900)])
 def test_fw_persistent_wait_limit_reruns_steps(dt, n):
     """A wait of the persistent closure that runs out, forced from the host
