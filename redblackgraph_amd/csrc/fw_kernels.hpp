@@ -2094,6 +2094,8 @@ struct FlowRowShared {
   U sY[BLK][BLK + 1];       // step K - 1's row snapshots of block J's columns: low bits [kk][j]
   uint8_t sM[BLK][BLK + 4]; // and shifts
   U Dc[BLK][BLK + 1];       // block K's diagonal column snapshots [kk][i]
+  int gL[BLK];              // per kk, the largest MSB among sL[kk][*]
+  int gD[BLK];              // and among Dc[kk][i], i > kk
   int64_t sMax[NT / 64];
   int sbad;
 };
@@ -2150,6 +2152,18 @@ __device__ __forceinline__ void fw_flow_row_body(FlowRowShared<typename unsigned
   }
   __syncthreads();
   if (L.sbad) return;
+  // the MSB guard per step: every column meets every row, so the largest left
+  // operand MSB of step kk plus a column's shift bounds that column's products
+  if (tid < 2 * BLK) {
+    const int kk = tid & (BLK - 1);
+    int g = 0;
+    for (int x = 0; x < BLK; x++) {
+      const int a = tid < BLK ? amsb(L.sL[kk][x]) : (x > kk ? amsb(L.Dc[kk][x]) : 0);
+      g = a > g ? a : g;
+    }
+    (tid < BLK ? L.gL : L.gD)[kk] = g;
+  }
+  __syncthreads();
   const int64_t seen = st->diameter_max;
   const int lim = width - 2;
   const int j = tid >> 3, i4 = tid & 7;
@@ -2163,11 +2177,11 @@ __device__ __forceinline__ void fw_flow_row_body(FlowRowShared<typename unsigned
   for (int kk = 0; kk < BLK; kk++) {
     const U m = L.sM[kk][j], y = L.sY[kk][j];
     const int mb = (T)y < 0 ? 0 : (int)m;
+    const int gk = L.gL[kk] + mb;
+    ga = gk > ga ? gk : ga;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       const U a = L.sL[kk][i4 + 8 * q];
-      const int gq = amsb(a) + mb;
-      ga = gq > ga ? gq : ga;
       const U v = (a << m) | y;
       r[q] = r[q] < v ? r[q] : v;
       dm = dm > (T)r[q] ? dm : (T)r[q];
@@ -2185,12 +2199,12 @@ __device__ __forceinline__ void fw_flow_row_body(FlowRowShared<typename unsigned
     const bool abs = (T)b < 0;
     const U m = abs ? 0 : (U)msb_u(b);
     const U y = abs ? TOP : b ^ ((U)1 << m);
+    const int gk = L.gD[kk] + (int)m;
+    ga = gk > ga ? gk : ga;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       if (i4 + 8 * q > kk) {
         const U a = L.Dc[kk][i4 + 8 * q];
-        const int gq = amsb(a) + (int)m;
-        ga = gq > ga ? gq : ga;
         const U v = (a << m) | y;
         r[q] = r[q] < v ? r[q] : v;
         dm = dm > (T)r[q] ? dm : (T)r[q];
